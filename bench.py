@@ -1,0 +1,191 @@
+"""Throughput of the explicit peridynamics step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Workload: the 1000x1000-node 2D RNP plate (h = 1e-4 m, horizon 4h,
+43,856,108 directed bonds) with the pre-notch and +/-0.5 m/s pulls, stepped
+with dt = 1e-8 s.  A "step" is one FDModel::integrateCD iteration (bond
+forces with break test, per-node sums, central-difference update, BCs).
+The timed region covers K steps with the state resident in HBM; bond
+updates/s = B * K / t.
+
+Reported: FP32-force mode (value) and FP64 plain mode, the dominant
+kernel's roofline from HIP events on the stepping stream, the displacement
+error and broken-bond mismatch of both modes against the FP64 exact GPU mode
+after the same steps, and the CPU oracle (OpenMP restatement) timed on the
+host on a bounded sample of steps of the same plate.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+# Compulsory HBM bytes of one step (per node: read x, u, v 3x16 B, volume
+# 8 B, flags 1 B, row offset 4 B, write u', v' 2x16 B = 93 B; per directed
+# bond: column index 4 B + broken bit 1/8 B = 4.125 B).
+BYTES_PER_NODE = 93.0
+BYTES_PER_BOND = 4.125
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="plate2d_1e6")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--modes", default="fp32,fp64")
+    return ap.parse_args()
+
+
+def traffic_from_profiles(workload: str, mode: str):
+    """HBM bytes per fused-step launch measured with rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), kept in
+    profiles/traffic_<workload>.json."""
+    f = ROOT / "profiles" / f"traffic_{workload}.json"
+    if not f.exists():
+        return None
+    try:
+        d = json.loads(f.read_text())
+        return d.get(mode, {}).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def run_mode(p, precision, steps, warmup):
+    from nlmech_amd.engine import GpuFD
+    g = GpuFD(p, precision=precision)
+    g.init()
+    if warmup > 0:
+        g.step(warmup)
+    t0 = time.perf_counter()
+    ms_total, ms_fused = g.timed_steps(steps)
+    wall = time.perf_counter() - t0
+    u, _, _ = g.get_state()
+    bonds = g.get_bonds()
+    nnz = g.nnz
+    return g, dict(ms_total=ms_total, ms_fused=ms_fused, wall=wall, u=u,
+                   bonds=bonds, nnz=nnz)
+
+
+def cpu_baseline(p, csr, seconds):
+    from oracle import pyoracle as po
+    o = po.OracleFD(p, csr)
+    o.init()
+    t0 = time.perf_counter()
+    o.step(1)
+    one = time.perf_counter() - t0
+    k = max(1, min(200, int(seconds / max(one, 1e-6))))
+    t0 = time.perf_counter()
+    o.step(k)
+    dt = time.perf_counter() - t0
+    nnz = int(csr[0][-1])
+    cores = po.load().ora_num_threads()
+    o.close()
+    return {"value": nnz * k / dt, "unit": "bond updates/s", "cores": cores,
+            "kind": "port",
+            "sample": f"{k} integrateCD steps of the same {p.n}-node plate "
+                      f"({nnz} bonds) after 1 untimed step, OpenMP "
+                      f"restatement, {dt:.1f} s"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not "
+              "supported (the sharding design is described in DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+    from nlmech_amd import problems as pr
+    from nlmech_amd.capi import (NLM_PREC_FP32_FORCE, NLM_PREC_FP64,
+                                 NLM_PREC_FP64_EXACT)
+
+    p = pr.workload(a.workload)
+    modes = {"fp32": NLM_PREC_FP32_FORCE, "fp64": NLM_PREC_FP64}
+    res = {}
+    csr = None
+    for name in a.modes.split(","):
+        g, r = run_mode(p, modes[name], a.steps, a.warmup)
+        if csr is None:
+            csr = g.neighbors()
+        g.close()
+        res[name] = r
+        print(f"[bench] {name}: {r['ms_total']:.1f} ms for {a.steps} steps, "
+              f"fused step kernel {r['ms_fused']*1e3:.1f} us", file=sys.stderr,
+              flush=True)
+    # accuracy against the FP64 exact mode after the same steps
+    ge, re_ = run_mode(p, NLM_PREC_FP64_EXACT, a.steps, a.warmup)
+    ge.close()
+    print(f"[bench] exact: fused step kernel {re_['ms_fused']*1e3:.1f} us",
+          file=sys.stderr, flush=True)
+    uref = re_["u"]
+    bref = re_["bonds"]
+    acc = {}
+    for name, r in res.items():
+        d = r["u"] - uref
+        acc[name] = {
+            "l2_rel": float(np.linalg.norm(d) / np.linalg.norm(uref)),
+            "linf_rel": float(np.abs(d).max() / np.abs(uref).max()),
+            "broken_bond_mismatch": int(np.count_nonzero(r["bonds"] != bref)),
+            "broken_bonds_ref": int(bref.sum()),
+        }
+    nnz = res[next(iter(res))]["nnz"]
+    bytes_launch = BYTES_PER_NODE * p.n + BYTES_PER_BOND * nnz
+    main_mode = "fp32" if "fp32" in res else next(iter(res))
+    r = res[main_mode]
+    value = nnz * a.steps / (r["ms_total"] * 1e-3)
+    achieved = bytes_launch / (r["ms_fused"] * 1e-3) / 1e9
+    traffic = traffic_from_profiles(a.workload, main_mode)
+    out = {
+        "metric": "bond updates/s (FP32-force mode; FP64 plain alongside)",
+        "value": value,
+        "unit": "bond updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": r["ms_total"] / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 bond force, f64 state" if main_mode == "fp32" else "f64",
+        "data": "synthetic",
+        "config": {"workload": a.workload, "nodes": p.n, "bonds": nnz,
+                   "horizon_over_h": int(round(p.horizon / p.h)),
+                   "dt": p.dt, "mode": main_mode},
+        "roofline": {"bound": "hbm", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic},
+        "modes": {},
+        "accuracy_vs_fp64_exact": acc,
+        "exact_mode_ms_per_step": re_["ms_total"] / a.steps,
+    }
+    for name, rr in res.items():
+        v = nnz * a.steps / (rr["ms_total"] * 1e-3)
+        ach = bytes_launch / (rr["ms_fused"] * 1e-3) / 1e9
+        out["modes"][name] = {"value": v, "ms_per_step": rr["ms_total"] / a.steps,
+                              "fused_kernel_us": rr["ms_fused"] * 1e3,
+                              "roofline_frac": ach / HBM_PEAK_GBS}
+    if "fp32" in res and "fp64" in res:
+        out["fp32_over_fp64"] = (out["modes"]["fp32"]["value"] /
+                                 out["modes"]["fp64"]["value"])
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(p, csr, a.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,252 @@
+/*
+ * nlmech_amd.h -- C-ABI of the MI355X peridynamics engine.
+ *
+ * The library replaces the explicit-dynamics hot path of NLMech's FDModel:
+ * neighbor search, the RNPBond bond force with its break test, and the
+ * central-difference update, computed by hand-written HIP kernels for gfx950.
+ *
+ * Conventions
+ *   - Every entry point returns NLM_OK (0) or a negative NLM_ERR_* code; the
+ *     message of the last failure is nlm_last_error(ctx).  Nothing in the
+ *     library calls exit() (the reference's error convention is
+ *     std::cerr + exit(1), e.g. src/material/pd/rnpBond.cpp:58-60).
+ *   - Host arrays are owned by the caller, passed with explicit sizes, copied
+ *     during the call and never retained.
+ *   - Node vectors at this boundary are AoS Point3, double[3*n] with z = 0 in
+ *     2D: the layout of the reference's std::vector<util::Point3>
+ *     (src/util/point.h:29-516, src/data/DataManager.h:74-372), so a binding
+ *     passes &u[0].d_x directly.
+ *   - Neighbor lists are CSR with int64 row offsets and int32 columns, each row
+ *     ascending by global id (the order of the reference's brute-force loop,
+ *     src/geometry/neighbor.cpp:69-89).  Bond bits are addressed as
+ *     (node i, local index jl) exactly as geometry::Fracture does
+ *     (src/geometry/fracture.cpp:162-187).
+ *   - One host thread per context; all device work is ordered on the
+ *     context's HIP stream.  Contexts are independent.
+ */
+#ifndef NLMECH_AMD_H
+#define NLMECH_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NLM_OK 0
+#define NLM_ERR_INVALID (-1) /* bad argument or unsupported option */
+#define NLM_ERR_HIP (-2)     /* HIP runtime failure */
+#define NLM_ERR_STATE (-3)   /* call out of order (e.g. step before setup) */
+#define NLM_ERR_MEMORY (-4)  /* allocation failed */
+
+typedef struct nlm_ctx nlm_ctx;
+
+/* Precision modes of the bond-force kernels. */
+#define NLM_PREC_FP64_EXACT 0 /* reference operation order, no FMA, IEEE div */
+#define NLM_PREC_FP64 1       /* FP64, break test as the reference, FMA elsewhere */
+#define NLM_PREC_FP32_FORCE 2 /* FP64 state / stretch / break test, FP32 force */
+
+/* Time integrators, inp::ModelDeck::d_timeDiscretization
+ * (src/inp/input.cpp:137-147). */
+#define NLM_TIME_CENTRAL_DIFFERENCE 0 /* FDModel::integrateCD, fDModel.cpp:377 */
+#define NLM_TIME_VELOCITY_VERLET 1    /* FDModel::integrateVerlet, :446 */
+
+/* ------------------------------------------------------------------------ */
+/* Context                                                                   */
+/* ------------------------------------------------------------------------ */
+int nlm_ctx_create(nlm_ctx** out, int device);
+void nlm_ctx_destroy(nlm_ctx* ctx);
+const char* nlm_last_error(const nlm_ctx* ctx);
+const char* nlm_version(void);
+/* Name of the GPU the context runs on (e.g. "gfx950"). */
+int nlm_device_arch(const nlm_ctx* ctx, char* buf, int len);
+
+/* ------------------------------------------------------------------------ */
+/* Material: RNPBond (src/material/pd/rnpBond.cpp)                           */
+/* ------------------------------------------------------------------------ */
+
+/* Fields of inp::MaterialDeck used by RNPBond (src/inp/decks/materialDeck.h:
+ * 226-288; MatData entries default to -1, :20-99) plus the model sizes. */
+typedef struct {
+  int dim;
+  double horizon;
+  double h; /* mesh size */
+  double density;
+  double E, K, Gc, KIc; /* < 0: not given */
+  int plane_strain;
+  int compute_from_classical;
+  int n_bond_params;
+  double bond_params[2];
+  int inf_type; /* 0 constant, 1 linear, 2 gaussian (influenceFn.h) */
+  int n_inf_params;
+  double inf_params[2];
+  double check_sc_factor;
+  int irreversible;
+  int no_penetration;
+} nlm_material_deck;
+
+/* Derived per-bond parameters: the state RNPBond keeps after its constructor
+ * (rnpBond.cpp:23-97, computeParameters :99-189). */
+typedef struct {
+  int dim;
+  double horizon, h, density;
+  double C, beta, rbar, inv_factor, factor_sc;
+  int irreversible;
+  int inf_type;
+  double inf_a, inf_b;
+  double inf_moment;
+  int contact;
+  double contact_kn, contact_rc;
+  double E, K, nu, lambda, G, Gc, KIc;
+} nlm_rnp_params;
+
+/* Replaces RNPBond::RNPBond + computeParameters (rnpBond.cpp:23-189). */
+int nlm_rnp_params_from_deck(const nlm_material_deck* deck,
+                             nlm_rnp_params* out);
+
+/* ------------------------------------------------------------------------ */
+/* Neighbor search: geometry::Neighbor::Neighbor (neighbor.cpp:24-92)       */
+/* ------------------------------------------------------------------------ */
+
+/* Builds the neighbor lists of n nodes on the GPU with a cell list.  Node j
+ * is a neighbor of i iff j != i and definitelyLessThan(|xi - xj|,
+ * horizon + 1e-10) (neighbor.cpp:81-83); rows come out ascending.  The lists
+ * stay on the device for nlm_fd_setup; *nnz_out receives the total count. */
+int nlm_neighbor_build(nlm_ctx* ctx, int dim, int64_t n, const double* x,
+                       double horizon, int64_t* nnz_out);
+/* Copies the lists of the last nlm_neighbor_build (row_ptr[n+1], col[nnz]).
+ * Replaces Neighbor::getNeighborsList (neighbor.cpp:106-112). */
+int nlm_neighbor_get(nlm_ctx* ctx, int64_t* row_ptr, int32_t* col);
+
+/* ------------------------------------------------------------------------ */
+/* Explicit FD model: FDModel<RNPBond> (src/model/fd/fDModel.cpp)            */
+/* ------------------------------------------------------------------------ */
+
+/* Mesh, neighbor lists and model constants.  row_ptr/col may be NULL to use
+ * the lists of the last nlm_neighbor_build on this context.  interior may be
+ * NULL (every node interior: geometry::BaseInterior, interiorFlags.cpp:27-30);
+ * otherwise interior[i] != 0 marks node i as breakable
+ * (InteriorFlags::getInteriorFlag, interiorFlags.cpp:228-231). */
+typedef struct {
+  int dim; /* 2 (3 accepted for the bond-based model) */
+  int64_t n;
+  const double* x;   /* [3n] reference coordinates, Mesh::getNodes */
+  const double* vol; /* [n] nodal volumes, Mesh::getNodalVolumes */
+  const int64_t* row_ptr;
+  const int32_t* col;
+  const uint8_t* interior;
+  double dt; /* ModelDeck::d_dt = Final_Time / Time_Steps (input.cpp:184) */
+  int precision;   /* NLM_PREC_* */
+  int time_scheme; /* NLM_TIME_* */
+  nlm_rnp_params mat;
+} nlm_fd_problem;
+
+/* Region / function codes of inp::BCData (src/inp/decks/loadingDeck.h). */
+#define NLM_REGION_RECTANGLE 0
+#define NLM_REGION_ANGLED_RECTANGLE 1
+#define NLM_REGION_CIRCLE 2
+#define NLM_REGION_TORUS 3
+#define NLM_REGION_LINE 4
+#define NLM_REGION_CUBOID 5
+#define NLM_TFN_CONSTANT 0
+#define NLM_TFN_LINEAR 1
+#define NLM_TFN_QUADRATIC 2
+#define NLM_TFN_SIN 3
+#define NLM_TFN_LINEAR_STEP 4
+#define NLM_TFN_LINEAR_SLOW_FAST 5
+#define NLM_SFN_CONSTANT 0
+#define NLM_SFN_SIN_X 1
+#define NLM_SFN_SIN_Y 2
+#define NLM_SFN_SIN_Z 3
+#define NLM_SFN_LINEAR_X 4
+#define NLM_SFN_LINEAR_Y 5
+#define NLM_SFN_LINEAR_Z 6
+#define NLM_SFN_HAT_X 7
+#define NLM_SFN_HAT_Y 8
+#define NLM_SFN_HAT_Z 9
+
+typedef struct {
+  int region;
+  double x1, x2, y1, y2, z1, z2, r1, r2, theta;
+  int ndir;
+  int dir[3]; /* 1-based dofs */
+  int time_fn;
+  int n_time_params;
+  double time_params[4];
+  int space_fn;
+  int n_space_params;
+  double space_params[4];
+} nlm_bc;
+
+/* Edge crack of inp::FractureDeck (src/inp/decks/fractureDeck.h:22-104). */
+typedef struct {
+  int o; /* 1 horizontal, -1 vertical, 0 angled */
+  double pb[2], pt[2];
+  double theta;
+  double activation_time; /* < 0: applied at setup (Fracture ctor) */
+} nlm_crack;
+
+/* Uploads mesh, lists and constants; allocates the device state (u = v = f
+ * = 0, every bond intact).  Replaces the allocation part of
+ * FDModel::initHObjects/init (fDModel.cpp:134-315). */
+int nlm_fd_setup(nlm_ctx* ctx, const nlm_fd_problem* p);
+/* Displacement BC sets: ULoading::ULoading node selection and fixity
+ * (uLoading.cpp:18-164); applied by nlm_fd_init / every step
+ * (ULoading::apply, :166-253).  At most 15 sets. */
+int nlm_fd_set_ubc(nlm_ctx* ctx, int nsets, const nlm_bc* sets);
+/* Force BC sets: FLoading (fLoading.cpp:18-279).  At most 8 sets. */
+int nlm_fd_set_fbc(nlm_ctx* ctx, int nsets, const nlm_bc* sets);
+/* Pre-cracks: cracks with activation_time < 0 break their crossing bonds now
+ * (Fracture::Fracture -> computeFracturedBondFd, fracture.cpp:20-160); the
+ * others are applied during stepping when the time passes their activation
+ * time (Fracture::addCrack, :45-72). */
+int nlm_fd_add_cracks(nlm_ctx* ctx, int ncracks, const nlm_crack* cracks);
+
+/* State transfer, AoS double[3n]; any pointer may be NULL. */
+int nlm_fd_set_state(nlm_ctx* ctx, const double* u, const double* v,
+                     const double* f);
+int nlm_fd_get_state(nlm_ctx* ctx, double* u, double* v, double* f);
+/* Bond bits, one byte (0/1) per directed bond in CSR order. */
+int nlm_fd_set_bonds(nlm_ctx* ctx, const uint8_t* bond_state);
+int nlm_fd_get_bonds(nlm_ctx* ctx, uint8_t* bond_state);
+/* Bond bits in the reference packing: ceil(n_i/8) bytes per node,
+ * concatenated over nodes (Fracture::d_fracture, fracture.cpp:24-33). */
+int nlm_fd_bond_bytes_size(nlm_ctx* ctx, int64_t* size);
+int nlm_fd_get_bond_bytes(nlm_ctx* ctx, uint8_t* bytes);
+/* Fixity bits per node (Mesh::d_fix, mesh.h:194-200). */
+int nlm_fd_get_fixity(nlm_ctx* ctx, uint8_t* fix);
+/* Model time and step count (FDModel::d_time, d_n). */
+int nlm_fd_get_time(nlm_ctx* ctx, double* time, int64_t* step);
+int nlm_fd_set_time(nlm_ctx* ctx, double time, int64_t step);
+
+/* integrate() prologue (fDModel.cpp:318-333): displacement and force BCs at
+ * the current time, then the internal forces (bond bits updated). */
+int nlm_fd_init(nlm_ctx* ctx);
+/* nsteps iterations of the integrate() loop (fDModel.cpp:344-373) with the
+ * configured scheme: update, BCs at t+dt, internal forces, crack activation.
+ * On return u, v, f, bond bits, time and step equal the reference's state
+ * after the same iterations. */
+int nlm_fd_step(nlm_ctx* ctx, int nsteps);
+/* computeForces (fDModel.cpp:528-541): f += internal force of the current u
+ * (force BCs are not added; bond bits updated). */
+int nlm_fd_compute_forces(nlm_ctx* ctx);
+/* Damage and energy fields of computePostProcFields (fDModel.cpp:652-778):
+ * Z_i = max_j |S|/Sc(r), phi_i = 1 - sum unbroken V_j / sum V_j (float like
+ * the reference), strain energy of node i.  Any pointer may be NULL. */
+int nlm_fd_postproc(nlm_ctx* ctx, float* Z, float* phi, double* energy);
+
+/* Measurement: runs nlm_fd_step(nsteps) bracketed by HIP events recorded on
+ * the context's stream; returns the elapsed milliseconds of the whole call
+ * and the mean duration of the fused step kernel. */
+int nlm_fd_timed_steps(nlm_ctx* ctx, int nsteps, double* ms_total,
+                       double* ms_per_step);
+/* Number of directed bonds of the configured model. */
+int nlm_fd_num_bonds(nlm_ctx* ctx, int64_t* nnz);
+/* Name of the fused step kernel launched for the configured model. */
+const char* nlm_fd_kernel_name(nlm_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NLMECH_AMD_H */

@@ -1,0 +1,1311 @@
+// fd.hip -- explicit peridynamics on the GPU: FDModel<RNPBond>'s force
+// computation and time integration (src/model/fd/fDModel.cpp).
+//
+// Per time step one fused kernel does, for every node i (one lane per node,
+// one wavefront per 64-node slice of the neighbor lists):
+//   - RNPBond::getBondEF over the row of i in list order (rnpBond.cpp:216-301)
+//     including the break test and the bond-bit update, summed sequentially
+//     like FDModel::computeForce (fDModel.cpp:544-582);
+//   - the force BCs of FLoading::apply at the force's time
+//     (fLoading.cpp:117-279), f_i = f_ext + sum (fDModel.cpp:536-537);
+//   - the central-difference update of FDModel::integrateCD
+//     (fDModel.cpp:377-425) into the other u/v buffer, and the displacement BCs
+//     of ULoading::apply at t+dt (uLoading.cpp:166-253) on fixed dofs.
+// The neighbor displacements are read from the current buffer only, so the
+// update of node i never races with the force of another node.
+//
+// Bond bits: slot k of the 64 nodes of a slice is one 64-bit word (bit =
+// lane); a wavefront reads it once, and when bonds break it ORs the ballot of
+// newly broken lanes into the word with one vector store from lane 0.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "bond_rnp.h"
+#include "device_common.h"
+#include "nlm_internal.h"
+
+namespace nlm {
+namespace {
+
+template <int DIM>
+struct FdArgs {
+  using V = typename VT<DIM>::T;
+  int64_t n;
+  int64_t nslices;
+  const V* __restrict__ x;
+  const double* __restrict__ vol;
+  const uint32_t* __restrict__ flags;
+  const int64_t* __restrict__ sl_ptr;
+  const int32_t* __restrict__ col;
+  const uint16_t* __restrict__ rowlen;
+  unsigned long long* __restrict__ bits;
+  const V* __restrict__ u;
+  const V* __restrict__ v;
+  V* __restrict__ u_out;
+  V* __restrict__ v_out;
+  const V* __restrict__ f_in;
+  V* __restrict__ f_out;
+  const BcTable* __restrict__ bc;
+  RnpDev m;
+  double dt;
+  double fact;
+  int has_fbc;
+  int has_nofail;
+};
+
+template <int DIM>
+__device__ __forceinline__ void load3(const typename VT<DIM>::T& a,
+                                      double* o) {
+  o[0] = a.x;
+  o[1] = a.y;
+  o[2] = 0.;
+  if constexpr (DIM == 3) o[2] = a.z;
+}
+
+template <int DIM>
+__device__ __forceinline__ typename VT<DIM>::T make_v(const double* o) {
+  typename VT<DIM>::T r;
+  r.x = o[0];
+  r.y = o[1];
+  if constexpr (DIM == 3) {
+    r.z = o[2];
+    r.w = 0.;
+  }
+  return r;
+}
+
+template <int DIM, int MODE>
+__device__ __forceinline__ void bond_mode(const RnpDev& m, const double* xi,
+                                          const double* ui, const double* xj,
+                                          const double* uj, double voli,
+                                          double volj, bool brk, bool& fs,
+                                          double* F) {
+  if constexpr (MODE == kModeExact)
+    bond_exact<DIM, false>(m, xi, ui, xj, uj, voli, volj, brk, fs, F, nullptr);
+  else if constexpr (MODE == kModeFp64)
+    bond_fp64<DIM>(m, xi, ui, xj, uj, voli, volj, brk, fs, F);
+  else
+    bond_fp32<DIM>(m, xi, ui, xj, uj, voli, volj, brk, fs, F);
+}
+
+// External force density of node i at time t (FLoading::apply order: sets in
+// order, each adding to every dof it loads).
+template <int DIM>
+__device__ __forceinline__ void fext_node(const BcTable* bc, uint32_t fl,
+                                          const double* xi, double t,
+                                          double* b) {
+  uint32_t mem = (fl >> kFlagFbcShift) & 0xffu;
+  for (int s = 0; s < bc->nf; s++) {
+    if (!((mem >> s) & 1u)) continue;
+    const BcSetDev& set = bc->f[s];
+    double val = fbc_eval(set, xi[0], xi[1], xi[2], t);
+    for (int d = 0; d < DIM; d++)
+      if ((set.dirmask >> d) & 1) b[d] += val;
+  }
+}
+
+// KIND_FORCE: f_out = base + internal force (base = f_in or f_ext(t_force)).
+// KIND_CD: fused force at t_force + central-difference update + BCs at t_next.
+constexpr int KIND_FORCE = 0;
+constexpr int KIND_CD = 1;
+constexpr int KIND_VERLET = 2;
+
+template <int DIM, int MODE, int KIND, bool BASE_F>
+__global__ __launch_bounds__(256) void k_fd(FdArgs<DIM> a, double t_force,
+                                            double t_next) {
+  using V = typename VT<DIM>::T;
+  int64_t nblk = (a.nslices * 64 + 255) / 256;
+  int64_t lb = xcd_block(nblk);
+  if (lb >= nblk) return;
+  int64_t i = lb * 256 + threadIdx.x;
+  int lane = threadIdx.x & 63;
+  int64_t s = i >> 6;
+  if (s >= a.nslices) return;  // whole wavefront
+  bool valid = i < a.n;
+  int64_t cb = a.sl_ptr[s];
+  int width = int((a.sl_ptr[s + 1] - cb) >> 6);
+  int64_t bb = cb >> 6;
+  int len = valid ? int(a.rowlen[i]) : 0;
+  double xi[3] = {0., 0., 0.}, ui[3] = {0., 0., 0.};
+  double voli = 1.;
+  uint32_t fl = 0;
+  if (valid) {
+    load3<DIM>(a.x[i], xi);
+    load3<DIM>(a.u[i], ui);
+    voli = a.vol[i];
+    fl = a.flags[i];
+  }
+  bool int_i = !(fl & kFlagNonInterior);
+  double acc[3] = {0., 0., 0.};
+  for (int k = 0; k < width; ++k) {
+    unsigned long long w = a.bits[bb + k];
+    bool newly = false;
+    if (k < len) {
+      int32_t j = a.col[cb + int64_t(k) * 64 + lane];
+      double xj[3], uj[3];
+      load3<DIM>(a.x[j], xj);
+      load3<DIM>(a.u[j], uj);
+      double volj = a.vol[j];
+      bool brk = int_i;
+      if (a.has_nofail) brk = brk && !(a.flags[j] & kFlagNonInterior);
+      bool fs0 = (w >> lane) & 1ull;
+      bool fs = fs0;
+      double F[3];
+      bond_mode<DIM, MODE>(a.m, xi, ui, xj, uj, voli, volj, brk, fs, F);
+      newly = fs && !fs0;
+      acc[0] += F[0];
+      acc[1] += F[1];
+      if (DIM == 3) acc[2] += F[2];
+    }
+    unsigned long long nb = __ballot(newly);
+    if (nb != 0ull && lane == 0) a.bits[bb + k] = w | nb;
+  }
+  if (!valid) return;
+  double f[3] = {0., 0., 0.};
+  if (BASE_F) {
+    load3<DIM>(a.f_in[i], f);
+  } else if (a.has_fbc) {
+    fext_node<DIM>(a.bc, fl, xi, t_force, f);
+  }
+  for (int d = 0; d < DIM; d++) f[d] += acc[d];
+  if (KIND == KIND_FORCE) {
+    a.f_out[i] = make_v<DIM>(f);
+    return;
+  }
+  double vi[3];
+  load3<DIM>(a.v[i], vi);
+  double un[3] = {0., 0., 0.}, vn[3] = {0., 0., 0.};
+  for (int d = 0; d < DIM; d++) {
+    uint32_t id = (fl >> (kFlagUbcShift + 4 * d)) & 0xfu;
+    if (id == 0) {
+      if (KIND == KIND_CD) {
+        // u += fact*f + dt*v; v = (u - u_old)/dt  (fDModel.cpp:386-394)
+        double uo = ui[d];
+        un[d] = uo + (a.fact * f[d] + a.dt * vi[d]);
+        vn[d] = (un[d] - uo) / a.dt;
+      }
+    } else {
+      double du, dv;
+      ubc_eval(a.bc->u[id - 1], xi[0], xi[1], xi[2], t_next, du, dv);
+      un[d] = du;
+      vn[d] = dv;
+    }
+  }
+  a.u_out[i] = make_v<DIM>(un);
+  a.v_out[i] = make_v<DIM>(vn);
+}
+
+// Update from a stored force (the first iteration of a stepping call):
+// integrateCD's update loop and the BCs at t_next.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_update_cd(FdArgs<DIM> a,
+                                                   double t_next) {
+  int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= a.n) return;
+  double xi[3], ui[3], vi[3], f[3];
+  load3<DIM>(a.x[i], xi);
+  load3<DIM>(a.u[i], ui);
+  load3<DIM>(a.v[i], vi);
+  load3<DIM>(a.f_in[i], f);
+  uint32_t fl = a.flags[i];
+  double un[3] = {0., 0., 0.}, vn[3] = {0., 0., 0.};
+  for (int d = 0; d < DIM; d++) {
+    uint32_t id = (fl >> (kFlagUbcShift + 4 * d)) & 0xfu;
+    if (id == 0) {
+      double uo = ui[d];
+      un[d] = uo + (a.fact * f[d] + a.dt * vi[d]);
+      vn[d] = (un[d] - uo) / a.dt;
+    } else {
+      double du, dv;
+      ubc_eval(a.bc->u[id - 1], xi[0], xi[1], xi[2], t_next, du, dv);
+      un[d] = du;
+      vn[d] = dv;
+    }
+  }
+  a.u_out[i] = make_v<DIM>(un);
+  a.v_out[i] = make_v<DIM>(vn);
+}
+
+// Velocity-Verlet pieces (FDModel::integrateVerlet, fDModel.cpp:446-525).
+// Step 1-2: v += fact f; u += dt v on free dofs; BCs at t_next.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_verlet_a(FdArgs<DIM> a,
+                                                  double t_next) {
+  int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= a.n) return;
+  double xi[3], ui[3], vi[3], f[3];
+  load3<DIM>(a.x[i], xi);
+  load3<DIM>(a.u[i], ui);
+  load3<DIM>(a.v[i], vi);
+  load3<DIM>(a.f_in[i], f);
+  uint32_t fl = a.flags[i];
+  for (int d = 0; d < DIM; d++) {
+    uint32_t id = (fl >> (kFlagUbcShift + 4 * d)) & 0xfu;
+    if (id == 0) {
+      vi[d] += a.fact * f[d];
+      ui[d] += a.dt * vi[d];
+    } else {
+      double du, dv;
+      ubc_eval(a.bc->u[id - 1], xi[0], xi[1], xi[2], t_next, du, dv);
+      ui[d] = du;
+      vi[d] = dv;
+    }
+  }
+  a.u_out[i] = make_v<DIM>(ui);
+  a.v_out[i] = make_v<DIM>(vi);
+}
+// Step 3: v += fact f on free dofs (in place).
+template <int DIM>
+__global__ __launch_bounds__(256) void k_verlet_b(FdArgs<DIM> a) {
+  int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= a.n) return;
+  double vi[3], f[3];
+  load3<DIM>(a.v[i], vi);
+  load3<DIM>(a.f_in[i], f);
+  uint32_t fl = a.flags[i];
+  for (int d = 0; d < DIM; d++) {
+    uint32_t id = (fl >> (kFlagUbcShift + 4 * d)) & 0xfu;
+    if (id == 0) vi[d] += a.fact * f[d];
+  }
+  a.v_out[i] = make_v<DIM>(vi);
+}
+
+// Displacement BCs at time t on fixed dofs (ULoading::apply).
+template <int DIM>
+__global__ __launch_bounds__(256) void k_apply_ubc(FdArgs<DIM> a, double t) {
+  int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= a.n) return;
+  uint32_t fl = a.flags[i];
+  if ((fl & 0xfffu) == 0) return;
+  double xi[3], ui[3], vi[3];
+  load3<DIM>(a.x[i], xi);
+  load3<DIM>(a.u[i], ui);
+  load3<DIM>(a.v[i], vi);
+  for (int d = 0; d < DIM; d++) {
+    uint32_t id = (fl >> (kFlagUbcShift + 4 * d)) & 0xfu;
+    if (id) ubc_eval(a.bc->u[id - 1], xi[0], xi[1], xi[2], t, ui[d], vi[d]);
+  }
+  a.u_out[i] = make_v<DIM>(ui);
+  a.v_out[i] = make_v<DIM>(vi);
+}
+
+// f_out = f_ext(t) (FLoading::apply after the force reset).
+template <int DIM>
+__global__ __launch_bounds__(256) void k_fext(FdArgs<DIM> a, double t,
+                                              int add) {
+  int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= a.n) return;
+  double xi[3], f[3] = {0., 0., 0.};
+  load3<DIM>(a.x[i], xi);
+  if (add) load3<DIM>(a.f_in[i], f);
+  fext_node<DIM>(a.bc, a.flags[i], xi, t, f);
+  a.f_out[i] = make_v<DIM>(f);
+}
+
+// ---- edge cracks: Fracture::computeFracturedBondFd (fracture.cpp:74-160) --
+
+__device__ __forceinline__ bool pt_outside(double px, double py, int o,
+                                           const nlm_crack& c) {
+  // EdgeCrack::ptOutside with its default theta = 0 (fractureDeck.h:115-147;
+  // computeFracturedBondFd does not pass the crack angle)
+  if (o == -1) return def_less(py, c.pb[1]) || def_greater(py, c.pt[1]);
+  if (o == 1) return def_less(px, c.pb[0]) || def_greater(px, c.pt[0]);
+  if (o == 0) {
+    double th = 0.0;
+    double qx = px - c.pb[0], qy = py - c.pb[1];
+    double pmx = qx * cos(th) + qy * sin(th);
+    double tx = c.pt[0] - c.pb[0], ty = c.pt[1] - c.pb[1];
+    double ptx = tx * cos(th) + ty * sin(th);
+    return def_less(pmx, 0.0) || def_greater(pmx, ptx);
+  }
+  return true;
+}
+__device__ __forceinline__ bool pt_left(double px, double py,
+                                        const nlm_crack& c) {
+  // EdgeCrack::ptLeftside, fractureDeck.h:156-174
+  double ar = 0.5 * ((c.pt[0] - c.pb[0]) * (py - c.pb[1]) -
+                     (px - c.pb[0]) * (c.pt[1] - c.pb[1]));
+  return !def_less(ar, 0.0);
+}
+
+template <int DIM>
+__global__ __launch_bounds__(256) void k_precrack(FdArgs<DIM> a,
+                                                  nlm_crack c) {
+  int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  int lane = threadIdx.x & 63;
+  int64_t s = i >> 6;
+  if (s >= a.nslices) return;
+  bool valid = i < a.n;
+  int64_t cb = a.sl_ptr[s];
+  int width = int((a.sl_ptr[s + 1] - cb) >> 6);
+  int64_t bb = cb >> 6;
+  int len = valid ? int(a.rowlen[i]) : 0;
+  double xi[3] = {0., 0., 0.};
+  if (valid) load3<DIM>(a.x[i], xi);
+  bool active = valid && !pt_outside(xi[0], xi[1], c.o, c);
+  bool left = active && pt_left(xi[0], xi[1], c);
+  for (int k = 0; k < width; ++k) {
+    bool set = false;
+    if (active && k < len) {
+      int32_t j = a.col[cb + int64_t(k) * 64 + lane];
+      double xj[3];
+      load3<DIM>(a.x[j], xj);
+      bool modify = !pt_outside(xj[0], xj[1], c.o, c);
+      bool jl = pt_left(xj[0], xj[1], c);
+      set = modify && (left ? !jl : jl);
+    }
+    unsigned long long m = __ballot(set);
+    if (m != 0ull && lane == 0) a.bits[bb + k] |= m;
+  }
+}
+
+// ---- bond-bit conversions ------------------------------------------------
+
+template <int DIM>
+__global__ __launch_bounds__(256) void k_bits_to_bytes(
+    FdArgs<DIM> a, const int64_t* __restrict__ row_ptr,
+    uint8_t* __restrict__ out) {
+  int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= a.n) return;
+  int lane = int(i & 63);
+  int64_t s = i >> 6;
+  int64_t bb = a.sl_ptr[s] >> 6;
+  int len = a.rowlen[i];
+  int64_t r0 = row_ptr[i];
+  for (int k = 0; k < len; k++)
+    out[r0 + k] = uint8_t((a.bits[bb + k] >> lane) & 1ull);
+}
+
+template <int DIM>
+__global__ __launch_bounds__(64) void k_bytes_to_bits(
+    FdArgs<DIM> a, const int64_t* __restrict__ row_ptr,
+    const uint8_t* __restrict__ in) {
+  int64_t s = blockIdx.x;
+  if (s >= a.nslices) return;
+  int lane = threadIdx.x;
+  int64_t i = s * 64 + lane;
+  bool valid = i < a.n;
+  int64_t cb = a.sl_ptr[s];
+  int width = int((a.sl_ptr[s + 1] - cb) >> 6);
+  int len = valid ? int(a.rowlen[i]) : 0;
+  int64_t r0 = valid ? row_ptr[i] : 0;
+  for (int k = 0; k < width; k++) {
+    bool b = k < len && in[r0 + k] != 0;
+    unsigned long long m = __ballot(b);
+    if (lane == 0) a.bits[(cb >> 6) + k] = m;
+  }
+}
+
+// ---- post-processing: computePostProcFields (fDModel.cpp:652-778) -------
+
+template <int DIM>
+__global__ __launch_bounds__(256) void k_postproc(FdArgs<DIM> a,
+                                                  float* __restrict__ Z,
+                                                  float* __restrict__ phi,
+                                                  double* __restrict__ energy) {
+  int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  int lane = threadIdx.x & 63;
+  int64_t s = i >> 6;
+  if (s >= a.nslices) return;
+  bool valid = i < a.n;
+  int64_t cb = a.sl_ptr[s];
+  int width = int((a.sl_ptr[s + 1] - cb) >> 6);
+  int64_t bb = cb >> 6;
+  int len = valid ? int(a.rowlen[i]) : 0;
+  double xi[3] = {0., 0., 0.}, ui[3] = {0., 0., 0.};
+  double voli = 1.;
+  uint32_t fl = 0;
+  if (valid) {
+    load3<DIM>(a.x[i], xi);
+    load3<DIM>(a.u[i], ui);
+    voli = a.vol[i];
+    fl = a.flags[i];
+  }
+  bool int_i = !(fl & kFlagNonInterior);
+  const RnpDev& m = a.m;
+  double energy_i = 0., sa = 0., sb = 0., z = 0.;
+  for (int k = 0; k < width; ++k) {
+    unsigned long long w = a.bits[bb + k];
+    bool newly = false;
+    if (k < len) {
+      int32_t j = a.col[cb + int64_t(k) * 64 + lane];
+      double xj[3], uj[3];
+      load3<DIM>(a.x[j], xj);
+      load3<DIM>(a.u[j], uj);
+      double volj0 = a.vol[j];
+      bool brk = int_i;
+      if (a.has_nofail) brk = brk && !(a.flags[j] & kFlagNonInterior);
+      bool fs0 = (w >> lane) & 1ull;
+      bool fs = fs0;
+      double F[3], E;
+      bond_exact<DIM, true>(m, xi, ui, xj, uj, voli, volj0, brk, fs, F, &E);
+      newly = fs && !fs0;
+      energy_i += E;
+      double dx0 = xj[0] - xi[0], dx1 = xj[1] - xi[1];
+      double dx2 = DIM == 3 ? xj[2] - xi[2] : 0.0;
+      double du0 = uj[0] - ui[0], du1 = uj[1] - ui[1];
+      double du2 = DIM == 3 ? uj[2] - ui[2] : 0.0;
+      double dd = dx0 * dx0 + dx1 * dx1;
+      dd = dd + dx2 * dx2;
+      double rji = sqrt(dd);
+      double num = dx0 * du0 + dx1 * du1;
+      num = num + dx2 * du2;
+      double S = num / dd;
+      double volj = volj0;
+      if (def_greater(rji, m.check_low)) volj *= (m.check_up - rji) / m.h;
+      if (!fs) sa += volj;
+      sb += volj;
+      double sr = 0.;
+      if (def_greater(rji, 1.0E-12)) sr = fabs(S) / (m.rbar / sqrt(rji));
+      if (def_less(z, sr)) z = sr;
+    }
+    unsigned long long nb = __ballot(newly);
+    if (nb != 0ull && lane == 0) a.bits[bb + k] = w | nb;
+  }
+  if (!valid) return;
+  if (Z) Z[i] = float(z);
+  if (phi) phi[i] = float(1. - sa / sb);
+  if (energy) energy[i] = energy_i * voli;
+}
+
+// ---- host side ------------------------------------------------------------
+
+template <int DIM>
+FdArgs<DIM> make_args(nlm_ctx* c) {
+  using V = typename VT<DIM>::T;
+  FdState& s = c->fd;
+  FdArgs<DIM> a;
+  a.n = s.n;
+  a.nslices = s.sell.nslices;
+  a.x = s.x.as<V>();
+  a.vol = s.vol.as<double>();
+  a.flags = s.flags.as<uint32_t>();
+  a.sl_ptr = s.sell.sl_ptr.as<int64_t>();
+  a.col = s.sell.col.as<int32_t>();
+  a.rowlen = s.sell.rowlen.as<uint16_t>();
+  a.bits = s.sell.bits.as<unsigned long long>();
+  a.u = s.u[s.cur].as<V>();
+  a.v = s.v[s.cur].as<V>();
+  a.u_out = s.u[1 - s.cur].as<V>();
+  a.v_out = s.v[1 - s.cur].as<V>();
+  a.f_in = s.f.as<V>();
+  a.f_out = s.f.as<V>();
+  a.bc = s.bctab.as<BcTable>();
+  a.m = s.rnp;
+  a.dt = s.dt;
+  a.fact = s.scheme == NLM_TIME_CENTRAL_DIFFERENCE
+               ? s.dt * s.dt / s.density
+               : 0.5 * s.dt / s.density;
+  a.has_fbc = s.has_fbc ? 1 : 0;
+  a.has_nofail = s.has_nofail ? 1 : 0;
+  return a;
+}
+
+int64_t force_blocks(const FdState& s) {
+  return (s.sell.nslices * 64 + 255) / 256;
+}
+
+template <int DIM, int MODE>
+void launch_force(nlm_ctx* c, bool base_f, double t) {
+  FdArgs<DIM> a = make_args<DIM>(c);
+  unsigned g = grid_xcd(force_blocks(c->fd) * 256, 256);
+  if (base_f)
+    hipLaunchKernelGGL((k_fd<DIM, MODE, KIND_FORCE, true>), dim3(g), dim3(256),
+                       0, c->stream, a, t, t);
+  else
+    hipLaunchKernelGGL((k_fd<DIM, MODE, KIND_FORCE, false>), dim3(g),
+                       dim3(256), 0, c->stream, a, t, t);
+}
+
+template <int DIM, int MODE>
+void launch_fused_cd(nlm_ctx* c, double t_force, double t_next) {
+  FdArgs<DIM> a = make_args<DIM>(c);
+  unsigned g = grid_xcd(force_blocks(c->fd) * 256, 256);
+  hipLaunchKernelGGL((k_fd<DIM, MODE, KIND_CD, false>), dim3(g), dim3(256), 0,
+                     c->stream, a, t_force, t_next);
+}
+
+template <int DIM>
+void launch_force_dispatch(nlm_ctx* c, bool base_f, double t) {
+  switch (c->fd.precision) {
+    case kModeExact:
+      launch_force<DIM, kModeExact>(c, base_f, t);
+      break;
+    case kModeFp64:
+      launch_force<DIM, kModeFp64>(c, base_f, t);
+      break;
+    default:
+      launch_force<DIM, kModeFp32>(c, base_f, t);
+      break;
+  }
+}
+
+template <int DIM>
+void launch_fused_dispatch(nlm_ctx* c, double tf, double tn) {
+  switch (c->fd.precision) {
+    case kModeExact:
+      launch_fused_cd<DIM, kModeExact>(c, tf, tn);
+      break;
+    case kModeFp64:
+      launch_fused_cd<DIM, kModeFp64>(c, tf, tn);
+      break;
+    default:
+      launch_fused_cd<DIM, kModeFp32>(c, tf, tn);
+      break;
+  }
+}
+
+unsigned node_blocks(const FdState& s) { return unsigned((s.n + 255) / 256); }
+
+// Fracture::addCrack (fracture.cpp:45-72): at most one pending crack per call
+// becomes active once definitelyLessThan(activation_time, t).
+template <int DIM>
+void check_cracks(nlm_ctx* c, double t) {
+  FdState& s = c->fd;
+  for (size_t k = 0; k < s.timed_cracks.size(); k++) {
+    if (s.crack_done[k]) continue;
+    if (def_less(s.timed_cracks[k].activation_time, t)) {
+      FdArgs<DIM> a = make_args<DIM>(c);
+      hipLaunchKernelGGL(k_precrack<DIM>,
+                         dim3(unsigned((s.sell.nslices * 64 + 255) / 256)),
+                         dim3(256), 0, c->stream, a, s.timed_cracks[k]);
+      s.crack_done[k] = true;
+      return;
+    }
+  }
+}
+
+template <int DIM>
+int do_steps(nlm_ctx* c, int nsteps, double* ms_fused) {
+  FdState& s = c->fd;
+  const unsigned nb = node_blocks(s);
+  if (s.scheme == NLM_TIME_VELOCITY_VERLET) {
+    for (int k = 0; k < nsteps; k++) {
+      double tn = s.time + s.dt;
+      FdArgs<DIM> a = make_args<DIM>(c);
+      hipLaunchKernelGGL(k_verlet_a<DIM>, dim3(nb), dim3(256), 0, c->stream, a,
+                         tn);
+      s.cur ^= 1;
+      s.time = tn;
+      s.step++;
+      launch_force_dispatch<DIM>(c, false, s.time);
+      FdArgs<DIM> b = make_args<DIM>(c);
+      b.v_out = const_cast<typename VT<DIM>::T*>(b.v);
+      hipLaunchKernelGGL(k_verlet_b<DIM>, dim3(nb), dim3(256), 0, c->stream, b);
+      check_cracks<DIM>(c, s.time);
+    }
+    if (ms_fused) *ms_fused = 0.;
+    return NLM_OK;
+  }
+  // first iteration: update from the stored force
+  {
+    double tn = s.time + s.dt;
+    FdArgs<DIM> a = make_args<DIM>(c);
+    hipLaunchKernelGGL(k_update_cd<DIM>, dim3(nb), dim3(256), 0, c->stream, a,
+                       tn);
+    s.cur ^= 1;
+    s.time = tn;
+    s.step++;
+  }
+  if (ms_fused) NLM_HIP(c, hipEventRecord(s.ev0, c->stream));
+  for (int k = 1; k < nsteps; k++) {
+    double tn = s.time + s.dt;
+    launch_fused_dispatch<DIM>(c, s.time, tn);
+    check_cracks<DIM>(c, s.time);
+    s.cur ^= 1;
+    s.time = tn;
+    s.step++;
+  }
+  if (ms_fused) NLM_HIP(c, hipEventRecord(s.ev1, c->stream));
+  launch_force_dispatch<DIM>(c, false, s.time);
+  check_cracks<DIM>(c, s.time);
+  NLM_HIP(c, hipGetLastError());
+  if (ms_fused) {
+    NLM_HIP(c, hipEventSynchronize(s.ev1));
+    float ms = 0.f;
+    NLM_HIP(c, hipEventElapsedTime(&ms, s.ev0, s.ev1));
+    *ms_fused = nsteps > 1 ? double(ms) / double(nsteps - 1) : 0.;
+  }
+  return NLM_OK;
+}
+
+int check_ready(nlm_ctx* c, const char* fn) {
+  if (!c) return NLM_ERR_INVALID;
+  if (!c->fd.ready)
+    return fail(c, NLM_ERR_STATE, std::string(fn) + ": nlm_fd_setup not done");
+  return NLM_OK;
+}
+
+// Host-side region tests for BC node selection: ULoading::ULoading
+// (uLoading.cpp:86-163) and FLoading::FLoading (fLoading.cpp:86-113).
+void rotate_cw(double x, double y, double th, double& ox, double& oy) {
+  // util::transformation::rotateCW2D, transformation.cpp:20-24
+  ox = x * std::cos(th) + y * std::sin(th);
+  oy = -x * std::sin(th) + y * std::cos(th);
+}
+bool h_in_rect(const double* p, double x1, double x2, double y1, double y2) {
+  // util::geometry::isPointInsideRectangle, utilGeom.cpp:43-50
+  return !(def_less(p[0], x1 - 1.0E-12) || def_less(p[1], y1 - 1.0E-12) ||
+           def_greater(p[0], x2 + 1.0E-12) || def_greater(p[1], y2 + 1.0E-12));
+}
+bool h_in_angled(const double* p, const nlm_bc& b) {
+  // util::geometry::isPointInsideAngledRectangle, utilGeom.cpp:60-95
+  double lx, ly, mx, my;
+  rotate_cw(b.x2 - b.x1, b.y2 - b.y1, b.theta, lx, ly);
+  rotate_cw(p[0] - b.x1, p[1] - b.y1, b.theta, mx, my);
+  return !(def_less(mx, -1.0E-12) || def_less(my, -1.0E-12) ||
+           def_greater(mx, lx + 1.0E-12) || def_greater(my, ly + 1.0E-12));
+}
+bool h_in_circle(const double* p, double cx, double cy, double r) {
+  // util::geometry::isPointinCircle, utilGeom.cpp:328-333 ((A-c).length())
+  double a = p[0] - cx, b = p[1] - cy, z = p[2] - 0.;
+  return std::sqrt(a * a + b * b + z * z) <= r;
+}
+bool h_in_cuboid(const double* p, const nlm_bc& b) {
+  // util::geometry::isPointInsideCuboid (dim 3), utilGeom.cpp:97-115
+  return !(def_less(p[0], b.x1 - 1.0E-12) || def_less(p[1], b.y1 - 1.0E-12) ||
+           def_less(p[2], b.z1 - 1.0E-12) ||
+           def_greater(p[0], b.x2 + 1.0E-12) ||
+           def_greater(p[1], b.y2 + 1.0E-12) ||
+           def_greater(p[2], b.z2 + 1.0E-12));
+}
+
+BcSetDev to_dev(const nlm_bc& b) {
+  BcSetDev d{};
+  d.time_fn = b.time_fn;
+  d.space_fn = b.space_fn;
+  d.dirmask = 0;
+  for (int k = 0; k < b.ndir && k < 3; k++)
+    if (b.dir[k] >= 1 && b.dir[k] <= 3) d.dirmask |= 1 << (b.dir[k] - 1);
+  d.n_space_params = b.n_space_params;
+  for (int k = 0; k < 4; k++) {
+    d.tp[k] = b.time_params[k];
+    d.sp[k] = b.space_params[k];
+  }
+  d.x1 = b.x1;
+  d.x2 = b.x2;
+  d.y1 = b.y1;
+  d.y2 = b.y2;
+  d.z1 = b.z1;
+  d.z2 = b.z2;
+  return d;
+}
+
+int n_time_params_u(int fn) {
+  // ULoading::ULoading parameter counts, uLoading.cpp:58-71
+  if (fn == NLM_TFN_QUADRATIC || fn == NLM_TFN_SIN) return 2;
+  if (fn == NLM_TFN_LINEAR_STEP) return 3;
+  if (fn == NLM_TFN_LINEAR_SLOW_FAST) return 4;
+  return 1;
+}
+
+}  // namespace
+}  // namespace nlm
+
+using namespace nlm;
+
+extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
+  if (!c) return NLM_ERR_INVALID;
+  if (!p || p->n <= 0 || !p->x || !p->vol)
+    return fail(c, NLM_ERR_INVALID, "nlm_fd_setup: missing mesh data");
+  if (p->dim != 2 && p->dim != 3)
+    return fail(c, NLM_ERR_INVALID, "nlm_fd_setup: dim must be 2 or 3");
+  if (p->precision < NLM_PREC_FP64_EXACT || p->precision > NLM_PREC_FP32_FORCE)
+    return fail(c, NLM_ERR_INVALID, "nlm_fd_setup: unknown precision mode");
+  if (p->time_scheme != NLM_TIME_CENTRAL_DIFFERENCE &&
+      p->time_scheme != NLM_TIME_VELOCITY_VERLET)
+    return fail(c, NLM_ERR_INVALID, "nlm_fd_setup: unknown time scheme");
+  if (!(p->dt > 0.) || !(p->mat.density > 0.) || !(p->mat.horizon > 0.) ||
+      !(p->mat.h > 0.))
+    return fail(c, NLM_ERR_INVALID,
+                "nlm_fd_setup: dt, density, horizon and h must be > 0");
+  if (p->n >= (int64_t(1) << 31))
+    return fail(c, NLM_ERR_INVALID, "nlm_fd_setup: n >= 2^31");
+  FdState& s = c->fd;
+  s.ready = false;
+  const int64_t n = p->n;
+  const int dim = p->dim;
+  hipStream_t st = c->stream;
+  int rc;
+  // neighbor lists: caller's CSR or the context's last build
+  const int64_t* h_row = p->row_ptr;
+  std::vector<int64_t> row_tmp;
+  if (!h_row) {
+    if (!c->nbr.ready || c->nbr.n != n)
+      return fail(c, NLM_ERR_STATE,
+                  "nlm_fd_setup: no CSR given and no matching neighbor build");
+    row_tmp.resize(n + 1);
+    NLM_HIP(c, hipMemcpy(row_tmp.data(), c->nbr.row_ptr.p,
+                         sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost));
+    h_row = row_tmp.data();
+  }
+  int64_t nnz = h_row[n];
+  for (int64_t i = 0; i < n; i++)
+    if (h_row[i + 1] - h_row[i] > 65535 || h_row[i + 1] < h_row[i])
+      return fail(c, NLM_ERR_INVALID, "nlm_fd_setup: bad row_ptr");
+  s.row_ptr_host.assign(h_row, h_row + n + 1);
+  if ((rc = alloc(c, s.row_ptr, sizeof(int64_t) * (n + 1)))) return rc;
+  if ((rc = alloc(c, s.col, sizeof(int32_t) * (nnz + 1)))) return rc;
+  NLM_HIP(c, hipMemcpyAsync(s.row_ptr.p, h_row, sizeof(int64_t) * (n + 1),
+                            hipMemcpyHostToDevice, st));
+  if (p->col) {
+    for (int64_t k = 0; k < nnz; k++)
+      if (p->col[k] < 0 || p->col[k] >= n)
+        return fail(c, NLM_ERR_INVALID, "nlm_fd_setup: column out of range");
+    NLM_HIP(c, hipMemcpyAsync(s.col.p, p->col, sizeof(int32_t) * nnz,
+                              hipMemcpyHostToDevice, st));
+  } else {
+    if (!c->nbr.ready || c->nbr.nnz != nnz)
+      return fail(c, NLM_ERR_STATE, "nlm_fd_setup: neighbor build mismatch");
+    NLM_HIP(c, hipMemcpyAsync(s.col.p, c->nbr.col.p, sizeof(int32_t) * nnz,
+                              hipMemcpyDeviceToDevice, st));
+  }
+  if ((rc = build_sell(c, n, s.row_ptr.as<int64_t>(), s.col.as<int32_t>(),
+                       nnz, s.sell)))
+    return rc;
+  // node arrays
+  const int stride = dim == 2 ? 2 : 4;
+  std::vector<double> hx(size_t(n) * stride, 0.);
+  for (int64_t i = 0; i < n; i++)
+    for (int d = 0; d < dim; d++) hx[i * stride + d] = p->x[3 * i + d];
+  size_t vbytes = sizeof(double) * stride * size_t(n);
+  if ((rc = alloc(c, s.x, vbytes))) return rc;
+  if ((rc = alloc(c, s.vol, sizeof(double) * n))) return rc;
+  if ((rc = alloc(c, s.flags, sizeof(uint32_t) * n))) return rc;
+  for (int b = 0; b < 2; b++) {
+    if ((rc = alloc(c, s.u[b], vbytes))) return rc;
+    if ((rc = alloc(c, s.v[b], vbytes))) return rc;
+    NLM_HIP(c, hipMemsetAsync(s.u[b].p, 0, vbytes, st));
+    NLM_HIP(c, hipMemsetAsync(s.v[b].p, 0, vbytes, st));
+  }
+  if ((rc = alloc(c, s.f, vbytes))) return rc;
+  NLM_HIP(c, hipMemsetAsync(s.f.p, 0, vbytes, st));
+  NLM_HIP(c, hipMemcpyAsync(s.x.p, hx.data(), vbytes, hipMemcpyHostToDevice,
+                            st));
+  NLM_HIP(c, hipMemcpyAsync(s.vol.p, p->vol, sizeof(double) * n,
+                            hipMemcpyHostToDevice, st));
+  std::vector<uint32_t> fl(n, 0u);
+  s.has_nofail = false;
+  if (p->interior)
+    for (int64_t i = 0; i < n; i++)
+      if (!p->interior[i]) {
+        fl[i] |= kFlagNonInterior;
+        s.has_nofail = true;
+      }
+  NLM_HIP(c, hipMemcpyAsync(s.flags.p, fl.data(), sizeof(uint32_t) * n,
+                            hipMemcpyHostToDevice, st));
+  if ((rc = alloc(c, s.bctab, sizeof(BcTable)))) return rc;
+  std::memset(&s.bc_host, 0, sizeof(BcTable));
+  NLM_HIP(c, hipMemcpyAsync(s.bctab.p, &s.bc_host, sizeof(BcTable),
+                            hipMemcpyHostToDevice, st));
+  NLM_HIP(c, hipStreamSynchronize(st));
+  s.fix_host.assign(n, 0);
+  s.n = n;
+  s.dim = dim;
+  s.nnz = nnz;
+  s.precision = p->precision;
+  s.scheme = p->time_scheme;
+  s.dt = p->dt;
+  s.density = p->mat.density;
+  s.time = 0.;
+  s.step = 0;
+  s.cur = 0;
+  s.has_fbc = false;
+  s.mat = p->mat;
+  s.timed_cracks.clear();
+  s.crack_done.clear();
+  RnpDev& m = s.rnp;
+  std::memset(&m, 0, sizeof(m));
+  m.C = p->mat.C;
+  m.beta = p->mat.beta;
+  m.rbar = p->mat.rbar;
+  m.inv_factor = p->mat.inv_factor;
+  m.factor_sc = p->mat.factor_sc;
+  m.horizon = p->mat.horizon;
+  m.h = p->mat.h;
+  // rnpBond.cpp:248-250
+  m.check_up = p->mat.horizon + 0.5 * p->mat.h;
+  m.check_low = p->mat.horizon - 0.5 * p->mat.h;
+  m.inf_type = p->mat.inf_type;
+  m.inf_a = p->mat.inf_a;
+  m.inf_b = p->mat.inf_b;
+  m.irreversible = p->mat.irreversible;
+  m.contact = p->mat.contact;
+  m.kn = p->mat.contact_kn;
+  m.rc = p->mat.contact_rc;
+  m.rcp_inv_factor = 1.0 / m.inv_factor;
+  m.rcp_h = 1.0 / m.h;
+  m.rcp_horizon = 1.0 / m.horizon;
+  m.krbar_sq = (m.factor_sc * m.rbar) * (m.factor_sc * m.rbar);
+  m.c4 = 4.0 * m.C * m.beta * m.rcp_inv_factor;
+  m.c4f = float(m.c4);
+  m.beta_f = float(m.beta);
+  if (!s.ev0) NLM_HIP(c, hipEventCreate(&s.ev0));
+  if (!s.ev1) NLM_HIP(c, hipEventCreate(&s.ev1));
+  s.ready = true;
+  return NLM_OK;
+}
+
+extern "C" int nlm_fd_set_ubc(nlm_ctx* c, int nsets, const nlm_bc* sets) {
+  int rc = check_ready(c, "nlm_fd_set_ubc");
+  if (rc) return rc;
+  FdState& s = c->fd;
+  if (nsets < 0 || nsets > kMaxUbcSets || (nsets > 0 && !sets))
+    return fail(c, NLM_ERR_INVALID, "nlm_fd_set_ubc: 0..15 sets");
+  std::vector<double> hx(size_t(s.n) * 3);
+  {
+    const int stride = s.dim == 2 ? 2 : 4;
+    std::vector<double> tmp(size_t(s.n) * stride);
+    NLM_HIP(c, hipMemcpy(tmp.data(), s.x.p, s.x.bytes, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < s.n; i++)
+      for (int d = 0; d < 3; d++)
+        hx[3 * i + d] = d < s.dim ? tmp[i * stride + d] : 0.;
+  }
+  std::vector<uint32_t> fl(s.n);
+  NLM_HIP(c, hipMemcpy(fl.data(), s.flags.p, sizeof(uint32_t) * s.n,
+                       hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < s.n; i++) fl[i] &= ~0xfffu;
+  s.fix_host.assign(s.n, 0);
+  for (int k = 0; k < nsets; k++) {
+    const nlm_bc& b = sets[k];
+    if (b.region < NLM_REGION_RECTANGLE || b.region > NLM_REGION_CUBOID)
+      return fail(c, NLM_ERR_INVALID, "nlm_fd_set_ubc: unknown region");
+    if (b.time_fn < NLM_TFN_CONSTANT || b.time_fn > NLM_TFN_LINEAR_SLOW_FAST)
+      return fail(c, NLM_ERR_INVALID, "nlm_fd_set_ubc: unknown time function");
+    if (b.space_fn < NLM_SFN_CONSTANT || b.space_fn > NLM_SFN_LINEAR_Z)
+      return fail(c, NLM_ERR_INVALID,
+                  "nlm_fd_set_ubc: unknown spatial function");
+    if (b.n_time_params != n_time_params_u(b.time_fn))
+      return fail(c, NLM_ERR_INVALID,
+                  "nlm_fd_set_ubc: wrong number of time parameters");
+    if (b.space_fn != NLM_SFN_CONSTANT && b.n_space_params < 1)
+      return fail(c, NLM_ERR_INVALID,
+                  "nlm_fd_set_ubc: spatial function needs a parameter");
+    if (b.ndir < 1 || b.ndir > 3)
+      return fail(c, NLM_ERR_INVALID, "nlm_fd_set_ubc: 1..3 directions");
+    for (int d = 0; d < b.ndir; d++)
+      if (b.dir[d] < 1 || b.dir[d] > s.dim)
+        return fail(c, NLM_ERR_INVALID, "nlm_fd_set_ubc: bad direction");
+    s.bc_host.u[k] = to_dev(b);
+    for (int64_t i = 0; i < s.n; i++) {
+      const double* p = &hx[3 * i];
+      bool sel = false;
+      switch (b.region) {
+        case NLM_REGION_RECTANGLE:
+          sel = h_in_rect(p, b.x1, b.x2, b.y1, b.y2);
+          break;
+        case NLM_REGION_ANGLED_RECTANGLE:
+          sel = h_in_angled(p, b);
+          break;
+        case NLM_REGION_CIRCLE:
+          sel = h_in_circle(p, b.x1, b.y1, b.r1);
+          break;
+        case NLM_REGION_TORUS:
+          sel = h_in_circle(p, b.x1, b.y1, b.r1) &&
+                !h_in_circle(p, b.x1, b.y1, b.r2);
+          break;
+        case NLM_REGION_LINE:
+          sel = def_greater(p[0], b.x1) && def_less(p[0], b.x2);
+          break;
+        case NLM_REGION_CUBOID:
+          // ULoading selects no node for a cuboid in the reference; this
+          // library selects the nodes inside it (3D pulls).
+          sel = h_in_cuboid(p, b);
+          break;
+      }
+      if (!sel) continue;
+      for (int d = 0; d < b.ndir; d++) {
+        int dof = b.dir[d] - 1;
+        fl[i] = (fl[i] & ~(0xfu << (4 * dof))) | (uint32_t(k + 1) << (4 * dof));
+        s.fix_host[i] |= uint8_t(1u << dof);
+      }
+    }
+  }
+  s.bc_host.nu = nsets;
+  NLM_HIP(c, hipMemcpy(s.flags.p, fl.data(), sizeof(uint32_t) * s.n,
+                       hipMemcpyHostToDevice));
+  NLM_HIP(c, hipMemcpy(s.bctab.p, &s.bc_host, sizeof(BcTable),
+                       hipMemcpyHostToDevice));
+  return NLM_OK;
+}
+
+extern "C" int nlm_fd_set_fbc(nlm_ctx* c, int nsets, const nlm_bc* sets) {
+  int rc = check_ready(c, "nlm_fd_set_fbc");
+  if (rc) return rc;
+  FdState& s = c->fd;
+  if (nsets < 0 || nsets > kMaxFbcSets || (nsets > 0 && !sets))
+    return fail(c, NLM_ERR_INVALID, "nlm_fd_set_fbc: 0..8 sets");
+  std::vector<double> hx(size_t(s.n) * 3);
+  {
+    const int stride = s.dim == 2 ? 2 : 4;
+    std::vector<double> tmp(size_t(s.n) * stride);
+    NLM_HIP(c, hipMemcpy(tmp.data(), s.x.p, s.x.bytes, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < s.n; i++)
+      for (int d = 0; d < 3; d++)
+        hx[3 * i + d] = d < s.dim ? tmp[i * stride + d] : 0.;
+  }
+  std::vector<uint32_t> fl(s.n);
+  NLM_HIP(c, hipMemcpy(fl.data(), s.flags.p, sizeof(uint32_t) * s.n,
+                       hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < s.n; i++) fl[i] &= ~(0xffu << kFlagFbcShift);
+  for (int k = 0; k < nsets; k++) {
+    const nlm_bc& b = sets[k];
+    if (b.region != NLM_REGION_LINE && b.region != NLM_REGION_RECTANGLE &&
+        b.region != NLM_REGION_ANGLED_RECTANGLE &&
+        b.region != NLM_REGION_CUBOID)
+      return fail(c, NLM_ERR_INVALID, "nlm_fd_set_fbc: unsupported region");
+    if (b.time_fn == NLM_TFN_QUADRATIC || b.time_fn < 0 ||
+        b.time_fn > NLM_TFN_LINEAR_SLOW_FAST)
+      return fail(c, NLM_ERR_INVALID,
+                  "nlm_fd_set_fbc: unsupported time function");
+    int ntp = b.time_fn == NLM_TFN_LINEAR_STEP
+                  ? 3
+                  : (b.time_fn == NLM_TFN_LINEAR_SLOW_FAST
+                         ? 4
+                         : (b.time_fn == NLM_TFN_SIN ? 2 : 1));
+    if (b.n_time_params != ntp)
+      return fail(c, NLM_ERR_INVALID,
+                  "nlm_fd_set_fbc: wrong number of time parameters");
+    if (b.space_fn < 0 || b.space_fn > NLM_SFN_HAT_Z)
+      return fail(c, NLM_ERR_INVALID,
+                  "nlm_fd_set_fbc: unknown spatial function");
+    if (b.space_fn != NLM_SFN_CONSTANT && b.n_space_params < 1)
+      return fail(c, NLM_ERR_INVALID,
+                  "nlm_fd_set_fbc: spatial function needs a parameter");
+    if (b.ndir < 1 || b.ndir > 3)
+      return fail(c, NLM_ERR_INVALID, "nlm_fd_set_fbc: 1..3 directions");
+    for (int d = 0; d < b.ndir; d++)
+      if (b.dir[d] < 1 || b.dir[d] > s.dim)
+        return fail(c, NLM_ERR_INVALID, "nlm_fd_set_fbc: bad direction");
+    s.bc_host.f[k] = to_dev(b);
+    for (int64_t i = 0; i < s.n; i++) {
+      const double* p = &hx[3 * i];
+      bool sel = false;
+      if (b.region == NLM_REGION_LINE)
+        sel = def_greater(p[0], b.x1) && def_less(p[0], b.x2);
+      else if (b.region == NLM_REGION_RECTANGLE)
+        sel = h_in_rect(p, b.x1, b.x2, b.y1, b.y2);
+      else if (b.region == NLM_REGION_ANGLED_RECTANGLE)
+        sel = h_in_angled(p, b);
+      else
+        sel = h_in_cuboid(p, b);
+      if (sel) fl[i] |= 1u << (kFlagFbcShift + k);
+    }
+  }
+  s.bc_host.nf = nsets;
+  s.has_fbc = nsets > 0;
+  NLM_HIP(c, hipMemcpy(s.flags.p, fl.data(), sizeof(uint32_t) * s.n,
+                       hipMemcpyHostToDevice));
+  NLM_HIP(c, hipMemcpy(s.bctab.p, &s.bc_host, sizeof(BcTable),
+                       hipMemcpyHostToDevice));
+  return NLM_OK;
+}
+
+extern "C" int nlm_fd_add_cracks(nlm_ctx* c, int ncracks,
+                                 const nlm_crack* cracks) {
+  int rc = check_ready(c, "nlm_fd_add_cracks");
+  if (rc) return rc;
+  FdState& s = c->fd;
+  if (ncracks < 0 || (ncracks > 0 && !cracks))
+    return fail(c, NLM_ERR_INVALID, "nlm_fd_add_cracks: bad arguments");
+  if (ncracks > 0 && s.dim != 2)
+    return fail(c, NLM_ERR_INVALID,
+                "nlm_fd_add_cracks: pre-cracks need dimension 2");
+  for (int k = 0; k < ncracks; k++) {
+    const nlm_crack& cr = cracks[k];
+    if (cr.o != -1 && cr.o != 0 && cr.o != 1)
+      return fail(c, NLM_ERR_INVALID, "nlm_fd_add_cracks: bad orientation");
+    if (cr.activation_time < 0.) {
+      if (s.dim == 2) {
+        FdArgs<2> a = make_args<2>(c);
+        hipLaunchKernelGGL(k_precrack<2>,
+                           dim3(unsigned((s.sell.nslices * 64 + 255) / 256)),
+                           dim3(256), 0, c->stream, a, cr);
+      }
+    } else {
+      s.timed_cracks.push_back(cr);
+      s.crack_done.push_back(false);
+    }
+  }
+  NLM_HIP(c, hipGetLastError());
+  NLM_HIP(c, hipStreamSynchronize(c->stream));
+  return NLM_OK;
+}
+
+namespace {
+int to_dev_vec(nlm_ctx* c, const double* src, DevBuf& dst) {
+  FdState& s = c->fd;
+  const int stride = s.dim == 2 ? 2 : 4;
+  std::vector<double> tmp(size_t(s.n) * stride, 0.);
+  for (int64_t i = 0; i < s.n; i++)
+    for (int d = 0; d < s.dim; d++) tmp[i * stride + d] = src[3 * i + d];
+  NLM_HIP(c, hipMemcpy(dst.p, tmp.data(), sizeof(double) * tmp.size(),
+                       hipMemcpyHostToDevice));
+  return NLM_OK;
+}
+int from_dev_vec(nlm_ctx* c, const DevBuf& src, double* dst) {
+  FdState& s = c->fd;
+  const int stride = s.dim == 2 ? 2 : 4;
+  std::vector<double> tmp(size_t(s.n) * stride, 0.);
+  NLM_HIP(c, hipMemcpy(tmp.data(), src.p, sizeof(double) * tmp.size(),
+                       hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < s.n; i++)
+    for (int d = 0; d < 3; d++)
+      dst[3 * i + d] = d < s.dim ? tmp[i * stride + d] : 0.;
+  return NLM_OK;
+}
+}  // namespace
+
+extern "C" int nlm_fd_set_state(nlm_ctx* c, const double* u, const double* v,
+                                const double* f) {
+  int rc = check_ready(c, "nlm_fd_set_state");
+  if (rc) return rc;
+  NLM_HIP(c, hipStreamSynchronize(c->stream));
+  FdState& s = c->fd;
+  if (u && (rc = to_dev_vec(c, u, s.u[s.cur]))) return rc;
+  if (v && (rc = to_dev_vec(c, v, s.v[s.cur]))) return rc;
+  if (f && (rc = to_dev_vec(c, f, s.f))) return rc;
+  return NLM_OK;
+}
+
+extern "C" int nlm_fd_get_state(nlm_ctx* c, double* u, double* v, double* f) {
+  int rc = check_ready(c, "nlm_fd_get_state");
+  if (rc) return rc;
+  NLM_HIP(c, hipStreamSynchronize(c->stream));
+  FdState& s = c->fd;
+  if (u && (rc = from_dev_vec(c, s.u[s.cur], u))) return rc;
+  if (v && (rc = from_dev_vec(c, s.v[s.cur], v))) return rc;
+  if (f && (rc = from_dev_vec(c, s.f, f))) return rc;
+  return NLM_OK;
+}
+
+extern "C" int nlm_fd_get_bonds(nlm_ctx* c, uint8_t* out) {
+  int rc = check_ready(c, "nlm_fd_get_bonds");
+  if (rc) return rc;
+  if (!out) return fail(c, NLM_ERR_INVALID, "nlm_fd_get_bonds: null output");
+  FdState& s = c->fd;
+  DevBuf tmp;
+  if ((rc = alloc(c, tmp, size_t(s.nnz) + 1))) return rc;
+  if (s.dim == 2) {
+    FdArgs<2> a = make_args<2>(c);
+    hipLaunchKernelGGL(k_bits_to_bytes<2>, dim3(node_blocks(s)), dim3(256), 0,
+                       c->stream, a, s.row_ptr.as<int64_t>(),
+                       tmp.as<uint8_t>());
+  } else {
+    FdArgs<3> a = make_args<3>(c);
+    hipLaunchKernelGGL(k_bits_to_bytes<3>, dim3(node_blocks(s)), dim3(256), 0,
+                       c->stream, a, s.row_ptr.as<int64_t>(),
+                       tmp.as<uint8_t>());
+  }
+  NLM_HIP(c, hipGetLastError());
+  NLM_HIP(c, hipMemcpyAsync(out, tmp.p, size_t(s.nnz), hipMemcpyDeviceToHost,
+                            c->stream));
+  NLM_HIP(c, hipStreamSynchronize(c->stream));
+  return NLM_OK;
+}
+
+extern "C" int nlm_fd_set_bonds(nlm_ctx* c, const uint8_t* in) {
+  int rc = check_ready(c, "nlm_fd_set_bonds");
+  if (rc) return rc;
+  if (!in) return fail(c, NLM_ERR_INVALID, "nlm_fd_set_bonds: null input");
+  FdState& s = c->fd;
+  DevBuf tmp;
+  if ((rc = alloc(c, tmp, size_t(s.nnz) + 1))) return rc;
+  NLM_HIP(c, hipMemcpyAsync(tmp.p, in, size_t(s.nnz), hipMemcpyHostToDevice,
+                            c->stream));
+  if (s.dim == 2) {
+    FdArgs<2> a = make_args<2>(c);
+    hipLaunchKernelGGL(k_bytes_to_bits<2>, dim3(unsigned(s.sell.nslices)),
+                       dim3(64), 0, c->stream, a, s.row_ptr.as<int64_t>(),
+                       tmp.as<uint8_t>());
+  } else {
+    FdArgs<3> a = make_args<3>(c);
+    hipLaunchKernelGGL(k_bytes_to_bits<3>, dim3(unsigned(s.sell.nslices)),
+                       dim3(64), 0, c->stream, a, s.row_ptr.as<int64_t>(),
+                       tmp.as<uint8_t>());
+  }
+  NLM_HIP(c, hipGetLastError());
+  NLM_HIP(c, hipStreamSynchronize(c->stream));
+  return NLM_OK;
+}
+
+extern "C" int nlm_fd_bond_bytes_size(nlm_ctx* c, int64_t* size) {
+  int rc = check_ready(c, "nlm_fd_bond_bytes_size");
+  if (rc) return rc;
+  FdState& s = c->fd;
+  int64_t t = 0;
+  for (int64_t i = 0; i < s.n; i++) {
+    int64_t ns = s.row_ptr_host[i + 1] - s.row_ptr_host[i];
+    t += (ns + 7) / 8;
+  }
+  *size = t;
+  return NLM_OK;
+}
+
+extern "C" int nlm_fd_get_bond_bytes(nlm_ctx* c, uint8_t* bytes) {
+  int rc = check_ready(c, "nlm_fd_get_bond_bytes");
+  if (rc) return rc;
+  FdState& s = c->fd;
+  std::vector<uint8_t> st(size_t(s.nnz) + 1);
+  if ((rc = nlm_fd_get_bonds(c, st.data()))) return rc;
+  int64_t o = 0;
+  for (int64_t i = 0; i < s.n; i++) {
+    int64_t r0 = s.row_ptr_host[i], ns = s.row_ptr_host[i + 1] - r0;
+    int64_t nb = (ns + 7) / 8;
+    for (int64_t b = 0; b < nb; b++) bytes[o + b] = 0;
+    for (int64_t k = 0; k < ns; k++)
+      if (st[r0 + k]) bytes[o + k / 8] |= uint8_t(1u << (k % 8));
+    o += nb;
+  }
+  return NLM_OK;
+}
+
+extern "C" int nlm_fd_get_fixity(nlm_ctx* c, uint8_t* fix) {
+  int rc = check_ready(c, "nlm_fd_get_fixity");
+  if (rc) return rc;
+  std::memcpy(fix, c->fd.fix_host.data(), size_t(c->fd.n));
+  return NLM_OK;
+}
+
+extern "C" int nlm_fd_get_time(nlm_ctx* c, double* t, int64_t* step) {
+  int rc = check_ready(c, "nlm_fd_get_time");
+  if (rc) return rc;
+  if (t) *t = c->fd.time;
+  if (step) *step = c->fd.step;
+  return NLM_OK;
+}
+
+extern "C" int nlm_fd_set_time(nlm_ctx* c, double t, int64_t step) {
+  int rc = check_ready(c, "nlm_fd_set_time");
+  if (rc) return rc;
+  c->fd.time = t;
+  c->fd.step = step;
+  return NLM_OK;
+}
+
+extern "C" int nlm_fd_init(nlm_ctx* c) {
+  int rc = check_ready(c, "nlm_fd_init");
+  if (rc) return rc;
+  FdState& s = c->fd;
+  unsigned nb = node_blocks(s);
+  // ULoading::apply in place (u_out = u)
+  if (s.dim == 2) {
+    FdArgs<2> a = make_args<2>(c);
+    a.u_out = const_cast<double2*>(a.u);
+    a.v_out = const_cast<double2*>(a.v);
+    hipLaunchKernelGGL(k_apply_ubc<2>, dim3(nb), dim3(256), 0, c->stream, a,
+                       s.time);
+    launch_force_dispatch<2>(c, false, s.time);
+  } else {
+    FdArgs<3> a = make_args<3>(c);
+    a.u_out = const_cast<double4*>(a.u);
+    a.v_out = const_cast<double4*>(a.v);
+    hipLaunchKernelGGL(k_apply_ubc<3>, dim3(nb), dim3(256), 0, c->stream, a,
+                       s.time);
+    launch_force_dispatch<3>(c, false, s.time);
+  }
+  NLM_HIP(c, hipGetLastError());
+  NLM_HIP(c, hipStreamSynchronize(c->stream));
+  return NLM_OK;
+}
+
+extern "C" int nlm_fd_step(nlm_ctx* c, int nsteps) {
+  int rc = check_ready(c, "nlm_fd_step");
+  if (rc) return rc;
+  if (nsteps < 0) return fail(c, NLM_ERR_INVALID, "nlm_fd_step: nsteps < 0");
+  if (nsteps == 0) return NLM_OK;
+  rc = c->fd.dim == 2 ? do_steps<2>(c, nsteps, nullptr)
+                      : do_steps<3>(c, nsteps, nullptr);
+  if (rc) return rc;
+  NLM_HIP(c, hipGetLastError());
+  NLM_HIP(c, hipStreamSynchronize(c->stream));
+  return NLM_OK;
+}
+
+extern "C" int nlm_fd_timed_steps(nlm_ctx* c, int nsteps, double* ms_total,
+                                  double* ms_per_step) {
+  int rc = check_ready(c, "nlm_fd_timed_steps");
+  if (rc) return rc;
+  if (nsteps < 2)
+    return fail(c, NLM_ERR_INVALID, "nlm_fd_timed_steps: nsteps >= 2");
+  hipEvent_t a0, a1;
+  NLM_HIP(c, hipEventCreate(&a0));
+  NLM_HIP(c, hipEventCreate(&a1));
+  NLM_HIP(c, hipEventRecord(a0, c->stream));
+  double per = 0.;
+  rc = c->fd.dim == 2 ? do_steps<2>(c, nsteps, &per)
+                      : do_steps<3>(c, nsteps, &per);
+  if (rc) return rc;
+  NLM_HIP(c, hipEventRecord(a1, c->stream));
+  NLM_HIP(c, hipEventSynchronize(a1));
+  float ms = 0.f;
+  NLM_HIP(c, hipEventElapsedTime(&ms, a0, a1));
+  (void)hipEventDestroy(a0);
+  (void)hipEventDestroy(a1);
+  if (ms_total) *ms_total = ms;
+  if (ms_per_step) *ms_per_step = per;
+  return NLM_OK;
+}
+
+extern "C" int nlm_fd_compute_forces(nlm_ctx* c) {
+  int rc = check_ready(c, "nlm_fd_compute_forces");
+  if (rc) return rc;
+  if (c->fd.dim == 2)
+    launch_force_dispatch<2>(c, true, c->fd.time);
+  else
+    launch_force_dispatch<3>(c, true, c->fd.time);
+  NLM_HIP(c, hipGetLastError());
+  NLM_HIP(c, hipStreamSynchronize(c->stream));
+  return NLM_OK;
+}
+
+extern "C" int nlm_fd_postproc(nlm_ctx* c, float* Z, float* phi,
+                               double* energy) {
+  int rc = check_ready(c, "nlm_fd_postproc");
+  if (rc) return rc;
+  FdState& s = c->fd;
+  DevBuf dz, dp, de;
+  if ((rc = alloc(c, dz, sizeof(float) * s.n))) return rc;
+  if ((rc = alloc(c, dp, sizeof(float) * s.n))) return rc;
+  if ((rc = alloc(c, de, sizeof(double) * s.n))) return rc;
+  unsigned g = unsigned((s.sell.nslices * 64 + 255) / 256);
+  if (s.dim == 2) {
+    FdArgs<2> a = make_args<2>(c);
+    hipLaunchKernelGGL(k_postproc<2>, dim3(g), dim3(256), 0, c->stream, a,
+                       dz.as<float>(), dp.as<float>(), de.as<double>());
+  } else {
+    FdArgs<3> a = make_args<3>(c);
+    hipLaunchKernelGGL(k_postproc<3>, dim3(g), dim3(256), 0, c->stream, a,
+                       dz.as<float>(), dp.as<float>(), de.as<double>());
+  }
+  NLM_HIP(c, hipGetLastError());
+  if (Z)
+    NLM_HIP(c, hipMemcpyAsync(Z, dz.p, sizeof(float) * s.n,
+                              hipMemcpyDeviceToHost, c->stream));
+  if (phi)
+    NLM_HIP(c, hipMemcpyAsync(phi, dp.p, sizeof(float) * s.n,
+                              hipMemcpyDeviceToHost, c->stream));
+  if (energy)
+    NLM_HIP(c, hipMemcpyAsync(energy, de.p, sizeof(double) * s.n,
+                              hipMemcpyDeviceToHost, c->stream));
+  NLM_HIP(c, hipStreamSynchronize(c->stream));
+  return NLM_OK;
+}
+
+extern "C" int nlm_fd_num_bonds(nlm_ctx* c, int64_t* nnz) {
+  int rc = check_ready(c, "nlm_fd_num_bonds");
+  if (rc) return rc;
+  *nnz = c->fd.nnz;
+  return NLM_OK;
+}
+
+extern "C" const char* nlm_fd_kernel_name(nlm_ctx* c) {
+  if (!c || !c->fd.ready) return "";
+  return "k_fd";
+}

@@ -1,0 +1,181 @@
+// nlm_internal.h -- context and device-side data of the nlmech_amd library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/nlmech_amd.h"
+
+namespace nlm {
+
+// Raw device buffer owned by a context (freed in the destructor).
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+// Bits of the per-node flag word.
+constexpr uint32_t kFlagUbcShift = 0;         // 4 bits per dof: u-BC set id + 1
+constexpr uint32_t kFlagNonInterior = 1u << 12;  // node never breaks bonds
+constexpr uint32_t kFlagFbcShift = 16;        // 8 bits: force-BC set membership
+constexpr int kMaxUbcSets = 15;
+constexpr int kMaxFbcSets = 8;
+
+// One BC set as evaluated on the device (inp::BCData with codes).
+struct BcSetDev {
+  int time_fn;
+  int space_fn;
+  int dirmask;  // bit d set: dof d (0-based) is loaded
+  int n_space_params;
+  double tp[4];
+  double sp[4];
+  double x1, x2, y1, y2, z1, z2;
+};
+
+struct BcTable {
+  int nu;
+  int nf;
+  int pad[2];
+  BcSetDev u[kMaxUbcSets];
+  BcSetDev f[kMaxFbcSets];
+};
+
+// RNPBond constants as used by the kernels.
+struct RnpDev {
+  double C, beta, rbar, inv_factor, factor_sc;
+  double horizon, h, check_up, check_low;
+  double inf_a, inf_b;
+  int inf_type;
+  int irreversible;
+  int contact;
+  int pad;
+  double kn, rc;
+  // derived for the FP64/FP32 fast paths
+  double rcp_inv_factor;   // 1 / inv_factor
+  double rcp_h;            // 1 / h
+  double rcp_horizon;      // 1 / horizon
+  double krbar_sq;         // (factor_sc * rbar)^2
+  double c4;               // 4 * C * beta / inv_factor
+  float c4f;               // c4 in FP32
+  float beta_f;            // beta in FP32
+};
+
+// Slice layout of the neighbor lists: 64 consecutive nodes (one wavefront)
+// share a slice; entry (k, lane) of slice s sits at col[sl_ptr[s] + 64k +
+// lane]; the broken bits of slot k of slice s form one 64-bit word
+// bits[sl_ptr[s]/64 + k], bit `lane`.
+struct SellDev {
+  int64_t nslices = 0;
+  int64_t nentries = 0;  // padded entries (multiple of 64)
+  DevBuf sl_ptr;         // int64[nslices+1]
+  DevBuf col;            // int32[nentries]
+  DevBuf rowlen;         // uint16[n]
+  DevBuf bits;           // uint64[nentries/64]
+};
+
+struct FdState {
+  bool ready = false;
+  int dim = 2;
+  int64_t n = 0;
+  int64_t nnz = 0;
+  int precision = NLM_PREC_FP64_EXACT;
+  int scheme = NLM_TIME_CENTRAL_DIFFERENCE;
+  double dt = 0.;
+  double density = 1.;
+  double time = 0.;
+  int64_t step = 0;
+  bool has_fbc = false;
+  bool has_nofail = false;
+  nlm_rnp_params mat{};
+  RnpDev rnp{};
+  // node data; 2D: double2, 3D: double4 (w unused)
+  DevBuf x, vol, flags;
+  DevBuf u[2], v[2], f;
+  int cur = 0;  // which of u[]/v[] holds the current state
+  DevBuf row_ptr, col;  // CSR copy (int64, int32)
+  SellDev sell;
+  DevBuf bctab;  // BcTable
+  BcTable bc_host{};
+  std::vector<uint8_t> fix_host;
+  std::vector<nlm_crack> timed_cracks;
+  std::vector<bool> crack_done;
+  std::vector<int64_t> row_ptr_host;  // for host-side conversions
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+struct NbrState {
+  bool ready = false;
+  int64_t n = 0;
+  int64_t nnz = 0;
+  DevBuf row_ptr;  // int64[n+1]
+  DevBuf col;      // int32[nnz]
+};
+
+}  // namespace nlm
+
+struct nlm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::string arch;
+  nlm::NbrState nbr;
+  nlm::FdState fd;
+};
+
+namespace nlm {
+
+inline int fail(nlm_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define NLM_HIP(ctx, expr)                                                   \
+  do {                                                                       \
+    hipError_t e_ = (expr);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      return ::nlm::fail((ctx), NLM_ERR_HIP,                                 \
+                         std::string(#expr ": ") + hipGetErrorString(e_));   \
+  } while (0)
+
+inline int alloc(nlm_ctx* c, DevBuf& b, size_t bytes) {
+  b.release();
+  if (bytes == 0) bytes = 8;
+  hipError_t e = hipMalloc(&b.p, bytes);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    return fail(c, NLM_ERR_MEMORY,
+                std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  b.bytes = bytes;
+  return NLM_OK;
+}
+
+// Grid helper: ceil-divide and pad to a multiple of 8 for the XCD remap.
+inline unsigned grid_xcd(int64_t items, int block) {
+  int64_t g = (items + block - 1) / block;
+  if (g < 1) g = 1;
+  return unsigned((g + 7) / 8 * 8);
+}
+
+// Builds the slice layout from a device CSR; defined in neighbor.hip.
+int build_sell(nlm_ctx* c, int64_t n, const int64_t* d_row_ptr,
+               const int32_t* d_col, int64_t nnz, SellDev& out);
+
+}  // namespace nlm

@@ -1,0 +1,241 @@
+"""Python driver of the C-ABI: one explicit FD model on one GPU.
+
+``GpuFD`` mirrors the part of model::FDModel<RNPBond> that the library
+replaces (src/model/fd/fDModel.cpp): setup of mesh, neighbor lists, fracture,
+loading and material (initHObjects, :134-189), the integrate() prologue
+(:318-333), the step loop (:344-373) and post-processing (:652-778).  All
+compute happens in libnlmech_amd.so; this module only marshals arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import capi
+from .capi import (NLM_PREC_FP64_EXACT, NLM_TIME_CENTRAL_DIFFERENCE, Bc, Crack,
+                   FdProblem, MaterialDeck, NlmError, RnpParams, ptr)
+from .problems import BC, CrackSpec, Problem
+
+
+def _check(lib, ctx, rc, what):
+    if rc != 0:
+        msg = lib.nlm_last_error(ctx)
+        raise NlmError(f"{what} failed ({rc}): "
+                       f"{msg.decode() if msg else ''}")
+
+
+def material_deck(p: Problem) -> MaterialDeck:
+    m = p.material
+    d = MaterialDeck()
+    d.dim = p.dim
+    d.horizon = p.horizon
+    d.h = p.h
+    d.density = m["density"]
+    d.E = m.get("E", -1.0)
+    d.K = m.get("K", -1.0)
+    d.Gc = m.get("Gc", -1.0)
+    d.KIc = m.get("KIc", -1.0)
+    d.plane_strain = int(m.get("plane_strain", 0))
+    d.compute_from_classical = int(m.get("compute_from_classical", 1))
+    bp = list(m.get("bond_params", []))
+    d.n_bond_params = len(bp)
+    for k, v in enumerate(bp[:2]):
+        d.bond_params[k] = v
+    d.inf_type = int(m.get("inf_type", 0))
+    ip = list(m.get("inf_params", []))
+    d.n_inf_params = len(ip)
+    for k, v in enumerate(ip[:2]):
+        d.inf_params[k] = v
+    d.check_sc_factor = m.get("check_sc_factor", 1.0)
+    d.irreversible = int(m.get("irreversible", 1))
+    d.no_penetration = int(m.get("no_penetration", 0))
+    return d
+
+
+def rnp_params(p: Problem) -> RnpParams:
+    lib = capi.load()
+    d = material_deck(p)
+    out = RnpParams()
+    rc = lib.nlm_rnp_params_from_deck(C.byref(d), C.byref(out))
+    if rc != 0:
+        raise NlmError(f"nlm_rnp_params_from_deck failed ({rc})")
+    return out
+
+
+def bc_struct(b: BC) -> Bc:
+    s = Bc()
+    s.region = capi.REGION[b.region]
+    for k in ("x1", "x2", "y1", "y2", "z1", "z2", "r1", "r2", "theta"):
+        setattr(s, k, getattr(b, k))
+    s.ndir = len(b.dirs)
+    for k, d in enumerate(b.dirs):
+        s.dir[k] = d
+    s.time_fn = capi.TIME_FN[b.time_fn]
+    s.n_time_params = len(b.time_params)
+    for k, v in enumerate(b.time_params):
+        s.time_params[k] = v
+    s.space_fn = capi.SPACE_FN[b.space_fn]
+    s.n_space_params = len(b.space_params)
+    for k, v in enumerate(b.space_params):
+        s.space_params[k] = v
+    return s
+
+
+def crack_struct(c: CrackSpec) -> Crack:
+    s = Crack()
+    s.o = c.o
+    s.pb[0], s.pb[1] = c.pb
+    s.pt[0], s.pt[1] = c.pt
+    s.theta = c.theta
+    s.activation_time = c.activation_time
+    return s
+
+
+class GpuFD:
+    """One FDModel<RNPBond> instance on the GPU."""
+
+    def __init__(self, p: Problem, precision: int = NLM_PREC_FP64_EXACT,
+                 scheme: int = NLM_TIME_CENTRAL_DIFFERENCE, device: int = 0,
+                 csr: tuple | None = None):
+        self.lib = lib = capi.load()
+        self.p = p
+        h = C.c_void_p()
+        rc = lib.nlm_ctx_create(C.byref(h), device)
+        if rc != 0:
+            raise NlmError(f"nlm_ctx_create failed ({rc}): "
+                           f"{lib.nlm_last_error(None).decode()}")
+        self.ctx = h
+        self._x = np.ascontiguousarray(p.x, dtype=np.float64)
+        self._vol = np.ascontiguousarray(p.vol, dtype=np.float64)
+        if csr is None:
+            nnz = C.c_int64()
+            self._call("nlm_neighbor_build", p.dim, p.n, ptr(self._x, C.c_double),
+                       p.horizon, C.byref(nnz))
+            self.nnz = nnz.value
+            row_ptr = col = None
+        else:
+            row_ptr = np.ascontiguousarray(csr[0], dtype=np.int64)
+            col = np.ascontiguousarray(csr[1], dtype=np.int32)
+            self.nnz = int(row_ptr[-1])
+        prob = FdProblem()
+        prob.dim = p.dim
+        prob.n = p.n
+        prob.x = ptr(self._x, C.c_double)
+        prob.vol = ptr(self._vol, C.c_double)
+        prob.row_ptr = ptr(row_ptr, C.c_int64)
+        prob.col = ptr(col, C.c_int32)
+        self._interior = (None if p.interior is None else
+                          np.ascontiguousarray(p.interior, dtype=np.uint8))
+        prob.interior = ptr(self._interior, C.c_uint8)
+        prob.dt = p.dt
+        prob.precision = precision
+        prob.time_scheme = scheme
+        prob.mat = rnp_params(p)
+        self.mat = prob.mat
+        self._call("nlm_fd_setup", C.byref(prob))
+        if p.ubc:
+            arr = (Bc * len(p.ubc))(*[bc_struct(b) for b in p.ubc])
+            self._call("nlm_fd_set_ubc", len(p.ubc), arr)
+        if p.fbc:
+            arr = (Bc * len(p.fbc))(*[bc_struct(b) for b in p.fbc])
+            self._call("nlm_fd_set_fbc", len(p.fbc), arr)
+        if p.cracks:
+            arr = (Crack * len(p.cracks))(*[crack_struct(c) for c in p.cracks])
+            self._call("nlm_fd_add_cracks", len(p.cracks), arr)
+
+    def _call(self, name, *args):
+        rc = getattr(self.lib, name)(self.ctx, *args)
+        _check(self.lib, self.ctx, rc, name)
+
+    def close(self):
+        if self.ctx:
+            self.lib.nlm_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- model ------------------------------------------------------------
+    def init(self):
+        self._call("nlm_fd_init")
+
+    def step(self, nsteps: int):
+        self._call("nlm_fd_step", int(nsteps))
+
+    def compute_forces(self):
+        self._call("nlm_fd_compute_forces")
+
+    def timed_steps(self, nsteps: int):
+        tot = C.c_double()
+        per = C.c_double()
+        self._call("nlm_fd_timed_steps", int(nsteps), C.byref(tot),
+                   C.byref(per))
+        return tot.value, per.value
+
+    def neighbors(self):
+        rp = np.zeros(self.p.n + 1, np.int64)
+        col = np.zeros(max(self.nnz, 1), np.int32)
+        self._call("nlm_neighbor_get", ptr(rp, C.c_int64), ptr(col, C.c_int32))
+        return rp, col[: self.nnz]
+
+    def get_state(self):
+        n = self.p.n
+        u = np.zeros((n, 3))
+        v = np.zeros((n, 3))
+        f = np.zeros((n, 3))
+        self._call("nlm_fd_get_state", ptr(u, C.c_double), ptr(v, C.c_double),
+                   ptr(f, C.c_double))
+        return u, v, f
+
+    def set_state(self, u=None, v=None, f=None):
+        def c(a):
+            return None if a is None else np.ascontiguousarray(a, np.float64)
+        u, v, f = c(u), c(v), c(f)
+        self._call("nlm_fd_set_state", ptr(u, C.c_double), ptr(v, C.c_double),
+                   ptr(f, C.c_double))
+
+    def get_bonds(self) -> np.ndarray:
+        b = np.zeros(max(self.nnz, 1), np.uint8)
+        self._call("nlm_fd_get_bonds", ptr(b, C.c_uint8))
+        return b[: self.nnz]
+
+    def set_bonds(self, b: np.ndarray):
+        b = np.ascontiguousarray(b, np.uint8)
+        self._call("nlm_fd_set_bonds", ptr(b, C.c_uint8))
+
+    def get_bond_bytes(self) -> np.ndarray:
+        sz = C.c_int64()
+        self._call("nlm_fd_bond_bytes_size", C.byref(sz))
+        b = np.zeros(max(sz.value, 1), np.uint8)
+        self._call("nlm_fd_get_bond_bytes", ptr(b, C.c_uint8))
+        return b[: sz.value]
+
+    def get_fixity(self) -> np.ndarray:
+        f = np.zeros(self.p.n, np.uint8)
+        self._call("nlm_fd_get_fixity", ptr(f, C.c_uint8))
+        return f
+
+    def time(self):
+        t = C.c_double()
+        s = C.c_int64()
+        self._call("nlm_fd_get_time", C.byref(t), C.byref(s))
+        return t.value, s.value
+
+    def postproc(self):
+        n = self.p.n
+        Z = np.zeros(n, np.float32)
+        phi = np.zeros(n, np.float32)
+        e = np.zeros(n)
+        self._call("nlm_fd_postproc", ptr(Z, C.c_float), ptr(phi, C.c_float),
+                   ptr(e, C.c_double))
+        return Z, phi, e

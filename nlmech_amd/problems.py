@@ -1,0 +1,165 @@
+"""Synthetic specimens for the explicit-dynamics workloads.
+
+A specimen is what NLMech reads from its YAML deck and mesh file: node
+coordinates and volumes, the Model / Material / Displacement_BC / Force_BC /
+Fracture decks.  Grids follow the reference mesh tool's conventions
+(tools/mesh/src/2d/fe2D.cpp:115-198): node (i, j) at x0 + (i h, j h), id
+j*nx + i, volume h^2 halved on each boundary edge (quartered at corners).  A
+seeded SplitMix64 counter hash perturbs every coordinate by up to jitter*h.
+"""
+from __future__ import annotations
+
+import dataclasses
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# Workloads: plate side in nodes, mesh size, horizon/h ratio, step, steps.
+PLATE_2D = {
+    # 100x100 nodes, h = 1e-3 m, horizon 3h, 1000 steps of 1e-7 s
+    "plate2d_1e4": dict(nx=100, ny=100, h=1e-3, ratio=3, dt=1e-7, nsteps=1000),
+    # 1000x1000 nodes, h = 1e-4 m, horizon 4h, 2000 steps of 1e-8 s
+    "plate2d_1e6": dict(nx=1000, ny=1000, h=1e-4, ratio=4, dt=1e-8,
+                        nsteps=2000),
+}
+
+
+@dataclass
+class BC:
+    """One Displacement_BC / Force_BC set (inp::BCData)."""
+    region: str
+    dirs: list
+    time_fn: str
+    time_params: list
+    space_fn: str = "constant"
+    space_params: list = field(default_factory=lambda: [1.0])
+    x1: float = 0.0
+    x2: float = 0.0
+    y1: float = 0.0
+    y2: float = 0.0
+    z1: float = 0.0
+    z2: float = 0.0
+    r1: float = 0.0
+    r2: float = 0.0
+    theta: float = 0.0
+
+
+@dataclass
+class CrackSpec:
+    """One Fracture: Cracks: Set (inp::EdgeCrack)."""
+    o: int
+    pb: tuple
+    pt: tuple
+    theta: float = 0.0
+    activation_time: float = -1.0
+
+
+@dataclass
+class Problem:
+    name: str
+    dim: int
+    x: np.ndarray          # (n, 3) float64 reference coordinates
+    vol: np.ndarray        # (n,) float64
+    horizon: float
+    h: float
+    dt: float
+    nsteps: int
+    material: dict         # MaterialDeck fields
+    ubc: list = field(default_factory=list)
+    fbc: list = field(default_factory=list)
+    cracks: list = field(default_factory=list)
+    interior: np.ndarray | None = None
+    shape: tuple = ()
+
+    @property
+    def n(self) -> int:
+        return int(self.x.shape[0])
+
+
+def splitmix64(counter: np.ndarray, seed: int) -> np.ndarray:
+    """SplitMix64 output for each counter value (Steele, Lea, Flood 2014)."""
+    with np.errstate(over="ignore"):
+        z = (np.uint64(seed) + (counter.astype(np.uint64) + np.uint64(1))
+             * np.uint64(0x9E3779B97F4A7C15))
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform01(counter: np.ndarray, seed: int) -> np.ndarray:
+    return (splitmix64(counter, seed) >> np.uint64(11)).astype(np.float64) * (
+        1.0 / 9007199254740992.0)
+
+
+def rnp_material(dim=2, E=3.24e9, Gc=500.0, density=1200.0, check_sc=1.0,
+                 inf_type=0, inf_params=(1.0,), plane_strain=False,
+                 irreversible=True, no_penetration=False) -> dict:
+    return dict(dim=dim, E=E, K=-1.0, Gc=Gc, KIc=-1.0, density=density,
+                plane_strain=int(plane_strain), compute_from_classical=1,
+                bond_params=[], inf_type=inf_type,
+                inf_params=list(inf_params), check_sc_factor=check_sc,
+                irreversible=int(irreversible),
+                no_penetration=int(no_penetration))
+
+
+def grid2d(nx: int, ny: int, h: float, jitter: float = 1e-6, seed: int = 7,
+           x0: float = 0.0, y0: float = 0.0):
+    ii, jj = np.meshgrid(np.arange(nx), np.arange(ny), indexing="xy")
+    ii = ii.ravel()
+    jj = jj.ravel()
+    n = nx * ny
+    x = np.zeros((n, 3))
+    x[:, 0] = x0 + ii.astype(np.float64) * h
+    x[:, 1] = y0 + jj.astype(np.float64) * h
+    if jitter:
+        cnt = np.arange(2 * n, dtype=np.uint64)
+        r = uniform01(cnt, seed).reshape(n, 2)
+        x[:, :2] += (2.0 * r - 1.0) * (jitter * h)
+    vol = np.full(n, h * h)
+    vol[(ii == 0) | (ii == nx - 1)] *= 0.5
+    vol[(jj == 0) | (jj == ny - 1)] *= 0.5
+    return x, vol
+
+
+def plate2d(nx: int, ny: int, h: float, ratio: int, dt: float, nsteps: int,
+            notch_len: float = 0.02, pull: float = 0.5, layers: int = 3,
+            jitter: float = 1e-6, seed: int = 7, name: str | None = None,
+            **mat) -> Problem:
+    """Square plate with a left-edge notch at mid-height and top/bottom layers
+    pulled apart at +/- pull m/s (Displacement_BC, linear in time, dof 2).
+
+    The notch is an Orientation 1 crack from (-h/2, ym) to (notch_len - h/2,
+    ym), ym = (ny-1) h / 2 the mid-height between two node rows; the bonds
+    crossing it are broken at t = 0.
+    """
+    x, vol = grid2d(nx, ny, h, jitter, seed)
+    horizon = ratio * h
+    xmax = (nx - 1) * h
+    ytop = (ny - 1) * h
+    ubc = []
+    if layers > 0 and pull != 0.0:
+        ubc.append(BC("rectangle", [2], "linear", [pull],
+                      x1=-h, x2=xmax + h, y1=ytop - (layers - 1) * h - 0.5 * h,
+                      y2=ytop + 0.5 * h))
+        ubc.append(BC("rectangle", [2], "linear", [-pull],
+                      x1=-h, x2=xmax + h, y1=-0.5 * h,
+                      y2=(layers - 1) * h + 0.5 * h))
+    cracks = []
+    if notch_len > 0:
+        ym = 0.5 * ytop
+        cracks.append(CrackSpec(1, (-0.5 * h, ym), (notch_len - 0.5 * h, ym)))
+    m = rnp_material(dim=2, **mat)
+    return Problem(name=name or f"plate2d_{nx}x{ny}", dim=2, x=x, vol=vol,
+                   horizon=horizon, h=h, dt=dt, nsteps=nsteps, material=m,
+                   ubc=ubc, cracks=cracks, shape=(nx, ny))
+
+
+def workload(key: str, **over) -> Problem:
+    spec = dict(PLATE_2D[key])
+    spec.update(over)
+    return plate2d(name=key, **spec)
+
+
+def with_(p: Problem, **kw) -> Problem:
+    return dataclasses.replace(p, **kw)

@@ -1,0 +1,190 @@
+/*
+ * nlm_oracle.h -- CPU restatement of NLMech's explicit-dynamics hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product
+ * library (nlmech_amd).  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The product never links it.
+ *
+ * Every routine restates one function of the reference (NLMech v0.1.0) in the
+ * same floating-point operation order, compiled with -ffp-contract=off so that
+ * no multiply-add is fused.  Citations are file:line into the reference tree.
+ *
+ * Layouts follow the reference containers:
+ *   - node vectors are AoS Point3, i.e. double[3*n] (z = 0 in 2D),
+ *     src/util/point.h:29-516;
+ *   - neighbor lists are CSR (row_ptr int64[n+1], col int32[nnz]), each row in
+ *     the order the reference's brute-force loop produces (ascending j),
+ *     src/geometry/neighbor.cpp:69-89;
+ *   - broken-bond state is one byte per directed bond in CSR order (0/1) at
+ *     this interface; internally the reference packing (ceil(n_i/8) bytes per
+ *     node, bit jl%8 of byte jl/8) is kept, src/geometry/fracture.cpp:24-33,
+ *     162-187.
+ */
+#ifndef NLM_ORACLE_H
+#define NLM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Material deck fields used by RNPBond (src/inp/decks/materialDeck.h:226-288,
+ * MatData defaults -1 at :20-99). */
+typedef struct {
+  int dim;
+  double horizon;
+  double h; /* mesh size */
+  double density;
+  double E, K, Gc, KIc; /* < 0 means "not given" */
+  int plane_strain;
+  int compute_from_classical;
+  int n_bond_params;
+  double bond_params[2];
+  int inf_type; /* 0 const, 1 linear, 2 gaussian */
+  int n_inf_params;
+  double inf_params[2];
+  double check_sc_factor;
+  int irreversible;
+  int no_penetration;
+} ora_mat_deck;
+
+/* Derived RNPBond parameters (src/material/pd/rnpBond.cpp:23-97). */
+typedef struct {
+  int dim;
+  double horizon, h, density;
+  double C, beta, rbar, inv_factor, factor_sc;
+  int irreversible;
+  int inf_type;
+  double inf_a, inf_b; /* const: a0; linear: a0,a1; gaussian: alpha,beta */
+  double inf_moment;   /* getMoment(dim) */
+  int contact;
+  double contact_kn, contact_rc;
+  /* elastic constants after computeParameters() */
+  double E, K, nu, lambda, G, Gc, KIc;
+} ora_rnp;
+
+/* Boundary condition set: inp::BCData (src/inp/decks/loadingDeck.h) with the
+ * region / function strings replaced by codes. */
+enum {
+  ORA_REGION_RECTANGLE = 0,
+  ORA_REGION_ANGLED_RECTANGLE = 1,
+  ORA_REGION_CIRCLE = 2,
+  ORA_REGION_TORUS = 3,
+  ORA_REGION_LINE = 4,
+  ORA_REGION_CUBOID = 5
+};
+enum {
+  ORA_TFN_CONSTANT = 0,
+  ORA_TFN_LINEAR = 1,
+  ORA_TFN_QUADRATIC = 2,
+  ORA_TFN_SIN = 3,
+  ORA_TFN_LINEAR_STEP = 4,
+  ORA_TFN_LINEAR_SLOW_FAST = 5
+};
+enum {
+  ORA_SFN_CONSTANT = 0,
+  ORA_SFN_SIN_X = 1,
+  ORA_SFN_SIN_Y = 2,
+  ORA_SFN_SIN_Z = 3,
+  ORA_SFN_LINEAR_X = 4,
+  ORA_SFN_LINEAR_Y = 5,
+  ORA_SFN_LINEAR_Z = 6,
+  ORA_SFN_HAT_X = 7,
+  ORA_SFN_HAT_Y = 8,
+  ORA_SFN_HAT_Z = 9
+};
+typedef struct {
+  int region;
+  double x1, x2, y1, y2, z1, z2, r1, r2, theta;
+  int ndir;
+  int dir[3]; /* 1-based dofs */
+  int time_fn;
+  int n_time_params;
+  double time_params[4];
+  int space_fn;
+  int n_space_params;
+  double space_params[4];
+} ora_bc;
+
+/* Edge crack: inp::EdgeCrack (src/inp/decks/fractureDeck.h:22-219). */
+typedef struct {
+  int o; /* 1 horizontal, -1 vertical, 0 angled */
+  double pb[2], pt[2];
+  double theta;
+  double activation_time;
+} ora_crack;
+
+/* util::compare (src/util/compare.cpp:13-33) */
+int ora_definitely_greater(double a, double b);
+int ora_definitely_less(double a, double b);
+
+/* RNPBond::RNPBond + computeParameters (src/material/pd/rnpBond.cpp:23-189) */
+int ora_rnp_params(const ora_mat_deck* deck, ora_rnp* out);
+
+/* One bond, RNPBond::getBondEF (src/material/pd/rnpBond.cpp:216-301) on
+ * explicit arguments: x/u are Point3, fs in/out is the bond bit. */
+void ora_bond_ef(const ora_rnp* m, const double* xi, const double* ui,
+                 const double* xj, const double* uj, double voli, double volj,
+                 int interior_i, int interior_j, int* fs, double* force3,
+                 double* energy);
+
+/* Neighbor::Neighbor brute-force path (src/geometry/neighbor.cpp:69-89).
+ * Returns a handle; fetch with ora_nbr_nnz / ora_nbr_copy; free after. */
+void* ora_neighbor_bruteforce(int64_t n, const double* x, double horizon);
+/* Same predicate, candidates from a uniform cell grid; rows sorted. */
+void* ora_neighbor_cells(int64_t n, const double* x, double horizon);
+int64_t ora_nbr_nnz(void* h);
+void ora_nbr_copy(void* h, int64_t* row_ptr, int32_t* col);
+void ora_nbr_free(void* h);
+
+/* Explicit FD model (src/model/fd/fDModel.cpp) on a given mesh and CSR. */
+void* ora_fd_create(int dim, int64_t n, const double* x, const double* vol,
+                    const int64_t* row_ptr, const int32_t* col,
+                    const ora_rnp* mat, double dt, int n_ubc,
+                    const ora_bc* ubc, int n_fbc, const ora_bc* fbc,
+                    int n_cracks, const ora_crack* cracks,
+                    const uint8_t* interior /* may be NULL => all interior */);
+void ora_fd_free(void* h);
+/* integrate() prologue: IC (none), ULoading/FLoading at t, computeForces
+ * (fDModel.cpp:318-333). */
+void ora_fd_init(void* h);
+/* nsteps x integrateCD (fDModel.cpp:377-443) */
+void ora_fd_step_cd(void* h, int nsteps);
+/* nsteps x integrateVerlet (fDModel.cpp:446-525) */
+void ora_fd_step_verlet(void* h, int nsteps);
+/* computeForces on current u, adding into f (fDModel.cpp:528-541) */
+void ora_fd_compute_forces(void* h);
+void ora_fd_get_state(void* h, double* u, double* v, double* f);
+void ora_fd_set_state(void* h, const double* u, const double* v,
+                      const double* f);
+void ora_fd_get_bonds(void* h, uint8_t* bond_state /* [nnz] */);
+void ora_fd_set_bonds(void* h, const uint8_t* bond_state);
+/* packed reference layout: ceil(n_i/8) bytes per node, concatenated */
+void ora_fd_get_bond_bytes(void* h, uint8_t* bytes);
+int64_t ora_fd_bond_bytes_size(void* h);
+void ora_fd_get_fixity(void* h, uint8_t* fix);
+double ora_fd_time(void* h);
+int64_t ora_fd_step_count(void* h);
+/* computePostProcFields damage Z / phi / strain energy density
+ * (fDModel.cpp:652-778): Z and phi stored as float like the reference. */
+void ora_fd_postproc(void* h, float* Z, float* phi, double* energy);
+/* per node sum_j |F_ij| at the current state (bond bits untouched) */
+void ora_fd_force_abs_sum(void* h, double* out);
+void ora_fd_set_time(void* h, double t, int64_t step);
+/* ComputeInterior::getInteriorFlag (interiorFlags.cpp:57-100) for rectangle
+ * and circle no-fail regions; regions[k] = {type(0 rect,1 circle), p0..p3}. */
+void ora_interior_flags(int64_t n, const double* x, const double* bbox_lo,
+                        const double* bbox_hi, double tol, int nreg,
+                        const double* regions /* [nreg*5] */, uint8_t* out);
+/* Mesh::computeBBox (src/fe/mesh.cpp:311-324) */
+void ora_bbox(int64_t n, const double* x, double* lo, double* hi);
+
+/* OpenMP thread count actually used by the force loop */
+int ora_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

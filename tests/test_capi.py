@@ -1,0 +1,70 @@
+"""CPU checks of the C-ABI library: it loads, exports every entry point
+include/nlmech_amd.h declares, and its host-side logic (RNPBond constants)
+equals the oracle's.  No GPU work is issued here."""
+import ctypes as C
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+from nlmech_amd import capi
+from nlmech_amd import problems as pr
+
+
+def declared_functions():
+    text = (ROOT / "include" / "nlmech_amd.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(nlm_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = capi.load()
+    names = declared_functions()
+    assert len(names) >= 25
+    for n in names:
+        assert hasattr(lib, n), n
+    # the ctypes table covers the header
+    assert set(names) <= set(capi.SIGNATURES)
+
+
+@pytest.mark.parametrize("variant", ["default", "plane_strain", "linear",
+                                     "gauss", "params", "contact", "K_only"])
+def test_rnp_params_equal_oracle(oracle, variant):
+    from nlmech_amd.engine import material_deck
+    p = pr.plate2d(4, 4, 1e-3, 4, 1e-8, 1)
+    mat = pr.rnp_material()
+    if variant == "plane_strain":
+        mat = pr.rnp_material(plane_strain=True)
+    elif variant == "linear":
+        mat = pr.rnp_material(inf_type=1, inf_params=())
+    elif variant == "gauss":
+        mat = pr.rnp_material(inf_type=2, inf_params=())
+    elif variant == "params":
+        mat.update(compute_from_classical=0, bond_params=[900.0, 3e7])
+    elif variant == "contact":
+        mat = pr.rnp_material(no_penetration=True)
+    elif variant == "K_only":
+        mat.update(E=-1.0, K=2.0e9, Gc=-1.0, KIc=1.2e6)
+    p = pr.with_(p, material=mat)
+    lib = capi.load()
+    out = capi.RnpParams()
+    assert lib.nlm_rnp_params_from_deck(C.byref(material_deck(p)),
+                                        C.byref(out)) == 0
+    ref = oracle.rnp_params(p)
+    for f, _ in capi.RnpParams._fields_:
+        a, b = getattr(out, f), getattr(ref, f)
+        assert a == b or (np.isnan(a) and np.isnan(b)), f
+
+
+def test_rnp_params_rejects_bad_deck():
+    from nlmech_amd.engine import material_deck
+    lib = capi.load()
+    p = pr.plate2d(4, 4, 1e-3, 3, 1e-7, 1)
+    bad = pr.with_(p, material=dict(pr.rnp_material(), E=-1.0, K=-1.0))
+    out = capi.RnpParams()
+    assert lib.nlm_rnp_params_from_deck(C.byref(material_deck(bad)),
+                                        C.byref(out)) != 0
+    bad = pr.with_(p, material=dict(pr.rnp_material(), inf_type=7))
+    assert lib.nlm_rnp_params_from_deck(C.byref(material_deck(bad)),
+                                        C.byref(out)) != 0

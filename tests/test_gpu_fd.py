@@ -1,0 +1,317 @@
+"""GPU parity of the explicit FD path against the CPU oracle.
+
+Bars (all through the C-ABI, libnlmech_amd.so):
+  - neighbor lists: identical CSR (sorted rows) -- integer, bit-exact;
+  - bond bits (pre-crack and every break decision): bit-exact;
+  - per-node force from an identical state, FP64 modes:
+        |f_gpu - f_ref|_inf <= 1e-12 * sum_j |F_ij^ref|   (per node)
+    FP32-force mode: <= 2e-6 * sum_j |F_ij^ref|;
+  - trajectories, FP64 modes: relative L2 displacement error <= 1e-9 and
+    identical broken-bond sets; FP32-force mode: <= 1e-5 and < 0.1 %
+    broken-bond mismatch.
+Device exp differs from glibc exp in the last ulp, so FP64 trajectories are
+compared with a tolerance, not bitwise.
+"""
+import numpy as np
+import pytest
+
+from conftest import progress
+from nlmech_amd import problems as pr
+from nlmech_amd.capi import (NLM_PREC_FP32_FORCE, NLM_PREC_FP64,
+                             NLM_PREC_FP64_EXACT, NLM_TIME_VELOCITY_VERLET)
+
+pytestmark = pytest.mark.gpu
+
+MODES = [NLM_PREC_FP64_EXACT, NLM_PREC_FP64, NLM_PREC_FP32_FORCE]
+FORCE_TOL = {NLM_PREC_FP64_EXACT: 1e-12, NLM_PREC_FP64: 1e-12,
+             NLM_PREC_FP32_FORCE: 2e-6}
+
+
+@pytest.fixture(scope="module")
+def engine():
+    from nlmech_amd import engine
+    return engine
+
+
+def rel_l2(a, b):
+    d = np.linalg.norm(a - b)
+    s = np.linalg.norm(b)
+    return d / s if s > 0 else d
+
+
+def force_err(f_gpu, f_ref, absF):
+    d = np.abs(f_gpu - f_ref).max(axis=1)
+    scale = np.where(absF > 0, absF, 1.0)
+    return (d / scale).max()
+
+
+# ------------------------------------------------------------ neighbors
+@pytest.mark.parametrize("case", ["2d_3h", "2d_4h", "3d_3h", "cloud"])
+def test_neighbor_lists_match_oracle(engine, oracle, case):
+    progress(f"[gpu] neighbor lists {case}")
+    if case.startswith("2d"):
+        ratio = 3 if case == "2d_3h" else 4
+        p = pr.plate2d(61, 47, 1e-3, ratio, 1e-7, 1, notch_len=0.0, pull=0.0)
+        x, horizon, dim = p.x, p.horizon, 2
+    elif case == "3d_3h":
+        g = np.arange(13) * 1e-3
+        z, y, xx = np.meshgrid(g, g, g, indexing="ij")
+        x = np.stack([xx.ravel(), y.ravel(), z.ravel()], 1)
+        x += pr.uniform01(np.arange(x.size, dtype=np.uint64), 11).reshape(x.shape) * 1e-9
+        horizon, dim = 3e-3, 3
+    else:
+        rng = np.random.default_rng(5)
+        x = np.zeros((3000, 3))
+        x[:, :2] = rng.random((3000, 2))
+        horizon, dim = 0.04, 2
+    rp_ref, col_ref = oracle.neighbors(x, horizon, "brute")
+    lib = engine.capi.load()
+    import ctypes as C
+    h = C.c_void_p()
+    assert lib.nlm_ctx_create(C.byref(h), 0) == 0
+    try:
+        xx = np.ascontiguousarray(x)
+        nnz = C.c_int64()
+        rc = lib.nlm_neighbor_build(h, dim, x.shape[0],
+                                    engine.capi.ptr(xx, C.c_double), horizon,
+                                    C.byref(nnz))
+        assert rc == 0, lib.nlm_last_error(h)
+        rp = np.zeros(x.shape[0] + 1, np.int64)
+        col = np.zeros(nnz.value, np.int32)
+        assert lib.nlm_neighbor_get(h, engine.capi.ptr(rp, C.c_int64),
+                                    engine.capi.ptr(col, C.c_int32)) == 0
+    finally:
+        lib.nlm_ctx_destroy(h)
+    assert np.array_equal(rp, rp_ref)
+    assert np.array_equal(col, col_ref)
+
+
+# ------------------------------------------------------------ setup
+def test_precrack_bits_and_packing_match(engine, oracle):
+    progress("[gpu] pre-crack bits")
+    p = pr.plate2d(40, 40, 1e-3, 3, 1e-7, 1)
+    o = oracle.OracleFD(p)
+    with engine.GpuFD(p) as g:
+        rp, col = g.neighbors()
+        assert np.array_equal(rp, o.row_ptr) and np.array_equal(col, o.col)
+        assert np.array_equal(g.get_bonds(), o.get_bonds())
+        assert np.array_equal(g.get_bond_bytes(), o.get_bond_bytes())
+        assert np.array_equal(g.get_fixity(), o.get_fixity())
+        assert o.get_bonds().sum() > 0
+
+
+def test_angled_and_vertical_cracks(engine, oracle):
+    progress("[gpu] angled/vertical cracks")
+    p = pr.plate2d(30, 30, 1e-3, 3, 1e-7, 1, notch_len=0.0, pull=0.0)
+    p.cracks = [pr.CrackSpec(-1, (0.0145, -0.001), (0.0145, 0.012)),
+                pr.CrackSpec(0, (0.003, 0.0205), (0.02, 0.0265), theta=0.34)]
+    o = oracle.OracleFD(p)
+    with engine.GpuFD(p) as g:
+        assert np.array_equal(g.get_bonds(), o.get_bonds())
+        assert o.get_bonds().sum() > 0
+
+
+# ------------------------------------------------------------ forces
+def _deformed_state(p, seed, sigma):
+    """Random displacements of standard deviation sigma [m]."""
+    rng = np.random.default_rng(seed)
+    u = np.zeros((p.n, 3))
+    u[:, :2] = rng.normal(scale=sigma, size=(p.n, 2))
+    v = np.zeros((p.n, 3))
+    v[:, :2] = rng.normal(scale=1e-2, size=(p.n, 2))
+    return u, v
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_force_parity_identical_state(engine, oracle, mode):
+    """One computeForces from the same (u, bits): per-node forces within the
+    condition-aware bar and identical break decisions, with bonds near and
+    across the break threshold."""
+    progress(f"[gpu] force parity mode={mode}")
+    p = pr.plate2d(50, 50, 1e-3, 3, 1e-7, 1)
+    o = oracle.OracleFD(p)
+    # bond stretches ~ sqrt(2) sigma / r ~ the critical stretch rbar/sqrt(r)
+    u, v = _deformed_state(p, 1, sigma=4e-6)
+    with engine.GpuFD(p, precision=mode) as g:
+        f0 = np.zeros((p.n, 3))
+        g.set_state(u=u, v=v, f=f0)
+        o.set_state(u=u, v=v, f=f0)
+        absF = o.force_abs_sum()
+        o.compute_forces()
+        g.compute_forces()
+        _, _, fr = o.get_state()
+        _, _, fg = g.get_state()
+        br, bg = o.get_bonds(), g.get_bonds()
+        assert br.sum() > 2000  # the state breaks bonds beyond the notch
+        assert np.array_equal(bg, br)
+        assert force_err(fg, fr, absF) <= FORCE_TOL[mode]
+
+
+# ------------------------------------------------------------ trajectories
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("pull", [0.5, 30.0])
+def test_trajectory_parity_small(engine, oracle, mode, pull):
+    progress(f"[gpu] trajectory 40x40 pull={pull} mode={mode}")
+    steps = 80
+    p = pr.plate2d(40, 40, 1e-3, 3, 1e-7, steps, pull=pull)
+    o = oracle.OracleFD(p)
+    o.init()
+    o.step(steps)
+    ur, vr, fr = o.get_state()
+    br = o.get_bonds()
+    with engine.GpuFD(p, precision=mode) as g:
+        g.init()
+        g.step(steps // 2)
+        g.step(steps - steps // 2)  # two calls: the state carries over
+        ug, vg, fg = g.get_state()
+        bg = g.get_bonds()
+        t, n = g.time()
+    to, no = o.time()
+    assert n == no and t == to
+    if mode == NLM_PREC_FP32_FORCE:
+        assert rel_l2(ug, ur) <= 1e-5
+        mism = np.count_nonzero(bg != br)
+        assert mism <= 1e-3 * max(br.sum(), 1)
+    else:
+        assert rel_l2(ug, ur) <= 1e-9
+        assert rel_l2(vg, vr) <= 1e-8
+        assert np.array_equal(bg, br)
+    if pull > 1:
+        assert br.sum() > 1000
+
+
+def test_trajectory_plate2d_1e4_exact(engine, oracle):
+    """100x100 plate, horizon 3h, notch, +/-0.5 m/s pulls, 1000 steps of
+    1e-7 s in FP64 exact mode."""
+    progress("[gpu] plate2d_1e4 1000 steps exact")
+    p = pr.workload("plate2d_1e4")
+    o = oracle.OracleFD(p)
+    o.init()
+    o.step(p.nsteps)
+    ur, vr, fr = o.get_state()
+    with engine.GpuFD(p, precision=NLM_PREC_FP64_EXACT) as g:
+        assert g.nnz == 234036
+        g.init()
+        g.step(p.nsteps)
+        ug, vg, fg = g.get_state()
+        assert np.array_equal(g.get_bonds(), o.get_bonds())
+    assert rel_l2(ug, ur) <= 1e-9
+    assert rel_l2(fg, fr) <= 1e-7
+
+
+def test_stepwise_parity_breaking(engine, oracle):
+    """Per-step parity judged from identical input state along a breaking
+    trajectory: at each checkpoint the GPU state is loaded into the oracle,
+    one step is taken on both, bits must match exactly and forces within
+    1e-12 of sum |F_ij|."""
+    progress("[gpu] stepwise parity breaking")
+    p = pr.plate2d(40, 40, 1e-3, 3, 1e-7, 1, pull=30.0)
+    o = oracle.OracleFD(p)
+    with engine.GpuFD(p, precision=NLM_PREC_FP64) as g:
+        g.init()
+        for chk in range(6):
+            g.step(15)
+            u, v, f = g.get_state()
+            b = g.get_bonds()
+            t, n = g.time()
+            o.set_state(u=u, v=v, f=f)
+            o.set_bonds(b)
+            o.set_time(t, n)
+            g.step(1)
+            o.step(1)
+            ur, vr, fr = o.get_state()
+            absF = o.force_abs_sum()
+            ug, vg, fg = g.get_state()
+            assert np.array_equal(g.get_bonds(), o.get_bonds())
+            assert force_err(fg, fr, absF) <= 1e-12
+            assert rel_l2(ug, ur) <= 1e-14
+        assert o.get_bonds().sum() > 1000
+
+
+def test_postproc_parity(engine, oracle):
+    progress("[gpu] postproc")
+    p = pr.plate2d(40, 40, 1e-3, 3, 1e-7, 60, pull=30.0)
+    o = oracle.OracleFD(p)
+    o.init()
+    o.step(60)
+    with engine.GpuFD(p) as g:
+        g.init()
+        g.step(60)
+        u, v, f = g.get_state()
+        o.set_state(u=u, v=v, f=f)
+        o.set_bonds(g.get_bonds())
+        Zr, phir, er = o.postproc()
+        Zg, phig, eg = g.postproc()
+        assert np.array_equal(g.get_bonds(), o.get_bonds())
+    assert np.array_equal(phig, phir)
+    assert np.max(np.abs(Zg - Zr) / np.maximum(Zr, 1e-30)) <= 1e-6
+    assert np.max(np.abs(eg - er) / np.maximum(np.abs(er), 1e-300)) <= 1e-12
+
+
+def test_verlet_parity(engine, oracle):
+    progress("[gpu] velocity verlet")
+    p = pr.plate2d(30, 30, 1e-3, 3, 1e-7, 40, pull=10.0)
+    o = oracle.OracleFD(p)
+    o.init()
+    o.step(40, scheme="verlet")
+    ur, vr, _ = o.get_state()
+    with engine.GpuFD(p, scheme=NLM_TIME_VELOCITY_VERLET) as g:
+        g.init()
+        g.step(40)
+        ug, vg, _ = g.get_state()
+        assert np.array_equal(g.get_bonds(), o.get_bonds())
+    assert rel_l2(ug, ur) <= 1e-9 and rel_l2(vg, vr) <= 1e-8
+
+
+def test_force_bc_nofail_contact_timed_crack(engine, oracle):
+    """Force BCs (hat / linear / sin time), a no-fail region, the contact
+    force of broken bonds and a crack activated during stepping."""
+    progress("[gpu] fbc + no-fail + contact + timed crack")
+    p = pr.plate2d(36, 36, 1e-3, 3, 1e-7, 50, pull=20.0, no_penetration=True)
+    p.fbc = [pr.BC("rectangle", [1], "linear", [4e9], "hat_y", [1.0],
+                   x1=-1e-3, x2=2.5e-3, y1=-1e-3, y2=0.036),
+             pr.BC("rectangle", [2], "sin", [2e9, 1e4], "constant", [1.0],
+                   x1=0.03, x2=0.036, y1=0.01, y2=0.02)]
+    p.cracks = list(p.cracks) + [pr.CrackSpec(-1, (0.03, 0.005), (0.03, 0.02),
+                                              activation_time=2.05e-6)]
+    lo, hi = np.zeros(3), np.zeros(3)
+    interior = np.ones(p.n, np.uint8)
+    interior[(p.x[:, 0] > 0.025) & (p.x[:, 1] > 0.025)] = 0
+    p.interior = interior
+    o = oracle.OracleFD(p)
+    o.init()
+    o.step(50)
+    ur, vr, fr = o.get_state()
+    with engine.GpuFD(p, precision=NLM_PREC_FP64_EXACT) as g:
+        g.init()
+        g.step(50)
+        ug, vg, fg = g.get_state()
+        assert np.array_equal(g.get_bonds(), o.get_bonds())
+    assert rel_l2(ug, ur) <= 1e-9
+    assert o.get_bonds().sum() > 0
+
+
+# ------------------------------------------------------------ full size
+def test_plate2d_1e6_properties(engine):
+    """1000x1000 plate at horizon 4h (no oracle run at this size):
+    B = 43,856,108 directed bonds, symmetric sorted lists, and sum_i V_i f_i
+    = 0 to rounding after 20 steps in every mode."""
+    progress("[gpu] plate2d_1e6 properties")
+    p = pr.workload("plate2d_1e6")
+    with engine.GpuFD(p, precision=NLM_PREC_FP64) as g:
+        assert g.nnz == 43856108
+        rp, col = g.neighbors()
+        rows = np.repeat(np.arange(p.n, dtype=np.int64), np.diff(rp))
+        assert np.all(np.diff(col.astype(np.int64))[np.diff(rows) == 0] > 0)
+        fwd = rows * p.n + col
+        bwd = col.astype(np.int64) * p.n + rows
+        assert np.array_equal(np.sort(fwd), np.sort(bwd))
+        del rows, fwd, bwd
+        g.init()
+        g.step(20)
+        u, v, f = g.get_state()
+        nb = g.get_bonds().sum()
+    assert np.isfinite(u).all()
+    assert nb > 0  # the notch
+    tot = (p.vol[:, None] * f).sum(0)
+    assert np.all(np.abs(tot) <= 1e-10 * (p.vol[:, None] * np.abs(f)).sum())

@@ -1,0 +1,423 @@
+"""Pins the CPU oracle (oracle/nlm_oracle.cpp) before it is trusted.
+
+Each check is chosen so that a plausible restatement mistake fails it:
+reference-held fixtures (QS vtu goldens, TestGeom node set, dc known
+answer), closed forms (lattice neighbor counts, RNP parameters, free-node
+kinematics, BC time functions) and invariants (energy-force consistency,
+Newton's third law, translation invariance).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+from nlmech_amd import problems as pr
+
+
+# ---------------------------------------------------------------- compare
+def test_compare_semantics(oracle):
+    # util::compare, src/util/compare.cpp:25-33, eps 1e-5 relative
+    dg, dl = oracle.definitely_greater, oracle.definitely_less
+    assert not dg(1.0, 1.0) and not dl(1.0, 1.0)
+    assert dg(1.0 + 2e-5, 1.0) and not dg(1.0 + 5e-6, 1.0)
+    assert dl(1.0, 1.0 + 2e-5) and not dl(1.0, 1.0 + 5e-6)
+    assert dg(0.0, -1e-300) and not dg(-1e-300, 0.0)
+    # relative band scales with the larger magnitude
+    assert not dl(2.99999, 3.0) and dl(2.9999, 3.0)
+
+
+# ---------------------------------------------------------------- params
+def test_rnp_params_closed_form(oracle):
+    p = pr.plate2d(10, 10, 1e-3, 3, 1e-7, 1)
+    m = oracle.rnp_params(p)
+    # plane stress nu = 1/3 (rnpBond.cpp:143-147); lambda = E nu/((1+nu)(1-2nu))
+    lam = 3.24e9 * (1 / 3) / ((4 / 3) * (1 / 3))
+    M = 1.0 / 3.0  # const J = 1, moment(2) = a0/3
+    C = math.pi * 500.0 / (4 * M)
+    beta = 4 * lam / (C * M)
+    assert m.lambda_ == pytest.approx(2.43e9, rel=1e-14)
+    assert m.C == pytest.approx(C, rel=1e-14)
+    assert m.C == pytest.approx(1178.0972450961724, rel=1e-12)
+    assert m.beta == pytest.approx(beta, rel=1e-14)
+    assert m.beta == pytest.approx(2.4752e7, rel=1e-4)
+    assert m.rbar == pytest.approx(math.sqrt(0.5 / beta), rel=1e-14)
+    assert m.rbar == pytest.approx(1.42129e-4, rel=1e-5)
+    assert m.inv_factor == pytest.approx(math.pi * (3e-3) ** 3, rel=1e-14)
+
+
+def test_rnp_params_variants(oracle):
+    # default constant influence a0 = dim + 1 (influenceFn.h:95)
+    p = pr.plate2d(4, 4, 1e-3, 3, 1e-7, 1, inf_params=())
+    m = oracle.rnp_params(p)
+    assert m.inf_a == 3.0 and m.inf_moment == pytest.approx(1.0)
+    # plane strain nu = 1/4; 3D formulas C = 2Gc/(3M), beta = 5 lambda/(C M)
+    p3 = pr.with_(p, dim=3, material=pr.rnp_material(dim=3, plane_strain=True))
+    m3 = oracle.rnp_params(p3)
+    nu = 0.25
+    lam = 3.24e9 * nu / ((1 + nu) * (1 - 2 * nu))
+    M = 1.0 / 4.0
+    assert m3.C == pytest.approx(2 * 500.0 / (3 * M), rel=1e-14)
+    assert m3.beta == pytest.approx(5 * lam / (m3.C * M), rel=1e-14)
+    assert m3.inv_factor == pytest.approx(4 * math.pi / 3 * (3e-3) ** 4)
+    # linear influence default a0 = 12 (2D), moment = a0/3 - a0/4
+    pl = pr.with_(p, material=pr.rnp_material(inf_type=1, inf_params=()))
+    ml = oracle.rnp_params(pl)
+    assert (ml.inf_a, ml.inf_b) == (12.0, -12.0)
+    assert ml.inf_moment == pytest.approx(12 / 3 - 12 / 4)
+    # parameters given directly (Compute_From_Classical false)
+    mat = pr.rnp_material()
+    mat.update(compute_from_classical=0, bond_params=[1000.0, 2.0e7])
+    md = oracle.rnp_params(pr.with_(p, material=mat))
+    assert md.C == 1000.0 and md.beta == 2.0e7
+    assert md.rbar == math.sqrt(0.5 / 2.0e7)
+
+
+# ---------------------------------------------------------------- neighbors
+def _lattice_count(m, dim):
+    r = range(-m, m + 1)
+    if dim == 2:
+        return sum(1 for a in r for b in r if 0 < a * a + b * b < m * m)
+    return sum(1 for a in r for b in r for c in r if 0 < a * a + b * b + c * c < m * m)
+
+
+@pytest.mark.parametrize("ratio,dim", [(3, 2), (4, 2), (5, 2), (3, 3)])
+def test_neighbor_lattice_counts(oracle, ratio, dim):
+    # strict criterion excludes lattice points at exactly the horizon:
+    # 24 / 44 (2D, 3h / 4h), 92 (3D, 3h)
+    h = 1e-3
+    n1 = 2 * ratio + 3
+    if dim == 2:
+        x, _ = pr.grid2d(n1, n1, h, jitter=1e-6)
+    else:
+        g = np.arange(n1) * h
+        z, y, xx = np.meshgrid(g, g, g, indexing="ij")
+        x = np.stack([xx.ravel(), y.ravel(), z.ravel()], 1)
+    rp, col = oracle.neighbors(x, ratio * h, "brute")
+    counts = np.diff(rp)
+    c = n1 // 2
+    centre = c * n1 + c if dim == 2 else (c * n1 + c) * n1 + c
+    assert counts[centre] == _lattice_count(ratio, dim)
+    assert counts.max() == _lattice_count(ratio, dim)
+    rp2, col2 = oracle.neighbors(x, ratio * h, "cells")
+    assert np.array_equal(rp, rp2) and np.array_equal(col, col2)
+    # rows ascending (brute-force loop order, neighbor.cpp:78-84)
+    for i in range(len(counts)):
+        row = col[rp[i]:rp[i + 1]]
+        assert np.all(np.diff(row) > 0)
+
+
+def test_neighbor_counts_match_reference_golden_2d(oracle):
+    # examples/qsModel/2D: 17x17 nodes h = 0.1, horizon 0.5; the golden's
+    # Neighbors field ranges 21..68 (strict); inclusive counting gives 80.
+    d = np.load(GOLDEN / "qs2d_output_res.npz")
+    g = np.arange(17) * (0.5 / 5)
+    yy, xx = np.meshgrid(g, g, indexing="ij")
+    x = np.stack([xx.ravel(), yy.ravel(), np.zeros(289)], 1)
+    xref = d["Points"].astype(np.float64) - d["Displacement"]
+    assert np.abs(xref - x).max() < 1e-6
+    rp, _ = oracle.neighbors(x, 0.5, "brute")
+    assert np.array_equal(np.diff(rp), d["Neighbors"].astype(np.int64))
+    assert np.diff(rp).max() == 68
+
+
+def test_neighbor_counts_match_reference_golden_1d(oracle):
+    # examples/qsModel/1D: 33 nodes on [0, 16] (h = 0.5), run horizon 1
+    d = np.load(GOLDEN / "qs1d_output_res.npz")
+    x = np.zeros((33, 3))
+    x[:, 0] = np.arange(33) * (2.0 / 4)
+    rp, _ = oracle.neighbors(x, 1.0, "brute")
+    assert np.array_equal(np.diff(rp), d["Neighbors"].astype(np.int64))
+
+
+# ---------------------------------------------------------------- fracture
+def test_fracture_bit_semantics_testgeom(oracle):
+    """TestGeom (src/test/geometry/testGeomLib.cpp:42-86): all-to-all lists
+    over testFracture_nodes.csv; bits start clear, even rows set, then odd
+    rows set / even rows cleared.  Also pins the packing of
+    Fracture::setBondState (byte jl/8, bit jl%8, fracture.cpp:170-171)."""
+    nodes = np.loadtxt(GOLDEN / "testFracture_nodes.csv", delimiter=",")
+    n = nodes.shape[0]
+    x = np.zeros((n, 3))
+    x[:, : nodes.shape[1]] = nodes
+    rp = np.arange(n + 1, dtype=np.int64) * n
+    col = np.tile(np.arange(n, dtype=np.int32), n)
+    p = pr.Problem("testgeom", 2, x, np.ones(n), 1.0, 0.1, 1e-7, 1,
+                   pr.rnp_material())
+    o = oracle.OracleFD(p, (rp, col))
+    assert o.get_bonds().sum() == 0
+    nb = (n + 7) // 8
+    state = np.repeat((np.arange(n) % 2 == 0).astype(np.uint8), n)
+    o.set_bonds(state)
+    packed = o.get_bond_bytes().reshape(n, nb)
+    full = np.zeros(nb, np.uint8)
+    for jl in range(n):
+        full[jl // 8] |= 1 << (jl % 8)
+    for i in range(n):
+        assert np.array_equal(packed[i], full if i % 2 == 0 else 0 * full)
+    o.set_bonds(1 - state)
+    assert np.array_equal(o.get_bonds(), 1 - state)
+
+
+def test_precrack_selection_geometry(oracle):
+    """A horizontal crack between rows j=2 and j=3 covering x in
+    [-h/2, 3.5h]: exactly the bonds whose endpoints both lie within the
+    crack's x-extent and on opposite sides are broken
+    (computeFracturedBondFd, fracture.cpp:74-160)."""
+    h = 1.0
+    x, vol = pr.grid2d(8, 6, h, jitter=0)
+    p = pr.Problem("crack", 2, x, vol, 2.0 * h, h, 1e-7, 1, pr.rnp_material(),
+                   cracks=[pr.CrackSpec(1, (-0.5, 2.5), (3.5, 2.5))])
+    o = oracle.OracleFD(p)
+    b = o.get_bonds()
+    got = set()
+    for i in range(p.n):
+        for k in range(o.row_ptr[i], o.row_ptr[i + 1]):
+            if b[k]:
+                got.add((i, int(o.col[k])))
+    want = set()
+    for i in range(p.n):
+        for k in range(o.row_ptr[i], o.row_ptr[i + 1]):
+            j = int(o.col[k])
+            xi, xj = x[i], x[j]
+            inside = -0.5 <= xi[0] <= 3.5 and -0.5 <= xj[0] <= 3.5
+            if inside and (xi[1] - 2.5) * (xj[1] - 2.5) < 0:
+                want.add((i, j))
+    assert got == want and len(want) > 0
+    # symmetric: (i, j) broken iff (j, i) broken
+    assert all((j, i) in got for (i, j) in got)
+
+
+def test_precrack_node_on_line_counts_left(oracle):
+    # a node exactly on the crack line is on the left (top) side
+    # (ptLeftside uses !definitelyLessThan(area, 0), fractureDeck.h:169-173)
+    h = 1.0
+    x, vol = pr.grid2d(5, 5, h, jitter=0)
+    p = pr.Problem("online", 2, x, vol, 1.5, h, 1e-7, 1, pr.rnp_material(),
+                   cracks=[pr.CrackSpec(1, (-0.5, 2.0), (4.5, 2.0))])
+    o = oracle.OracleFD(p)
+    b = o.get_bonds()
+    for i in range(p.n):
+        for k in range(o.row_ptr[i], o.row_ptr[i + 1]):
+            j = int(o.col[k])
+            yi, yj = x[i, 1], x[j, 1]
+            left_i, left_j = yi >= 2.0, yj >= 2.0
+            assert bool(b[k]) == (left_i != left_j)
+
+
+# ---------------------------------------------------------------- bond
+def _mat(oracle):
+    return oracle.rnp_params(pr.plate2d(4, 4, 1e-3, 3, 1e-7, 1))
+
+
+def test_bond_zero_stretch_and_translation(oracle):
+    m = _mat(oracle)
+    xi, xj = np.array([0.0, 0, 0]), np.array([2e-3, 1e-3, 0])
+    F, E, fs = oracle.bond_ef(m, xi, np.zeros(3), xj, np.zeros(3), 1e-6, 1e-6)
+    assert np.all(F == 0) and E == 0 and fs == 0
+    t = np.array([3e-5, -7e-6, 0])
+    F, E, fs = oracle.bond_ef(m, xi, t, xj, t, 1e-6, 1e-6)
+    assert np.all(F == 0) and E == 0
+    # pure rotation by a small angle gives S = dx.du/|dx|^2 = 0 to first order
+    th = 1e-6
+    R = np.array([[math.cos(th), -math.sin(th)], [math.sin(th), math.cos(th)]])
+    ui = np.zeros(3)
+    uj = np.zeros(3)
+    uj[:2] = R @ xj[:2] - xj[:2]
+    F, E, _ = oracle.bond_ef(m, xi, ui, xj, uj, 1e-6, 1e-6)
+    # the same displacement magnitude as a stretch S = th
+    Fs, _, _ = oracle.bond_ef(m, xi, ui, xj, th * xj, 1e-6, 1e-6)
+    assert np.linalg.norm(F) < 1e-5 * np.linalg.norm(Fs)
+
+
+def test_bond_energy_force_consistency(oracle):
+    """F = (2/r) dE/dS e, rnpBond.h:56-83 (E = J C (1-exp(-beta r S^2)) V/f)."""
+    m = _mat(oracle)
+    xi, xj = np.array([0.0, 0, 0]), np.array([2e-3, 1e-3, 0])
+    r = np.linalg.norm(xj - xi)
+    e = (xj - xi) / r
+
+    def E_of(S):
+        uj = S * (xj - xi)  # du parallel to dx gives exactly stretch S
+        return oracle.bond_ef(m, xi, np.zeros(3), xj, uj, 1e-6, 1e-6)
+
+    S = 1e-3
+    F, _, fs = E_of(S)
+    assert fs == 0
+    dS = 1e-8
+    dE = (E_of(S + dS)[1] - E_of(S - dS)[1]) / (2 * dS)
+    assert np.allclose(F, (2 / r) * dE * e, rtol=1e-6, atol=0)
+    # magnitude closed form
+    s = 1.0 * 4 * S * m.C * m.beta * math.exp(-m.beta * r * S * S) / m.inv_factor
+    assert np.allclose(F, s * 1e-6 * e, rtol=1e-13)
+
+
+def test_bond_break_threshold_and_branches(oracle):
+    m = _mat(oracle)
+    xi, xj = np.array([0.0, 0, 0]), np.array([2e-3, 0, 0])
+    r = 2e-3
+    T = m.factor_sc * (m.rbar / math.sqrt(r))
+    edge = T / (1 - 1e-5)
+    for S, broken in ((edge * (1 + 1e-9), 1), (edge * (1 - 1e-9), 0),
+                      (-edge * (1 + 1e-9), 1)):
+        uj = np.array([S * r, 0, 0])
+        F, E, fs = oracle.bond_ef(m, xi, np.zeros(3), xj, uj, 1e-6, 1e-6)
+        assert fs == broken
+        if broken:
+            # broken bond: no force without contact, energy J C V / f
+            assert np.all(F == 0)
+            assert E == pytest.approx(m.C / m.inv_factor * 1e-6, rel=1e-14)
+    # non-interior: linearised force, never breaks
+    uj = np.array([2 * edge * r, 0, 0])
+    F, E, fs = oracle.bond_ef(m, xi, np.zeros(3), xj, uj, 1e-6, 1e-6, 0, 1)
+    assert fs == 0
+    assert F[0] == pytest.approx(4 * (2 * edge) * m.C * m.beta / m.inv_factor * 1e-6,
+                                 rel=1e-13)
+    # small S: nonlinear force tends to the linearised one
+    Ss = 1e-7
+    Fn, _, _ = oracle.bond_ef(m, xi, np.zeros(3), xj, np.array([Ss * r, 0, 0]),
+                              1e-6, 1e-6)
+    Fl, _, _ = oracle.bond_ef(m, xi, np.zeros(3), xj, np.array([Ss * r, 0, 0]),
+                              1e-6, 1e-6, 0, 1)
+    assert Fn[0] == pytest.approx(Fl[0], rel=1e-9)
+
+
+def test_volume_correction(oracle):
+    # volj scaled by (horizon + h/2 - r)/h when r > horizon - h/2
+    m = _mat(oracle)  # horizon 3e-3, h 1e-3
+    xi = np.zeros(3)
+    S = 1e-4
+    for r, fac in ((2.0e-3, 1.0), (2.8e-3, (3.5e-3 - 2.8e-3) / 1e-3)):
+        xj = np.array([r, 0, 0])
+        F, _, _ = oracle.bond_ef(m, xi, np.zeros(3), xj, np.array([S * r, 0, 0]),
+                                 1e-6, 1e-6)
+        s = 4 * S * m.C * m.beta * math.exp(-m.beta * r * S * S) / m.inv_factor
+        assert F[0] == pytest.approx(s * 1e-6 * fac, rel=1e-12)
+
+
+# ---------------------------------------------------------------- model
+def test_newton_third_law(oracle):
+    """sum_i V_i f_i = 0: bond (i,j) and (j,i) carry opposite forces with the
+    same corrected volumes."""
+    p = pr.plate2d(30, 30, 1e-3, 3, 1e-7, 1, notch_len=0.0, pull=0.0)
+    o = oracle.OracleFD(p)
+    rng = np.random.default_rng(3)
+    u = np.zeros((p.n, 3))
+    u[:, :2] = rng.normal(scale=1e-6, size=(p.n, 2))
+    o.set_state(u=u)
+    o.compute_forces()
+    _, _, f = o.get_state()
+    tot = (p.vol[:, None] * f).sum(0)
+    scale = (p.vol[:, None] * np.abs(f)).sum()
+    assert np.all(np.abs(tot) <= 1e-13 * scale)
+
+
+def test_cd_free_node_kinematics(oracle):
+    """An isolated node under a constant force density f:
+    v_k = k dt f/rho, u_k = dt^2 f/rho k(k+1)/2 (integrateCD,
+    fDModel.cpp:377-443)."""
+    x = np.array([[0.0, 0.0, 0.0], [10.0, 0.0, 0.0]])
+    p = pr.Problem("free", 2, x, np.ones(2), 1e-3, 1e-3 / 3, 1e-4, 1,
+                   pr.rnp_material(),
+                   fbc=[pr.BC("rectangle", [1], "constant", [2.0],
+                              space_params=[3.0], x1=-1, x2=1, y1=-1, y2=1)])
+    o = oracle.OracleFD(p)
+    o.init()
+    K = 200
+    o.step(K)
+    u, v, f = o.get_state()
+    a = 6.0 / 1200.0
+    assert f[0, 0] == 6.0 and f[1, 0] == 0.0
+    assert v[0, 0] == pytest.approx(K * 1e-4 * a, rel=1e-12)
+    assert u[0, 0] == pytest.approx(1e-8 * a * K * (K + 1) / 2, rel=1e-12)
+    assert np.all(u[1] == 0) and np.all(v[1] == 0)
+
+
+def test_displacement_bc_time_functions(oracle):
+    p = pr.plate2d(12, 12, 1e-3, 3, 1e-7, 1, notch_len=0.0, pull=0.7)
+    o = oracle.OracleFD(p)
+    o.init()
+    o.step(7)
+    u, v, _ = o.get_state()
+    t, n = o.time()
+    assert n == 7
+    tacc = 0.0
+    for _ in range(7):
+        tacc += 1e-7  # time accumulates by repeated addition (fDModel.cpp:432)
+    assert t == tacc
+    fix = o.get_fixity()
+    top = x_top = p.x[:, 1] > (12 - 3.5) * 1e-3
+    bot = p.x[:, 1] < 2.5e-3
+    assert np.all(fix[top] == 2) and np.all(fix[bot] == 2)
+    assert np.all(u[top, 1] == 0.7 * tacc) and np.all(v[top, 1] == 0.7)
+    assert np.all(u[bot, 1] == -0.7 * tacc) and np.all(v[bot, 1] == -0.7)
+    assert not np.all(u[x_top, 0] == 0)  # x stays free on the layers
+
+
+def test_quadratic_and_sin_bc(oracle):
+    x, vol = pr.grid2d(3, 3, 1.0, jitter=0)
+    bcq = pr.BC("rectangle", [1], "quadratic", [2.0, 3.0], x1=-.1, x2=.1,
+                y1=-.1, y2=.1)
+    bcs = pr.BC("rectangle", [2], "sin", [2.0, 0.5], x1=1.9, x2=2.1, y1=-.1,
+                y2=.1)
+    p = pr.Problem("bcs", 2, x, vol, 1.5, 1.0, 0.25, 1, pr.rnp_material(),
+                   ubc=[bcq, bcs])
+    o = oracle.OracleFD(p)
+    o.init()
+    o.step(3)
+    u, v, _ = o.get_state()
+    t = 0.75
+    assert u[0, 0] == pytest.approx(2 * t + 3 * t * t, rel=1e-15)
+    assert v[0, 0] == pytest.approx(2 + 3 * t, rel=1e-15)
+    assert u[2, 1] == pytest.approx(2 * math.sin(0.5 * math.pi * t), rel=1e-15)
+    assert v[2, 1] == pytest.approx(2 * 0.5 * math.pi * math.cos(0.5 * math.pi * t),
+                                    rel=1e-15)
+
+
+def test_postproc_damage(oracle):
+    # undeformed plate: Z = 0; notched plate: phi > 0 exactly at notch nodes
+    p = pr.plate2d(20, 20, 1e-3, 3, 1e-7, 1, notch_len=0.006, pull=0.0)
+    o = oracle.OracleFD(p)
+    Z, phi, e = o.postproc()
+    assert np.all(Z == 0)
+    b = o.get_bonds()
+    has_broken = np.array([b[o.row_ptr[i]:o.row_ptr[i + 1]].any()
+                           for i in range(p.n)])
+    assert np.array_equal(phi > 0, has_broken)
+    # only broken bonds carry energy at u = 0: J C / f V_j per broken bond
+    assert np.array_equal(e > 0, has_broken)
+
+
+@pytest.mark.parametrize("name", ["oracle_plate24_pull05",
+                                  "oracle_plate24_pull30"])
+def test_oracle_trajectory_golden(oracle, name):
+    """The committed trajectories (tests/golden/make_fixtures.py) reproduce
+    bit for bit."""
+    g = np.load(GOLDEN / f"{name}.npz")
+    pull = 0.5 if name.endswith("05") else 30.0
+    steps = int(g["steps"])
+    p = pr.plate2d(24, 24, 1e-3, 3, 1e-7, steps, notch_len=0.008, pull=pull)
+    o = oracle.OracleFD(p)
+    assert np.array_equal(o.row_ptr, g["row_ptr"])
+    assert np.array_equal(o.col, g["col"])
+    o.init()
+    o.step(steps)
+    u, v, f = o.get_state()
+    assert np.array_equal(u, g["u"]) and np.array_equal(v, g["v"])
+    assert np.array_equal(f, g["f"])
+    assert np.array_equal(o.get_bonds(), g["bonds"])
+    if pull > 1:
+        assert g["bonds"].sum() > 0
+
+
+def test_dc_metric_known_answer(golden):
+    """tools/dc fd_simple (fdSimple.cpp:164-176): L2 = sqrt(sum |dv|^2),
+    sup = max |dv|; known answer examples/tools/dc/fd_simple/compare.txt."""
+    d = np.load(golden / "dc_fd_simple.npz")
+    dv = d["velocity_2"] - d["velocity_4"]
+    ln = np.sqrt((dv * dv).sum(1))
+    l2 = math.sqrt((ln * ln).sum())
+    sup = ln.max()
+    exp_l2, exp_sup = d["expected_l2_sup"]
+    assert float(f"{l2:.6g}") == pytest.approx(exp_l2, abs=5e-5)
+    assert float(f"{sup:.6g}") == pytest.approx(exp_sup, abs=5e-7)
