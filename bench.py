@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--modes", default="fp32,fp64")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the FP64-exact comparison run (profiling)")
     return ap.parse_args()
 
 
@@ -67,6 +69,7 @@ def run_mode(p, precision, steps, warmup):
     from nlmech_amd.engine import GpuFD
     g = GpuFD(p, precision=precision)
     g.init()
+    unroll, tune_us = g.autotune()
     if warmup > 0:
         g.step(warmup)
     t0 = time.perf_counter()
@@ -76,7 +79,7 @@ def run_mode(p, precision, steps, warmup):
     bonds = g.get_bonds()
     nnz = g.nnz
     return g, dict(ms_total=ms_total, ms_fused=ms_fused, wall=wall, u=u,
-                   bonds=bonds, nnz=nnz)
+                   bonds=bonds, nnz=nnz, unroll=unroll)
 
 
 def cpu_baseline(p, csr, seconds):
@@ -125,14 +128,16 @@ def main():
               f"fused step kernel {r['ms_fused']*1e3:.1f} us", file=sys.stderr,
               flush=True)
     # accuracy against the FP64 exact mode after the same steps
-    ge, re_ = run_mode(p, NLM_PREC_FP64_EXACT, a.steps, a.warmup)
-    ge.close()
-    print(f"[bench] exact: fused step kernel {re_['ms_fused']*1e3:.1f} us",
-          file=sys.stderr, flush=True)
-    uref = re_["u"]
-    bref = re_["bonds"]
     acc = {}
-    for name, r in res.items():
+    re_ = None
+    if not a.no_accuracy:
+        ge, re_ = run_mode(p, NLM_PREC_FP64_EXACT, a.steps, a.warmup)
+        ge.close()
+        print(f"[bench] exact: fused step kernel {re_['ms_fused']*1e3:.1f} us",
+              file=sys.stderr, flush=True)
+    for name, r in (res.items() if re_ is not None else []):
+        uref = re_["u"]
+        bref = re_["bonds"]
         d = r["u"] - uref
         acc[name] = {
             "l2_rel": float(np.linalg.norm(d) / np.linalg.norm(uref)),
@@ -169,14 +174,16 @@ def main():
                      "traffic": traffic},
         "modes": {},
         "accuracy_vs_fp64_exact": acc,
-        "exact_mode_ms_per_step": re_["ms_total"] / a.steps,
+        "exact_mode_ms_per_step": (re_["ms_total"] / a.steps
+                                   if re_ is not None else None),
     }
     for name, rr in res.items():
         v = nnz * a.steps / (rr["ms_total"] * 1e-3)
         ach = bytes_launch / (rr["ms_fused"] * 1e-3) / 1e9
         out["modes"][name] = {"value": v, "ms_per_step": rr["ms_total"] / a.steps,
                               "fused_kernel_us": rr["ms_fused"] * 1e3,
-                              "roofline_frac": ach / HBM_PEAK_GBS}
+                              "roofline_frac": ach / HBM_PEAK_GBS,
+                              "autotuned_unroll": rr["unroll"]}
     if "fp32" in res and "fp64" in res:
         out["fp32_over_fp64"] = (out["modes"]["fp32"]["value"] /
                                  out["modes"]["fp64"]["value"])

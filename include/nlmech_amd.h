@@ -240,6 +240,14 @@ int nlm_fd_postproc(nlm_ctx* ctx, float* Z, float* phi, double* energy);
  * and the mean duration of the fused step kernel. */
 int nlm_fd_timed_steps(nlm_ctx* ctx, int nsteps, double* ms_total,
                        double* ms_per_step);
+/* Autotuner (no counterpart in the reference): times the fused step kernel
+ * for each neighbor-slot chunk width (2, 4, 8 slots per wavefront iteration)
+ * on the current state without changing it, and keeps the fastest.  Returns
+ * the choice and its time per step in microseconds.  The FP64 exact mode
+ * always uses 4. */
+int nlm_fd_autotune(nlm_ctx* ctx, int* chosen, double* best_us);
+/* Sets the chunk width by hand (2, 4 or 8). */
+int nlm_fd_set_unroll(nlm_ctx* ctx, int unroll);
 /* Number of directed bonds of the configured model. */
 int nlm_fd_num_bonds(nlm_ctx* ctx, int64_t* nnz);
 /* Name of the fused step kernel launched for the configured model. */

@@ -122,6 +122,8 @@ SIGNATURES = {
     "nlm_fd_postproc": (C.c_int, [_P, _F, _F, _D]),
     "nlm_fd_timed_steps": (C.c_int, [_P, C.c_int, _D, _D]),
     "nlm_fd_num_bonds": (C.c_int, [_P, _I64]),
+    "nlm_fd_autotune": (C.c_int, [_P, C.POINTER(C.c_int), _D]),
+    "nlm_fd_set_unroll": (C.c_int, [_P, C.c_int]),
     "nlm_fd_kernel_name": (C.c_char_p, [_P]),
 }
 

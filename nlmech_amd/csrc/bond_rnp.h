@@ -9,8 +9,9 @@
 //  FP64   the stretch S, the reference length r and the break test as EXACT
 //         decides them; the force magnitude and direction from reciprocal
 //         square roots and fused multiply-adds.
-//  FP32F  displacement difference and stretch in FP64, bond force magnitude
-//         and direction in FP32, accumulated in FP64 by the caller.
+//  FP32F  position and displacement differences formed in FP64, then bond
+//         length, stretch, force magnitude and direction in FP32; forces
+//         accumulated in FP64 by the caller.
 //
 // Break decision.  The reference breaks a bond when
 //   definitelyGreaterThan(|S|, k * (rbar / sqrt(r)))   (rnpBond.cpp:263-266)
@@ -162,8 +163,46 @@ __device__ __forceinline__ double rsqrt_f64(double a) {
   return __builtin_fma(0.5 * y, e, y);
 }
 
-// FP64 fast variant.
+// Influence function value for the fast paths (INF: 0 constant, 1 linear,
+// 2 gaussian; r the reference bond length).
+template <int INF>
+__device__ __forceinline__ double influence_f64(const RnpDev& m, double r) {
+  if constexpr (INF == 0) return m.inf_a;
+  if constexpr (INF == 1) return __builtin_fma(m.inf_b, r * m.rcp_horizon, m.inf_a);
+  double q = r * m.rcp_horizon;
+  return m.inf_a * exp(-(q * q) * m.rcp_inf_b);
+}
+template <int INF>
+__device__ __forceinline__ float influence_f32(const RnpDev& m, float r) {
+  if constexpr (INF == 0) return m.inf_a_f;
+  if constexpr (INF == 1) return __builtin_fmaf(m.inf_b_f, r * m.rcp_horizon_f, m.inf_a_f);
+  float q = r * m.rcp_horizon_f;
+  return m.inf_a_f * __expf(-(q * q) * m.rcp_inf_b_f);
+}
+
+// Contact force of a broken bond (rnpBond.cpp:285-291), FP64.
 template <int DIM>
+__device__ __forceinline__ void contact_f64(const RnpDev& m, const double* xi,
+                                            const double* ui,
+                                            const double* xj,
+                                            const double* uj, double voli,
+                                            double volj, double* F) {
+  double y0 = (xj[0] + uj[0]) - (xi[0] + ui[0]);
+  double y1 = (xj[1] + uj[1]) - (xi[1] + ui[1]);
+  double y2 = DIM == 3 ? (xj[2] + uj[2]) - (xi[2] + ui[2]) : 0.0;
+  double R2 = __builtin_fma(y1, y1, y0 * y0);
+  if (DIM == 3) R2 = __builtin_fma(y2, y2, R2);
+  double Rinv = rsqrt_f64(R2);
+  double R = R2 * Rinv;
+  double sf = m.kn * (voli * volj / (voli + volj)) * (m.rc - R) * Rinv;
+  if (sf < 0.) sf = 0.;
+  F[0] = -sf * y0;
+  F[1] = -sf * y1;
+  if (DIM == 3) F[2] = -sf * y2;
+}
+
+// FP64 fast variant.
+template <int DIM, int INF>
 __device__ __forceinline__ void bond_fp64(const RnpDev& m, const double* xi,
                                           const double* ui, const double* xj,
                                           const double* uj, double voli,
@@ -182,24 +221,17 @@ __device__ __forceinline__ void bond_fp64(const RnpDev& m, const double* xi,
   double rinv = rsqrt_f64(dd);
   double r = dd * rinv;
   double S = num * (rinv * rinv);
-  // volume correction with guard band around the relative-epsilon boundary
-  double lo = m.check_low * (1.0 + 1e-5);
-  double band = r - lo;
-  if (fabs(band) <= 1e-9 * lo)
+  // volume correction, exact decision inside the guard band
+  double band = r - m.low_band;
+  if (fabs(band) <= 1e-9 * m.low_band)
     volj = corrected_volj_exact<DIM>(m, xi, xj, volj);
   else if (band > 0.)
     volj *= (m.check_up - r) * m.rcp_h;
-  double infl = m.inf_a;
-  if (m.inf_type == 1)
-    infl = __builtin_fma(m.inf_b, r * m.rcp_horizon, m.inf_a);
-  else if (m.inf_type == 2) {
-    double q = r * m.rcp_horizon;
-    infl = m.inf_a * exp(-q * q / m.inf_b);
-  }
+  double infl = influence_f64<INF>(m, r);
   F[0] = F[1] = F[2] = 0.;
   if (brk) {
     if (m.irreversible && !fs) {
-      double rho2 = (S * S) * r / m.krbar_sq;
+      double rho2 = (S * S) * r * m.rcp_krbar_sq;
       double d = rho2 - kBreakRho2;
       if (fabs(d) <= kGuard64 * kBreakRho2)
         fs = break_exact<DIM>(m, xi, ui, xj, uj);
@@ -208,38 +240,25 @@ __device__ __forceinline__ void bond_fp64(const RnpDev& m, const double* xi,
     }
     if (!fs) {
       double ex = exp(-m.beta * r * S * S);
-      double s = (infl * m.c4) * S * ex * volj;
-      double sr = s * rinv;
+      double sr = (infl * m.c4) * S * ex * volj * rinv;
       F[0] = sr * dx0;
       F[1] = sr * dx1;
       if (DIM == 3) F[2] = sr * dx2;
       return;
     }
-    if (m.contact) {
-      double y0 = (xj[0] + uj[0]) - (xi[0] + ui[0]);
-      double y1 = (xj[1] + uj[1]) - (xi[1] + ui[1]);
-      double y2 = DIM == 3 ? (xj[2] + uj[2]) - (xi[2] + ui[2]) : 0.0;
-      double R2 = __builtin_fma(y1, y1, y0 * y0);
-      if (DIM == 3) R2 = __builtin_fma(y2, y2, R2);
-      double Rinv = rsqrt_f64(R2);
-      double R = R2 * Rinv;
-      double sf = m.kn * (voli * volj / (voli + volj)) * (m.rc - R) * Rinv;
-      if (sf < 0.) sf = 0.;
-      F[0] = -sf * y0;
-      F[1] = -sf * y1;
-      if (DIM == 3) F[2] = -sf * y2;
-    }
+    if (m.contact) contact_f64<DIM>(m, xi, ui, xj, uj, voli, volj, F);
     return;
   }
-  double s = (infl * m.c4) * S * volj;
-  double sr = s * rinv;
+  double sr = (infl * m.c4) * S * volj * rinv;
   F[0] = sr * dx0;
   F[1] = sr * dx1;
   if (DIM == 3) F[2] = sr * dx2;
 }
 
-// FP32-force variant: FP64 differences, FP32 geometry and force.
-template <int DIM>
+// FP32-force variant: displacement and position differences in FP64 (no
+// cancellation: u_j - u_i is formed before rounding), then bond geometry,
+// stretch and force in FP32; the break decision is exact (guard band).
+template <int DIM, int INF>
 __device__ __forceinline__ void bond_fp32(const RnpDev& m, const double* xi,
                                           const double* ui, const double* xj,
                                           const double* uj, double voli,
@@ -258,24 +277,17 @@ __device__ __forceinline__ void bond_fp32(const RnpDev& m, const double* xi,
   float rinv = __builtin_amdgcn_rsqf(dd);
   float r = dd * rinv;
   float S = num * (rinv * rinv);
-  float lo = float(m.check_low * (1.0 + 1e-5));
-  float band = r - lo;
+  float band = r - m.low_band_f;
   float vj = float(volj);
-  if (fabsf(band) <= 1e-5f * lo)
+  if (fabsf(band) <= 1e-5f * m.low_band_f)
     vj = float(corrected_volj_exact<DIM>(m, xi, xj, volj));
   else if (band > 0.f)
-    vj *= (float(m.check_up) - r) * float(m.rcp_h);
-  float infl = float(m.inf_a);
-  if (m.inf_type == 1)
-    infl = float(m.inf_a) + float(m.inf_b) * (r * float(m.rcp_horizon));
-  else if (m.inf_type == 2) {
-    float q = r * float(m.rcp_horizon);
-    infl = float(m.inf_a) * __expf(-q * q / float(m.inf_b));
-  }
+    vj *= (m.check_up_f - r) * m.rcp_h_f;
+  float infl = influence_f32<INF>(m, r);
   F[0] = F[1] = F[2] = 0.;
   if (brk) {
     if (m.irreversible && !fs) {
-      float rho2 = (S * S) * r / float(m.krbar_sq);
+      float rho2 = (S * S) * r * m.rcp_krbar_sq_f;
       float d = rho2 - float(kBreakRho2);
       if (fabsf(d) <= kGuard32 * float(kBreakRho2))
         fs = break_exact<DIM>(m, xi, ui, xj, uj);
@@ -284,34 +296,107 @@ __device__ __forceinline__ void bond_fp32(const RnpDev& m, const double* xi,
     }
     if (!fs) {
       float ex = __expf(-m.beta_f * r * S * S);
-      float s = infl * m.c4f * S * ex * vj;
-      float sr = s * rinv;
+      float sr = (infl * m.c4f) * S * ex * vj * rinv;
       F[0] = double(sr * dx0);
       F[1] = double(sr * dx1);
       if (DIM == 3) F[2] = double(sr * dx2);
       return;
     }
-    if (m.contact) {
-      double y0 = (xj[0] + uj[0]) - (xi[0] + ui[0]);
-      double y1 = (xj[1] + uj[1]) - (xi[1] + ui[1]);
-      double y2 = DIM == 3 ? (xj[2] + uj[2]) - (xi[2] + ui[2]) : 0.0;
-      float R2 = float(y0 * y0 + y1 * y1 + y2 * y2);
-      float Rinv = __builtin_amdgcn_rsqf(R2);
-      float R = R2 * Rinv;
-      float sf = float(m.kn * (voli * volj / (voli + volj))) *
-                 (float(m.rc) - R) * Rinv;
-      if (sf < 0.f) sf = 0.f;
-      F[0] = -double(sf) * y0;
-      F[1] = -double(sf) * y1;
-      if (DIM == 3) F[2] = -double(sf) * y2;
-    }
+    if (m.contact) contact_f64<DIM>(m, xi, ui, xj, uj, voli, volj, F);
     return;
   }
-  float s = infl * m.c4f * S * vj;
-  float sr = s * rinv;
+  float sr = (infl * m.c4f) * S * vj * rinv;
   F[0] = double(sr * dx0);
   F[1] = double(sr * dx1);
   if (DIM == 3) F[2] = double(sr * dx2);
+}
+
+// Branch-free common path of the fast variants.  Computes the force of a
+// bond whose two boundary decisions (volume correction, break test) are
+// clear of their guard bands; `exact_needed` flags the rare bonds inside a
+// band, which the caller recomputes with bond_fp64 / bond_fp32 (those take
+// the exact decision).  Broken bonds contribute zero (contact forces are also
+// left to the caller's slow path).  brk: both ends breakable.
+template <int DIM, int INF>
+__device__ __forceinline__ void bond_fp64_bf(const RnpDev& m,
+                                             const double* xi,
+                                             const double* ui,
+                                             const double* xj,
+                                             const double* uj, double volj,
+                                             bool act, bool brk, bool fs0,
+                                             bool& fs, bool& exact_needed,
+                                             double* F) {
+  double dx0 = xj[0] - xi[0], dx1 = xj[1] - xi[1];
+  double dx2 = DIM == 3 ? xj[2] - xi[2] : 0.0;
+  double du0 = uj[0] - ui[0], du1 = uj[1] - ui[1];
+  double du2 = DIM == 3 ? uj[2] - ui[2] : 0.0;
+  double dd = __builtin_fma(dx1, dx1, dx0 * dx0);
+  double num = __builtin_fma(dx1, du1, dx0 * du0);
+  if (DIM == 3) {
+    dd = __builtin_fma(dx2, dx2, dd);
+    num = __builtin_fma(dx2, du2, num);
+  }
+  double rinv = rsqrt_f64(dd);
+  double r = dd * rinv;
+  double S = num * (rinv * rinv);
+  // (check_up - r)/h >= 1 exactly when r <= check_low, so the clamp gives
+  // the reference's factor away from the relative-epsilon edge
+  bool vex = fabs(r - m.low_band) <= 2e-9 * m.low_band;
+  double fac = fmin((m.check_up - r) * m.rcp_h, 1.0);
+  double d = (S * S) * r * m.rcp_krbar_sq - kBreakRho2;
+  bool bex = brk && m.irreversible && !fs0 &&
+             fabs(d) <= kGuard64 * kBreakRho2;
+  fs = fs0 || (brk && m.irreversible && d > 0.);
+  // contact forces of broken bonds are left to the slow path
+  exact_needed = vex || bex || (m.contact && brk && fs);
+  double infl = influence_f64<INF>(m, r);
+  // bonds touching a no-fail node use the linearised force whatever their
+  // bit (rnpBond.cpp:296-300)
+  double ex = brk ? exp(-m.beta * r * S * S) : 1.0;
+  double sr = (infl * m.c4) * S * ex * (volj * fac) * rinv;
+  // masked slots (neighbor = the node itself: dx = 0) and broken bonds add 0
+  sr = (act && !(brk && fs)) ? sr : 0.0;
+  F[0] = sr * dx0;
+  F[1] = sr * dx1;
+  F[2] = DIM == 3 ? sr * dx2 : 0.0;
+}
+
+template <int DIM, int INF>
+__device__ __forceinline__ void bond_fp32_bf(const RnpDev& m,
+                                             const double* xi,
+                                             const double* ui,
+                                             const double* xj,
+                                             const double* uj, double volj,
+                                             bool act, bool brk, bool fs0,
+                                             bool& fs, bool& exact_needed,
+                                             double* F) {
+  float dx0 = float(xj[0] - xi[0]), dx1 = float(xj[1] - xi[1]);
+  float dx2 = DIM == 3 ? float(xj[2] - xi[2]) : 0.f;
+  float du0 = float(uj[0] - ui[0]), du1 = float(uj[1] - ui[1]);
+  float du2 = DIM == 3 ? float(uj[2] - ui[2]) : 0.f;
+  float dd = __builtin_fmaf(dx1, dx1, dx0 * dx0);
+  float num = __builtin_fmaf(dx1, du1, dx0 * du0);
+  if (DIM == 3) {
+    dd = __builtin_fmaf(dx2, dx2, dd);
+    num = __builtin_fmaf(dx2, du2, num);
+  }
+  float rinv = __builtin_amdgcn_rsqf(dd);
+  float r = dd * rinv;
+  float S = num * (rinv * rinv);
+  bool vex = fabsf(r - m.low_band_f) <= 3e-5f * m.low_band_f;
+  float fac = fminf((m.check_up_f - r) * m.rcp_h_f, 1.0f);
+  float d = (S * S) * r * m.rcp_krbar_sq_f - float(kBreakRho2);
+  bool bex = brk && m.irreversible && !fs0 &&
+             fabsf(d) <= kGuard32 * float(kBreakRho2);
+  fs = fs0 || (brk && m.irreversible && d > 0.f);
+  exact_needed = vex || bex || (m.contact && brk && fs);
+  float infl = influence_f32<INF>(m, r);
+  float ex = brk ? __expf(-m.beta_f * r * S * S) : 1.0f;
+  float sr = (infl * m.c4f) * S * ex * (float(volj) * fac) * rinv;
+  sr = (act && !(brk && fs)) ? sr : 0.f;
+  F[0] = double(sr * dx0);
+  F[1] = double(sr * dx1);
+  F[2] = DIM == 3 ? double(sr * dx2) : 0.0;
 }
 
 }  // namespace nlm

@@ -80,7 +80,7 @@ __device__ __forceinline__ typename VT<DIM>::T make_v(const double* o) {
   return r;
 }
 
-template <int DIM, int MODE>
+template <int DIM, int MODE, int INF>
 __device__ __forceinline__ void bond_mode(const RnpDev& m, const double* xi,
                                           const double* ui, const double* xj,
                                           const double* uj, double voli,
@@ -89,9 +89,9 @@ __device__ __forceinline__ void bond_mode(const RnpDev& m, const double* xi,
   if constexpr (MODE == kModeExact)
     bond_exact<DIM, false>(m, xi, ui, xj, uj, voli, volj, brk, fs, F, nullptr);
   else if constexpr (MODE == kModeFp64)
-    bond_fp64<DIM>(m, xi, ui, xj, uj, voli, volj, brk, fs, F);
+    bond_fp64<DIM, INF>(m, xi, ui, xj, uj, voli, volj, brk, fs, F);
   else
-    bond_fp32<DIM>(m, xi, ui, xj, uj, voli, volj, brk, fs, F);
+    bond_fp32<DIM, INF>(m, xi, ui, xj, uj, voli, volj, brk, fs, F);
 }
 
 // External force density of node i at time t (FLoading::apply order: sets in
@@ -116,21 +116,33 @@ constexpr int KIND_FORCE = 0;
 constexpr int KIND_CD = 1;
 constexpr int KIND_VERLET = 2;
 
-template <int DIM, int MODE, int KIND, bool BASE_F>
+// Element idx of a device array addressed as base + 32-bit byte offset, so
+// the load uses the scalar-base + vector-offset form (no 64-bit address
+// arithmetic per lane).
+template <class T>
+__device__ __forceinline__ T ld32(const T* __restrict__ base, uint32_t idx) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
+                                     idx * uint32_t(sizeof(T)));
+}
+
+template <int DIM, int MODE, int KIND, bool BASE_F, int INF, int U,
+          bool NOFAIL>
 __global__ __launch_bounds__(256) void k_fd(FdArgs<DIM> a, double t_force,
                                             double t_next) {
-  using V = typename VT<DIM>::T;
   int64_t nblk = (a.nslices * 64 + 255) / 256;
   int64_t lb = xcd_block(nblk);
   if (lb >= nblk) return;
-  int64_t i = lb * 256 + threadIdx.x;
-  int lane = threadIdx.x & 63;
-  int64_t s = i >> 6;
+  // The slice index is the same for the 64 lanes of a wavefront; reading it
+  // from the first lane makes the slice bounds, the slot loop and the
+  // bond-bit addresses scalar.
+  const int lane = threadIdx.x & 63;
+  const int64_t s = lb * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (s >= a.nslices) return;  // whole wavefront
+  const int64_t i = s * 64 + lane;
   bool valid = i < a.n;
-  int64_t cb = a.sl_ptr[s];
-  int width = int((a.sl_ptr[s + 1] - cb) >> 6);
-  int64_t bb = cb >> 6;
+  const int64_t cb = a.sl_ptr[s];
+  const int width = int((a.sl_ptr[s + 1] - cb) >> 6);
+  const int64_t bb = cb >> 6;
   int len = valid ? int(a.rowlen[i]) : 0;
   double xi[3] = {0., 0., 0.}, ui[3] = {0., 0., 0.};
   double voli = 1.;
@@ -142,29 +154,88 @@ __global__ __launch_bounds__(256) void k_fd(FdArgs<DIM> a, double t_force,
     fl = a.flags[i];
   }
   bool int_i = !(fl & kFlagNonInterior);
+  // this slice's neighbor slots: slot k of this lane at colS[64 k + lane]
+  const int32_t* __restrict__ colS = a.col + cb;
+  const unsigned long long* __restrict__ bits_in = a.bits + bb;
+  const int wlast = width > 0 ? width - 1 : 0;
   double acc[3] = {0., 0., 0.};
-  for (int k = 0; k < width; ++k) {
-    unsigned long long w = a.bits[bb + k];
-    bool newly = false;
-    if (k < len) {
-      int32_t j = a.col[cb + int64_t(k) * 64 + lane];
-      double xj[3], uj[3];
-      load3<DIM>(a.x[j], xj);
-      load3<DIM>(a.u[j], uj);
-      double volj = a.vol[j];
-      bool brk = int_i;
-      if (a.has_nofail) brk = brk && !(a.flags[j] & kFlagNonInterior);
-      bool fs0 = (w >> lane) & 1ull;
+  // Neighbor indices are fetched one chunk of U slots ahead of the chunk
+  // whose positions and displacements are gathered, so the index latency
+  // overlaps the previous chunk's arithmetic.  Every load is unconditional:
+  // slots are clamped to the slice width (padding slots hold the node
+  // itself), and slots at or past the row length are masked out of the
+  // arithmetic.
+  int32_t jn[U];
+#pragma unroll
+  for (int q = 0; q < U; ++q)
+    jn[q] = ld32(colS, uint32_t(64 * (q < wlast ? q : wlast) + lane));
+  for (int k0 = 0; k0 < width; k0 += U) {
+    int32_t jc[U];
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      jc[q] = jn[q];
+      int kk = k0 + U + q;
+      jn[q] = ld32(colS, uint32_t(64 * (kk < wlast ? kk : wlast) + lane));
+    }
+    double xj[U][3], uj[U][3], vj[U];
+    unsigned long long w[U];
+    uint32_t flj[U];
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      uint32_t j = uint32_t(jc[q]);
+      load3<DIM>(ld32(a.x, j), xj[q]);
+      load3<DIM>(ld32(a.u, j), uj[q]);
+      vj[q] = ld32(a.vol, j);
+      if (NOFAIL) flj[q] = ld32(a.flags, j);
+      int kq = k0 + q;
+      w[q] = bits_in[kq < wlast ? kq : wlast];
+    }
+    // The arithmetic runs on every lane; slots past the row length (their
+    // neighbor is the node itself, giving NaN) are removed by selects, so no
+    // branch separates the gathers from their use.
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      const bool act = k0 + q < len;
+      bool brk = NOFAIL ? (int_i && !(flj[q] & kFlagNonInterior)) : true;
+      bool fs0 = (w[q] >> lane) & 1ull;
       bool fs = fs0;
       double F[3];
-      bond_mode<DIM, MODE>(a.m, xi, ui, xj, uj, voli, volj, brk, fs, F);
-      newly = fs && !fs0;
-      acc[0] += F[0];
-      acc[1] += F[1];
-      if (DIM == 3) acc[2] += F[2];
+      if constexpr (MODE == kModeExact) {
+        bond_mode<DIM, MODE, INF>(a.m, xi, ui, xj[q], uj[q], voli, vj[q], brk,
+                                  fs, F);
+      } else {
+        bool exn;
+        if constexpr (MODE == kModeFp64)
+          bond_fp64_bf<DIM, INF>(a.m, xi, ui, xj[q], uj[q], vj[q], act, brk,
+                                 fs0, fs, exn, F);
+        else
+          bond_fp32_bf<DIM, INF>(a.m, xi, ui, xj[q], uj[q], vj[q], act, brk,
+                                 fs0, fs, exn, F);
+        // rare: a decision inside its guard band, or a contact force
+        exn = exn && act;
+        if (__ballot(exn) != 0ull) {
+          if (exn) {
+            fs = fs0;
+            bond_mode<DIM, MODE, INF>(a.m, xi, ui, xj[q], uj[q], voli, vj[q],
+                                      brk, fs, F);
+          }
+        }
+      }
+      bool newly = act && fs && !fs0;
+      if constexpr (MODE == kModeExact) {
+        acc[0] += act ? F[0] : 0.0;
+        acc[1] += act ? F[1] : 0.0;
+        if (DIM == 3) acc[2] += act ? F[2] : 0.0;
+      } else {  // F is already 0 on masked slots
+        acc[0] += F[0];
+        acc[1] += F[1];
+        if (DIM == 3) acc[2] += F[2];
+      }
+      if (k0 + q < width) {
+        unsigned long long nb = __ballot(newly);
+        if (nb != 0ull && lane == 0) a.bits[bb + k0 + q] = w[q] | nb;
+      }
     }
-    unsigned long long nb = __ballot(newly);
-    if (nb != 0ull && lane == 0) a.bits[bb + k] = w | nb;
   }
   if (!valid) return;
   double f[3] = {0., 0., 0.};
@@ -511,54 +582,71 @@ int64_t force_blocks(const FdState& s) {
   return (s.sell.nslices * 64 + 255) / 256;
 }
 
-template <int DIM, int MODE>
-void launch_force(nlm_ctx* c, bool base_f, double t) {
-  FdArgs<DIM> a = make_args<DIM>(c);
+// Launch helpers.  The neighbor-slot chunk U of the fused step kernel is a
+// tuning knob (nlm_fd_autotune); force-only launches use U = 4.
+template <int DIM, int MODE, int KIND, bool BASE_F, int INF, int U>
+void launch_k(nlm_ctx* c, const FdArgs<DIM>& a, double tf, double tn) {
   unsigned g = grid_xcd(force_blocks(c->fd) * 256, 256);
-  if (base_f)
-    hipLaunchKernelGGL((k_fd<DIM, MODE, KIND_FORCE, true>), dim3(g), dim3(256),
-                       0, c->stream, a, t, t);
+  if (a.has_nofail)
+    hipLaunchKernelGGL((k_fd<DIM, MODE, KIND, BASE_F, INF, U, true>), dim3(g),
+                       dim3(256), 0, c->stream, a, tf, tn);
   else
-    hipLaunchKernelGGL((k_fd<DIM, MODE, KIND_FORCE, false>), dim3(g),
-                       dim3(256), 0, c->stream, a, t, t);
+    hipLaunchKernelGGL((k_fd<DIM, MODE, KIND, BASE_F, INF, U, false>),
+                       dim3(g), dim3(256), 0, c->stream, a, tf, tn);
 }
 
-template <int DIM, int MODE>
-void launch_fused_cd(nlm_ctx* c, double t_force, double t_next) {
-  FdArgs<DIM> a = make_args<DIM>(c);
-  unsigned g = grid_xcd(force_blocks(c->fd) * 256, 256);
-  hipLaunchKernelGGL((k_fd<DIM, MODE, KIND_CD, false>), dim3(g), dim3(256), 0,
-                     c->stream, a, t_force, t_next);
+template <int DIM, int MODE, int KIND, bool BASE_F>
+void launch_inf(nlm_ctx* c, const FdArgs<DIM>& a, double tf, double tn,
+                int U) {
+  const int inf = c->fd.rnp.inf_type;
+  if constexpr (KIND == KIND_CD && MODE != kModeExact) {
+    if (U == 2) {
+      if (inf == 0) launch_k<DIM, MODE, KIND, BASE_F, 0, 2>(c, a, tf, tn);
+      else if (inf == 1) launch_k<DIM, MODE, KIND, BASE_F, 1, 2>(c, a, tf, tn);
+      else launch_k<DIM, MODE, KIND, BASE_F, 2, 2>(c, a, tf, tn);
+      return;
+    }
+    if (U == 8) {
+      if (inf == 0) launch_k<DIM, MODE, KIND, BASE_F, 0, 8>(c, a, tf, tn);
+      else if (inf == 1) launch_k<DIM, MODE, KIND, BASE_F, 1, 8>(c, a, tf, tn);
+      else launch_k<DIM, MODE, KIND, BASE_F, 2, 8>(c, a, tf, tn);
+      return;
+    }
+  }
+  if (inf == 0) launch_k<DIM, MODE, KIND, BASE_F, 0, 4>(c, a, tf, tn);
+  else if (inf == 1) launch_k<DIM, MODE, KIND, BASE_F, 1, 4>(c, a, tf, tn);
+  else launch_k<DIM, MODE, KIND, BASE_F, 2, 4>(c, a, tf, tn);
+}
+
+template <int DIM, int KIND, bool BASE_F>
+void launch_mode(nlm_ctx* c, const FdArgs<DIM>& a, double tf, double tn,
+                 int U) {
+  switch (c->fd.precision) {
+    case kModeExact:
+      launch_inf<DIM, kModeExact, KIND, BASE_F>(c, a, tf, tn, U);
+      break;
+    case kModeFp64:
+      launch_inf<DIM, kModeFp64, KIND, BASE_F>(c, a, tf, tn, U);
+      break;
+    default:
+      launch_inf<DIM, kModeFp32, KIND, BASE_F>(c, a, tf, tn, U);
+      break;
+  }
 }
 
 template <int DIM>
 void launch_force_dispatch(nlm_ctx* c, bool base_f, double t) {
-  switch (c->fd.precision) {
-    case kModeExact:
-      launch_force<DIM, kModeExact>(c, base_f, t);
-      break;
-    case kModeFp64:
-      launch_force<DIM, kModeFp64>(c, base_f, t);
-      break;
-    default:
-      launch_force<DIM, kModeFp32>(c, base_f, t);
-      break;
-  }
+  FdArgs<DIM> a = make_args<DIM>(c);
+  if (base_f)
+    launch_mode<DIM, KIND_FORCE, true>(c, a, t, t, 4);
+  else
+    launch_mode<DIM, KIND_FORCE, false>(c, a, t, t, 4);
 }
 
 template <int DIM>
 void launch_fused_dispatch(nlm_ctx* c, double tf, double tn) {
-  switch (c->fd.precision) {
-    case kModeExact:
-      launch_fused_cd<DIM, kModeExact>(c, tf, tn);
-      break;
-    case kModeFp64:
-      launch_fused_cd<DIM, kModeFp64>(c, tf, tn);
-      break;
-    default:
-      launch_fused_cd<DIM, kModeFp32>(c, tf, tn);
-      break;
-  }
+  FdArgs<DIM> a = make_args<DIM>(c);
+  launch_mode<DIM, KIND_CD, false>(c, a, tf, tn, c->fd.unroll);
 }
 
 unsigned node_blocks(const FdState& s) { return unsigned((s.n + 255) / 256); }
@@ -843,9 +931,21 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
   m.rcp_h = 1.0 / m.h;
   m.rcp_horizon = 1.0 / m.horizon;
   m.krbar_sq = (m.factor_sc * m.rbar) * (m.factor_sc * m.rbar);
+  m.rcp_krbar_sq = 1.0 / m.krbar_sq;
   m.c4 = 4.0 * m.C * m.beta * m.rcp_inv_factor;
+  m.low_band = m.check_low * (1.0 + 1e-5);
+  m.rcp_inf_b = m.inf_type == 2 ? 1.0 / m.inf_b : 0.0;
   m.c4f = float(m.c4);
   m.beta_f = float(m.beta);
+  m.check_up_f = float(m.check_up);
+  m.low_band_f = float(m.low_band);
+  m.rcp_h_f = float(m.rcp_h);
+  m.rcp_horizon_f = float(m.rcp_horizon);
+  m.inf_a_f = float(m.inf_a);
+  m.inf_b_f = float(m.inf_b);
+  m.rcp_inf_b_f = float(m.rcp_inf_b);
+  m.rcp_krbar_sq_f = float(m.rcp_krbar_sq);
+  s.unroll = 4;
   if (!s.ev0) NLM_HIP(c, hipEventCreate(&s.ev0));
   if (!s.ev1) NLM_HIP(c, hipEventCreate(&s.ev1));
   s.ready = true;
@@ -1295,6 +1395,76 @@ extern "C" int nlm_fd_postproc(nlm_ctx* c, float* Z, float* phi,
     NLM_HIP(c, hipMemcpyAsync(energy, de.p, sizeof(double) * s.n,
                               hipMemcpyDeviceToHost, c->stream));
   NLM_HIP(c, hipStreamSynchronize(c->stream));
+  return NLM_OK;
+}
+
+namespace {
+// Times the fused step kernel for each neighbor-slot chunk width on the
+// current state.  The trials write the spare u/v buffers (overwritten by the
+// next step anyway) and a scratch copy of the bond bits, so the model state
+// is unchanged.
+template <int DIM>
+int autotune(nlm_ctx* c, int* chosen, double* best_us) {
+  FdState& s = c->fd;
+  DevBuf scratch;
+  int rc;
+  if ((rc = alloc(c, scratch, s.sell.bits.bytes))) return rc;
+  NLM_HIP(c, hipMemcpyAsync(scratch.p, s.sell.bits.p, s.sell.bits.bytes,
+                            hipMemcpyDeviceToDevice, c->stream));
+  FdArgs<DIM> a = make_args<DIM>(c);
+  a.bits = scratch.as<unsigned long long>();
+  const int cands[3] = {2, 4, 8};
+  double best = 1e300;
+  int pick = 4;
+  hipEvent_t e0, e1;
+  NLM_HIP(c, hipEventCreate(&e0));
+  NLM_HIP(c, hipEventCreate(&e1));
+  const int reps = 6;
+  for (int U : cands) {
+    for (int r = 0; r < 2; r++)
+      launch_mode<DIM, KIND_CD, false>(c, a, s.time, s.time + s.dt, U);
+    NLM_HIP(c, hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; r++)
+      launch_mode<DIM, KIND_CD, false>(c, a, s.time, s.time + s.dt, U);
+    NLM_HIP(c, hipEventRecord(e1, c->stream));
+    NLM_HIP(c, hipEventSynchronize(e1));
+    float ms = 0.f;
+    NLM_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+    double us = 1e3 * double(ms) / reps;
+    if (us < best) {
+      best = us;
+      pick = U;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  NLM_HIP(c, hipGetLastError());
+  s.unroll = pick;
+  if (chosen) *chosen = pick;
+  if (best_us) *best_us = best;
+  return NLM_OK;
+}
+}  // namespace
+
+extern "C" int nlm_fd_autotune(nlm_ctx* c, int* chosen, double* best_us) {
+  int rc = check_ready(c, "nlm_fd_autotune");
+  if (rc) return rc;
+  if (c->fd.precision == NLM_PREC_FP64_EXACT) {
+    c->fd.unroll = 4;
+    if (chosen) *chosen = 4;
+    if (best_us) *best_us = 0.;
+    return NLM_OK;
+  }
+  return c->fd.dim == 2 ? autotune<2>(c, chosen, best_us)
+                        : autotune<3>(c, chosen, best_us);
+}
+
+extern "C" int nlm_fd_set_unroll(nlm_ctx* c, int unroll) {
+  int rc = check_ready(c, "nlm_fd_set_unroll");
+  if (rc) return rc;
+  if (unroll != 2 && unroll != 4 && unroll != 8)
+    return fail(c, NLM_ERR_INVALID, "nlm_fd_set_unroll: 2, 4 or 8");
+  c->fd.unroll = unroll;
   return NLM_OK;
 }
 

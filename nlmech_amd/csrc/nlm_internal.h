@@ -72,9 +72,12 @@ struct RnpDev {
   double rcp_h;            // 1 / h
   double rcp_horizon;      // 1 / horizon
   double krbar_sq;         // (factor_sc * rbar)^2
+  double rcp_krbar_sq;     // 1 / krbar_sq
   double c4;               // 4 * C * beta / inv_factor
-  float c4f;               // c4 in FP32
-  float beta_f;            // beta in FP32
+  double low_band;         // check_low * (1 + 1e-5): volume-correction edge
+  double rcp_inf_b;        // 1 / inf_b (gaussian influence)
+  float c4f, beta_f, check_up_f, low_band_f, rcp_h_f, rcp_horizon_f;
+  float inf_a_f, inf_b_f, rcp_inf_b_f, rcp_krbar_sq_f;
 };
 
 // Slice layout of the neighbor lists: 64 consecutive nodes (one wavefront)
@@ -109,6 +112,7 @@ struct FdState {
   DevBuf x, vol, flags;
   DevBuf u[2], v[2], f;
   int cur = 0;  // which of u[]/v[] holds the current state
+  int unroll = 4;  // neighbor slots per chunk in the fused step kernel
   DevBuf row_ptr, col;  // CSR copy (int64, int32)
   SellDev sell;
   DevBuf bctab;  // BcTable

@@ -175,6 +175,15 @@ class GpuFD:
     def compute_forces(self):
         self._call("nlm_fd_compute_forces")
 
+    def autotune(self):
+        u = C.c_int()
+        us = C.c_double()
+        self._call("nlm_fd_autotune", C.byref(u), C.byref(us))
+        return u.value, us.value
+
+    def set_unroll(self, unroll: int):
+        self._call("nlm_fd_set_unroll", int(unroll))
+
     def timed_steps(self, nsteps: int):
         tot = C.c_double()
         per = C.c_double()

@@ -291,6 +291,30 @@ def test_force_bc_nofail_contact_timed_crack(engine, oracle):
     assert o.get_bonds().sum() > 0
 
 
+@pytest.mark.parametrize("mode", [NLM_PREC_FP64, NLM_PREC_FP32_FORCE])
+def test_autotune_and_unroll_do_not_change_results(engine, mode):
+    """The chunk width changes the schedule only: every width gives the same
+    bits and bitwise the same displacements, and autotuning leaves the state
+    untouched."""
+    progress(f"[gpu] autotune invariance mode={mode}")
+    p = pr.plate2d(48, 40, 1e-3, 3, 1e-7, 1, pull=30.0)
+    outs = []
+    for unroll in (None, 2, 4, 8):
+        with engine.GpuFD(p, precision=mode) as g:
+            g.init()
+            if unroll is None:
+                chosen, us = g.autotune()
+                assert chosen in (2, 4, 8) and us > 0
+            else:
+                g.set_unroll(unroll)
+            g.step(30)
+            u, v, f = g.get_state()
+            outs.append((u, v, f, g.get_bonds()))
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert np.array_equal(a, b)
+
+
 # ------------------------------------------------------------ full size
 def test_plate2d_1e6_properties(engine):
     """1000x1000 plate at horizon 4h (no oracle run at this size):
