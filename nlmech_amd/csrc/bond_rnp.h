@@ -383,17 +383,20 @@ __device__ __forceinline__ void bond_fp32_bf(const RnpDev& m,
   float rinv = __builtin_amdgcn_rsqf(dd);
   float r = dd * rinv;
   float S = num * (rinv * rinv);
+  float rS2 = r * (S * S);        // beta r S^2 / beta, shared by the
+  float rho2 = rS2 * m.rcp_krbar_sq_f;  // exponent and the break ratio
   bool vex = fabsf(r - m.low_band_f) <= 3e-5f * m.low_band_f;
   float fac = fminf((m.check_up_f - r) * m.rcp_h_f, 1.0f);
-  float d = (S * S) * r * m.rcp_krbar_sq_f - float(kBreakRho2);
-  bool bex = brk && m.irreversible && !fs0 &&
-             fabsf(d) <= kGuard32 * float(kBreakRho2);
-  fs = fs0 || (brk && m.irreversible && d > 0.f);
+  const bool brk_irr = brk && m.irreversible;
+  bool bex = brk_irr && !fs0 &&
+             fabsf(rho2 - float(kBreakRho2)) <= kGuard32 * float(kBreakRho2);
+  fs = fs0 || (brk_irr && rho2 > float(kBreakRho2));
   exact_needed = vex || bex || (m.contact && brk && fs);
   float infl = influence_f32<INF>(m, r);
-  float ex = brk ? __expf(-m.beta_f * r * S * S) : 1.0f;
+  float ex = brk ? __builtin_amdgcn_exp2f(rS2 * m.nbeta_log2e_f) : 1.0f;
   float sr = (infl * m.c4f) * S * ex * (float(volj) * fac) * rinv;
-  sr = (act && !(brk && fs)) ? sr : 0.f;
+  const bool keep = act & !(brk & fs);
+  sr = keep ? sr : 0.f;
   F[0] = double(sr * dx0);
   F[1] = double(sr * dx1);
   F[2] = DIM == 3 ? double(sr * dx2) : 0.0;

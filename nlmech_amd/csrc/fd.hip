@@ -28,6 +28,7 @@
 
 #include "bond_rnp.h"
 #include "device_common.h"
+#include "layout.h"
 #include "nlm_internal.h"
 
 namespace nlm {
@@ -52,6 +53,10 @@ struct FdArgs {
   const V* __restrict__ f_in;
   V* __restrict__ f_out;
   const BcTable* __restrict__ bc;
+  const int64_t* __restrict__ halo_ptr;
+  const int32_t* __restrict__ halo;
+  const uint16_t* __restrict__ colL;
+  int64_t halo_cap;  // LDS entries per halo array
   RnpDev m;
   double dt;
   double fact;
@@ -125,13 +130,45 @@ __device__ __forceinline__ T ld32(const T* __restrict__ base, uint32_t idx) {
                                      idx * uint32_t(sizeof(T)));
 }
 
+// One halo node staged in LDS by the tiled step kernel (2D): reference
+// position, displacement, volume and flags in one 48-byte record, so a
+// neighbor costs one address computation and three LDS reads.
+struct alignas(16) HaloRec {
+  double2 x;
+  double2 u;
+  double vol;
+  uint32_t flags;
+  uint32_t pad;
+};
+
 template <int DIM, int MODE, int KIND, bool BASE_F, int INF, int U,
-          bool NOFAIL>
-__global__ __launch_bounds__(256) void k_fd(FdArgs<DIM> a, double t_force,
-                                            double t_next) {
+          bool NOFAIL, bool TILED>
+__global__ __launch_bounds__(256) void k_fd(
+    FdArgs<DIM> a, const unsigned long long* __restrict__ bits_ro,
+    double t_force, double t_next) {
   int64_t nblk = (a.nslices * 64 + 255) / 256;
   int64_t lb = xcd_block(nblk);
   if (lb >= nblk) return;
+  // TILED: the workgroup's 256 nodes are one block of the layout; its halo
+  // (every node its rows reference) is staged in LDS before the bond loop
+  // and neighbors are read through halo-local indices.
+  static_assert(!TILED || DIM == 2, "the LDS-staged kernel is 2D");
+  extern __shared__ HaloRec smem_halo[];
+  if constexpr (TILED) {
+    const int64_t hb = a.halo_ptr[lb];
+    const int hn = int(a.halo_ptr[lb + 1] - hb);
+    for (int t = threadIdx.x; t < hn; t += 256) {
+      const uint32_t g = uint32_t(a.halo[hb + t]);
+      HaloRec r;
+      r.x = ld32(a.x, g);
+      r.u = ld32(a.u, g);
+      r.vol = ld32(a.vol, g);
+      r.flags = NOFAIL ? ld32(a.flags, g) : 0u;
+      r.pad = 0u;
+      smem_halo[t] = r;
+    }
+    __syncthreads();
+  }
   // The slice index is the same for the 64 lanes of a wavefront; reading it
   // from the first lane makes the slice bounds, the slot loop and the
   // bond-bit addresses scalar.
@@ -154,9 +191,12 @@ __global__ __launch_bounds__(256) void k_fd(FdArgs<DIM> a, double t_force,
     fl = a.flags[i];
   }
   bool int_i = !(fl & kFlagNonInterior);
-  // this slice's neighbor slots: slot k of this lane at colS[64 k + lane]
+  // this slice's neighbor slots: slot k of this lane at entry 64 k + lane
   const int32_t* __restrict__ colS = a.col + cb;
-  const unsigned long long* __restrict__ bits_in = a.bits + bb;
+  const uint16_t* __restrict__ colL = a.colL + cb;
+  // bond-bit words of this slice, read through the scalar path (each word is
+  // read once, before this wavefront may rewrite it)
+  const unsigned long long* __restrict__ bits_in = bits_ro + bb;
   const int wlast = width > 0 ? width - 1 : 0;
   double acc[3] = {0., 0., 0.};
   // Neighbor indices are fetched one chunk of U slots ahead of the chunk
@@ -165,28 +205,45 @@ __global__ __launch_bounds__(256) void k_fd(FdArgs<DIM> a, double t_force,
   // slots are clamped to the slice width (padding slots hold the node
   // itself), and slots at or past the row length are masked out of the
   // arithmetic.
-  int32_t jn[U];
+  auto slot_index = [&](int k) -> uint32_t {
+    const uint32_t e = uint32_t(64 * (k < wlast ? k : wlast) + lane);
+    if constexpr (TILED) return uint32_t(ld32(colL, e));
+    else return uint32_t(ld32(colS, e));
+  };
+  uint32_t jn[U];
 #pragma unroll
-  for (int q = 0; q < U; ++q)
-    jn[q] = ld32(colS, uint32_t(64 * (q < wlast ? q : wlast) + lane));
+  for (int q = 0; q < U; ++q) jn[q] = slot_index(q);
   for (int k0 = 0; k0 < width; k0 += U) {
-    int32_t jc[U];
+    uint32_t jc[U];
 #pragma unroll
     for (int q = 0; q < U; ++q) {
       jc[q] = jn[q];
-      int kk = k0 + U + q;
-      jn[q] = ld32(colS, uint32_t(64 * (kk < wlast ? kk : wlast) + lane));
+      jn[q] = slot_index(k0 + U + q);
     }
     double xj[U][3], uj[U][3], vj[U];
     unsigned long long w[U];
     uint32_t flj[U];
 #pragma unroll
     for (int q = 0; q < U; ++q) {
-      uint32_t j = uint32_t(jc[q]);
-      load3<DIM>(ld32(a.x, j), xj[q]);
-      load3<DIM>(ld32(a.u, j), uj[q]);
-      vj[q] = ld32(a.vol, j);
-      if (NOFAIL) flj[q] = ld32(a.flags, j);
+      const uint32_t j = jc[q];
+      if constexpr (TILED) {
+        const HaloRec& r = *reinterpret_cast<const HaloRec*>(
+            reinterpret_cast<const char*>(smem_halo) +
+            j * uint32_t(sizeof(HaloRec)));
+        xj[q][0] = r.x.x;
+        xj[q][1] = r.x.y;
+        xj[q][2] = 0.;
+        uj[q][0] = r.u.x;
+        uj[q][1] = r.u.y;
+        uj[q][2] = 0.;
+        vj[q] = r.vol;
+        if (NOFAIL) flj[q] = r.flags;
+      } else {
+        load3<DIM>(ld32(a.x, j), xj[q]);
+        load3<DIM>(ld32(a.u, j), uj[q]);
+        vj[q] = ld32(a.vol, j);
+        if (NOFAIL) flj[q] = ld32(a.flags, j);
+      }
       int kq = k0 + q;
       w[q] = bits_in[kq < wlast ? kq : wlast];
     }
@@ -197,7 +254,8 @@ __global__ __launch_bounds__(256) void k_fd(FdArgs<DIM> a, double t_force,
     for (int q = 0; q < U; ++q) {
       const bool act = k0 + q < len;
       bool brk = NOFAIL ? (int_i && !(flj[q] & kFlagNonInterior)) : true;
-      bool fs0 = (w[q] >> lane) & 1ull;
+      // w[q] is wavefront-uniform: its bits are the lanes' broken flags
+      bool fs0 = __builtin_amdgcn_inverse_ballot_w64(w[q]);
       bool fs = fs0;
       double F[3];
       if constexpr (MODE == kModeExact) {
@@ -568,6 +626,10 @@ FdArgs<DIM> make_args(nlm_ctx* c) {
   a.f_in = s.f.as<V>();
   a.f_out = s.f.as<V>();
   a.bc = s.bctab.as<BcTable>();
+  a.halo_ptr = s.halo_ptr.as<int64_t>();
+  a.halo = s.halo.as<int32_t>();
+  a.colL = s.colL.as<uint16_t>();
+  a.halo_cap = (s.halo_max + 1) & ~int64_t(1);
   a.m = s.rnp;
   a.dt = s.dt;
   a.fact = s.scheme == NLM_TIME_CENTRAL_DIFFERENCE
@@ -584,15 +646,32 @@ int64_t force_blocks(const FdState& s) {
 
 // Launch helpers.  The neighbor-slot chunk U of the fused step kernel is a
 // tuning knob (nlm_fd_autotune); force-only launches use U = 4.
+template <int DIM, int MODE, int KIND, bool BASE_F, int INF, int U,
+          bool NOFAIL>
+void launch_t(nlm_ctx* c, const FdArgs<DIM>& a, double tf, double tn) {
+  unsigned g = grid_xcd(force_blocks(c->fd) * 256, 256);
+  const unsigned long long* bits_ro = a.bits;
+  if constexpr (DIM == 2) {
+    if (c->fd.tiled && c->fd.use_tiled) {
+      size_t lds = size_t(a.halo_cap) * sizeof(HaloRec);
+      hipLaunchKernelGGL((k_fd<DIM, MODE, KIND, BASE_F, INF, U, NOFAIL, true>),
+                         dim3(g), dim3(256), lds, c->stream, a, bits_ro, tf,
+                         tn);
+      return;
+    }
+  }
+  {
+    hipLaunchKernelGGL((k_fd<DIM, MODE, KIND, BASE_F, INF, U, NOFAIL, false>),
+                       dim3(g), dim3(256), 0, c->stream, a, bits_ro, tf, tn);
+  }
+}
+
 template <int DIM, int MODE, int KIND, bool BASE_F, int INF, int U>
 void launch_k(nlm_ctx* c, const FdArgs<DIM>& a, double tf, double tn) {
-  unsigned g = grid_xcd(force_blocks(c->fd) * 256, 256);
   if (a.has_nofail)
-    hipLaunchKernelGGL((k_fd<DIM, MODE, KIND, BASE_F, INF, U, true>), dim3(g),
-                       dim3(256), 0, c->stream, a, tf, tn);
+    launch_t<DIM, MODE, KIND, BASE_F, INF, U, true>(c, a, tf, tn);
   else
-    hipLaunchKernelGGL((k_fd<DIM, MODE, KIND, BASE_F, INF, U, false>),
-                       dim3(g), dim3(256), 0, c->stream, a, tf, tn);
+    launch_t<DIM, MODE, KIND, BASE_F, INF, U, false>(c, a, tf, tn);
 }
 
 template <int DIM, int MODE, int KIND, bool BASE_F>
@@ -838,30 +917,61 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
     if (h_row[i + 1] - h_row[i] > 65535 || h_row[i + 1] < h_row[i])
       return fail(c, NLM_ERR_INVALID, "nlm_fd_setup: bad row_ptr");
   s.row_ptr_host.assign(h_row, h_row + n + 1);
-  if ((rc = alloc(c, s.row_ptr, sizeof(int64_t) * (n + 1)))) return rc;
-  if ((rc = alloc(c, s.col, sizeof(int32_t) * (nnz + 1)))) return rc;
-  NLM_HIP(c, hipMemcpyAsync(s.row_ptr.p, h_row, sizeof(int64_t) * (n + 1),
-                            hipMemcpyHostToDevice, st));
-  if (p->col) {
+  std::vector<int32_t> col_tmp;
+  const int32_t* h_col = p->col;
+  if (h_col) {
     for (int64_t k = 0; k < nnz; k++)
-      if (p->col[k] < 0 || p->col[k] >= n)
+      if (h_col[k] < 0 || h_col[k] >= n)
         return fail(c, NLM_ERR_INVALID, "nlm_fd_setup: column out of range");
-    NLM_HIP(c, hipMemcpyAsync(s.col.p, p->col, sizeof(int32_t) * nnz,
-                              hipMemcpyHostToDevice, st));
   } else {
     if (!c->nbr.ready || c->nbr.nnz != nnz)
       return fail(c, NLM_ERR_STATE, "nlm_fd_setup: neighbor build mismatch");
-    NLM_HIP(c, hipMemcpyAsync(s.col.p, c->nbr.col.p, sizeof(int32_t) * nnz,
-                              hipMemcpyDeviceToDevice, st));
+    col_tmp.resize(nnz + 1);
+    NLM_HIP(c, hipMemcpy(col_tmp.data(), c->nbr.col.p, sizeof(int32_t) * nnz,
+                         hipMemcpyDeviceToHost));
+    h_col = col_tmp.data();
   }
-  if ((rc = build_sell(c, n, s.row_ptr.as<int64_t>(), s.col.as<int32_t>(),
-                       nnz, s.sell)))
+  // internal order, slices and halos (layout.cpp)
+  HostLayout L;
+  build_layout(dim, n, p->x, p->mat.h, h_row, h_col, L);
+  s.perm_host = L.perm;
+  s.inv_host = L.inv;
+  s.sell.nslices = L.nslices;
+  s.sell.nentries = L.sl_ptr[L.nslices];
+  const int64_t nent = s.sell.nentries;
+  auto up = [&](DevBuf& d, const void* src, size_t bytes) -> int {
+    int r = alloc(c, d, bytes + 256);
+    if (r) return r;
+    NLM_HIP(c, hipMemsetAsync(d.p, 0, d.bytes, st));
+    if (src && bytes)
+      NLM_HIP(c, hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, st));
+    return NLM_OK;
+  };
+  if ((rc = up(s.row_ptr, L.row_ptr.data(), sizeof(int64_t) * (n + 1)))) return rc;
+  if ((rc = up(s.col, L.col.data(), sizeof(int32_t) * nnz))) return rc;
+  if ((rc = up(s.ref_off, L.ref_off.data(), sizeof(int64_t) * n))) return rc;
+  if ((rc = up(s.sell.sl_ptr, L.sl_ptr.data(), sizeof(int64_t) * (L.nslices + 1))))
     return rc;
-  // node arrays
+  if ((rc = up(s.sell.col, L.colS.data(), sizeof(int32_t) * nent))) return rc;
+  if ((rc = up(s.sell.rowlen, L.rowlen.data(), sizeof(uint16_t) * n))) return rc;
+  if ((rc = up(s.sell.bits, nullptr, sizeof(uint64_t) * (nent / 64 + 1)))) return rc;
+  if ((rc = up(s.halo_ptr, L.halo_ptr.data(), sizeof(int64_t) * (L.nblocks + 1))))
+    return rc;
+  if ((rc = up(s.halo, L.halo.data(), sizeof(int32_t) * L.halo.size()))) return rc;
+  if ((rc = up(s.colL, L.colL.data(), sizeof(uint16_t) * nent))) return rc;
+  s.halo_max = L.halo_max;
+  // LDS budget of the staged kernel: one 48-byte record per halo node
+  s.tiled = dim == 2 && L.halo_max > 0 &&
+            (L.halo_max + 1) * 48 <= 64 * 1024;
+  // node arrays in internal order
   const int stride = dim == 2 ? 2 : 4;
   std::vector<double> hx(size_t(n) * stride, 0.);
-  for (int64_t i = 0; i < n; i++)
-    for (int d = 0; d < dim; d++) hx[i * stride + d] = p->x[3 * i + d];
+  std::vector<double> hv(n);
+  for (int64_t a = 0; a < n; a++) {
+    int64_t i = L.perm[a];
+    for (int d = 0; d < dim; d++) hx[a * stride + d] = p->x[3 * i + d];
+    hv[a] = p->vol[i];
+  }
   size_t vbytes = sizeof(double) * stride * size_t(n);
   if ((rc = alloc(c, s.x, vbytes))) return rc;
   if ((rc = alloc(c, s.vol, sizeof(double) * n))) return rc;
@@ -876,14 +986,14 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
   NLM_HIP(c, hipMemsetAsync(s.f.p, 0, vbytes, st));
   NLM_HIP(c, hipMemcpyAsync(s.x.p, hx.data(), vbytes, hipMemcpyHostToDevice,
                             st));
-  NLM_HIP(c, hipMemcpyAsync(s.vol.p, p->vol, sizeof(double) * n,
+  NLM_HIP(c, hipMemcpyAsync(s.vol.p, hv.data(), sizeof(double) * n,
                             hipMemcpyHostToDevice, st));
   std::vector<uint32_t> fl(n, 0u);
   s.has_nofail = false;
   if (p->interior)
-    for (int64_t i = 0; i < n; i++)
-      if (!p->interior[i]) {
-        fl[i] |= kFlagNonInterior;
+    for (int64_t a = 0; a < n; a++)
+      if (!p->interior[L.perm[a]]) {
+        fl[a] |= kFlagNonInterior;
         s.has_nofail = true;
       }
   NLM_HIP(c, hipMemcpyAsync(s.flags.p, fl.data(), sizeof(uint32_t) * n,
@@ -945,6 +1055,7 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
   m.inf_b_f = float(m.inf_b);
   m.rcp_inf_b_f = float(m.rcp_inf_b);
   m.rcp_krbar_sq_f = float(m.rcp_krbar_sq);
+  m.nbeta_log2e_f = float(-m.beta * 1.4426950408889634);
   s.unroll = 4;
   if (!s.ev0) NLM_HIP(c, hipEventCreate(&s.ev0));
   if (!s.ev1) NLM_HIP(c, hipEventCreate(&s.ev1));
@@ -1023,7 +1134,7 @@ extern "C" int nlm_fd_set_ubc(nlm_ctx* c, int nsets, const nlm_bc* sets) {
       for (int d = 0; d < b.ndir; d++) {
         int dof = b.dir[d] - 1;
         fl[i] = (fl[i] & ~(0xfu << (4 * dof))) | (uint32_t(k + 1) << (4 * dof));
-        s.fix_host[i] |= uint8_t(1u << dof);
+        s.fix_host[s.perm_host[i]] |= uint8_t(1u << dof);
       }
     }
   }
@@ -1139,12 +1250,15 @@ extern "C" int nlm_fd_add_cracks(nlm_ctx* c, int ncracks,
 }
 
 namespace {
+// AoS reference-order host vectors <-> internal-order device vectors
 int to_dev_vec(nlm_ctx* c, const double* src, DevBuf& dst) {
   FdState& s = c->fd;
   const int stride = s.dim == 2 ? 2 : 4;
   std::vector<double> tmp(size_t(s.n) * stride, 0.);
-  for (int64_t i = 0; i < s.n; i++)
-    for (int d = 0; d < s.dim; d++) tmp[i * stride + d] = src[3 * i + d];
+  for (int64_t a = 0; a < s.n; a++) {
+    const int64_t i = s.perm_host[a];
+    for (int d = 0; d < s.dim; d++) tmp[a * stride + d] = src[3 * i + d];
+  }
   NLM_HIP(c, hipMemcpy(dst.p, tmp.data(), sizeof(double) * tmp.size(),
                        hipMemcpyHostToDevice));
   return NLM_OK;
@@ -1155,9 +1269,11 @@ int from_dev_vec(nlm_ctx* c, const DevBuf& src, double* dst) {
   std::vector<double> tmp(size_t(s.n) * stride, 0.);
   NLM_HIP(c, hipMemcpy(tmp.data(), src.p, sizeof(double) * tmp.size(),
                        hipMemcpyDeviceToHost));
-  for (int64_t i = 0; i < s.n; i++)
+  for (int64_t a = 0; a < s.n; a++) {
+    const int64_t i = s.perm_host[a];
     for (int d = 0; d < 3; d++)
-      dst[3 * i + d] = d < s.dim ? tmp[i * stride + d] : 0.;
+      dst[3 * i + d] = d < s.dim ? tmp[a * stride + d] : 0.;
+  }
   return NLM_OK;
 }
 }  // namespace
@@ -1195,12 +1311,12 @@ extern "C" int nlm_fd_get_bonds(nlm_ctx* c, uint8_t* out) {
   if (s.dim == 2) {
     FdArgs<2> a = make_args<2>(c);
     hipLaunchKernelGGL(k_bits_to_bytes<2>, dim3(node_blocks(s)), dim3(256), 0,
-                       c->stream, a, s.row_ptr.as<int64_t>(),
+                       c->stream, a, s.ref_off.as<int64_t>(),
                        tmp.as<uint8_t>());
   } else {
     FdArgs<3> a = make_args<3>(c);
     hipLaunchKernelGGL(k_bits_to_bytes<3>, dim3(node_blocks(s)), dim3(256), 0,
-                       c->stream, a, s.row_ptr.as<int64_t>(),
+                       c->stream, a, s.ref_off.as<int64_t>(),
                        tmp.as<uint8_t>());
   }
   NLM_HIP(c, hipGetLastError());
@@ -1222,12 +1338,12 @@ extern "C" int nlm_fd_set_bonds(nlm_ctx* c, const uint8_t* in) {
   if (s.dim == 2) {
     FdArgs<2> a = make_args<2>(c);
     hipLaunchKernelGGL(k_bytes_to_bits<2>, dim3(unsigned(s.sell.nslices)),
-                       dim3(64), 0, c->stream, a, s.row_ptr.as<int64_t>(),
+                       dim3(64), 0, c->stream, a, s.ref_off.as<int64_t>(),
                        tmp.as<uint8_t>());
   } else {
     FdArgs<3> a = make_args<3>(c);
     hipLaunchKernelGGL(k_bytes_to_bits<3>, dim3(unsigned(s.sell.nslices)),
-                       dim3(64), 0, c->stream, a, s.row_ptr.as<int64_t>(),
+                       dim3(64), 0, c->stream, a, s.ref_off.as<int64_t>(),
                        tmp.as<uint8_t>());
   }
   NLM_HIP(c, hipGetLastError());
@@ -1385,16 +1501,21 @@ extern "C" int nlm_fd_postproc(nlm_ctx* c, float* Z, float* phi,
                        dz.as<float>(), dp.as<float>(), de.as<double>());
   }
   NLM_HIP(c, hipGetLastError());
-  if (Z)
-    NLM_HIP(c, hipMemcpyAsync(Z, dz.p, sizeof(float) * s.n,
-                              hipMemcpyDeviceToHost, c->stream));
-  if (phi)
-    NLM_HIP(c, hipMemcpyAsync(phi, dp.p, sizeof(float) * s.n,
-                              hipMemcpyDeviceToHost, c->stream));
-  if (energy)
-    NLM_HIP(c, hipMemcpyAsync(energy, de.p, sizeof(double) * s.n,
-                              hipMemcpyDeviceToHost, c->stream));
+  std::vector<float> hz(s.n), hp(s.n);
+  std::vector<double> he(s.n);
+  NLM_HIP(c, hipMemcpyAsync(hz.data(), dz.p, sizeof(float) * s.n,
+                            hipMemcpyDeviceToHost, c->stream));
+  NLM_HIP(c, hipMemcpyAsync(hp.data(), dp.p, sizeof(float) * s.n,
+                            hipMemcpyDeviceToHost, c->stream));
+  NLM_HIP(c, hipMemcpyAsync(he.data(), de.p, sizeof(double) * s.n,
+                            hipMemcpyDeviceToHost, c->stream));
   NLM_HIP(c, hipStreamSynchronize(c->stream));
+  for (int64_t a = 0; a < s.n; a++) {
+    const int64_t i = s.perm_host[a];
+    if (Z) Z[i] = hz[a];
+    if (phi) phi[i] = hp[a];
+    if (energy) energy[i] = he[a];
+  }
   return NLM_OK;
 }
 

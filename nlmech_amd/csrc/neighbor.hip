@@ -1,5 +1,4 @@
-// neighbor.hip -- GPU cell-list neighbor search and the slice (SELL-64)
-// layout of the neighbor lists.
+// neighbor.hip -- GPU cell-list neighbor search.
 //
 // Replaces geometry::Neighbor::Neighbor (src/geometry/neighbor.cpp:24-92).
 // The reference tests all N^2 pairs; here the candidates of node i come from
@@ -176,83 +175,7 @@ __global__ __launch_bounds__(64) void k_nbr_rows(
   for (int t = lane; t < count; t += 64) col[base + t] = row[t];
 }
 
-// ---- slice layout -------------------------------------------------------
-
-__global__ void k_rowlen_u16(int64_t n, const int64_t* __restrict__ row_ptr,
-                             uint16_t* __restrict__ rowlen) {
-  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  rowlen[i] = uint16_t(row_ptr[i + 1] - row_ptr[i]);
-}
-
-// One wavefront per slice: width = longest row among its 64 nodes; stored as
-// 64*width entries in wcount[s].
-__global__ __launch_bounds__(64) void k_slice_width(
-    int64_t n, int64_t nslices, const int64_t* __restrict__ row_ptr,
-    int32_t* __restrict__ wcount) {
-  int64_t s = blockIdx.x;
-  if (s >= nslices) return;
-  int64_t i = s * 64 + threadIdx.x;
-  int len = i < n ? int(row_ptr[i + 1] - row_ptr[i]) : 0;
-  for (int off = 32; off > 0; off >>= 1) {
-    int o = __shfl_xor(len, off);
-    len = len > o ? len : o;
-  }
-  if (threadIdx.x == 0) wcount[s] = len * 64;
-}
-
-__global__ __launch_bounds__(64) void k_slice_fill(
-    int64_t n, int64_t nslices, const int64_t* __restrict__ row_ptr,
-    const int32_t* __restrict__ col, const int64_t* __restrict__ sl_ptr,
-    int32_t* __restrict__ colS) {
-  int64_t s = blockIdx.x;
-  if (s >= nslices) return;
-  int lane = threadIdx.x;
-  int64_t i = s * 64 + lane;
-  int64_t base = sl_ptr[s];
-  int width = int((sl_ptr[s + 1] - base) >> 6);
-  int64_t rb = i < n ? row_ptr[i] : 0;
-  int len = i < n ? int(row_ptr[i + 1] - rb) : 0;
-  int32_t self = i < n ? int32_t(i) : 0;
-  for (int k = 0; k < width; k++)
-    colS[base + int64_t(k) * 64 + lane] = k < len ? col[rb + k] : self;
-}
-
 }  // namespace
-
-int build_sell(nlm_ctx* c, int64_t n, const int64_t* d_row_ptr,
-               const int32_t* d_col, int64_t nnz, SellDev& out) {
-  (void)nnz;
-  int64_t nslices = (n + 63) / 64;
-  out.nslices = nslices;
-  int rc;
-  if ((rc = alloc(c, out.sl_ptr, sizeof(int64_t) * (nslices + 1)))) return rc;
-  if ((rc = alloc(c, out.rowlen, sizeof(uint16_t) * (n + 1)))) return rc;
-  DevBuf wcount;
-  if ((rc = alloc(c, wcount, sizeof(int32_t) * (nslices + 1)))) return rc;
-  hipStream_t st = c->stream;
-  hipLaunchKernelGGL(k_rowlen_u16, dim3(unsigned((n + 255) / 256)), dim3(256),
-                     0, st, n, d_row_ptr, out.rowlen.as<uint16_t>());
-  hipLaunchKernelGGL(k_slice_width, dim3(unsigned(nslices)), dim3(64), 0, st,
-                     n, nslices, d_row_ptr, wcount.as<int32_t>());
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, nslices,
-                     wcount.as<int32_t>(), out.sl_ptr.as<int64_t>());
-  NLM_HIP(c, hipGetLastError());
-  int64_t total = 0;
-  NLM_HIP(c, hipMemcpyAsync(&total, out.sl_ptr.as<int64_t>() + nslices,
-                            sizeof(int64_t), hipMemcpyDeviceToHost, st));
-  NLM_HIP(c, hipStreamSynchronize(st));
-  out.nentries = total;
-  if ((rc = alloc(c, out.col, sizeof(int32_t) * (total + 64)))) return rc;
-  if ((rc = alloc(c, out.bits, sizeof(uint64_t) * (total / 64 + 1)))) return rc;
-  NLM_HIP(c, hipMemsetAsync(out.bits.p, 0, out.bits.bytes, st));
-  hipLaunchKernelGGL(k_slice_fill, dim3(unsigned(nslices)), dim3(64), 0, st,
-                     n, nslices, d_row_ptr, d_col, out.sl_ptr.as<int64_t>(),
-                     out.col.as<int32_t>());
-  NLM_HIP(c, hipGetLastError());
-  NLM_HIP(c, hipStreamSynchronize(st));
-  return NLM_OK;
-}
 
 }  // namespace nlm
 

@@ -78,6 +78,7 @@ struct RnpDev {
   double rcp_inf_b;        // 1 / inf_b (gaussian influence)
   float c4f, beta_f, check_up_f, low_band_f, rcp_h_f, rcp_horizon_f;
   float inf_a_f, inf_b_f, rcp_inf_b_f, rcp_krbar_sq_f;
+  float nbeta_log2e_f;     // -beta * log2(e): exp(-beta t) = exp2(t * this)
 };
 
 // Slice layout of the neighbor lists: 64 consecutive nodes (one wavefront)
@@ -113,8 +114,17 @@ struct FdState {
   DevBuf u[2], v[2], f;
   int cur = 0;  // which of u[]/v[] holds the current state
   int unroll = 4;  // neighbor slots per chunk in the fused step kernel
-  DevBuf row_ptr, col;  // CSR copy (int64, int32)
+  DevBuf row_ptr, col;  // CSR in internal ids (int64, int32)
   SellDev sell;
+  // internal node order (see layout.cpp): perm[a] = reference id of a
+  std::vector<int32_t> perm_host, inv_host;
+  DevBuf ref_off;   // int64 per internal node: row start in reference CSR
+  DevBuf halo_ptr;  // int64 per 256-node block
+  DevBuf halo;      // int32 internal ids
+  DevBuf colL;      // uint16 halo-local slice entries
+  int64_t halo_max = 0;
+  bool tiled = false;   // LDS-staged step kernel usable
+  bool use_tiled = true;
   DevBuf bctab;  // BcTable
   BcTable bc_host{};
   std::vector<uint8_t> fix_host;
@@ -178,8 +188,5 @@ inline unsigned grid_xcd(int64_t items, int block) {
   return unsigned((g + 7) / 8 * 8);
 }
 
-// Builds the slice layout from a device CSR; defined in neighbor.hip.
-int build_sell(nlm_ctx* c, int64_t n, const int64_t* d_row_ptr,
-               const int32_t* d_col, int64_t nnz, SellDev& out);
 
 }  // namespace nlm
