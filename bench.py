@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--modes", default="fp32,fp64")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the FP64-exact comparison run (profiling)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 256 MiB device copy (counter scale)")
     return ap.parse_args()
 
 
@@ -103,8 +105,25 @@ def cpu_baseline(p, csr, seconds):
                       f"restatement, {dt:.1f} s"}
 
 
+def copy_probe(nbytes: int = 1 << 28) -> None:
+    """A plain device-to-device copy of nbytes, the reference point that
+    scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    src.fill_(1.0)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes}))
+
+
 def main():
     a = parse()
+    if a.copy_probe:
+        copy_probe()
+        return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not "
               "supported (the sharding design is described in DESIGN.md)",
