@@ -70,6 +70,7 @@ class Problem:
     cracks: list = field(default_factory=list)
     interior: np.ndarray | None = None
     shape: tuple = ()
+    v0: np.ndarray | None = None   # (n, 3) initial velocity, None: zero
 
     @property
     def n(self) -> int:
@@ -155,7 +156,88 @@ def plate2d(nx: int, ny: int, h: float, ratio: int, dt: float, nsteps: int,
                    ubc=ubc, cracks=cracks, shape=(nx, ny))
 
 
+def es_material(dim=3, K=2.16e9, G=1.30e9, E=-1.0, density=1200.0,
+                plane_strain=False, sc=1e-3) -> dict:
+    """ElasticState deck (Material: Type: ElasticState, K, G; E in 1D) plus
+    the critical stretch sc of the explicit model's bond breaking (sc <= 0:
+    no breaking)."""
+    return dict(model="elastic_state", dim=dim, K=K, G=G, E=E,
+                density=density, plane_strain=int(plane_strain), sc=sc)
+
+
+def gaussian(counter: np.ndarray, seed: int) -> np.ndarray:
+    """Standard normal variates from two SplitMix64 streams (Box-Muller)."""
+    u1 = uniform01(counter, seed ^ 0x5DEECE66D)
+    u2 = uniform01(counter, seed ^ 0x2545F4914F6CDD1D)
+    return np.sqrt(-2.0 * np.log1p(-u1)) * np.cos(2.0 * np.pi * u2)
+
+
+def grid3d(nx: int, ny: int, nz: int, h: float, jitter: float = 1e-6,
+           seed: int = 7):
+    """Node (i, j, k) at (i h, j h, k h), id (k ny + j) nx + i, volume h^3
+    halved on each boundary face it lies on (the 3D analogue of the mesh
+    tool's 2D rule, fe2D.cpp:148-158)."""
+    kk, jj, ii = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx),
+                             indexing="ij")
+    ii, jj, kk = ii.ravel(), jj.ravel(), kk.ravel()
+    n = nx * ny * nz
+    x = np.empty((n, 3))
+    x[:, 0] = ii.astype(np.float64) * h
+    x[:, 1] = jj.astype(np.float64) * h
+    x[:, 2] = kk.astype(np.float64) * h
+    if jitter:
+        r = uniform01(np.arange(3 * n, dtype=np.uint64), seed).reshape(n, 3)
+        x += (2.0 * r - 1.0) * (jitter * h)
+    vol = np.full(n, h * h * h)
+    vol[(ii == 0) | (ii == nx - 1)] *= 0.5
+    vol[(jj == 0) | (jj == ny - 1)] *= 0.5
+    vol[(kk == 0) | (kk == nz - 1)] *= 0.5
+    return x, vol
+
+
+def cube3d(nx: int, h: float, ratio: int, dt: float, nsteps: int,
+           pull: float = 1.0, layers: int = 3, sigma_v: float = 1e-3,
+           jitter: float = 1e-6, seed: int = 7, name: str | None = None,
+           ny: int | None = None, nz: int | None = None, **mat) -> Problem:
+    """Block of linear state-based material (ElasticState) with its z faces
+    pulled apart at +/- pull m/s (layers node planes each, dof 3, linear in
+    time) and a seeded Gaussian initial velocity of standard deviation
+    sigma_v in every component."""
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    x, vol = grid3d(nx, ny, nz, h, jitter, seed)
+    n = x.shape[0]
+    xmax, ymax, zmax = (nx - 1) * h, (ny - 1) * h, (nz - 1) * h
+    ubc = []
+    if layers > 0 and pull != 0.0:
+        box = dict(x1=-h, x2=xmax + h, y1=-h, y2=ymax + h)
+        ubc.append(BC("cuboid", [3], "linear", [pull],
+                      z1=zmax - (layers - 1) * h - 0.5 * h, z2=zmax + 0.5 * h,
+                      **box))
+        ubc.append(BC("cuboid", [3], "linear", [-pull], z1=-0.5 * h,
+                      z2=(layers - 1) * h + 0.5 * h, **box))
+    v0 = None
+    if sigma_v:
+        v0 = sigma_v * gaussian(np.arange(3 * n, dtype=np.uint64),
+                                seed + 1).reshape(n, 3)
+    m = es_material(dim=3, **mat)
+    return Problem(name=name or f"cube3d_{nx}x{ny}x{nz}", dim=3, x=x, vol=vol,
+                   horizon=ratio * h, h=h, dt=dt, nsteps=nsteps, material=m,
+                   ubc=ubc, shape=(nx, ny, nz), v0=v0)
+
+
+# 3D ElasticState block: side in nodes, mesh size, horizon/h, step, steps.
+CUBE_3D = {
+    # 160^3 nodes, h = 6.25e-4 m, horizon 3h, 500 steps
+    "cube3d_4e6": dict(nx=160, h=6.25e-4, ratio=3, dt=2e-8, nsteps=500),
+}
+
+
 def workload(key: str, **over) -> Problem:
+    if key in CUBE_3D:
+        spec = dict(CUBE_3D[key])
+        spec.update(over)
+        return cube3d(name=key, **spec)
     spec = dict(PLATE_2D[key])
     spec.update(over)
     return plate2d(name=key, **spec)

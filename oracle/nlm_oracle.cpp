@@ -497,6 +497,13 @@ struct FD {
         else if (bc.region == ORA_REGION_LINE && dgt(xi.x, bc.x1) &&
                  dlt(xi.x, bc.x2))
           fixed = true;
+        // ULoading has no cuboid region (uLoading.cpp:27-31); the 3D
+        // specimens select their face layers with the FLoading cuboid test
+        // (utilGeom.cpp:97-115) as an extension.
+        else if (bc.region == ORA_REGION_CUBOID &&
+                 inside_cuboid3(xi, P3{bc.x1, bc.y1, bc.z1},
+                                P3{bc.x2, bc.y2, bc.z2}))
+          fixed = true;
         if (fixed) {
           for (int k = 0; k < bc.ndir; k++)
             fix[i] |= uint8_t(1U << (bc.dir[k] - 1));
@@ -865,9 +872,299 @@ struct FD {
   }
 };
 
+// ---------------------------------------------------------------------------
+// Explicit linear state-based model (ElasticState) -- the reference runs this
+// material only in QuasiStaticModel; the explicit model here combines its
+// force (scatter form of QuasiStaticModel::computeForces) with integrateCD.
+// Broken bonds (stretch definitelyGreaterThan sc, irreversible) are left out
+// of the dilatation and the force; the reference ElasticState has no damage.
+// ---------------------------------------------------------------------------
+struct ES {
+  int dim = 3;
+  int64_t n = 0;
+  std::vector<P3> x;
+  std::vector<double> vol;
+  std::vector<int64_t> row_ptr;
+  std::vector<int32_t> col;
+  ora_es m{};
+  double dt = 0.;
+  std::vector<ora_bc> ubc, fbc;
+  std::vector<std::vector<int64_t>> ubc_nodes, fbc_nodes;
+  std::vector<uint8_t> fix;
+  std::vector<uint8_t> broken;          // per bond in CSR order
+  std::vector<double> corr;             // volume correction per bond
+  std::vector<double> mw;               // weighted volume per node
+  std::vector<double> theta;            // dilatation per node
+  std::vector<double> ext;              // extension per bond
+  std::vector<P3> u, v, f;
+  double time = 0.;
+  FD* loader = nullptr;                 // BC selection/application helper
+
+  // VolumeCorrection::correctVolume / weightedVolume, volumeCorrection.cpp
+  void init_volume() {
+    corr.assign(col.size(), 1.);
+    mw.assign(n, 1.);
+    for (int64_t i = 0; i < n; i++) {
+      for (int64_t k = row_ptr[i]; k < row_ptr[i + 1]; k++) {
+        P3 X = x[col[k]] - x[i];
+        double r = m.h * 0.5;
+        if (dgt(length(X), m.horizon - r))
+          corr[k] = (m.horizon + r - length(X)) / m.h;
+      }
+      double tmp = 0;
+      for (int64_t k = row_ptr[i]; k < row_ptr[i + 1]; k++) {
+        P3 X = x[col[k]] - x[i];
+        tmp += (length(X) * length(X) * corr[k] * vol[col[k]]);
+      }
+      mw[i] = tmp;
+    }
+  }
+
+  // StateBasedHelperFunctions::dilatation, stateBasedHelperFunctions.cpp:
+  // 26-83, with the break test of the explicit model first
+  void dilatation() {
+    theta.assign(n, 0.);
+    ext.assign(col.size(), 0.);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+      double w = 1;
+      for (int64_t k = row_ptr[i]; k < row_ptr[i + 1]; k++) {
+        int64_t j = col[k];
+        P3 Y = (x[j] + u[j]) - (x[i] + u[i]);
+        P3 X = x[j] - x[i];
+        ext[k] = length(Y) - length(X);
+        if (m.sc > 0. && !broken[k] && dgt(ext[k] / length(X), m.sc))
+          broken[k] = 1;
+        if (broken[k]) continue;
+        switch (dim) {
+          case 1:
+            theta[i] += (1. / mw[i]) * w * length(X) * ext[k] * corr[k] *
+                        vol[j];
+            break;
+          case 2:
+            theta[i] += (2. / mw[i]) * m.factor2D * w * length(X) * ext[k] *
+                        corr[k] * vol[j];
+            break;
+          default:
+            theta[i] += (3. / mw[i]) * w * length(X) * ext[k] * corr[k] *
+                        vol[j];
+            break;
+        }
+      }
+    }
+  }
+
+  // ElasticState::getBondEF, ElasticState.cpp:160-271 (force part)
+  P3 bond_force(int64_t i, int64_t k) {
+    int64_t j = col[k];
+    double w = 1;
+    double t = 0., alpha_s = 0., alpha_d = 0., e_s = 0., e_d = 0., alpha = 0.;
+    double t_s = 0., t_d = 0.;
+    P3 Y = (x[j] + u[j]) - (x[i] + u[i]);
+    P3 X = x[j] - x[i];
+    P3 M = Y / length(Y);
+    switch (dim) {
+      case 1:
+        alpha = m.E / mw[i];
+        t = alpha * w * ext[k];
+        break;
+      case 2:
+        if (!m.plane_strain)
+          alpha_s = (9. / mw[i]) *
+                    (m.K + std::pow((m.nu + 1.) / (2. * m.nu - 1.), 2) * m.mu /
+                               9.);
+        else
+          alpha_s = (9. / mw[i]) * (m.K + m.mu / 9.);
+        alpha_d = (8. / mw[i]) * m.mu;
+        e_s = theta[i] * length(X) / 3.;
+        e_d = ext[k] - e_s;
+        t_s = (2. * m.factor2D * alpha_s - (3. - 2. * m.factor2D) * alpha_d) *
+              w * e_s / 3.;
+        t_d = alpha_d * w * e_d;
+        t = t_s + t_d;
+        break;
+      default:
+        alpha_s = (9. / mw[i]) * m.K;
+        alpha_d = (15. / mw[i]) * m.mu;
+        e_s = theta[i] * length(X) / 3.;
+        t_s = alpha_s * w * e_s;
+        t_d = alpha_d * w * e_d;  // e_d stays 0 in 3D (reference)
+        t = t_s + t_d;
+        break;
+    }
+    P3 r = t * M;
+    return corr[k] * r;
+  }
+
+  // QuasiStaticModel::computeForces (QuasiStaticModel.cpp:169-239), run
+  // serially: f_i += T_ij V_j, f_j -= T_ij V_i.  Adds into ff.
+  void forces(std::vector<P3>& ff) {
+    dilatation();
+    for (int64_t i = 0; i < n; i++) {
+      P3 force_i{};
+      for (int64_t k = row_ptr[i]; k < row_ptr[i + 1]; k++) {
+        if (broken[k]) continue;
+        int64_t j = col[k];
+        P3 T = bond_force(i, k);
+        force_i = force_i + vol[j] * T;
+        ff[j] = ff[j] - vol[i] * T;
+      }
+      ff[i] = ff[i] + force_i;
+    }
+  }
+
+  void apply_ubc(double t) {
+    loader->u = u;
+    loader->v = v;
+    loader->apply_ubc(t);
+    u = loader->u;
+    v = loader->v;
+  }
+  void apply_fbc(double t) { loader->apply_fbc(t, f); }
+
+  void init() {
+    apply_ubc(time);
+    apply_fbc(time);
+    forces(f);
+  }
+
+  void step_cd() {
+    double delta_t = dt;
+    double fact = delta_t * delta_t / m.density;
+    for (int64_t i = 0; i < n; i++) {
+      double* ui = &u[i].x;
+      double* vi = &v[i].x;
+      double* fi = &f[i].x;
+      for (int d = 0; d < dim; d++) {
+        if (!(fix[i] >> d & 1UL)) {
+          double u_old = ui[d];
+          ui[d] += fact * fi[d] + delta_t * vi[d];
+          vi[d] = (ui[d] - u_old) / delta_t;
+        }
+      }
+      f[i] = P3{};
+    }
+    time += dt;
+    apply_ubc(time);
+    apply_fbc(time);
+    forces(f);
+  }
+};
+
 }  // namespace
 
 extern "C" {
+
+int ora_es_params(const ora_es_deck* d, ora_es* o) {
+  std::memset(o, 0, sizeof(*o));
+  o->dim = d->dim;
+  o->horizon = d->horizon;
+  o->h = d->h;
+  o->density = d->density;
+  o->E = d->E;
+  o->K = d->K;
+  o->G = d->G;
+  o->plane_strain = d->plane_strain;
+  o->sc = d->sc;
+  if (d->dim == 1) {
+    if (dlt(d->E, 0.)) return -1;
+  } else {
+    if (d->K <= 0. || d->G <= 0.) return -2;
+  }
+  // inp::MatData::toNuClassical, materialDeck.h:120
+  o->nu = (3 * d->K - 2 * d->G) / (2 * (3 * d->K + d->G));
+  o->mu = d->G;
+  if (d->plane_strain == 0 && d->dim == 2)
+    o->factor2D = (2. * o->nu - 1.) / (o->nu - 1.);
+  else
+    o->factor2D = 1;
+  return 0;
+}
+
+void* ora_es_create(int dim, int64_t n, const double* x, const double* vol,
+                    const int64_t* row_ptr, const int32_t* col,
+                    const ora_es* mat, double dt, int n_ubc, const ora_bc* ubc,
+                    int n_fbc, const ora_bc* fbc) {
+  ES* s = new ES;
+  s->dim = dim;
+  s->n = n;
+  s->x.resize(n);
+  for (int64_t i = 0; i < n; i++) s->x[i] = p3(x + 3 * i);
+  s->vol.assign(vol, vol + n);
+  s->row_ptr.assign(row_ptr, row_ptr + n + 1);
+  s->col.assign(col, col + row_ptr[n]);
+  s->m = *mat;
+  s->dt = dt;
+  s->broken.assign(s->col.size(), 0);
+  s->u.assign(n, P3{});
+  s->v.assign(n, P3{});
+  s->f.assign(n, P3{});
+  // BC node selection through the FD loader (same ULoading/FLoading code)
+  FD* L = new FD;
+  L->dim = dim;
+  L->n = n;
+  L->x = s->x;
+  L->ubc.assign(ubc, ubc + n_ubc);
+  L->fbc.assign(fbc, fbc + n_fbc);
+  L->fix.assign(n, 0);
+  L->u.assign(n, P3{});
+  L->v.assign(n, P3{});
+  L->init_ubc();
+  L->init_fbc();
+  s->fix = L->fix;
+  s->loader = L;
+  s->init_volume();
+  return s;
+}
+void ora_es_free(void* h) {
+  ES* s = static_cast<ES*>(h);
+  delete s->loader;
+  delete s;
+}
+void ora_es_init(void* h) { static_cast<ES*>(h)->init(); }
+void ora_es_step_cd(void* h, int nsteps) {
+  ES* s = static_cast<ES*>(h);
+  for (int k = 0; k < nsteps; k++) s->step_cd();
+}
+void ora_es_compute_forces(void* h) {
+  ES* s = static_cast<ES*>(h);
+  s->f.assign(s->n, P3{});
+  s->forces(s->f);
+}
+void ora_es_get_state(void* h, double* u, double* v, double* f) {
+  ES* s = static_cast<ES*>(h);
+  for (int64_t i = 0; i < s->n; i++)
+    for (int d = 0; d < 3; d++) {
+      if (u) u[3 * i + d] = (&s->u[i].x)[d];
+      if (v) v[3 * i + d] = (&s->v[i].x)[d];
+      if (f) f[3 * i + d] = (&s->f[i].x)[d];
+    }
+}
+void ora_es_set_state(void* h, const double* u, const double* v,
+                      const double* f) {
+  ES* s = static_cast<ES*>(h);
+  for (int64_t i = 0; i < s->n; i++) {
+    if (u) s->u[i] = p3(u + 3 * i);
+    if (v) s->v[i] = p3(v + 3 * i);
+    if (f) s->f[i] = p3(f + 3 * i);
+  }
+}
+void ora_es_get_bonds(void* h, uint8_t* b) {
+  ES* s = static_cast<ES*>(h);
+  std::copy(s->broken.begin(), s->broken.end(), b);
+}
+void ora_es_set_bonds(void* h, const uint8_t* b) {
+  ES* s = static_cast<ES*>(h);
+  for (size_t k = 0; k < s->broken.size(); k++) s->broken[k] = b[k] ? 1 : 0;
+}
+void ora_es_get_node_fields(void* h, double* theta, double* mvol) {
+  ES* s = static_cast<ES*>(h);
+  for (int64_t i = 0; i < s->n; i++) {
+    if (theta) theta[i] = s->theta.empty() ? 0. : s->theta[i];
+    if (mvol) mvol[i] = s->mw[i];
+  }
+}
+double ora_es_time(void* h) { return static_cast<ES*>(h)->time; }
 
 int ora_definitely_greater(double a, double b) { return dgt(a, b) ? 1 : 0; }
 int ora_definitely_less(double a, double b) { return dlt(a, b) ? 1 : 0; }

@@ -183,6 +183,47 @@ void ora_bbox(int64_t n, const double* x, double* lo, double* hi);
 /* OpenMP thread count actually used by the force loop */
 int ora_num_threads(void);
 
+/* ---- Linear state-based material (ElasticState) ------------------------ */
+typedef struct {
+  int dim;
+  double horizon, h, density;
+  double E, K, G; /* < 0: not given */
+  int plane_strain;
+  double sc; /* critical stretch of the explicit model (<= 0: no breaking) */
+} ora_es_deck;
+
+typedef struct {
+  int dim;
+  double horizon, h, density;
+  double E, K, G, nu, mu, factor2D;
+  int plane_strain;
+  double sc;
+} ora_es;
+
+/* ElasticState::computeParameters (src/material/pd/ElasticState.cpp:47-158) */
+int ora_es_params(const ora_es_deck* d, ora_es* out);
+/* Explicit state-based model: VolumeCorrection (volumeCorrection.cpp:25-67),
+ * dilatation (stateBasedHelperFunctions.cpp:26-83), ElasticState::getBondEF
+ * (ElasticState.cpp:160-271), QuasiStaticModel::computeForces scatter form
+ * (QuasiStaticModel.cpp:169-239), FDModel::integrateCD update. */
+void* ora_es_create(int dim, int64_t n, const double* x, const double* vol,
+                    const int64_t* row_ptr, const int32_t* col,
+                    const ora_es* mat, double dt, int n_ubc, const ora_bc* ubc,
+                    int n_fbc, const ora_bc* fbc);
+void ora_es_free(void* h);
+void ora_es_init(void* h);
+void ora_es_step_cd(void* h, int nsteps);
+/* f = internal force density of the current u (dilatation recomputed, bonds
+ * break first); external forces not added */
+void ora_es_compute_forces(void* h);
+void ora_es_get_state(void* h, double* u, double* v, double* f);
+void ora_es_set_state(void* h, const double* u, const double* v,
+                      const double* f);
+void ora_es_get_bonds(void* h, uint8_t* bond_state);
+void ora_es_set_bonds(void* h, const uint8_t* bond_state);
+void ora_es_get_node_fields(void* h, double* theta, double* mvol);
+double ora_es_time(void* h);
+
 #ifdef __cplusplus
 }
 #endif

@@ -63,6 +63,24 @@ class OraBc(C.Structure):
     ]
 
 
+class OraEsDeck(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int), ("horizon", C.c_double), ("h", C.c_double),
+        ("density", C.c_double), ("E", C.c_double), ("K", C.c_double),
+        ("G", C.c_double), ("plane_strain", C.c_int), ("sc", C.c_double),
+    ]
+
+
+class OraEs(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int), ("horizon", C.c_double), ("h", C.c_double),
+        ("density", C.c_double), ("E", C.c_double), ("K", C.c_double),
+        ("G", C.c_double), ("nu", C.c_double), ("mu", C.c_double),
+        ("factor2D", C.c_double), ("plane_strain", C.c_int),
+        ("sc", C.c_double),
+    ]
+
+
 class OraCrack(C.Structure):
     _fields_ = [
         ("o", C.c_int), ("pb", C.c_double * 2), ("pt", C.c_double * 2),
@@ -113,6 +131,20 @@ SIGNATURES = {
                                   _D, _U8]),
     "ora_bbox": (None, [C.c_int64, _D, _D, _D]),
     "ora_num_threads": (C.c_int, []),
+    "ora_es_params": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "ora_es_create": (_P, [C.c_int, C.c_int64, _D, _D, _I64, _I32,
+                           C.c_void_p, C.c_double, C.c_int,
+                           C.POINTER(OraBc), C.c_int, C.POINTER(OraBc)]),
+    "ora_es_free": (None, [_P]),
+    "ora_es_init": (None, [_P]),
+    "ora_es_step_cd": (None, [_P, C.c_int]),
+    "ora_es_compute_forces": (None, [_P]),
+    "ora_es_get_state": (None, [_P, _D, _D, _D]),
+    "ora_es_set_state": (None, [_P, _D, _D, _D]),
+    "ora_es_get_bonds": (None, [_P, _U8]),
+    "ora_es_set_bonds": (None, [_P, _U8]),
+    "ora_es_get_node_fields": (None, [_P, _D, _D]),
+    "ora_es_time": (C.c_double, [_P]),
 }
 
 _lib = None
@@ -337,3 +369,102 @@ def definitely_greater(a, b):
 
 def definitely_less(a, b):
     return bool(load().ora_definitely_less(a, b))
+
+
+def es_params(p) -> OraEs:
+    """ElasticState constants of a problem whose material is elastic_state."""
+    m = p.material
+    d = OraEsDeck()
+    d.dim = p.dim
+    d.horizon = p.horizon
+    d.h = p.h
+    d.density = m["density"]
+    d.E = m.get("E", -1.0)
+    d.K = m.get("K", -1.0)
+    d.G = m.get("G", -1.0)
+    d.plane_strain = int(m.get("plane_strain", 0))
+    d.sc = m.get("sc", -1.0)
+    out = OraEs()
+    rc = load().ora_es_params(C.byref(d), C.byref(out))
+    if rc != 0:
+        raise ValueError(f"ora_es_params failed ({rc})")
+    return out
+
+
+class OracleES:
+    """Explicit linear state-based model (ElasticState force, QS scatter
+    form, integrateCD update) restated on the CPU (see nlm_oracle.cpp)."""
+
+    def __init__(self, p, csr=None):
+        self.lib = lib = load()
+        self.p = p
+        if csr is None:
+            csr = neighbors(p.x, p.horizon, "cells" if p.n > 4000 else "brute")
+        self.row_ptr = np.ascontiguousarray(csr[0], np.int64)
+        self.col = np.ascontiguousarray(csr[1], np.int32)
+        self.nnz = int(self.row_ptr[-1])
+        self._x = np.ascontiguousarray(p.x, np.float64)
+        self._vol = np.ascontiguousarray(p.vol, np.float64)
+        self.mat = es_params(p)
+        ub = (OraBc * max(len(p.ubc), 1))(*[bc_struct(b) for b in p.ubc])
+        fb = (OraBc * max(len(p.fbc), 1))(*[bc_struct(b) for b in p.fbc])
+        self.h = lib.ora_es_create(
+            p.dim, p.n, _p(self._x, C.c_double), _p(self._vol, C.c_double),
+            _p(self.row_ptr, C.c_int64), _p(self.col, C.c_int32),
+            C.byref(self.mat), p.dt, len(p.ubc), ub, len(p.fbc), fb)
+        if getattr(p, "v0", None) is not None:
+            self.set_state(v=p.v0)
+
+    def close(self):
+        if self.h:
+            self.lib.ora_es_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init(self):
+        self.lib.ora_es_init(self.h)
+
+    def step(self, n):
+        self.lib.ora_es_step_cd(self.h, int(n))
+
+    def compute_forces(self):
+        self.lib.ora_es_compute_forces(self.h)
+
+    def get_state(self):
+        n = self.p.n
+        u, v, f = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+        self.lib.ora_es_get_state(self.h, _p(u, C.c_double), _p(v, C.c_double),
+                                  _p(f, C.c_double))
+        return u, v, f
+
+    def set_state(self, u=None, v=None, f=None):
+        def c(a):
+            return None if a is None else np.ascontiguousarray(a, np.float64)
+        u, v, f = c(u), c(v), c(f)
+        self.lib.ora_es_set_state(self.h, _p(u, C.c_double), _p(v, C.c_double),
+                                  _p(f, C.c_double))
+
+    def get_bonds(self):
+        b = np.zeros(max(self.nnz, 1), np.uint8)
+        self.lib.ora_es_get_bonds(self.h, _p(b, C.c_uint8))
+        return b[: self.nnz]
+
+    def set_bonds(self, b):
+        b = np.ascontiguousarray(b, np.uint8)
+        self.lib.ora_es_set_bonds(self.h, _p(b, C.c_uint8))
+
+    def node_fields(self):
+        """(dilatation theta_i, weighted volume m_i) of the last force pass."""
+        th = np.zeros(self.p.n)
+        mw = np.zeros(self.p.n)
+        self.lib.ora_es_get_node_fields(self.h, _p(th, C.c_double),
+                                        _p(mw, C.c_double))
+        return th, mw
+
+    def time(self):
+        return self.lib.ora_es_time(self.h)

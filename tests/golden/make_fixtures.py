@@ -10,6 +10,8 @@ container, where the reference tree is mounted):
         quasistatic.2D.elastic, test/CMakeLists.txt:36-54)
    - qs1d_output_res.npz  <- examples/qsModel/1D/output_res.vtu
        (33 nodes; golden of quasistatic.1D.elastic, test/CMakeLists.txt:12-32)
+   - qs1d_mesh.npz        <- examples/qsModel/1D/mesh.vtu (the input mesh of
+       that test: Points as stored, float32, and Node_Volume)
    - dc_fd_simple.npz     <- examples/tools/dc/fd_simple/output_{2,4}.vtu and
        compare.txt (the dc tool's known answer, test/CMakeLists.txt:61-64)
    - testFracture_nodes.csv <- data/test/geometry/testFracture_nodes.csv
@@ -40,6 +42,9 @@ def reference_fixtures(ref: Path) -> None:
                          "Fixity", "Node_Volume", "Neighbors",
                          "Strain_Tensor", "Stress_Tensor")}
         np.savez_compressed(HERE / f"qs{dim}d_output_res.npz", **keep)
+    m = read_vtu(str(ref / "examples/qsModel/1D/mesh.vtu"))
+    np.savez_compressed(HERE / "qs1d_mesh.npz", Points=m["Points"],
+                        Node_Volume=m["Node_Volume"])
     a = read_vtu(str(ref / "examples/tools/dc/fd_simple/output_2.vtu"))
     b = read_vtu(str(ref / "examples/tools/dc/fd_simple/output_4.vtu"))
     txt = (ref / "examples/tools/dc/fd_simple/compare.txt").read_text()
