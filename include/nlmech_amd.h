@@ -105,6 +105,40 @@ int nlm_rnp_params_from_deck(const nlm_material_deck* deck,
                              nlm_rnp_params* out);
 
 /* ------------------------------------------------------------------------ */
+/* Material: ElasticState (src/material/pd/ElasticState.cpp)                 */
+/* ------------------------------------------------------------------------ */
+
+/* Fields of inp::MaterialDeck used by ElasticState (materialDeck.h:226-288)
+ * plus the critical stretch of the explicit model's bond breaking (the
+ * reference ElasticState has no damage, ElasticState.cpp:273; here a bond
+ * whose stretch (|Y|-|X|)/|X| is definitelyGreaterThan critical_stretch at a
+ * force evaluation breaks irreversibly; <= 0 disables breaking). */
+typedef struct {
+  int dim;
+  double horizon;
+  double h;
+  double density;
+  double E, K, G; /* < 0: not given; 1D needs E, 2D/3D need K and G */
+  int plane_strain;
+  double critical_stretch;
+} nlm_es_deck;
+
+/* ElasticState constants after computeParameters (ElasticState.cpp:47-158):
+ * nu = (3K - 2G) / (2 (3K + G)) (materialDeck.h:120), mu = G, factor2D =
+ * (2 nu - 1) / (nu - 1) for 2D plane stress, else 1. */
+typedef struct {
+  int dim;
+  double horizon, h, density;
+  double E, K, G, nu, mu, factor2D;
+  int plane_strain;
+  double critical_stretch;
+} nlm_es_params;
+
+/* Replaces ElasticState::ElasticState + computeParameters
+ * (ElasticState.cpp:17-158); errors where the reference exits. */
+int nlm_es_params_from_deck(const nlm_es_deck* deck, nlm_es_params* out);
+
+/* ------------------------------------------------------------------------ */
 /* Neighbor search: geometry::Neighbor::Neighbor (neighbor.cpp:24-92)       */
 /* ------------------------------------------------------------------------ */
 
@@ -128,7 +162,7 @@ int nlm_neighbor_get(nlm_ctx* ctx, int64_t* row_ptr, int32_t* col);
  * otherwise interior[i] != 0 marks node i as breakable
  * (InteriorFlags::getInteriorFlag, interiorFlags.cpp:228-231). */
 typedef struct {
-  int dim; /* 2 (3 accepted for the bond-based model) */
+  int dim; /* 2 or 3 */
   int64_t n;
   const double* x;   /* [3n] reference coordinates, Mesh::getNodes */
   const double* vol; /* [n] nodal volumes, Mesh::getNodalVolumes */
@@ -138,8 +172,19 @@ typedef struct {
   double dt; /* ModelDeck::d_dt = Final_Time / Time_Steps (input.cpp:184) */
   int precision;   /* NLM_PREC_* */
   int time_scheme; /* NLM_TIME_* */
-  nlm_rnp_params mat;
+  nlm_rnp_params mat; /* material == NLM_MATERIAL_RNP_BOND */
+  int material;       /* NLM_MATERIAL_* (0: RNPBond) */
+  nlm_es_params es;   /* material == NLM_MATERIAL_ELASTIC_STATE */
 } nlm_fd_problem;
+
+/* The model's material: FDModel<T>'s template argument.  The state-based
+ * material computes, per force evaluation, the weighted-volume dilatation
+ * (StateBasedHelperFunctions::dilatation, stateBasedHelperFunctions.cpp:
+ * 26-83) and the ElasticState bond force of both bond directions, summed as
+ * QuasiStaticModel::computeForces does (f_i += T_ij V_j, f_j -= T_ij V_i,
+ * QuasiStaticModel.cpp:169-239); central difference only. */
+#define NLM_MATERIAL_RNP_BOND 0
+#define NLM_MATERIAL_ELASTIC_STATE 1
 
 /* Region / function codes of inp::BCData (src/inp/decks/loadingDeck.h). */
 #define NLM_REGION_RECTANGLE 0
@@ -234,6 +279,11 @@ int nlm_fd_compute_forces(nlm_ctx* ctx);
  * Z_i = max_j |S|/Sc(r), phi_i = 1 - sum unbroken V_j / sum V_j (float like
  * the reference), strain energy of node i.  Any pointer may be NULL. */
 int nlm_fd_postproc(nlm_ctx* ctx, float* Z, float* phi, double* energy);
+
+/* State-based material: dilatation theta_i of the last force evaluation and
+ * the weighted volume m_i (VolumeCorrection::weightedVolume,
+ * volumeCorrection.cpp:51-67), reference node order; either may be NULL. */
+int nlm_fd_get_node_fields(nlm_ctx* ctx, double* theta, double* mw);
 
 /* Measurement: runs nlm_fd_step(nsteps) bracketed by HIP events recorded on
  * the context's stream; returns the elapsed milliseconds of the whole call

@@ -19,6 +19,8 @@ NLM_PREC_FP64 = 1
 NLM_PREC_FP32_FORCE = 2
 NLM_TIME_CENTRAL_DIFFERENCE = 0
 NLM_TIME_VELOCITY_VERLET = 1
+NLM_MATERIAL_RNP_BOND = 0
+NLM_MATERIAL_ELASTIC_STATE = 1
 
 REGION = {"rectangle": 0, "angled_rectangle": 1, "circle": 2, "torus": 3,
           "line": 4, "cuboid": 5}
@@ -56,13 +58,32 @@ class RnpParams(C.Structure):
     ]
 
 
+class EsDeck(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int), ("horizon", C.c_double), ("h", C.c_double),
+        ("density", C.c_double), ("E", C.c_double), ("K", C.c_double),
+        ("G", C.c_double), ("plane_strain", C.c_int),
+        ("critical_stretch", C.c_double),
+    ]
+
+
+class EsParams(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int), ("horizon", C.c_double), ("h", C.c_double),
+        ("density", C.c_double), ("E", C.c_double), ("K", C.c_double),
+        ("G", C.c_double), ("nu", C.c_double), ("mu", C.c_double),
+        ("factor2D", C.c_double), ("plane_strain", C.c_int),
+        ("critical_stretch", C.c_double),
+    ]
+
+
 class FdProblem(C.Structure):
     _fields_ = [
         ("dim", C.c_int), ("n", C.c_int64), ("x", C.POINTER(C.c_double)),
         ("vol", C.POINTER(C.c_double)), ("row_ptr", C.POINTER(C.c_int64)),
         ("col", C.POINTER(C.c_int32)), ("interior", C.POINTER(C.c_uint8)),
         ("dt", C.c_double), ("precision", C.c_int), ("time_scheme", C.c_int),
-        ("mat", RnpParams),
+        ("mat", RnpParams), ("material", C.c_int), ("es", EsParams),
     ]
 
 
@@ -125,6 +146,9 @@ SIGNATURES = {
     "nlm_fd_autotune": (C.c_int, [_P, C.POINTER(C.c_int), _D]),
     "nlm_fd_set_unroll": (C.c_int, [_P, C.c_int]),
     "nlm_fd_kernel_name": (C.c_char_p, [_P]),
+    "nlm_es_params_from_deck": (C.c_int, [C.POINTER(EsDeck),
+                                          C.POINTER(EsParams)]),
+    "nlm_fd_get_node_fields": (C.c_int, [_P, _D, _D]),
 }
 
 _lib = None

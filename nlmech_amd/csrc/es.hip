@@ -1,0 +1,523 @@
+// es.hip -- explicit dynamics of the linear state-based material
+// (material::pd::ElasticState) on the GPU.
+//
+// A force evaluation is two kernels over the neighbor slices (one lane per
+// node, one wavefront per 64-node slice, rows in list order):
+//   k_es_dil    extensions, critical-stretch break test with the bond-bit
+//               update, dilatation theta_i (StateBasedHelperFunctions::
+//               dilatation, stateBasedHelperFunctions.cpp:26-83) and the
+//               node's force coefficients;
+//   k_es_force  the ElasticState bond force of both directions of every bond
+//               (ElasticState.cpp:160-271) gathered at node i as
+//               QuasiStaticModel::computeForces scatters it
+//               (QuasiStaticModel.cpp:169-239), plus the force BCs, fused
+//               with the central-difference update and the displacement BCs
+//               at t+dt (FDModel::integrateCD, fDModel.cpp:377-443).
+// The second kernel reads the coefficients of the neighbors, so the two are
+// separate launches.  The weighted volume m_i (volumeCorrection.cpp:51-67) is
+// computed once at setup.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "bond_es.h"
+#include "fd_common.h"
+#include "nlm_internal.h"
+
+namespace nlm {
+namespace {
+
+template <int DIM>
+struct EsArgs {
+  FdArgs<DIM> f;
+  const double* __restrict__ mw;
+  double* __restrict__ theta;
+  double* __restrict__ P;
+  double* __restrict__ Q;
+  EsDev m;
+};
+
+// m_i = sum_j |X| |X| c_ij V_j in list order (weightedVolume, :51-67), over
+// the internal CSR whose rows keep the reference order.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_es_mw(FdArgs<DIM> a,
+                                               const int64_t* __restrict__ rp,
+                                               const int32_t* __restrict__ col,
+                                               EsDev m,
+                                               double* __restrict__ mw) {
+  int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= a.n) return;
+  double xi[3];
+  load3<DIM>(a.x[i], xi);
+  double tmp = 0.;
+  for (int64_t k = rp[i]; k < rp[i + 1]; k++) {
+    const int32_t j = col[k];
+    double xj[3];
+    load3<DIM>(a.x[j], xj);
+    double X0 = xj[0] - xi[0], X1 = xj[1] - xi[1], X2 = xj[2] - xi[2];
+    double lx = sqrt(X0 * X0 + X1 * X1 + X2 * X2);
+    double c = es_corr_exact(m, lx);
+    tmp += (lx * lx * c * a.vol[j]);
+  }
+  mw[i] = tmp;
+}
+
+// Pass 1: break test, dilatation, force coefficients.
+template <int DIM, int MODE, int U>
+__global__ __launch_bounds__(256) void k_es_dil(
+    EsArgs<DIM> e, const unsigned long long* __restrict__ bits_ro) {
+  const FdArgs<DIM>& a = e.f;
+  const int64_t nblk = (a.nslices * 64 + 255) / 256;
+  const int64_t lb = xcd_block(nblk);
+  if (lb >= nblk) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t s = lb * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (s >= a.nslices) return;  // whole wavefront
+  const int64_t i = s * 64 + lane;
+  const bool valid = i < a.n;
+  const int64_t cb = a.sl_ptr[s];
+  const int width = int((a.sl_ptr[s + 1] - cb) >> 6);
+  const int64_t bb = cb >> 6;
+  const int len = valid ? int(a.rowlen[i]) : 0;
+  double xi[3] = {0., 0., 0.}, ui[3] = {0., 0., 0.};
+  double mwi = 1.;
+  if (valid) {
+    load3<DIM>(a.x[i], xi);
+    load3<DIM>(a.u[i], ui);
+    mwi = e.mw[i];
+  }
+  // (3/m_i) in 3D, (2/m_i) factor2D in 2D, leading factors of the reference
+  // product (stateBasedHelperFunctions.cpp:58-71)
+  const double c3 = DIM == 3 ? 3. / mwi : (2. / mwi) * e.m.f2d;
+  const float c3f = float(c3);
+  const int32_t* __restrict__ colS = a.col + cb;
+  const unsigned long long* __restrict__ bits_in = bits_ro + bb;
+  const int wlast = width > 0 ? width - 1 : 0;
+  auto slot_index = [&](int k) -> uint32_t {
+    const uint32_t en = uint32_t(64 * (k < wlast ? k : wlast) + lane);
+    return uint32_t(ld32(colS, en));
+  };
+  double th = 0.;
+  uint32_t jn[U];
+#pragma unroll
+  for (int q = 0; q < U; ++q) jn[q] = slot_index(q);
+  for (int k0 = 0; k0 < width; k0 += U) {
+    uint32_t jc[U];
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      jc[q] = jn[q];
+      jn[q] = slot_index(k0 + U + q);
+    }
+    double xj[U][3], uj[U][3], vj[U];
+    unsigned long long w[U];
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      load3<DIM>(ld32(a.x, jc[q]), xj[q]);
+      load3<DIM>(ld32(a.u, jc[q]), uj[q]);
+      vj[q] = ld32(a.vol, jc[q]);
+      const int kq = k0 + q;
+      w[q] = bits_in[kq < wlast ? kq : wlast];
+    }
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      const bool act = k0 + q < len;
+      const bool fs0 = __builtin_amdgcn_inverse_ballot_w64(w[q]);
+      bool fs = fs0;
+      double d;
+      if constexpr (MODE == kModeExact) {
+        d = es_dil_exact(e.m, xi, ui, xj[q], uj[q], vj[q], c3, fs);
+      } else {
+        bool exn;
+        if constexpr (MODE == kModeFp64)
+          d = es_dil_fp64<DIM>(e.m, xi, ui, xj[q], uj[q], vj[q], c3, fs0, fs,
+                               exn);
+        else
+          d = es_dil_fp32<DIM>(e.m, xi, ui, xj[q], uj[q], vj[q], c3f, fs0,
+                               fs, exn);
+        exn = exn && act;
+        if (__ballot(exn) != 0ull) {
+          if (exn) {
+            fs = fs0;
+            d = es_dil_exact(e.m, xi, ui, xj[q], uj[q], vj[q], c3, fs);
+          }
+        }
+      }
+      if (act && !fs) th += d;
+      const bool newly = act && fs && !fs0;
+      if (k0 + q < width) {
+        const unsigned long long nb = __ballot(newly);
+        if (nb != 0ull && lane == 0) a.bits[bb + k0 + q] = w[q] | nb;
+      }
+    }
+  }
+  if (!valid) return;
+  e.theta[i] = th;
+  if constexpr (DIM == 3) {
+    // EXACT keeps alpha_s = (9/m) K (ElasticState.cpp:230-233); the fast
+    // modes fold t_ij = alpha_s theta_i |X| / 3 into P_i |X|.
+    e.P[i] = MODE == kModeExact ? (9. / mwi) * e.m.K : 3. * e.m.K * th / mwi;
+    e.Q[i] = 0.;
+  } else {
+    const double as = (9. / mwi) * e.m.Kc;
+    const double ad = (8. / mwi) * e.m.mu;
+    const double C = 2. * e.m.f2d * as - (3. - 2. * e.m.f2d) * ad;
+    // fast modes: t_ij = theta_i (C/9 - alpha_d/3) |X| + alpha_d e_ij
+    e.P[i] = MODE == kModeExact ? C : th * (C / 9. - ad / 3.);
+    e.Q[i] = ad;
+  }
+}
+
+// Pass 2: bond forces of both directions gathered at node i, force BCs,
+// central-difference update.
+template <int DIM, int MODE, int KIND, bool BASE_F, int U>
+__global__ __launch_bounds__(256) void k_es_force(
+    EsArgs<DIM> e, const unsigned long long* __restrict__ bits_ro,
+    double t_force, double t_next) {
+  const FdArgs<DIM>& a = e.f;
+  const int64_t nblk = (a.nslices * 64 + 255) / 256;
+  const int64_t lb = xcd_block(nblk);
+  if (lb >= nblk) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t s = lb * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (s >= a.nslices) return;
+  const int64_t i = s * 64 + lane;
+  const bool valid = i < a.n;
+  const int64_t cb = a.sl_ptr[s];
+  const int width = int((a.sl_ptr[s + 1] - cb) >> 6);
+  const int64_t bb = cb >> 6;
+  const int len = valid ? int(a.rowlen[i]) : 0;
+  constexpr bool kExact = MODE == kModeExact;
+  constexpr bool kNeedQ = DIM == 2;
+  double xi[3] = {0., 0., 0.}, ui[3] = {0., 0., 0.};
+  double Pi = 0., Qi = 0., thi = 0.;
+  uint32_t fl = 0;
+  if (valid) {
+    load3<DIM>(a.x[i], xi);
+    load3<DIM>(a.u[i], ui);
+    Pi = e.P[i];
+    if (kNeedQ) Qi = e.Q[i];
+    if (kExact) thi = e.theta[i];
+    fl = a.flags[i];
+  }
+  const float Pif = float(Pi), Qif = float(Qi);
+  const int32_t* __restrict__ colS = a.col + cb;
+  const unsigned long long* __restrict__ bits_in = bits_ro + bb;
+  const int wlast = width > 0 ? width - 1 : 0;
+  auto slot_index = [&](int k) -> uint32_t {
+    const uint32_t en = uint32_t(64 * (k < wlast ? k : wlast) + lane);
+    return uint32_t(ld32(colS, en));
+  };
+  double acc[3] = {0., 0., 0.}, oth[3] = {0., 0., 0.};
+  uint32_t jn[U];
+#pragma unroll
+  for (int q = 0; q < U; ++q) jn[q] = slot_index(q);
+  for (int k0 = 0; k0 < width; k0 += U) {
+    uint32_t jc[U];
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      jc[q] = jn[q];
+      jn[q] = slot_index(k0 + U + q);
+    }
+    double xj[U][3], uj[U][3], vj[U], Pj[U], Qj[U], thj[U];
+    unsigned long long w[U];
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      const uint32_t j = jc[q];
+      load3<DIM>(ld32(a.x, j), xj[q]);
+      load3<DIM>(ld32(a.u, j), uj[q]);
+      vj[q] = ld32(a.vol, j);
+      Pj[q] = ld32(e.P, j);
+      Qj[q] = kNeedQ ? ld32(e.Q, j) : 0.;
+      thj[q] = kExact ? ld32(e.theta, j) : 0.;
+      const int kq = k0 + q;
+      w[q] = bits_in[kq < wlast ? kq : wlast];
+    }
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      const bool act = k0 + q < len;
+      const bool fs0 = __builtin_amdgcn_inverse_ballot_w64(w[q]);
+      const bool keep = act && !fs0;
+      if constexpr (kExact) {
+        double own[3], ot[3];
+        es_force_exact<DIM>(e.m, xi, ui, xj[q], uj[q], vj[q], Pi, Qi, thi,
+                            Pj[q], Qj[q], thj[q], own, ot);
+        for (int d = 0; d < DIM; d++) {
+          if (keep) {
+            acc[d] += own[d];
+            oth[d] += ot[d];
+          }
+        }
+      } else if constexpr (MODE == kModeFp64) {
+        double dx[3], du[3], Y[3];
+        for (int d = 0; d < 3; d++) {
+          dx[d] = d < DIM ? xj[q][d] - xi[d] : 0.;
+          du[d] = d < DIM ? uj[q][d] - ui[d] : 0.;
+          Y[d] = dx[d] + du[d];
+        }
+        double yy = __builtin_fma(Y[1], Y[1], Y[0] * Y[0]);
+        double dd = __builtin_fma(dx[1], dx[1], dx[0] * dx[0]);
+        if (DIM == 3) {
+          yy = __builtin_fma(Y[2], Y[2], yy);
+          dd = __builtin_fma(dx[2], dx[2], dd);
+        }
+        const double ry = rsqrt_f64(yy);
+        const double rx = rsqrt_f64(dd);
+        const double lx = dd * rx;
+        const bool vex = keep && fabs(lx - e.m.low_band) <=
+                                     kEsGuard64 * e.m.low_band;
+        double c = lx > e.m.low_band ? (e.m.check_up - lx) * e.m.rcp_h : 1.;
+        if (__ballot(vex) != 0ull) {
+          if (vex) c = es_corr_exact(e.m, sqrt(dx[0] * dx[0] + dx[1] * dx[1] +
+                                               dx[2] * dx[2]));
+        }
+        double t = (Pi + Pj[q]) * lx;
+        if (DIM == 2) {
+          double g = __builtin_fma(du[1], __builtin_fma(2., dx[1], du[1]),
+                                   du[0] * __builtin_fma(2., dx[0], du[0]));
+          double ext = g / (yy * ry + lx);
+          t = __builtin_fma(Qi + Qj[q], ext, t);
+        }
+        double sr = t * c * vj[q] * ry;
+        sr = keep ? sr : 0.;
+        for (int d = 0; d < DIM; d++) acc[d] = __builtin_fma(sr, Y[d], acc[d]);
+      } else {
+        float dx[3], du[3], Y[3];
+        for (int d = 0; d < 3; d++) {
+          dx[d] = d < DIM ? float(xj[q][d] - xi[d]) : 0.f;
+          du[d] = d < DIM ? float(uj[q][d] - ui[d]) : 0.f;
+          Y[d] = dx[d] + du[d];
+        }
+        float yy = __builtin_fmaf(Y[1], Y[1], Y[0] * Y[0]);
+        float dd = __builtin_fmaf(dx[1], dx[1], dx[0] * dx[0]);
+        if (DIM == 3) {
+          yy = __builtin_fmaf(Y[2], Y[2], yy);
+          dd = __builtin_fmaf(dx[2], dx[2], dd);
+        }
+        const float ry = __builtin_amdgcn_rsqf(yy);
+        const float rx = __builtin_amdgcn_rsqf(dd);
+        const float lx = dd * rx;
+        const bool vex = keep && fabsf(lx - e.m.low_band_f) <=
+                                     3e-5f * e.m.low_band_f;
+        float c = fminf((e.m.check_up_f - lx) * e.m.rcp_h_f, 1.0f);
+        if (__ballot(vex) != 0ull) {
+          if (vex) {
+            double X0 = xj[q][0] - xi[0], X1 = xj[q][1] - xi[1];
+            double X2 = DIM == 3 ? xj[q][2] - xi[2] : 0.;
+            c = float(es_corr_exact(e.m, sqrt(X0 * X0 + X1 * X1 + X2 * X2)));
+          }
+        }
+        float t = (Pif + float(Pj[q])) * lx;
+        if (DIM == 2) {
+          float g = __builtin_fmaf(du[1], __builtin_fmaf(2.f, dx[1], du[1]),
+                                   du[0] * __builtin_fmaf(2.f, dx[0], du[0]));
+          float ext = g * __builtin_amdgcn_rcpf(yy * ry + lx);
+          t = __builtin_fmaf(Qif + float(Qj[q]), ext, t);
+        }
+        float sr = t * c * float(vj[q]) * ry;
+        sr = keep ? sr : 0.f;
+        for (int d = 0; d < DIM; d++) acc[d] += double(sr * Y[d]);
+      }
+    }
+  }
+  if (!valid) return;
+  double f[3] = {0., 0., 0.};
+  if (BASE_F) {
+    load3<DIM>(a.f_in[i], f);
+  } else if (a.has_fbc) {
+    fext_node<DIM>(a.bc, fl, xi, t_force, f);
+  }
+  for (int d = 0; d < DIM; d++) f[d] += kExact ? acc[d] - oth[d] : acc[d];
+  node_tail<DIM, KIND>(a, i, fl, xi, ui, f, t_next);
+}
+
+// ---- host side ------------------------------------------------------------
+
+template <int DIM>
+EsArgs<DIM> make_es_args(nlm_ctx* c) {
+  FdState& s = c->fd;
+  EsArgs<DIM> e;
+  e.f = make_args<DIM>(c);
+  e.mw = s.es_mw.as<double>();
+  e.theta = s.es_theta.as<double>();
+  e.P = s.es_P.as<double>();
+  e.Q = s.es_Q.as<double>();
+  e.m = s.es;
+  return e;
+}
+
+template <int DIM, int MODE>
+void launch_dil(nlm_ctx* c, const EsArgs<DIM>& e) {
+  unsigned g = grid_xcd(force_blocks(c->fd) * 256, 256);
+  hipLaunchKernelGGL((k_es_dil<DIM, MODE, 4>), dim3(g), dim3(256), 0,
+                     c->stream, e, e.f.bits);
+}
+
+template <int DIM, int MODE, int KIND, bool BASE_F, int U>
+void launch_force_k(nlm_ctx* c, const EsArgs<DIM>& e, double tf, double tn) {
+  unsigned g = grid_xcd(force_blocks(c->fd) * 256, 256);
+  hipLaunchKernelGGL((k_es_force<DIM, MODE, KIND, BASE_F, U>), dim3(g),
+                     dim3(256), 0, c->stream, e, e.f.bits, tf, tn);
+}
+
+template <int DIM, int MODE>
+void launch_pair(nlm_ctx* c, int kind, bool base_f, double tf, double tn,
+                 int U) {
+  EsArgs<DIM> e = make_es_args<DIM>(c);
+  launch_dil<DIM, MODE>(c, e);
+  if (kind == KIND_FORCE) {
+    if (base_f)
+      launch_force_k<DIM, MODE, KIND_FORCE, true, 4>(c, e, tf, tn);
+    else
+      launch_force_k<DIM, MODE, KIND_FORCE, false, 4>(c, e, tf, tn);
+    return;
+  }
+  if constexpr (MODE != kModeExact) {
+    if (U == 2) return launch_force_k<DIM, MODE, KIND_CD, false, 2>(c, e, tf, tn);
+    if (U == 8) return launch_force_k<DIM, MODE, KIND_CD, false, 8>(c, e, tf, tn);
+  }
+  launch_force_k<DIM, MODE, KIND_CD, false, 4>(c, e, tf, tn);
+}
+
+template <int DIM>
+void launch_dim(nlm_ctx* c, int kind, bool base_f, double tf, double tn,
+                int U) {
+  switch (c->fd.precision) {
+    case kModeExact:
+      launch_pair<DIM, kModeExact>(c, kind, base_f, tf, tn, U);
+      break;
+    case kModeFp64:
+      launch_pair<DIM, kModeFp64>(c, kind, base_f, tf, tn, U);
+      break;
+    default:
+      launch_pair<DIM, kModeFp32>(c, kind, base_f, tf, tn, U);
+      break;
+  }
+}
+
+void launch_any(nlm_ctx* c, int kind, bool base_f, double tf, double tn,
+                int U) {
+  if (c->fd.dim == 2)
+    launch_dim<2>(c, kind, base_f, tf, tn, U);
+  else
+    launch_dim<3>(c, kind, base_f, tf, tn, U);
+}
+
+}  // namespace
+
+// Device arrays and the weighted volume; called by nlm_fd_setup after the
+// node arrays and slices are in place.
+int es_prepare(nlm_ctx* c) {
+  FdState& s = c->fd;
+  int rc;
+  const size_t nb = sizeof(double) * size_t(s.n) + 256;
+  if ((rc = alloc(c, s.es_mw, nb))) return rc;
+  if ((rc = alloc(c, s.es_theta, nb))) return rc;
+  if ((rc = alloc(c, s.es_P, nb))) return rc;
+  if ((rc = alloc(c, s.es_Q, nb))) return rc;
+  NLM_HIP(c, hipMemsetAsync(s.es_theta.p, 0, nb, c->stream));
+  NLM_HIP(c, hipMemsetAsync(s.es_P.p, 0, nb, c->stream));
+  NLM_HIP(c, hipMemsetAsync(s.es_Q.p, 0, nb, c->stream));
+  const unsigned g = unsigned((s.n + 255) / 256);
+  if (s.dim == 2)
+    hipLaunchKernelGGL(k_es_mw<2>, dim3(g), dim3(256), 0, c->stream,
+                       make_args<2>(c), s.row_ptr.as<int64_t>(),
+                       s.col.as<int32_t>(), s.es, s.es_mw.as<double>());
+  else
+    hipLaunchKernelGGL(k_es_mw<3>, dim3(g), dim3(256), 0, c->stream,
+                       make_args<3>(c), s.row_ptr.as<int64_t>(),
+                       s.col.as<int32_t>(), s.es, s.es_mw.as<double>());
+  NLM_HIP(c, hipGetLastError());
+  NLM_HIP(c, hipStreamSynchronize(c->stream));
+  return NLM_OK;
+}
+
+void es_launch_force(nlm_ctx* c, bool base_f, double t) {
+  launch_any(c, KIND_FORCE, base_f, t, t, 4);
+}
+
+void es_launch_fused(nlm_ctx* c, double tf, double tn) {
+  launch_any(c, KIND_CD, false, tf, tn, c->fd.unroll);
+}
+
+// Times the fused pass pair for each chunk width of the force pass on the
+// current state; the trials write the spare u/v buffers, the node
+// coefficient arrays (rewritten by the next pass 1) and a scratch copy of the
+// bond bits, so the model state is unchanged.
+int es_autotune(nlm_ctx* c, int* chosen, double* best_us) {
+  FdState& s = c->fd;
+  DevBuf scratch;
+  int rc;
+  if ((rc = alloc(c, scratch, s.sell.bits.bytes))) return rc;
+  NLM_HIP(c, hipMemcpyAsync(scratch.p, s.sell.bits.p, s.sell.bits.bytes,
+                            hipMemcpyDeviceToDevice, c->stream));
+  DevBuf th_save;
+  if ((rc = alloc(c, th_save, s.es_theta.bytes))) return rc;
+  NLM_HIP(c, hipMemcpyAsync(th_save.p, s.es_theta.p, s.es_theta.bytes,
+                            hipMemcpyDeviceToDevice, c->stream));
+  // the trials run on the scratch bits
+  std::swap(s.sell.bits.p, scratch.p);
+  const int cands[3] = {2, 4, 8};
+  double best = 1e300;
+  int pick = 4;
+  hipEvent_t e0, e1;
+  NLM_HIP(c, hipEventCreate(&e0));
+  NLM_HIP(c, hipEventCreate(&e1));
+  const int reps = 4;
+  for (int U : cands) {
+    launch_any(c, KIND_CD, false, s.time, s.time + s.dt, U);
+    NLM_HIP(c, hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; r++)
+      launch_any(c, KIND_CD, false, s.time, s.time + s.dt, U);
+    NLM_HIP(c, hipEventRecord(e1, c->stream));
+    NLM_HIP(c, hipEventSynchronize(e1));
+    float ms = 0.f;
+    NLM_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+    const double us = 1e3 * double(ms) / reps;
+    if (us < best) {
+      best = us;
+      pick = U;
+    }
+  }
+  std::swap(s.sell.bits.p, scratch.p);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  // restore pass 1's outputs of the current state
+  NLM_HIP(c, hipMemcpyAsync(s.es_theta.p, th_save.p, s.es_theta.bytes,
+                            hipMemcpyDeviceToDevice, c->stream));
+  NLM_HIP(c, hipGetLastError());
+  NLM_HIP(c, hipStreamSynchronize(c->stream));
+  s.unroll = pick;
+  if (chosen) *chosen = pick;
+  if (best_us) *best_us = best;
+  return NLM_OK;
+}
+
+}  // namespace nlm
+
+extern "C" int nlm_fd_get_node_fields(nlm_ctx* c, double* theta, double* mw) {
+  using namespace nlm;
+  if (!c) return NLM_ERR_INVALID;
+  FdState& s = c->fd;
+  if (!s.ready)
+    return fail(c, NLM_ERR_STATE, "nlm_fd_get_node_fields: no setup");
+  if (s.material != NLM_MATERIAL_ELASTIC_STATE)
+    return fail(c, NLM_ERR_INVALID,
+                "nlm_fd_get_node_fields: state-based material only");
+  std::vector<double> a(s.n), b(s.n);
+  NLM_HIP(c, hipMemcpyAsync(a.data(), s.es_theta.p, sizeof(double) * s.n,
+                            hipMemcpyDeviceToHost, c->stream));
+  NLM_HIP(c, hipMemcpyAsync(b.data(), s.es_mw.p, sizeof(double) * s.n,
+                            hipMemcpyDeviceToHost, c->stream));
+  NLM_HIP(c, hipStreamSynchronize(c->stream));
+  for (int64_t k = 0; k < s.n; k++) {
+    const int64_t i = s.perm_host[k];
+    if (theta) theta[i] = a[k];
+    if (mw) mw[i] = b[k];
+  }
+  return NLM_OK;
+}

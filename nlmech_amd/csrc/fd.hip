@@ -28,62 +28,12 @@
 
 #include "bond_rnp.h"
 #include "device_common.h"
+#include "fd_common.h"
 #include "layout.h"
 #include "nlm_internal.h"
 
 namespace nlm {
 namespace {
-
-template <int DIM>
-struct FdArgs {
-  using V = typename VT<DIM>::T;
-  int64_t n;
-  int64_t nslices;
-  const V* __restrict__ x;
-  const double* __restrict__ vol;
-  const uint32_t* __restrict__ flags;
-  const int64_t* __restrict__ sl_ptr;
-  const int32_t* __restrict__ col;
-  const uint16_t* __restrict__ rowlen;
-  unsigned long long* __restrict__ bits;
-  const V* __restrict__ u;
-  const V* __restrict__ v;
-  V* __restrict__ u_out;
-  V* __restrict__ v_out;
-  const V* __restrict__ f_in;
-  V* __restrict__ f_out;
-  const BcTable* __restrict__ bc;
-  const int64_t* __restrict__ halo_ptr;
-  const int32_t* __restrict__ halo;
-  const uint16_t* __restrict__ colL;
-  int64_t halo_cap;  // LDS entries per halo array
-  RnpDev m;
-  double dt;
-  double fact;
-  int has_fbc;
-  int has_nofail;
-};
-
-template <int DIM>
-__device__ __forceinline__ void load3(const typename VT<DIM>::T& a,
-                                      double* o) {
-  o[0] = a.x;
-  o[1] = a.y;
-  o[2] = 0.;
-  if constexpr (DIM == 3) o[2] = a.z;
-}
-
-template <int DIM>
-__device__ __forceinline__ typename VT<DIM>::T make_v(const double* o) {
-  typename VT<DIM>::T r;
-  r.x = o[0];
-  r.y = o[1];
-  if constexpr (DIM == 3) {
-    r.z = o[2];
-    r.w = 0.;
-  }
-  return r;
-}
 
 template <int DIM, int MODE, int INF>
 __device__ __forceinline__ void bond_mode(const RnpDev& m, const double* xi,
@@ -97,37 +47,6 @@ __device__ __forceinline__ void bond_mode(const RnpDev& m, const double* xi,
     bond_fp64<DIM, INF>(m, xi, ui, xj, uj, voli, volj, brk, fs, F);
   else
     bond_fp32<DIM, INF>(m, xi, ui, xj, uj, voli, volj, brk, fs, F);
-}
-
-// External force density of node i at time t (FLoading::apply order: sets in
-// order, each adding to every dof it loads).
-template <int DIM>
-__device__ __forceinline__ void fext_node(const BcTable* bc, uint32_t fl,
-                                          const double* xi, double t,
-                                          double* b) {
-  uint32_t mem = (fl >> kFlagFbcShift) & 0xffu;
-  for (int s = 0; s < bc->nf; s++) {
-    if (!((mem >> s) & 1u)) continue;
-    const BcSetDev& set = bc->f[s];
-    double val = fbc_eval(set, xi[0], xi[1], xi[2], t);
-    for (int d = 0; d < DIM; d++)
-      if ((set.dirmask >> d) & 1) b[d] += val;
-  }
-}
-
-// KIND_FORCE: f_out = base + internal force (base = f_in or f_ext(t_force)).
-// KIND_CD: fused force at t_force + central-difference update + BCs at t_next.
-constexpr int KIND_FORCE = 0;
-constexpr int KIND_CD = 1;
-constexpr int KIND_VERLET = 2;
-
-// Element idx of a device array addressed as base + 32-bit byte offset, so
-// the load uses the scalar-base + vector-offset form (no 64-bit address
-// arithmetic per lane).
-template <class T>
-__device__ __forceinline__ T ld32(const T* __restrict__ base, uint32_t idx) {
-  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
-                                     idx * uint32_t(sizeof(T)));
 }
 
 // One halo node staged in LDS by the tiled step kernel (2D): reference
@@ -605,45 +524,6 @@ __global__ __launch_bounds__(256) void k_postproc(FdArgs<DIM> a,
 
 // ---- host side ------------------------------------------------------------
 
-template <int DIM>
-FdArgs<DIM> make_args(nlm_ctx* c) {
-  using V = typename VT<DIM>::T;
-  FdState& s = c->fd;
-  FdArgs<DIM> a;
-  a.n = s.n;
-  a.nslices = s.sell.nslices;
-  a.x = s.x.as<V>();
-  a.vol = s.vol.as<double>();
-  a.flags = s.flags.as<uint32_t>();
-  a.sl_ptr = s.sell.sl_ptr.as<int64_t>();
-  a.col = s.sell.col.as<int32_t>();
-  a.rowlen = s.sell.rowlen.as<uint16_t>();
-  a.bits = s.sell.bits.as<unsigned long long>();
-  a.u = s.u[s.cur].as<V>();
-  a.v = s.v[s.cur].as<V>();
-  a.u_out = s.u[1 - s.cur].as<V>();
-  a.v_out = s.v[1 - s.cur].as<V>();
-  a.f_in = s.f.as<V>();
-  a.f_out = s.f.as<V>();
-  a.bc = s.bctab.as<BcTable>();
-  a.halo_ptr = s.halo_ptr.as<int64_t>();
-  a.halo = s.halo.as<int32_t>();
-  a.colL = s.colL.as<uint16_t>();
-  a.halo_cap = (s.halo_max + 1) & ~int64_t(1);
-  a.m = s.rnp;
-  a.dt = s.dt;
-  a.fact = s.scheme == NLM_TIME_CENTRAL_DIFFERENCE
-               ? s.dt * s.dt / s.density
-               : 0.5 * s.dt / s.density;
-  a.has_fbc = s.has_fbc ? 1 : 0;
-  a.has_nofail = s.has_nofail ? 1 : 0;
-  return a;
-}
-
-int64_t force_blocks(const FdState& s) {
-  return (s.sell.nslices * 64 + 255) / 256;
-}
-
 // Launch helpers.  The neighbor-slot chunk U of the fused step kernel is a
 // tuning knob (nlm_fd_autotune); force-only launches use U = 4.
 template <int DIM, int MODE, int KIND, bool BASE_F, int INF, int U,
@@ -715,6 +595,8 @@ void launch_mode(nlm_ctx* c, const FdArgs<DIM>& a, double tf, double tn,
 
 template <int DIM>
 void launch_force_dispatch(nlm_ctx* c, bool base_f, double t) {
+  if (c->fd.material == NLM_MATERIAL_ELASTIC_STATE)
+    return es_launch_force(c, base_f, t);
   FdArgs<DIM> a = make_args<DIM>(c);
   if (base_f)
     launch_mode<DIM, KIND_FORCE, true>(c, a, t, t, 4);
@@ -724,6 +606,8 @@ void launch_force_dispatch(nlm_ctx* c, bool base_f, double t) {
 
 template <int DIM>
 void launch_fused_dispatch(nlm_ctx* c, double tf, double tn) {
+  if (c->fd.material == NLM_MATERIAL_ELASTIC_STATE)
+    return es_launch_fused(c, tf, tn);
   FdArgs<DIM> a = make_args<DIM>(c);
   launch_mode<DIM, KIND_CD, false>(c, a, tf, tn, c->fd.unroll);
 }
@@ -888,10 +772,25 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
   if (p->time_scheme != NLM_TIME_CENTRAL_DIFFERENCE &&
       p->time_scheme != NLM_TIME_VELOCITY_VERLET)
     return fail(c, NLM_ERR_INVALID, "nlm_fd_setup: unknown time scheme");
-  if (!(p->dt > 0.) || !(p->mat.density > 0.) || !(p->mat.horizon > 0.) ||
-      !(p->mat.h > 0.))
+  if (p->material != NLM_MATERIAL_RNP_BOND &&
+      p->material != NLM_MATERIAL_ELASTIC_STATE)
+    return fail(c, NLM_ERR_INVALID, "nlm_fd_setup: unknown material");
+  const bool es_mat = p->material == NLM_MATERIAL_ELASTIC_STATE;
+  const double density = es_mat ? p->es.density : p->mat.density;
+  const double horizon = es_mat ? p->es.horizon : p->mat.horizon;
+  const double mesh_h = es_mat ? p->es.h : p->mat.h;
+  if (!(p->dt > 0.) || !(density > 0.) || !(horizon > 0.) || !(mesh_h > 0.))
     return fail(c, NLM_ERR_INVALID,
                 "nlm_fd_setup: dt, density, horizon and h must be > 0");
+  if (es_mat && p->time_scheme != NLM_TIME_CENTRAL_DIFFERENCE)
+    return fail(c, NLM_ERR_INVALID,
+                "nlm_fd_setup: the state-based material uses central "
+                "difference");
+  if (es_mat && (p->es.dim != p->dim || !(p->es.K > 0.) ||
+                 !(p->es.mu > 0.) || !(p->es.factor2D > 0.)))
+    return fail(c, NLM_ERR_INVALID,
+                "nlm_fd_setup: state-based material needs dim, K, G "
+                "(nlm_es_params_from_deck)");
   if (p->n >= (int64_t(1) << 31))
     return fail(c, NLM_ERR_INVALID, "nlm_fd_setup: n >= 2^31");
   FdState& s = c->fd;
@@ -933,7 +832,7 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
   }
   // internal order, slices and halos (layout.cpp)
   HostLayout L;
-  build_layout(dim, n, p->x, p->mat.h, h_row, h_col, L);
+  build_layout(dim, n, p->x, mesh_h, h_row, h_col, L);
   s.perm_host = L.perm;
   s.inv_host = L.inv;
   s.sell.nslices = L.nslices;
@@ -961,7 +860,7 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
   if ((rc = up(s.colL, L.colL.data(), sizeof(uint16_t) * nent))) return rc;
   s.halo_max = L.halo_max;
   // LDS budget of the staged kernel: one 48-byte record per halo node
-  s.tiled = dim == 2 && L.halo_max > 0 &&
+  s.tiled = !es_mat && dim == 2 && L.halo_max > 0 &&
             (L.halo_max + 1) * 48 <= 64 * 1024;
   // node arrays in internal order
   const int stride = dim == 2 ? 2 : 4;
@@ -1010,7 +909,8 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
   s.precision = p->precision;
   s.scheme = p->time_scheme;
   s.dt = p->dt;
-  s.density = p->mat.density;
+  s.density = density;
+  s.material = p->material;
   s.time = 0.;
   s.step = 0;
   s.cur = 0;
@@ -1057,6 +957,36 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
   m.rcp_krbar_sq_f = float(m.rcp_krbar_sq);
   m.nbeta_log2e_f = float(-m.beta * 1.4426950408889634);
   s.unroll = 4;
+  if (es_mat) {
+    // ElasticState constants (ElasticState.cpp:47-158, volumeCorrection.cpp)
+    const nlm_es_params& q = p->es;
+    s.es_par = q;
+    EsDev& e = s.es;
+    std::memset(&e, 0, sizeof(e));
+    e.K = q.K;
+    e.mu = q.mu;
+    e.f2d = q.factor2D;
+    // alpha_s = (9/m) (K + ((nu+1)/(2 nu-1))^2 mu / 9) for plane stress,
+    // (9/m) (K + mu/9) for plane strain (ElasticState.cpp:201-212)
+    e.Kc = q.plane_strain
+               ? q.K + q.mu / 9.
+               : q.K + std::pow((q.nu + 1.) / (2. * q.nu - 1.), 2) * q.mu / 9.;
+    e.horizon = q.horizon;
+    e.h = q.h;
+    const double r = q.h * 0.5;
+    e.check_low = q.horizon - r;
+    e.check_up = q.horizon + r;
+    e.rcp_h = 1.0 / q.h;
+    e.low_band = e.check_low * (1.0 + 1e-5);
+    e.breaking = q.critical_stretch > 0. ? 1 : 0;
+    e.sc = q.critical_stretch;
+    e.sc_break = e.breaking ? q.critical_stretch / (1.0 - 1e-5) : 0.;
+    e.check_up_f = float(e.check_up);
+    e.low_band_f = float(e.low_band);
+    e.rcp_h_f = float(e.rcp_h);
+    e.sc_break_f = float(e.sc_break);
+    if ((rc = es_prepare(c))) return rc;
+  }
   if (!s.ev0) NLM_HIP(c, hipEventCreate(&s.ev0));
   if (!s.ev1) NLM_HIP(c, hipEventCreate(&s.ev1));
   s.ready = true;
@@ -1486,6 +1416,9 @@ extern "C" int nlm_fd_postproc(nlm_ctx* c, float* Z, float* phi,
   int rc = check_ready(c, "nlm_fd_postproc");
   if (rc) return rc;
   FdState& s = c->fd;
+  if (s.material != NLM_MATERIAL_RNP_BOND)
+    return fail(c, NLM_ERR_INVALID,
+                "nlm_fd_postproc: bond-based material only");
   DevBuf dz, dp, de;
   if ((rc = alloc(c, dz, sizeof(float) * s.n))) return rc;
   if ((rc = alloc(c, dp, sizeof(float) * s.n))) return rc;
@@ -1576,6 +1509,8 @@ extern "C" int nlm_fd_autotune(nlm_ctx* c, int* chosen, double* best_us) {
     if (best_us) *best_us = 0.;
     return NLM_OK;
   }
+  if (c->fd.material == NLM_MATERIAL_ELASTIC_STATE)
+    return es_autotune(c, chosen, best_us);
   return c->fd.dim == 2 ? autotune<2>(c, chosen, best_us)
                         : autotune<3>(c, chosen, best_us);
 }
@@ -1598,5 +1533,6 @@ extern "C" int nlm_fd_num_bonds(nlm_ctx* c, int64_t* nnz) {
 
 extern "C" const char* nlm_fd_kernel_name(nlm_ctx* c) {
   if (!c || !c->fd.ready) return "";
+  if (c->fd.material == NLM_MATERIAL_ELASTIC_STATE) return "k_es_force";
   return "k_fd";
 }

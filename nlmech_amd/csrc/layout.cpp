@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstdint>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "layout.h"
@@ -44,6 +45,23 @@ uint64_t spread3(uint64_t v) {  // 21 bits -> every 3rd bit
   v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
   v = (v | (v << 2)) & 0x1249249249249249ull;
   return v;
+}
+
+// Runs body(lo, hi) over [0, count) split into contiguous chunks, one per
+// host thread (at most 16).
+template <class F>
+void parallel_chunks(int64_t count, F body) {
+  int64_t nt = int64_t(std::thread::hardware_concurrency());
+  nt = std::max<int64_t>(1, std::min<int64_t>(nt, 16));
+  if (count < 4096) nt = 1;
+  const int64_t chunk = (count + nt - 1) / nt;
+  std::vector<std::thread> th;
+  for (int64_t t = 0; t < nt; t++) {
+    const int64_t lo = t * chunk, hi = std::min(count, lo + chunk);
+    if (lo >= hi) break;
+    th.emplace_back([=, &body] { body(lo, hi); });
+  }
+  for (auto& x : th) x.join();
 }
 
 }  // namespace
@@ -83,13 +101,15 @@ void build_layout(int dim, int64_t n, const double* x, double cell,
   const int64_t nnz = L.row_ptr[n];
   L.col.resize(nnz);
   L.ref_off.resize(n);
-  for (int64_t a = 0; a < n; a++) {
-    int64_t i = L.perm[a];
-    L.ref_off[a] = rp_ref[i];
-    int64_t o = L.row_ptr[a];
-    for (int64_t k = rp_ref[i]; k < rp_ref[i + 1]; k++)
-      L.col[o++] = L.inv[col_ref[k]];
-  }
+  parallel_chunks(n, [&](int64_t a0, int64_t a1) {
+    for (int64_t a = a0; a < a1; a++) {
+      int64_t i = L.perm[a];
+      L.ref_off[a] = rp_ref[i];
+      int64_t o = L.row_ptr[a];
+      for (int64_t k = rp_ref[i]; k < rp_ref[i + 1]; k++)
+        L.col[o++] = L.inv[col_ref[k]];
+    }
+  });
 
   // --- slices ----------------------------------------------------------------
   const int64_t nslices = (n + 63) / 64;
@@ -106,43 +126,55 @@ void build_layout(int dim, int64_t n, const double* x, double cell,
     L.sl_ptr[s + 1] = L.sl_ptr[s] + 64 * w;
   }
   const int64_t nent = L.sl_ptr[nslices];
-  L.colS.assign(nent, 0);
-  for (int64_t s = 0; s < nslices; s++) {
-    int64_t w = (L.sl_ptr[s + 1] - L.sl_ptr[s]) / 64;
-    for (int l = 0; l < 64; l++) {
-      int64_t a = s * 64 + l;
-      int64_t len = a < n ? L.row_ptr[a + 1] - L.row_ptr[a] : 0;
-      int32_t self = a < n ? int32_t(a) : 0;
-      for (int64_t k = 0; k < w; k++)
-        L.colS[L.sl_ptr[s] + 64 * k + l] =
-            k < len ? L.col[L.row_ptr[a] + k] : self;
+  L.colS.resize(nent);
+  parallel_chunks(nslices, [&](int64_t s0, int64_t s1) {
+    for (int64_t s = s0; s < s1; s++) {
+      int64_t w = (L.sl_ptr[s + 1] - L.sl_ptr[s]) / 64;
+      for (int64_t k = 0; k < w; k++) {
+        for (int l = 0; l < 64; l++) {
+          int64_t a = s * 64 + l;
+          int64_t len = a < n ? L.row_ptr[a + 1] - L.row_ptr[a] : 0;
+          int32_t self = a < n ? int32_t(a) : 0;
+          L.colS[L.sl_ptr[s] + 64 * k + l] =
+              k < len ? L.col[L.row_ptr[a] + k] : self;
+        }
+      }
     }
-  }
+  });
 
   // --- halos of 256-node blocks ----------------------------------------------
   const int64_t nblocks = (nslices + 3) / 4;
   L.nblocks = nblocks;
   L.halo_ptr.assign(nblocks + 1, 0);
-  L.colL.assign(nent, 0);
-  L.halo.clear();
-  L.halo_max = 0;
-  std::vector<int32_t> h;
-  for (int64_t b = 0; b < nblocks; b++) {
-    int64_t s0 = 4 * b, s1 = std::min(nslices, s0 + 4);
-    int64_t e0 = L.sl_ptr[s0], e1 = L.sl_ptr[s1];
-    h.assign(L.colS.begin() + e0, L.colS.begin() + e1);
-    for (int64_t a = s0 * 64; a < std::min(n, s1 * 64); a++)
-      h.push_back(int32_t(a));
-    std::sort(h.begin(), h.end());
-    h.erase(std::unique(h.begin(), h.end()), h.end());
-    L.halo_max = std::max<int64_t>(L.halo_max, int64_t(h.size()));
-    for (int64_t e = e0; e < e1; e++) {
-      auto it = std::lower_bound(h.begin(), h.end(), L.colS[e]);
-      L.colL[e] = uint16_t(std::min<int64_t>(it - h.begin(), 65535));
+  L.colL.resize(nent);
+  std::vector<std::vector<int32_t>> hb(nblocks);
+  parallel_chunks(nblocks, [&](int64_t b0, int64_t b1) {
+    std::vector<int32_t> h;
+    for (int64_t b = b0; b < b1; b++) {
+      int64_t s0 = 4 * b, s1 = std::min(nslices, s0 + 4);
+      int64_t e0 = L.sl_ptr[s0], e1 = L.sl_ptr[s1];
+      h.assign(L.colS.begin() + e0, L.colS.begin() + e1);
+      for (int64_t a = s0 * 64; a < std::min(n, s1 * 64); a++)
+        h.push_back(int32_t(a));
+      std::sort(h.begin(), h.end());
+      h.erase(std::unique(h.begin(), h.end()), h.end());
+      for (int64_t e = e0; e < e1; e++) {
+        auto it = std::lower_bound(h.begin(), h.end(), L.colS[e]);
+        L.colL[e] = uint16_t(std::min<int64_t>(it - h.begin(), 65535));
+      }
+      hb[b] = h;
     }
-    L.halo.insert(L.halo.end(), h.begin(), h.end());
-    L.halo_ptr[b + 1] = int64_t(L.halo.size());
+  });
+  L.halo_max = 0;
+  for (int64_t b = 0; b < nblocks; b++) {
+    L.halo_max = std::max<int64_t>(L.halo_max, int64_t(hb[b].size()));
+    L.halo_ptr[b + 1] = L.halo_ptr[b] + int64_t(hb[b].size());
   }
+  L.halo.resize(L.halo_ptr[nblocks]);
+  parallel_chunks(nblocks, [&](int64_t b0, int64_t b1) {
+    for (int64_t b = b0; b < b1; b++)
+      std::copy(hb[b].begin(), hb[b].end(), L.halo.begin() + L.halo_ptr[b]);
+  });
 }
 
 }  // namespace nlm

@@ -170,3 +170,36 @@ extern "C" int nlm_rnp_params_from_deck(const nlm_material_deck* d,
   }
   return NLM_OK;
 }
+
+// ElasticState::ElasticState / computeParameters (ElasticState.cpp:17-158):
+// the checks where the reference exits become NLM_ERR_INVALID.
+extern "C" int nlm_es_params_from_deck(const nlm_es_deck* d,
+                                       nlm_es_params* o) {
+  if (!d || !o) return NLM_ERR_INVALID;
+  if (d->dim < 1 || d->dim > 3) return NLM_ERR_INVALID;
+  if (!(d->horizon > 0.) || !(d->h > 0.) || !(d->density > 0.))
+    return NLM_ERR_INVALID;
+  // 1D needs E (:61-66); 2D rejects E together with K and needs K and G
+  // (:70-91); K and G drive the 3D constants
+  if (d->dim == 1 && !(d->E > 0.)) return NLM_ERR_INVALID;
+  if (d->dim == 2 && d->E > 0. && d->K > 0.) return NLM_ERR_INVALID;
+  if (d->dim >= 2 && (!(d->K > 0.) || !(d->G > 0.))) return NLM_ERR_INVALID;
+  std::memset(o, 0, sizeof(*o));
+  o->dim = d->dim;
+  o->horizon = d->horizon;
+  o->h = d->h;
+  o->density = d->density;
+  o->E = d->E;
+  o->K = d->K;
+  o->G = d->G;
+  o->plane_strain = d->plane_strain ? 1 : 0;
+  o->critical_stretch = d->critical_stretch;
+  // MatData::toNuClassical (materialDeck.h:120), mu = G (:94-96)
+  o->nu = (3 * d->K - 2 * d->G) / (2 * (3 * d->K + d->G));
+  o->mu = d->G;
+  if (!o->plane_strain && d->dim == 2)
+    o->factor2D = (2. * o->nu - 1.) / (o->nu - 1.);
+  else
+    o->factor2D = 1;
+  return NLM_OK;
+}

@@ -81,6 +81,22 @@ struct RnpDev {
   float nbeta_log2e_f;     // -beta * log2(e): exp(-beta t) = exp2(t * this)
 };
 
+// ElasticState constants as used by the kernels.
+struct EsDev {
+  double K, mu, f2d;      // bulk and shear modulus, factor2D
+  double Kc;              // 2D: alpha_s = (9/m) Kc (ElasticState.cpp:201-212)
+  double horizon, h;
+  double check_low;       // horizon - h/2 (volumeCorrection.cpp:40-42)
+  double check_up;        // horizon + h/2
+  double rcp_h;
+  double low_band;        // check_low (1 + 1e-5): definitelyGreaterThan edge
+  double sc;              // critical stretch
+  double sc_break;        // sc / (1 - 1e-5): definitelyGreaterThan edge
+  int breaking;           // sc > 0
+  int pad;
+  float check_up_f, low_band_f, rcp_h_f, sc_break_f;
+};
+
 // Slice layout of the neighbor lists: 64 consecutive nodes (one wavefront)
 // share a slice; entry (k, lane) of slice s sits at col[sl_ptr[s] + 64k +
 // lane]; the broken bits of slot k of slice s form one 64-bit word
@@ -96,6 +112,7 @@ struct SellDev {
 
 struct FdState {
   bool ready = false;
+  int material = NLM_MATERIAL_RNP_BOND;
   int dim = 2;
   int64_t n = 0;
   int64_t nnz = 0;
@@ -109,6 +126,11 @@ struct FdState {
   bool has_nofail = false;
   nlm_rnp_params mat{};
   RnpDev rnp{};
+  nlm_es_params es_par{};
+  EsDev es{};
+  // state-based material: weighted volume m_i, dilatation theta_i and the
+  // per-node force coefficients of the last pass 1 (internal order)
+  DevBuf es_mw, es_theta, es_P, es_Q;
   // node data; 2D: double2, 3D: double4 (w unused)
   DevBuf x, vol, flags;
   DevBuf u[2], v[2], f;

@@ -13,8 +13,10 @@ import ctypes as C
 import numpy as np
 
 from . import capi
-from .capi import (NLM_PREC_FP64_EXACT, NLM_TIME_CENTRAL_DIFFERENCE, Bc, Crack,
-                   FdProblem, MaterialDeck, NlmError, RnpParams, ptr)
+from .capi import (NLM_MATERIAL_ELASTIC_STATE, NLM_MATERIAL_RNP_BOND,
+                   NLM_PREC_FP64_EXACT, NLM_TIME_CENTRAL_DIFFERENCE, Bc, Crack,
+                   EsDeck, EsParams, FdProblem, MaterialDeck, NlmError,
+                   RnpParams, ptr)
 from .problems import BC, CrackSpec, Problem
 
 
@@ -63,6 +65,31 @@ def rnp_params(p: Problem) -> RnpParams:
     return out
 
 
+def is_state_based(p: Problem) -> bool:
+    return p.material.get("model") == "elastic_state"
+
+
+def es_params(p: Problem) -> EsParams:
+    """ElasticState constants (nlm_es_params_from_deck)."""
+    lib = capi.load()
+    m = p.material
+    d = EsDeck()
+    d.dim = p.dim
+    d.horizon = p.horizon
+    d.h = p.h
+    d.density = m["density"]
+    d.E = m.get("E", -1.0)
+    d.K = m.get("K", -1.0)
+    d.G = m.get("G", -1.0)
+    d.plane_strain = int(m.get("plane_strain", 0))
+    d.critical_stretch = m.get("sc", -1.0)
+    out = EsParams()
+    rc = lib.nlm_es_params_from_deck(C.byref(d), C.byref(out))
+    if rc != 0:
+        raise NlmError(f"nlm_es_params_from_deck failed ({rc})")
+    return out
+
+
 def bc_struct(b: BC) -> Bc:
     s = Bc()
     s.region = capi.REGION[b.region]
@@ -93,7 +120,8 @@ def crack_struct(c: CrackSpec) -> Crack:
 
 
 class GpuFD:
-    """One FDModel<RNPBond> instance on the GPU."""
+    """One FDModel<T> instance on the GPU, T = RNPBond or (explicit)
+    ElasticState as the problem's material says."""
 
     def __init__(self, p: Problem, precision: int = NLM_PREC_FP64_EXACT,
                  scheme: int = NLM_TIME_CENTRAL_DIFFERENCE, device: int = 0,
@@ -131,8 +159,15 @@ class GpuFD:
         prob.dt = p.dt
         prob.precision = precision
         prob.time_scheme = scheme
-        prob.mat = rnp_params(p)
-        self.mat = prob.mat
+        if is_state_based(p):
+            prob.material = NLM_MATERIAL_ELASTIC_STATE
+            prob.es = es_params(p)
+            prob.mat.density = p.material["density"]
+            self.mat = prob.es
+        else:
+            prob.material = NLM_MATERIAL_RNP_BOND
+            prob.mat = rnp_params(p)
+            self.mat = prob.mat
         self._call("nlm_fd_setup", C.byref(prob))
         if p.ubc:
             arr = (Bc * len(p.ubc))(*[bc_struct(b) for b in p.ubc])
@@ -143,6 +178,8 @@ class GpuFD:
         if p.cracks:
             arr = (Crack * len(p.cracks))(*[crack_struct(c) for c in p.cracks])
             self._call("nlm_fd_add_cracks", len(p.cracks), arr)
+        if getattr(p, "v0", None) is not None:
+            self.set_state(v=p.v0)
 
     def _call(self, name, *args):
         rc = getattr(self.lib, name)(self.ctx, *args)
@@ -239,6 +276,14 @@ class GpuFD:
         s = C.c_int64()
         self._call("nlm_fd_get_time", C.byref(t), C.byref(s))
         return t.value, s.value
+
+    def node_fields(self):
+        """State-based material: (dilatation, weighted volume) per node."""
+        th = np.zeros(self.p.n)
+        mw = np.zeros(self.p.n)
+        self._call("nlm_fd_get_node_fields", ptr(th, C.c_double),
+                   ptr(mw, C.c_double))
+        return th, mw
 
     def postproc(self):
         n = self.p.n

@@ -229,7 +229,7 @@ def cube3d(nx: int, h: float, ratio: int, dt: float, nsteps: int,
 # 3D ElasticState block: side in nodes, mesh size, horizon/h, step, steps.
 CUBE_3D = {
     # 160^3 nodes, h = 6.25e-4 m, horizon 3h, 500 steps
-    "cube3d_4e6": dict(nx=160, h=6.25e-4, ratio=3, dt=2e-8, nsteps=500),
+    "cube3d_4e6": dict(nx=160, h=6.25e-4, ratio=3, dt=5e-8, nsteps=500),
 }
 
 

@@ -1013,6 +1013,33 @@ struct ES {
     }
   }
 
+  // per node sum_j |V_j T_ij| + |V_j T_ji| over unbroken bonds at the
+  // current u (dilatation recomputed, bond states untouched)
+  void force_abs_sum(double* out) {
+    std::vector<uint8_t> keep = broken;
+    double sc = m.sc;
+    m.sc = -1.;
+    dilatation();
+    m.sc = sc;
+    broken = keep;
+    for (int64_t i = 0; i < n; i++) {
+      double acc = 0.;
+      for (int64_t k = row_ptr[i]; k < row_ptr[i + 1]; k++) {
+        if (broken[k]) continue;
+        int64_t j = col[k];
+        P3 T = bond_force(i, k);
+        acc += length(vol[j] * T);
+        for (int64_t kk = row_ptr[j]; kk < row_ptr[j + 1]; kk++)
+          if (col[kk] == i) {
+            P3 T2 = bond_force(j, kk);
+            acc += length(vol[j] * T2);
+            break;
+          }
+      }
+      out[i] = acc;
+    }
+  }
+
   void apply_ubc(double t) {
     loader->u = u;
     loader->v = v;
@@ -1165,6 +1192,9 @@ void ora_es_get_node_fields(void* h, double* theta, double* mvol) {
   }
 }
 double ora_es_time(void* h) { return static_cast<ES*>(h)->time; }
+void ora_es_force_abs_sum(void* h, double* out) {
+  static_cast<ES*>(h)->force_abs_sum(out);
+}
 
 int ora_definitely_greater(double a, double b) { return dgt(a, b) ? 1 : 0; }
 int ora_definitely_less(double a, double b) { return dlt(a, b) ? 1 : 0; }

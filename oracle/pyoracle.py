@@ -145,6 +145,7 @@ SIGNATURES = {
     "ora_es_set_bonds": (None, [_P, _U8]),
     "ora_es_get_node_fields": (None, [_P, _D, _D]),
     "ora_es_time": (C.c_double, [_P]),
+    "ora_es_force_abs_sum": (None, [_P, _D]),
 }
 
 _lib = None
@@ -468,3 +469,8 @@ class OracleES:
 
     def time(self):
         return self.lib.ora_es_time(self.h)
+
+    def force_abs_sum(self):
+        out = np.zeros(self.p.n)
+        self.lib.ora_es_force_abs_sum(self.h, _p(out, C.c_double))
+        return out

@@ -68,3 +68,32 @@ def test_rnp_params_rejects_bad_deck():
     bad = pr.with_(p, material=dict(pr.rnp_material(), inf_type=7))
     assert lib.nlm_rnp_params_from_deck(C.byref(material_deck(bad)),
                                         C.byref(out)) != 0
+
+
+@pytest.mark.parametrize("case", ["qs2d", "qs1d", "cube", "plane_strain"])
+def test_es_params_equal_oracle(oracle, case):
+    from es_cases import qs1d, qs2d, small_cube
+    from nlmech_amd.engine import es_params
+    if case == "qs2d":
+        p, _ = qs2d()
+    elif case == "qs1d":
+        p, _ = qs1d()
+    elif case == "cube":
+        p = small_cube()
+    else:
+        p, _ = qs2d()
+        p = pr.with_(p, material=dict(p.material, plane_strain=1))
+    out = es_params(p)
+    ref = oracle.es_params(p)
+    for f, _ in capi.EsParams._fields_:
+        g = "sc" if f == "critical_stretch" else f
+        assert getattr(out, f) == getattr(ref, g), f
+
+
+def test_es_params_reject_bad_deck():
+    from es_cases import qs2d
+    from nlmech_amd.engine import es_params
+    p, _ = qs2d()
+    for bad in (dict(K=-1.0), dict(G=-1.0), dict(E=1e3)):
+        with pytest.raises(capi.NlmError):
+            es_params(pr.with_(p, material=dict(p.material, **bad)))

@@ -1,0 +1,230 @@
+"""GPU parity of the state-based (ElasticState) explicit path against the
+CPU oracle, through the C-ABI.
+
+Bars:
+  - weighted volume m_i and, in the FP64 exact mode, the dilatation theta_i:
+    bit-exact (same operation order, rows in list order);
+  - bond bits (critical-stretch breaks): bit-exact from an identical state;
+  - per-node force from an identical state:
+        FP64 modes  |f_gpu - f_ref|_inf <= 1e-12 * sum_j (|V_j T_ij| + |V_j T_ji|)
+        FP32-force  <= 2e-6 * the same scale
+    (the reference scatters f_j -= V_i T_ij from other rows, so only the
+    summation order differs in the exact mode);
+  - the quasi-static 2D golden's Force (= f V) at its displacement:
+        <= 1e-12 * max |Force| (exact and FP64 modes);
+  - trajectories: FP64 modes relative L2 displacement error <= 1e-9 and
+    identical broken-bond sets; FP32-force <= 1e-5 and < 0.1 % mismatch.
+"""
+import numpy as np
+import pytest
+
+from conftest import progress
+from es_cases import qs2d, small_cube
+from nlmech_amd import problems as pr
+from nlmech_amd.capi import (NLM_PREC_FP32_FORCE, NLM_PREC_FP64,
+                             NLM_PREC_FP64_EXACT)
+
+pytestmark = pytest.mark.gpu
+
+MODES = [NLM_PREC_FP64_EXACT, NLM_PREC_FP64, NLM_PREC_FP32_FORCE]
+NAMES = {NLM_PREC_FP64_EXACT: "exact", NLM_PREC_FP64: "fp64",
+         NLM_PREC_FP32_FORCE: "fp32"}
+FORCE_TOL = {NLM_PREC_FP64_EXACT: 1e-12, NLM_PREC_FP64: 1e-12,
+             NLM_PREC_FP32_FORCE: 2e-6}
+
+
+def rel_l2(a, b):
+    s = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / s if s > 0 else np.linalg.norm(a - b)
+
+
+def force_err(f_gpu, f_ref, scale):
+    d = np.abs(f_gpu - f_ref).max(axis=1)
+    return (d / np.where(scale > 0, scale, 1.0)).max()
+
+
+def oracle_forces(oracle, p, u, bonds=None):
+    o = oracle.OracleES(p)
+    if bonds is not None:
+        o.set_bonds(bonds)
+    z = np.zeros_like(u)
+    o.set_state(u=u, v=z, f=z)
+    o.compute_forces()
+    _, _, f = o.get_state()
+    th, mw = o.node_fields()
+    b = o.get_bonds()
+    scale = o.force_abs_sum()
+    csr = (o.row_ptr, o.col)
+    o.close()
+    return f, th, mw, b, scale, csr
+
+
+def gpu_forces(p, u, mode, csr, bonds=None):
+    from nlmech_amd.engine import GpuFD
+    g = GpuFD(p, precision=mode, csr=csr)
+    if bonds is not None:
+        g.set_bonds(bonds)
+    z = np.zeros_like(u)
+    g.set_state(u=u, v=z, f=z)
+    g.compute_forces()
+    _, _, f = g.get_state()
+    th, mw = g.node_fields()
+    b = g.get_bonds()
+    g.close()
+    return f, th, mw, b
+
+
+def random_state(p, amp, seed):
+    rng = np.random.default_rng(seed)
+    u = amp * rng.standard_normal((p.n, 3))
+    if p.dim == 2:
+        u[:, 2] = 0.0
+    return u
+
+
+@pytest.mark.parametrize("mode", MODES, ids=lambda m: NAMES[m])
+def test_es_forces_3d_identical_state(oracle, mode):
+    progress(f"[gpu] es forces 3d {NAMES[mode]}")
+    p = small_cube(n=9, sc=1e-3, jitter=0.05)
+    # stretch plus noise: some bonds cross the critical stretch
+    u = 0.9e-3 * p.x + random_state(p, 0.2e-3 * p.h, 1)
+    f_ref, th_ref, mw_ref, b_ref, scale, csr = oracle_forces(oracle, p, u)
+    assert 0 < b_ref.sum() < b_ref.size
+    f, th, mw, b = gpu_forces(p, u, mode, csr)
+    assert np.array_equal(mw, mw_ref)
+    assert np.array_equal(b, b_ref)
+    if mode == NLM_PREC_FP64_EXACT:
+        assert np.array_equal(th, th_ref)
+    else:
+        assert np.abs(th - th_ref).max() <= 1e-5 * np.abs(th_ref).max()
+    err = force_err(f, f_ref, scale)
+    assert err <= FORCE_TOL[mode], err
+
+
+@pytest.mark.parametrize("mode", MODES, ids=lambda m: NAMES[m])
+def test_es_forces_2d_identical_state(oracle, mode):
+    progress(f"[gpu] es forces 2d {NAMES[mode]}")
+    p, _ = qs2d()
+    p = pr.with_(p, material=dict(p.material, sc=2e-3))
+    u = 1.5e-3 * p.x + random_state(p, 1e-3 * p.h, 2)
+    f_ref, th_ref, mw_ref, b_ref, scale, csr = oracle_forces(oracle, p, u)
+    assert b_ref.sum() > 0
+    f, th, mw, b = gpu_forces(p, u, mode, csr)
+    assert np.array_equal(mw, mw_ref)
+    assert np.array_equal(b, b_ref)
+    if mode == NLM_PREC_FP64_EXACT:
+        assert np.array_equal(th, th_ref)
+    err = force_err(f, f_ref, scale)
+    assert err <= FORCE_TOL[mode], err
+
+
+@pytest.mark.parametrize("mode", [NLM_PREC_FP64_EXACT, NLM_PREC_FP64],
+                         ids=lambda m: NAMES[m])
+def test_es_qs2d_golden_force_gpu(oracle, mode):
+    progress(f"[gpu] es qs2d golden {NAMES[mode]}")
+    p, g = qs2d()
+    csr = oracle.neighbors(p.x, p.horizon, "brute")
+    f, _, _, _ = gpu_forces(p, g["Displacement"], mode, csr)
+    F = f * g["Node_Volume"][:, None]
+    err = np.abs(F - g["Force"]).max() / np.abs(g["Force"]).max()
+    assert err <= 1e-12, err
+
+
+@pytest.mark.parametrize("mode", MODES, ids=lambda m: NAMES[m])
+def test_es_trajectory_cube(oracle, mode):
+    progress(f"[gpu] es trajectory cube {NAMES[mode]}")
+    from nlmech_amd.engine import GpuFD
+    p = small_cube(n=10, dt=5e-8, pull=1.0, sigma_v=1e-3)
+    steps = 60
+    o = oracle.OracleES(p)
+    o.init()
+    o.step(steps)
+    u_ref, v_ref, _ = o.get_state()
+    b_ref = o.get_bonds()
+    csr = (o.row_ptr, o.col)
+    o.close()
+    assert 0 < b_ref.sum() < b_ref.size
+    g = GpuFD(p, precision=mode, csr=csr)
+    g.init()
+    g.step(steps // 2)
+    g.step(steps - steps // 2)
+    u, v, _ = g.get_state()
+    b = g.get_bonds()
+    t, n = g.time()
+    g.close()
+    assert n == steps
+    assert t == pytest.approx(steps * p.dt, rel=1e-12)
+    e = rel_l2(u, u_ref)
+    mism = np.count_nonzero(b != b_ref)
+    if mode == NLM_PREC_FP32_FORCE:
+        assert e <= 1e-5, e
+        assert mism <= 1e-3 * b_ref.sum(), mism
+    else:
+        assert e <= 1e-9, e
+        assert mism == 0, mism
+
+
+def test_es_trajectory_plate_2d(oracle):
+    progress("[gpu] es trajectory 2d plate exact")
+    from nlmech_amd.engine import GpuFD
+    base = pr.plate2d(30, 30, 1e-3, 3, 2e-7, 40, notch_len=0.0, pull=2.0)
+    mat = pr.es_material(dim=2, K=2.16e9, G=1.30e9, sc=2e-4)
+    p = pr.with_(base, material=mat)
+    o = oracle.OracleES(p)
+    o.init()
+    o.step(40)
+    u_ref, _, _ = o.get_state()
+    b_ref = o.get_bonds()
+    csr = (o.row_ptr, o.col)
+    o.close()
+    g = GpuFD(p, precision=NLM_PREC_FP64_EXACT, csr=csr)
+    g.init()
+    g.step(40)
+    u, _, _ = g.get_state()
+    b = g.get_bonds()
+    g.close()
+    assert b_ref.sum() > 0
+    assert rel_l2(u, u_ref) <= 1e-9
+    assert np.array_equal(b, b_ref)
+
+
+def test_es_autotune_keeps_state(oracle):
+    progress("[gpu] es autotune invariance")
+    from nlmech_amd.engine import GpuFD
+    p = small_cube(n=12, dt=5e-8, pull=1.0)
+    runs = []
+    for tune in (False, True):
+        g = GpuFD(p, precision=NLM_PREC_FP32_FORCE)
+        g.init()
+        g.step(5)
+        if tune:
+            U, us = g.autotune()
+            assert U in (2, 4, 8) and us > 0
+        g.step(10)
+        u, v, f = g.get_state()
+        th, _ = g.node_fields()
+        runs.append((u, v, f, g.get_bonds(), th))
+        g.close()
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)
+
+
+def test_es_cube_full_size_properties():
+    """160^3 block (4.1e6 nodes): Newton's third law of the internal force
+    and the closed-form dilatation of a homogeneous stretch."""
+    progress("[gpu] es cube 160^3 properties")
+    from nlmech_amd.engine import GpuFD
+    p = pr.workload("cube3d_4e6")
+    g = GpuFD(p, precision=NLM_PREC_FP64)
+    assert g.nnz == 369653064
+    s = 2e-4
+    z = np.zeros((p.n, 3))
+    g.set_state(u=s * p.x, v=z, f=z)
+    g.compute_forces()
+    th, mw = g.node_fields()
+    assert np.abs(th - 3 * s).max() <= 1e-12 * 3 * s
+    _, _, f = g.get_state()
+    tot = (p.vol[:, None] * f).sum(0)
+    scale = (p.vol[:, None] * np.abs(f)).sum()
+    assert np.abs(tot).max() <= 1e-10 * scale
+    g.close()
