@@ -14,6 +14,11 @@ kernel's roofline from HIP events on the stepping stream, the displacement
 error and broken-bond mismatch of both modes against the FP64 exact GPU mode
 after the same steps, and the CPU oracle (OpenMP restatement) timed on the
 host on a bounded sample of steps of the same plate.
+
+Beside it ("state_based_3d"): the 160^3-node ElasticState block (horizon 3h,
+369,653,064 bonds, z faces pulled at +/-1 m/s, seeded Gaussian initial
+velocity), 500 steps of the two-pass dilatation/force step in the same
+modes.
 """
 from __future__ import annotations
 
@@ -35,6 +40,13 @@ HBM_PEAK_GBS = 8000.0
 # bond: column index 4 B + broken bit 1/8 B = 4.125 B).
 BYTES_PER_NODE = 93.0
 BYTES_PER_BOND = 4.125
+# State-based (3D) step = dilatation pass + force pass.  Pass 1 per node:
+# read x, u (2x24 B), weighted volume 8 B, write theta and the force
+# coefficient (2x8 B) = 72 B; pass 2 per node: read x, u, v (3x24 B), the
+# coefficient 8 B, flags 4 B, write u', v' (2x24 B) = 132 B; per bond and
+# pass: column index 4 B + broken bit 1/8 B.
+ES_BYTES_PER_NODE = 72.0 + 132.0
+ES_BYTES_PER_BOND = 2 * 4.125
 
 
 def parse():
@@ -50,6 +62,10 @@ def parse():
                     help="skip the FP64-exact comparison run (profiling)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device copy (counter scale)")
+    ap.add_argument("--es-workload", default="cube3d_4e6",
+                    help="state-based workload reported beside the plate")
+    ap.add_argument("--es-steps", type=int, default=500)
+    ap.add_argument("--no-es", action="store_true")
     return ap.parse_args()
 
 
@@ -82,6 +98,64 @@ def run_mode(p, precision, steps, warmup):
     nnz = g.nnz
     return g, dict(ms_total=ms_total, ms_fused=ms_fused, wall=wall, u=u,
                    bonds=bonds, nnz=nnz, unroll=unroll)
+
+
+def state_based(a):
+    """The 3D ElasticState block: FP32-force and FP64 plain throughput, the
+    two-pass step against HBM, and both modes' error against the FP64 exact
+    mode after the same steps."""
+    from nlmech_amd import problems as pr
+    from nlmech_amd.capi import (NLM_PREC_FP32_FORCE, NLM_PREC_FP64,
+                                 NLM_PREC_FP64_EXACT)
+    p = pr.workload(a.es_workload)
+    steps = a.es_steps
+    res = {}
+    csr = None
+    for name, mode in (("fp32", NLM_PREC_FP32_FORCE), ("fp64", NLM_PREC_FP64),
+                       ("exact", NLM_PREC_FP64_EXACT)):
+        if name == "exact" and a.no_accuracy:
+            continue
+        from nlmech_amd.engine import GpuFD
+        g = GpuFD(p, precision=mode, csr=csr)
+        if csr is None:
+            csr = g.neighbors()
+        g.init()
+        unroll, _ = g.autotune()
+        g.step(5)
+        ms_total, ms_pair = g.timed_steps(steps)
+        u, _, _ = g.get_state()
+        res[name] = dict(ms_total=ms_total, ms_pair=ms_pair, u=u,
+                         bonds=g.get_bonds(), nnz=g.nnz, unroll=unroll)
+        g.close()
+        print(f"[bench] state-based {name}: {ms_total / steps * 1e3:.1f} us "
+              f"per step", file=sys.stderr, flush=True)
+    nnz = res["fp32"]["nnz"]
+    bytes_step = ES_BYTES_PER_NODE * p.n + ES_BYTES_PER_BOND * nnz
+    out = {"workload": a.es_workload, "nodes": p.n, "bonds": nnz,
+           "steps": steps, "dt": p.dt, "modes": {}}
+    for name, r in res.items():
+        ach = bytes_step / (r["ms_pair"] * 1e-3) / 1e9
+        out["modes"][name] = {
+            "value": nnz * steps / (r["ms_total"] * 1e-3),
+            "unit": "bond updates/s",
+            "ms_per_step": r["ms_total"] / steps,
+            "pass_pair_us": r["ms_pair"] * 1e3,
+            "roofline_frac": ach / HBM_PEAK_GBS,
+            "autotuned_unroll": r["unroll"]}
+    if "exact" in res:
+        uref, bref = res["exact"]["u"], res["exact"]["bonds"]
+        out["accuracy_vs_fp64_exact"] = {
+            name: {"l2_rel": float(np.linalg.norm(r["u"] - uref) /
+                                   np.linalg.norm(uref)),
+                   "linf_rel": float(np.abs(r["u"] - uref).max() /
+                                     np.abs(uref).max()),
+                   "broken_bond_mismatch": int(np.count_nonzero(
+                       r["bonds"] != bref)),
+                   "broken_bonds_ref": int(bref.sum())}
+            for name, r in res.items() if name != "exact"}
+    out["fp32_over_fp64"] = (out["modes"]["fp32"]["value"] /
+                             out["modes"]["fp64"]["value"])
+    return out
 
 
 def cpu_baseline(p, csr, seconds):
@@ -206,6 +280,8 @@ def main():
     if "fp32" in res and "fp64" in res:
         out["fp32_over_fp64"] = (out["modes"]["fp32"]["value"] /
                                  out["modes"]["fp64"]["value"])
+    if not a.no_es:
+        out["state_based_3d"] = state_based(a)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, csr, a.cpu_seconds)
     else:

@@ -42,6 +42,46 @@ struct EsArgs {
   EsDev m;
 };
 
+// One halo node staged in LDS by the tiled passes (3D): reference position,
+// displacement, volume and the force coefficient (pass 2) in one 64-byte
+// record.
+struct alignas(16) EsRec {
+  double x0, x1, x2;
+  double u0, u1, u2;
+  double vol, P;
+};
+
+// Stages the halo of layout block lb (every node its four slices reference)
+// in LDS; WITH_P also copies the force coefficient of pass 1.
+template <bool WITH_P>
+__device__ __forceinline__ void es_stage_halo(const FdArgs<3>& a,
+                                              const double* __restrict__ P,
+                                              int64_t lb, EsRec* smem) {
+  const int64_t hb = a.halo_ptr[lb];
+  const int hn = int(a.halo_ptr[lb + 1] - hb);
+  for (int t = threadIdx.x; t < hn; t += 256) {
+    const uint32_t g = uint32_t(a.halo[hb + t]);
+    const double4 xx = ld32(a.x, g);
+    const double4 uu = ld32(a.u, g);
+    EsRec r;
+    r.x0 = xx.x;
+    r.x1 = xx.y;
+    r.x2 = xx.z;
+    r.u0 = uu.x;
+    r.u1 = uu.y;
+    r.u2 = uu.z;
+    r.vol = ld32(a.vol, g);
+    r.P = WITH_P ? ld32(P, g) : 0.;
+    smem[t] = r;
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ const EsRec& es_rec(const EsRec* smem, uint32_t j) {
+  return *reinterpret_cast<const EsRec*>(reinterpret_cast<const char*>(smem) +
+                                         j * uint32_t(sizeof(EsRec)));
+}
+
 // m_i = sum_j |X| |X| c_ij V_j in list order (weightedVolume, :51-67), over
 // the internal CSR whose rows keep the reference order.
 template <int DIM>
@@ -68,13 +108,18 @@ __global__ __launch_bounds__(256) void k_es_mw(FdArgs<DIM> a,
 }
 
 // Pass 1: break test, dilatation, force coefficients.
-template <int DIM, int MODE, int U>
+// TILED (3D): the block's halo is staged in LDS and neighbors are read
+// through 16-bit halo-local indices.
+template <int DIM, int MODE, int U, bool TILED>
 __global__ __launch_bounds__(256) void k_es_dil(
     EsArgs<DIM> e, const unsigned long long* __restrict__ bits_ro) {
+  static_assert(!TILED || DIM == 3, "the LDS-staged passes are 3D");
   const FdArgs<DIM>& a = e.f;
   const int64_t nblk = (a.nslices * 64 + 255) / 256;
   const int64_t lb = xcd_block(nblk);
   if (lb >= nblk) return;
+  extern __shared__ EsRec es_smem[];
+  if constexpr (TILED) es_stage_halo<false>(a, e.P, lb, es_smem);
   const int lane = threadIdx.x & 63;
   const int64_t s = lb * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (s >= a.nslices) return;  // whole wavefront
@@ -96,11 +141,13 @@ __global__ __launch_bounds__(256) void k_es_dil(
   const double c3 = DIM == 3 ? 3. / mwi : (2. / mwi) * e.m.f2d;
   const float c3f = float(c3);
   const int32_t* __restrict__ colS = a.col + cb;
+  const uint16_t* __restrict__ colL = a.colL + cb;
   const unsigned long long* __restrict__ bits_in = bits_ro + bb;
   const int wlast = width > 0 ? width - 1 : 0;
   auto slot_index = [&](int k) -> uint32_t {
     const uint32_t en = uint32_t(64 * (k < wlast ? k : wlast) + lane);
-    return uint32_t(ld32(colS, en));
+    if constexpr (TILED) return uint32_t(ld32(colL, en));
+    else return uint32_t(ld32(colS, en));
   };
   double th = 0.;
   uint32_t jn[U];
@@ -117,9 +164,20 @@ __global__ __launch_bounds__(256) void k_es_dil(
     unsigned long long w[U];
 #pragma unroll
     for (int q = 0; q < U; ++q) {
-      load3<DIM>(ld32(a.x, jc[q]), xj[q]);
-      load3<DIM>(ld32(a.u, jc[q]), uj[q]);
-      vj[q] = ld32(a.vol, jc[q]);
+      if constexpr (TILED) {
+        const EsRec& r = es_rec(es_smem, jc[q]);
+        xj[q][0] = r.x0;
+        xj[q][1] = r.x1;
+        xj[q][2] = r.x2;
+        uj[q][0] = r.u0;
+        uj[q][1] = r.u1;
+        uj[q][2] = r.u2;
+        vj[q] = r.vol;
+      } else {
+        load3<DIM>(ld32(a.x, jc[q]), xj[q]);
+        load3<DIM>(ld32(a.u, jc[q]), uj[q]);
+        vj[q] = ld32(a.vol, jc[q]);
+      }
       const int kq = k0 + q;
       w[q] = bits_in[kq < wlast ? kq : wlast];
     }
@@ -161,7 +219,6 @@ __global__ __launch_bounds__(256) void k_es_dil(
     // EXACT keeps alpha_s = (9/m) K (ElasticState.cpp:230-233); the fast
     // modes fold t_ij = alpha_s theta_i |X| / 3 into P_i |X|.
     e.P[i] = MODE == kModeExact ? (9. / mwi) * e.m.K : 3. * e.m.K * th / mwi;
-    e.Q[i] = 0.;
   } else {
     const double as = (9. / mwi) * e.m.Kc;
     const double ad = (8. / mwi) * e.m.mu;
@@ -174,14 +231,18 @@ __global__ __launch_bounds__(256) void k_es_dil(
 
 // Pass 2: bond forces of both directions gathered at node i, force BCs,
 // central-difference update.
-template <int DIM, int MODE, int KIND, bool BASE_F, int U>
+template <int DIM, int MODE, int KIND, bool BASE_F, int U, bool TILED>
 __global__ __launch_bounds__(256) void k_es_force(
     EsArgs<DIM> e, const unsigned long long* __restrict__ bits_ro,
     double t_force, double t_next) {
+  static_assert(!TILED || (DIM == 3 && MODE != kModeExact),
+                "the LDS-staged force pass is 3D, fast modes");
   const FdArgs<DIM>& a = e.f;
   const int64_t nblk = (a.nslices * 64 + 255) / 256;
   const int64_t lb = xcd_block(nblk);
   if (lb >= nblk) return;
+  extern __shared__ EsRec es_smem[];
+  if constexpr (TILED) es_stage_halo<true>(a, e.P, lb, es_smem);
   const int lane = threadIdx.x & 63;
   const int64_t s = lb * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (s >= a.nslices) return;
@@ -206,11 +267,13 @@ __global__ __launch_bounds__(256) void k_es_force(
   }
   const float Pif = float(Pi), Qif = float(Qi);
   const int32_t* __restrict__ colS = a.col + cb;
+  const uint16_t* __restrict__ colL = a.colL + cb;
   const unsigned long long* __restrict__ bits_in = bits_ro + bb;
   const int wlast = width > 0 ? width - 1 : 0;
   auto slot_index = [&](int k) -> uint32_t {
     const uint32_t en = uint32_t(64 * (k < wlast ? k : wlast) + lane);
-    return uint32_t(ld32(colS, en));
+    if constexpr (TILED) return uint32_t(ld32(colL, en));
+    else return uint32_t(ld32(colS, en));
   };
   double acc[3] = {0., 0., 0.}, oth[3] = {0., 0., 0.};
   uint32_t jn[U];
@@ -228,12 +291,26 @@ __global__ __launch_bounds__(256) void k_es_force(
 #pragma unroll
     for (int q = 0; q < U; ++q) {
       const uint32_t j = jc[q];
-      load3<DIM>(ld32(a.x, j), xj[q]);
-      load3<DIM>(ld32(a.u, j), uj[q]);
-      vj[q] = ld32(a.vol, j);
-      Pj[q] = ld32(e.P, j);
-      Qj[q] = kNeedQ ? ld32(e.Q, j) : 0.;
-      thj[q] = kExact ? ld32(e.theta, j) : 0.;
+      if constexpr (TILED) {
+        const EsRec& r = es_rec(es_smem, j);
+        xj[q][0] = r.x0;
+        xj[q][1] = r.x1;
+        xj[q][2] = r.x2;
+        uj[q][0] = r.u0;
+        uj[q][1] = r.u1;
+        uj[q][2] = r.u2;
+        vj[q] = r.vol;
+        Pj[q] = r.P;
+        Qj[q] = 0.;
+        thj[q] = 0.;
+      } else {
+        load3<DIM>(ld32(a.x, j), xj[q]);
+        load3<DIM>(ld32(a.u, j), uj[q]);
+        vj[q] = ld32(a.vol, j);
+        Pj[q] = ld32(e.P, j);
+        Qj[q] = kNeedQ ? ld32(e.Q, j) : 0.;
+        thj[q] = kExact ? ld32(e.theta, j) : 0.;
+      }
       const int kq = k0 + q;
       w[q] = bits_in[kq < wlast ? kq : wlast];
     }
@@ -350,17 +427,54 @@ EsArgs<DIM> make_es_args(nlm_ctx* c) {
   return e;
 }
 
+// LDS bytes of a tiled launch: one record per halo entry.
+size_t es_lds_bytes(const FdState& s) {
+  return size_t(s.halo_max + 1) * sizeof(EsRec);
+}
+
+// Raises a kernel's dynamic-LDS limit to what the launch needs (the first
+// time the kernel is launched with that size or more).
+template <class K>
+void allow_lds(K kernel, size_t bytes) {
+  static size_t granted = 0;
+  if (bytes > granted) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              int(bytes));
+    granted = bytes;
+  }
+}
+
 template <int DIM, int MODE>
 void launch_dil(nlm_ctx* c, const EsArgs<DIM>& e) {
   unsigned g = grid_xcd(force_blocks(c->fd) * 256, 256);
-  hipLaunchKernelGGL((k_es_dil<DIM, MODE, 4>), dim3(g), dim3(256), 0,
+  if constexpr (DIM == 3) {
+    if (c->fd.es_tiled && c->fd.use_tiled) {
+      const size_t lds = es_lds_bytes(c->fd);
+      allow_lds(k_es_dil<DIM, MODE, 4, true>, lds);
+      hipLaunchKernelGGL((k_es_dil<DIM, MODE, 4, true>), dim3(g), dim3(256),
+                         lds, c->stream, e, e.f.bits);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_es_dil<DIM, MODE, 4, false>), dim3(g), dim3(256), 0,
                      c->stream, e, e.f.bits);
 }
 
 template <int DIM, int MODE, int KIND, bool BASE_F, int U>
 void launch_force_k(nlm_ctx* c, const EsArgs<DIM>& e, double tf, double tn) {
   unsigned g = grid_xcd(force_blocks(c->fd) * 256, 256);
-  hipLaunchKernelGGL((k_es_force<DIM, MODE, KIND, BASE_F, U>), dim3(g),
+  if constexpr (DIM == 3 && MODE != kModeExact) {
+    if (c->fd.es_tiled && c->fd.use_tiled) {
+      const size_t lds = es_lds_bytes(c->fd);
+      allow_lds(k_es_force<DIM, MODE, KIND, BASE_F, U, true>, lds);
+      hipLaunchKernelGGL((k_es_force<DIM, MODE, KIND, BASE_F, U, true>),
+                         dim3(g), dim3(256), lds, c->stream, e, e.f.bits, tf,
+                         tn);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_es_force<DIM, MODE, KIND, BASE_F, U, false>), dim3(g),
                      dim3(256), 0, c->stream, e, e.f.bits, tf, tn);
 }
 
@@ -422,6 +536,13 @@ int es_prepare(nlm_ctx* c) {
   NLM_HIP(c, hipMemsetAsync(s.es_theta.p, 0, nb, c->stream));
   NLM_HIP(c, hipMemsetAsync(s.es_P.p, 0, nb, c->stream));
   NLM_HIP(c, hipMemsetAsync(s.es_Q.p, 0, nb, c->stream));
+  // LDS-staged passes: 3D, when one block's halo fits the device's LDS
+  int lds_max = 0;
+  NLM_HIP(c, hipDeviceGetAttribute(&lds_max,
+                                   hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                   c->device));
+  s.es_tiled = s.dim == 3 && s.halo_max > 0 &&
+               es_lds_bytes(s) <= size_t(lds_max);
   const unsigned g = unsigned((s.n + 255) / 256);
   if (s.dim == 2)
     hipLaunchKernelGGL(k_es_mw<2>, dim3(g), dim3(256), 0, c->stream,

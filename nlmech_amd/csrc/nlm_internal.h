@@ -146,6 +146,7 @@ struct FdState {
   DevBuf colL;      // uint16 halo-local slice entries
   int64_t halo_max = 0;
   bool tiled = false;   // LDS-staged step kernel usable
+  bool es_tiled = false;  // LDS-staged state-based passes usable
   bool use_tiled = true;
   DevBuf bctab;  // BcTable
   BcTable bc_host{};
