@@ -303,6 +303,113 @@ int nlm_fd_num_bonds(nlm_ctx* ctx, int64_t* nnz);
 /* Name of the fused step kernel launched for the configured model. */
 const char* nlm_fd_kernel_name(nlm_ctx* ctx);
 
+/* ------------------------------------------------------------------------ */
+/* Quasi-static model: QuasiStaticModel<T> (src/model/quasistatic/)          */
+/* ------------------------------------------------------------------------ */
+
+/* The quasi-static solver runs on a context set up with nlm_fd_setup (2D,
+ * bond-based material) and its displacement / force BC sets.  Load step n
+ * is t = t0 + n dt: FLoading gives the body force b(t), ULoading the
+ * prescribed displacements, and Newton iterations solve R(u) = f_int(u) + b
+ * = 0 on the free dofs (QuasiStaticModel::solver, QuasiStaticModel.cpp:
+ * 527-592; computeResidual :597-646).  The tangent is the analytic derivative
+ * of the bond force (on the neighbor sparsity, 2x2 blocks); each Newton
+ * system is solved by BiCGSTAB to a relative residual. */
+#define NLM_QS_TANGENT_FULL 0      /* exact derivative, all 2x2 entries */
+#define NLM_QS_TANGENT_COMPONENT 1 /* same-component entries only, the
+                                      entries the reference's central-
+                                      difference Jacobian keeps (:436-440) */
+#define NLM_QS_PRECOND_NONE 0
+#define NLM_QS_PRECOND_JACOBI 1  /* point Jacobi */
+#define NLM_QS_PRECOND_BLOCKS 2  /* sparse approximate inverse with 2x2
+                                    blocks on the neighbor sparsity, set by
+                                    nlm_qs_set_precond or predicted by the
+                                    graph network (nlm_gnn_*) */
+
+typedef struct {
+  int nsteps;         /* load steps (ModelDeck Time_Steps) */
+  double dt;          /* load-step increment of t (Final_Time / Time_Steps) */
+  double tol_abs;     /* Newton stops when ||R||_2 <= tol_abs ... */
+  double tol_rel;     /* ... or ||R||_2 <= tol_rel ||R_0|| (0: unused) */
+  int max_newton;     /* SolverDeck Max_Iteration */
+  int tangent;        /* NLM_QS_TANGENT_* */
+  int precond;        /* NLM_QS_PRECOND_* */
+  double lin_tol;     /* BiCGSTAB: ||r|| <= lin_tol ||b|| */
+  int lin_max_iter;
+  int predictor;      /* 1: linearized first iteration with the prescribed
+                         increment as Dirichlet data (no force evaluation at
+                         the jumped state); 0: the reference's order */
+  int gnn_refresh;    /* NLM_QS_PRECOND_BLOCKS with a graph network: 1
+                         predict the preconditioner at every Newton
+                         iteration, 0 once per load step */
+} nlm_qs_options;
+
+typedef struct {
+  int load_steps;
+  int newton_iters;
+  int linear_iters;
+  int linear_failures;   /* solves that hit lin_max_iter or broke down */
+  double residual;       /* ||R||_2 after the last load step */
+  double residual0;      /* ||R_0|| of the last load step */
+  double ms_total;
+  double ms_linear;      /* BiCGSTAB iterations */
+  double ms_precond;     /* preconditioner construction (incl. network) */
+  double ms_tangent;     /* tangent assembly */
+} nlm_qs_stats;
+
+int nlm_qs_solve(nlm_ctx* ctx, const nlm_qs_options* opt, nlm_qs_stats* st);
+/* Tangent A = -df_int/du at the current u and bond state (no force
+ * evaluation); tangent = NLM_QS_TANGENT_*. */
+int nlm_qs_assemble(nlm_ctx* ctx, int tangent);
+/* The assembled blocks: blocks[4 k .. 4 k + 3] = A_{i,j} (row-major 2x2) for
+ * directed bond k = (i, j) in the reference CSR order, diag[4 i ..] = A_ii.
+ * Either pointer may be NULL. */
+int nlm_qs_get_tangent(nlm_ctx* ctx, double* blocks, double* diag);
+/* y = A_ff x on the free dofs (x, y AoS double[3n]; entries of prescribed
+ * dofs of y are 0 and those of x are ignored). */
+int nlm_qs_apply(nlm_ctx* ctx, const double* x, double* y);
+/* Preconditioner blocks in the same layout as nlm_qs_get_tangent. */
+int nlm_qs_set_precond(nlm_ctx* ctx, const double* blocks, const double* diag);
+int nlm_qs_get_precond(nlm_ctx* ctx, double* blocks, double* diag);
+/* Solves A_ff x = b with the assembled tangent (b, x AoS double[3n]). */
+int nlm_qs_linear_solve(nlm_ctx* ctx, const double* b, double* x,
+                        int precond, double tol, int max_iter, int* iters,
+                        double* relres);
+
+/* ------------------------------------------------------------------------ */
+/* Learned preconditioner (no counterpart in the reference)                  */
+/* ------------------------------------------------------------------------ */
+
+/* A graph network over the neighbor graph predicts the blocks of a sparse
+ * approximate inverse M ~ A_ff^-1 on the tangent's sparsity (one 2x2 block
+ * per directed bond and per node).  Inputs: the tangent blocks scaled by the
+ * nodes' diagonal magnitudes, bond geometry, fixity.  Four message-passing
+ * layers of width 64: m_ij = relu(g_ij Wc + h_i Wa + h_j Wb + bm), g <- m,
+ * h_i <- relu(h_i Wd + mean_j(m_ij) Wf + bu); linear decoders give the
+ * edge and node blocks.  Weights are BF16 copies of FP32 master weights;
+ * every layer product runs on the matrix cores (BF16 inputs, FP32
+ * accumulation), forward and backward.  Training minimizes
+ * sum_k ||M r_k - e_k||^2 / ||e_k||^2 over residual / error pairs of the
+ * tangent (A e_k = r_k) with Adam. */
+#define NLM_GNN_LAYERS 4
+#define NLM_GNN_WIDTH 64
+/* Number of FP32 parameters of the network. */
+int64_t nlm_gnn_num_params(void);
+/* Attaches a network with deterministic initial weights to the context
+ * (after nlm_fd_setup); NLM_QS_PRECOND_BLOCKS then uses its prediction. */
+int nlm_gnn_create(nlm_ctx* ctx, uint64_t seed);
+int nlm_gnn_get_weights(nlm_ctx* ctx, float* w);
+int nlm_gnn_set_weights(nlm_ctx* ctx, const float* w);
+/* Predicts the preconditioner blocks from the assembled tangent. */
+int nlm_gnn_predict(nlm_ctx* ctx);
+/* Loss and gradient for K pairs (r_k, e_k), AoS double[K][3n] each, on the
+ * assembled tangent; the gradient is added to the context's accumulator and
+ * optionally copied to grad (FP32, nlm_gnn_num_params entries). */
+int nlm_gnn_loss_grad(nlm_ctx* ctx, int K, const double* r, const double* e,
+                      double* loss, float* grad);
+/* Adam step with the accumulated gradient (then cleared). */
+int nlm_gnn_adam_step(nlm_ctx* ctx, double lr);
+
 #ifdef __cplusplus
 }
 #endif

@@ -21,6 +21,11 @@ NLM_TIME_CENTRAL_DIFFERENCE = 0
 NLM_TIME_VELOCITY_VERLET = 1
 NLM_MATERIAL_RNP_BOND = 0
 NLM_MATERIAL_ELASTIC_STATE = 1
+NLM_QS_TANGENT_FULL = 0
+NLM_QS_TANGENT_COMPONENT = 1
+NLM_QS_PRECOND_NONE = 0
+NLM_QS_PRECOND_JACOBI = 1
+NLM_QS_PRECOND_BLOCKS = 2
 
 REGION = {"rectangle": 0, "angled_rectangle": 1, "circle": 2, "torus": 3,
           "line": 4, "cuboid": 5}
@@ -106,6 +111,26 @@ class Crack(C.Structure):
     ]
 
 
+class QsOptions(C.Structure):
+    _fields_ = [
+        ("nsteps", C.c_int), ("dt", C.c_double), ("tol_abs", C.c_double),
+        ("tol_rel", C.c_double), ("max_newton", C.c_int),
+        ("tangent", C.c_int), ("precond", C.c_int), ("lin_tol", C.c_double),
+        ("lin_max_iter", C.c_int), ("predictor", C.c_int),
+        ("gnn_refresh", C.c_int),
+    ]
+
+
+class QsStats(C.Structure):
+    _fields_ = [
+        ("load_steps", C.c_int), ("newton_iters", C.c_int),
+        ("linear_iters", C.c_int), ("linear_failures", C.c_int),
+        ("residual", C.c_double), ("residual0", C.c_double),
+        ("ms_total", C.c_double), ("ms_linear", C.c_double),
+        ("ms_precond", C.c_double), ("ms_tangent", C.c_double),
+    ]
+
+
 # exported symbol -> (restype, argtypes)
 _P = C.c_void_p
 _D = C.POINTER(C.c_double)
@@ -149,6 +174,21 @@ SIGNATURES = {
     "nlm_es_params_from_deck": (C.c_int, [C.POINTER(EsDeck),
                                           C.POINTER(EsParams)]),
     "nlm_fd_get_node_fields": (C.c_int, [_P, _D, _D]),
+    "nlm_qs_solve": (C.c_int, [_P, C.POINTER(QsOptions), C.POINTER(QsStats)]),
+    "nlm_qs_assemble": (C.c_int, [_P, C.c_int]),
+    "nlm_qs_get_tangent": (C.c_int, [_P, _D, _D]),
+    "nlm_qs_apply": (C.c_int, [_P, _D, _D]),
+    "nlm_qs_set_precond": (C.c_int, [_P, _D, _D]),
+    "nlm_qs_get_precond": (C.c_int, [_P, _D, _D]),
+    "nlm_gnn_num_params": (C.c_int64, []),
+    "nlm_gnn_create": (C.c_int, [_P, C.c_uint64]),
+    "nlm_gnn_get_weights": (C.c_int, [_P, _F]),
+    "nlm_gnn_set_weights": (C.c_int, [_P, _F]),
+    "nlm_gnn_predict": (C.c_int, [_P]),
+    "nlm_gnn_loss_grad": (C.c_int, [_P, C.c_int, _D, _D, _D, _F]),
+    "nlm_gnn_adam_step": (C.c_int, [_P, C.c_double]),
+    "nlm_qs_linear_solve": (C.c_int, [_P, _D, _D, C.c_int, C.c_double,
+                                      C.c_int, C.POINTER(C.c_int), _D]),
 }
 
 _lib = None

@@ -604,6 +604,18 @@ void launch_force_dispatch(nlm_ctx* c, bool base_f, double t) {
     launch_mode<DIM, KIND_FORCE, false>(c, a, t, t, 4);
 }
 
+}  // namespace
+
+// f = f_ext(t) + f_int(u) (base_f: f += f_int(u)), bond bits updated.
+void fd_launch_force(nlm_ctx* c, bool base_f, double t) {
+  if (c->fd.dim == 2)
+    launch_force_dispatch<2>(c, base_f, t);
+  else
+    launch_force_dispatch<3>(c, base_f, t);
+}
+
+namespace {
+
 template <int DIM>
 void launch_fused_dispatch(nlm_ctx* c, double tf, double tn) {
   if (c->fd.material == NLM_MATERIAL_ELASTIC_STATE)

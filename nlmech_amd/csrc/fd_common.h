@@ -165,6 +165,9 @@ inline int64_t force_blocks(const FdState& s) {
   return (s.sell.nslices * 64 + 255) / 256;
 }
 
+// Force of the configured material at the current u (fd.hip).
+void fd_launch_force(nlm_ctx* c, bool base_f, double t);
+
 // State-based material (es.hip).
 int es_prepare(nlm_ctx* c);
 void es_launch_force(nlm_ctx* c, bool base_f, double t);

@@ -157,6 +157,47 @@ struct FdState {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
+// Quasi-static solver data (2D): tangent and preconditioner blocks in the
+// slice layout (double4 = row-major 2x2 per slice entry), per-node diagonal
+// blocks, Krylov vectors (double2 per node, internal order) and the
+// per-block partial sums and per-iteration scalars of BiCGSTAB.
+struct QsState {
+  bool assembled = false;
+  bool has_m = false;
+  int tangent = 0;
+  DevBuf blk, diag, dinv;    // A = -J blocks, diagonal blocks, Jacobi 1/a_dd
+  DevBuf mblk, mdiag;        // preconditioner blocks
+  DevBuf r, rh, p, v, s, t, x, ph, sh, b, y, dup, u0;
+  DevBuf part;               // 4 * nb partial sums
+  DevBuf scal;               // per-iteration scalars (rho, alpha, omega, rr)
+  DevBuf ctl;                // int control words (done flag, iteration)
+  int64_t nb = 0;            // 256-node blocks
+  int max_iter_alloc = 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+
+// Graph-network preconditioner: FP32 master weights, gradient, Adam
+// moments, BF16 copies of the layer matrices (forward: transposed [64][K],
+// backward: [K][64]) and the activations of the last forward pass.
+struct GnnState {
+  bool ready = false;
+  int64_t np = 0;            // parameters
+  DevBuf w, grad, am, av;    // FP32
+  DevBuf wfwd, wbwd;         // BF16 copies
+  int64_t adam_t = 0;
+  int64_t n = 0, ne = 0;     // nodes, slice entries of the sized buffers
+  DevBuf xn, xe;             // features (BF16)
+  DevBuf h, g, agg;          // per-layer node / edge states (BF16)
+  DevBuf P, Q;               // FP32 node projections
+  DevBuf scale, isq, deg;    // node diagonal scale, its 1/sqrt, degree
+  DevBuf erow, rev, valid;   // per slice entry: node, reverse entry, mask
+  DevBuf dh, dh16, dg, dg2, dpre, dP, dQ, dPQ16, dagg;  // backward
+  DevBuf part;               // partial sums of weight gradients
+  DevBuf vr, ve, vz;         // training vectors
+  DevBuf dme, dmn;           // dL/d(decoder outputs)
+  DevBuf lpart;              // loss partials
+};
+
 struct NbrState {
   bool ready = false;
   int64_t n = 0;
@@ -174,6 +215,8 @@ struct nlm_ctx {
   std::string arch;
   nlm::NbrState nbr;
   nlm::FdState fd;
+  nlm::QsState qs;
+  nlm::GnnState gnn;
 };
 
 namespace nlm {

@@ -1,0 +1,966 @@
+// qs.hip -- quasi-static Newton solver of the bond-based material on the GPU
+// (model::QuasiStaticModel<T>, src/model/quasistatic/QuasiStaticModel.cpp).
+//
+// Tangent.  The bond force of RNPBond::getBondEF (rnpBond.cpp:216-301) is
+// F_ij = g(S) e with e = dx/|dx| fixed and S = e.du/|dx|, so
+// dF_ij/du_j = k_ij e e^T = -dF_ij/du_i with k_ij = g'(S)/|dx|.  The solver
+// works with A = -df_int/du: per slice entry (i, j) the block -k_ij e e^T, per
+// node the block sum_j k_ij e e^T.  The reference assembles a central-
+// difference Jacobian that keeps only same-component entries (:363-454);
+// NLM_QS_TANGENT_COMPONENT keeps the same entries of the exact derivative.
+//
+// Krylov solve.  BiCGSTAB on A_ff (prescribed dofs held at zero in every
+// vector, their rows masked in every product).  The scalars live on the
+// device: each kernel that needs a dot product reduces the 256-node blocks'
+// partial sums itself (every workgroup in the same order, so all agree), and
+// workgroup 0 records the scalar for later kernels.  Convergence sets a flag
+// that turns the remaining launches into no-ops, so the host checks it only
+// every few iterations.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "bond_rnp.h"
+#include "device_common.h"
+#include "fd_common.h"
+#include "nlm_internal.h"
+
+namespace nlm {
+namespace {
+
+struct QsArgs {
+  int64_t n;
+  int64_t nslices;
+  int64_t nb;
+  const double2* __restrict__ x;
+  const double2* __restrict__ u;
+  const double* __restrict__ vol;
+  const uint32_t* __restrict__ flags;
+  const int64_t* __restrict__ sl_ptr;
+  const int32_t* __restrict__ col;
+  const uint16_t* __restrict__ rowlen;
+  const unsigned long long* __restrict__ bits;
+  RnpDev m;
+};
+
+__device__ __forceinline__ bool dof_fixed(uint32_t fl, int d) {
+  return ((fl >> (kFlagUbcShift + 4 * d)) & 0xfu) != 0u;
+}
+
+// Sum of nb partials, the same order in every workgroup.
+__device__ double block_sum(const double* __restrict__ P, int64_t nb) {
+  __shared__ double red[256];
+  double s = 0.;
+  for (int64_t k = threadIdx.x; k < nb; k += 256) s += P[k];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+// Workgroup sum of one value per thread, stored to P[b] by thread 0.
+__device__ void block_partial(double v, double* __restrict__ P, int64_t b) {
+  __shared__ double red[256];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) P[b] = red[0];
+  __syncthreads();
+}
+
+// Tangent blocks of every slice entry and the diagonal blocks.
+__global__ __launch_bounds__(256) void k_qs_tangent(
+    QsArgs a, int component, double4* __restrict__ blk,
+    double4* __restrict__ diag, double2* __restrict__ dinv) {
+  const int64_t nblk = (a.nslices * 64 + 255) / 256;
+  const int64_t lb = xcd_block(nblk);
+  if (lb >= nblk) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t s = lb * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (s >= a.nslices) return;
+  const int64_t i = s * 64 + lane;
+  const bool valid = i < a.n;
+  const int64_t cb = a.sl_ptr[s];
+  const int width = int((a.sl_ptr[s + 1] - cb) >> 6);
+  const int64_t bb = cb >> 6;
+  const int len = valid ? int(a.rowlen[i]) : 0;
+  double2 xi = valid ? a.x[i] : make_double2(0., 0.);
+  double2 ui = valid ? a.u[i] : make_double2(0., 0.);
+  const uint32_t fli = valid ? a.flags[i] : 0u;
+  const bool int_i = !(fli & kFlagNonInterior);
+  const RnpDev& m = a.m;
+  double4 D = make_double4(0., 0., 0., 0.);
+  for (int k = 0; k < width; k++) {
+    const int64_t e = cb + int64_t(k) * 64 + lane;
+    const unsigned long long w = a.bits[bb + k];
+    double4 B = make_double4(0., 0., 0., 0.);
+    if (k < len) {
+      const int32_t j = a.col[e];
+      const double2 xj = a.x[j], uj = a.u[j];
+      const bool brk = int_i && !(a.flags[j] & kFlagNonInterior);
+      const bool fs = (w >> lane) & 1ull;
+      const double dx0 = xj.x - xi.x, dx1 = xj.y - xi.y;
+      const double du0 = uj.x - ui.x, du1 = uj.y - ui.y;
+      const double dd = dx0 * dx0 + dx1 * dx1;
+      const double r = sqrt(dd);
+      const double S = (dx0 * du0 + dx1 * du1) / dd;
+      double volj = a.vol[j];
+      if (def_greater(r, m.check_low)) volj *= (m.check_up - r) / m.h;
+      const double infl = influence_exact(m, r);
+      const double c = infl * 4. * m.C * m.beta / m.inv_factor * volj;
+      double kk;
+      if (brk) {
+        const double q = m.beta * r * S * S;
+        kk = fs ? 0. : c * exp(-q) * (1. - 2. * q) / r;
+      } else {
+        kk = c / r;
+      }
+      const double e0 = dx0 / r, e1 = dx1 / r;
+      B.x = kk * e0 * e0;
+      B.y = component ? 0. : kk * e0 * e1;
+      B.z = component ? 0. : kk * e1 * e0;
+      B.w = kk * e1 * e1;
+      D.x += B.x;
+      D.y += B.y;
+      D.z += B.z;
+      D.w += B.w;
+    }
+    blk[e] = make_double4(-B.x, -B.y, -B.z, -B.w);
+  }
+  if (!valid) return;
+  diag[i] = D;
+  dinv[i] = make_double2(!dof_fixed(fli, 0) && D.x != 0. ? 1. / D.x : 0.,
+                         !dof_fixed(fli, 1) && D.w != 0. ? 1. / D.w : 0.);
+}
+
+// y = (D x_i + sum_j B_ij x_j) on free dofs, 0 on prescribed dofs.
+// DOT 1: partial w.y into P0; DOT 2: partials w.y into P0 and y.y into P1.
+template <int DOT>
+__global__ __launch_bounds__(256) void k_qs_spmv(
+    QsArgs a, const double4* __restrict__ blk, const double4* __restrict__ dg,
+    const double2* __restrict__ xv, double2* __restrict__ y,
+    const double2* __restrict__ wv, double* __restrict__ P0,
+    double* __restrict__ P1, const int* __restrict__ ctl) {
+  if (ctl && ctl[0]) return;
+  const int64_t nblk = (a.nslices * 64 + 255) / 256;
+  const int64_t lb = xcd_block(nblk);
+  if (lb >= nblk) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t s = lb * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const bool slice_ok = s < a.nslices;
+  const int64_t i = s * 64 + lane;
+  const bool valid = slice_ok && i < a.n;
+  double p0 = 0., p1 = 0.;
+  if (slice_ok) {
+    const int64_t cb = a.sl_ptr[s];
+    const int width = int((a.sl_ptr[s + 1] - cb) >> 6);
+    double2 acc = make_double2(0., 0.);
+    for (int k = 0; k < width; k++) {
+      const int64_t e = cb + int64_t(k) * 64 + lane;
+      const double4 B = blk[e];
+      const double2 xj = xv[a.col[e]];
+      acc.x += B.x * xj.x + B.y * xj.y;
+      acc.y += B.z * xj.x + B.w * xj.y;
+    }
+    if (valid) {
+      const double4 D = dg[i];
+      const double2 xi = xv[i];
+      acc.x += D.x * xi.x + D.y * xi.y;
+      acc.y += D.z * xi.x + D.w * xi.y;
+      const uint32_t fl = a.flags[i];
+      if (dof_fixed(fl, 0)) acc.x = 0.;
+      if (dof_fixed(fl, 1)) acc.y = 0.;
+      y[i] = acc;
+      if (DOT >= 1) {
+        const double2 w = wv[i];
+        p0 = w.x * acc.x + w.y * acc.y;
+      }
+      if (DOT == 2) p1 = acc.x * acc.x + acc.y * acc.y;
+    }
+  }
+  if (DOT >= 1) block_partial(p0, P0, lb);
+  if (DOT == 2) block_partial(p1, P1, lb);
+}
+
+// Scalars of iteration it: rho[it] = (rh, r_it), alpha[it], omega[it],
+// rr[it] = (r_it, r_it), at offsets 8 + {0, 1, 2, 3} * L + it.
+struct Scal {
+  double* base;
+  int L;
+  __device__ double& tol2() const { return base[0]; }
+  __device__ double& rho(int it) const { return base[8 + it]; }
+  __device__ double& alpha(int it) const { return base[8 + L + it]; }
+  __device__ double& omega(int it) const { return base[8 + 2 * L + it]; }
+  __device__ double& rr(int it) const { return base[8 + 3 * L + it]; }
+};
+
+struct VecArgs {
+  int64_t n, nb;
+  const uint32_t* __restrict__ flags;
+  double2 *r, *rh, *p, *v, *s, *t, *x, *ph, *sh;
+  const double2* __restrict__ dinv;
+  double *P0, *P1, *P2, *P3;
+  int* ctl;
+  Scal sc;
+};
+
+// r = rh = b (masked), p = v = x = 0; partials (rh, r) and (r, r).
+__global__ __launch_bounds__(256) void k_bicg_init(VecArgs a,
+                                                   const double2* __restrict__ b) {
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  double p0 = 0.;
+  if (i < a.n) {
+    double2 bi = b[i];
+    const uint32_t fl = a.flags[i];
+    if (dof_fixed(fl, 0)) bi.x = 0.;
+    if (dof_fixed(fl, 1)) bi.y = 0.;
+    const double2 z = make_double2(0., 0.);
+    a.r[i] = bi;
+    a.rh[i] = bi;
+    a.p[i] = z;
+    a.v[i] = z;
+    a.x[i] = z;
+    a.ph[i] = z;
+    a.sh[i] = z;
+    p0 = bi.x * bi.x + bi.y * bi.y;
+  }
+  block_partial(p0, a.P0, blockIdx.x);
+  block_partial(p0, a.P1, blockIdx.x);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.ctl[0] = 0;
+    a.ctl[1] = 0;
+  }
+}
+
+// Iteration it, step 1: convergence test on r_{it-1}; p = r + beta (p -
+// omega v); ph = M p for the diagonal preconditioners (PRE 0 none, 1 Jacobi;
+// PRE 2 leaves ph to a block product).
+template <int PRE>
+__global__ __launch_bounds__(256) void k_bicg_p(VecArgs a, int it) {
+  if (a.ctl[0]) return;
+  const double rho_i = block_sum(a.P0, a.nb);
+  const double rr = block_sum(a.P1, a.nb);
+  const bool conv = !(rr > a.sc.tol2());
+  const bool brk = !conv && rho_i == 0.;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.sc.rho(it - 1) = rho_i;
+    a.sc.rr(it - 1) = rr;
+    if (conv || brk) {
+      a.ctl[0] = conv ? 1 : 2;
+      a.ctl[1] = it - 1;
+    }
+  }
+  if (conv || brk) return;
+  double beta = 0., om = 0.;
+  if (it > 1) {
+    om = a.sc.omega(it - 1);
+    beta = (rho_i / a.sc.rho(it - 2)) * (a.sc.alpha(it - 1) / om);
+  }
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= a.n) return;
+  const double2 r = a.r[i], p = a.p[i], v = a.v[i];
+  double2 pn;
+  pn.x = r.x + beta * (p.x - om * v.x);
+  pn.y = r.y + beta * (p.y - om * v.y);
+  a.p[i] = pn;
+  if (PRE == 0) a.ph[i] = pn;
+  if (PRE == 1) {
+    const double2 di = a.dinv[i];
+    a.ph[i] = make_double2(di.x * pn.x, di.y * pn.y);
+  }
+}
+
+// Step 2 (after v = A ph with partial (rh, v) in P2): alpha, s = r - alpha v.
+template <int PRE>
+__global__ __launch_bounds__(256) void k_bicg_s(VecArgs a, int it) {
+  if (a.ctl[0]) return;
+  const double rv = block_sum(a.P2, a.nb);
+  const double rho_i = a.sc.rho(it - 1);
+  const bool brk = rv == 0.;
+  const double alpha = brk ? 0. : rho_i / rv;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.sc.alpha(it) = alpha;
+    if (brk) {
+      a.ctl[0] = 2;
+      a.ctl[1] = it - 1;
+    }
+  }
+  if (brk) return;
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= a.n) return;
+  const double2 r = a.r[i], v = a.v[i];
+  const double2 sn = make_double2(r.x - alpha * v.x, r.y - alpha * v.y);
+  a.s[i] = sn;
+  if (PRE == 0) a.sh[i] = sn;
+  if (PRE == 1) {
+    const double2 di = a.dinv[i];
+    a.sh[i] = make_double2(di.x * sn.x, di.y * sn.y);
+  }
+}
+
+// Step 3 (after t = A sh with partials (t, s) in P2 and (t, t) in P3):
+// omega, x += alpha ph + omega sh, r = s - omega t; partials (rh, r), (r, r).
+__global__ __launch_bounds__(256) void k_bicg_x(VecArgs a, int it) {
+  if (a.ctl[0]) return;
+  const double ts = block_sum(a.P2, a.nb);
+  const double tt = block_sum(a.P3, a.nb);
+  const double omega = tt > 0. ? ts / tt : 0.;
+  const double alpha = a.sc.alpha(it);
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.sc.omega(it) = omega;
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  double p0 = 0., p1 = 0.;
+  if (i < a.n) {
+    const double2 ph = a.ph[i], sh = a.sh[i], s = a.s[i], t = a.t[i];
+    double2 x = a.x[i];
+    x.x += alpha * ph.x + omega * sh.x;
+    x.y += alpha * ph.y + omega * sh.y;
+    a.x[i] = x;
+    const double2 r = make_double2(s.x - omega * t.x, s.y - omega * t.y);
+    a.r[i] = r;
+    const double2 rh = a.rh[i];
+    p0 = rh.x * r.x + rh.y * r.y;
+    p1 = r.x * r.x + r.y * r.y;
+  }
+  block_partial(p0, a.P0, blockIdx.x);
+  block_partial(p1, a.P1, blockIdx.x);
+}
+
+// Newton helpers ------------------------------------------------------------
+
+// R = f on free dofs (f = f_int + b), 0 on prescribed; partial (R, R).
+__global__ __launch_bounds__(256) void k_qs_residual(
+    int64_t n, const uint32_t* __restrict__ flags,
+    const double2* __restrict__ f, double2* __restrict__ R,
+    double* __restrict__ P) {
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  double p = 0.;
+  if (i < n) {
+    double2 v = f[i];
+    const uint32_t fl = flags[i];
+    if (dof_fixed(fl, 0)) v.x = 0.;
+    if (dof_fixed(fl, 1)) v.y = 0.;
+    R[i] = v;
+    p = v.x * v.x + v.y * v.y;
+  }
+  block_partial(p, P, blockIdx.x);
+}
+
+// dup = ULoading(t) - u on prescribed dofs, 0 on free dofs.
+__global__ __launch_bounds__(256) void k_qs_dup(
+    int64_t n, const uint32_t* __restrict__ flags,
+    const double2* __restrict__ x, const double2* __restrict__ u,
+    const BcTable* __restrict__ bc, double t, double2* __restrict__ dup) {
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t fl = flags[i];
+  const double2 xi = x[i], ui = u[i];
+  double d[2] = {0., 0.};
+  const double uu[2] = {ui.x, ui.y};
+  for (int c = 0; c < 2; c++) {
+    const uint32_t id = (fl >> (kFlagUbcShift + 4 * c)) & 0xfu;
+    if (id) {
+      double du, dv;
+      ubc_eval(bc->u[id - 1], xi.x, xi.y, 0., t, du, dv);
+      d[c] = du - uu[c];
+    }
+  }
+  dup[i] = make_double2(d[0], d[1]);
+}
+
+// b = R - y on free dofs; partial (b, b).
+__global__ __launch_bounds__(256) void k_qs_sub(int64_t n,
+                                                const uint32_t* __restrict__ flags,
+                                                double2* __restrict__ R,
+                                                const double2* __restrict__ y,
+                                                double* __restrict__ P) {
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  double p = 0.;
+  if (i < n) {
+    double2 v = R[i];
+    const double2 w = y[i];
+    v.x -= w.x;
+    v.y -= w.y;
+    const uint32_t fl = flags[i];
+    if (dof_fixed(fl, 0)) v.x = 0.;
+    if (dof_fixed(fl, 1)) v.y = 0.;
+    R[i] = v;
+    p = v.x * v.x + v.y * v.y;
+  }
+  block_partial(p, P, blockIdx.x);
+}
+
+// u += delta on free dofs; u += dup on prescribed dofs (dup may be null:
+// prescribed dofs unchanged).
+__global__ __launch_bounds__(256) void k_qs_update_u(
+    int64_t n, const uint32_t* __restrict__ flags, double2* __restrict__ u,
+    const double2* __restrict__ delta, const double2* __restrict__ dup) {
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t fl = flags[i];
+  double2 ui = u[i];
+  const double2 d = delta[i];
+  const double2 p = dup ? dup[i] : make_double2(0., 0.);
+  ui.x += dof_fixed(fl, 0) ? p.x : d.x;
+  ui.y += dof_fixed(fl, 1) ? p.y : d.y;
+  u[i] = ui;
+}
+
+// ---- host side ------------------------------------------------------------
+
+QsArgs qs_args(nlm_ctx* c) {
+  FdState& s = c->fd;
+  QsArgs a;
+  a.n = s.n;
+  a.nslices = s.sell.nslices;
+  a.nb = (s.n + 255) / 256;
+  a.x = s.x.as<double2>();
+  a.u = s.u[s.cur].as<double2>();
+  a.vol = s.vol.as<double>();
+  a.flags = s.flags.as<uint32_t>();
+  a.sl_ptr = s.sell.sl_ptr.as<int64_t>();
+  a.col = s.sell.col.as<int32_t>();
+  a.rowlen = s.sell.rowlen.as<uint16_t>();
+  a.bits = s.sell.bits.as<unsigned long long>();
+  a.m = s.rnp;
+  return a;
+}
+
+int qs_check(nlm_ctx* c, const char* fn) {
+  if (!c) return NLM_ERR_INVALID;
+  if (!c->fd.ready)
+    return fail(c, NLM_ERR_STATE, std::string(fn) + ": nlm_fd_setup not done");
+  if (c->fd.dim != 2 || c->fd.material != NLM_MATERIAL_RNP_BOND)
+    return fail(c, NLM_ERR_INVALID,
+                std::string(fn) + ": 2D bond-based material only");
+  return NLM_OK;
+}
+
+int qs_alloc(nlm_ctx* c, int max_iter) {
+  FdState& s = c->fd;
+  QsState& q = c->qs;
+  const int64_t n = s.n;
+  const size_t nent = size_t(s.sell.nentries);
+  int rc;
+  if (!q.blk.p || q.blk.bytes < nent * sizeof(double4)) {
+    if ((rc = alloc(c, q.blk, nent * sizeof(double4) + 256))) return rc;
+    if ((rc = alloc(c, q.diag, n * sizeof(double4) + 256))) return rc;
+    if ((rc = alloc(c, q.dinv, n * sizeof(double2) + 256))) return rc;
+    DevBuf* vec[] = {&q.r, &q.rh, &q.p, &q.v, &q.s, &q.t, &q.x,
+                     &q.ph, &q.sh, &q.b, &q.y, &q.dup, &q.u0};
+    for (DevBuf* d : vec)
+      if ((rc = alloc(c, *d, n * sizeof(double2) + 256))) return rc;
+    q.nb = (n + 255) / 256;
+    if ((rc = alloc(c, q.part, 4 * q.nb * sizeof(double) + 256))) return rc;
+    if ((rc = alloc(c, q.ctl, 64))) return rc;
+    if (!q.e0) NLM_HIP(c, hipEventCreate(&q.e0));
+    if (!q.e1) NLM_HIP(c, hipEventCreate(&q.e1));
+  }
+  if (max_iter + 2 > q.max_iter_alloc) {
+    q.max_iter_alloc = max_iter + 2;
+    if ((rc = alloc(c, q.scal, (8 + 4 * size_t(q.max_iter_alloc)) *
+                                   sizeof(double))))
+      return rc;
+  }
+  return NLM_OK;
+}
+
+VecArgs vec_args(nlm_ctx* c) {
+  QsState& q = c->qs;
+  VecArgs a;
+  a.n = c->fd.n;
+  a.nb = q.nb;
+  a.flags = c->fd.flags.as<uint32_t>();
+  a.r = q.r.as<double2>();
+  a.rh = q.rh.as<double2>();
+  a.p = q.p.as<double2>();
+  a.v = q.v.as<double2>();
+  a.s = q.s.as<double2>();
+  a.t = q.t.as<double2>();
+  a.x = q.x.as<double2>();
+  a.ph = q.ph.as<double2>();
+  a.sh = q.sh.as<double2>();
+  a.dinv = q.dinv.as<double2>();
+  double* P = q.part.as<double>();
+  a.P0 = P;
+  a.P1 = P + q.nb;
+  a.P2 = P + 2 * q.nb;
+  a.P3 = P + 3 * q.nb;
+  a.ctl = q.ctl.as<int>();
+  a.sc.base = q.scal.as<double>();
+  a.sc.L = q.max_iter_alloc;
+  return a;
+}
+
+unsigned node_grid(const FdState& s) { return unsigned((s.n + 255) / 256); }
+unsigned slice_grid(const FdState& s) {
+  return grid_xcd(force_blocks(s) * 256, 256);
+}
+
+// Host sum of one partial slot (nb doubles).
+int partial_sum(nlm_ctx* c, const double* P, double* out) {
+  QsState& q = c->qs;
+  std::vector<double> h(q.nb);
+  NLM_HIP(c, hipMemcpyAsync(h.data(), P, sizeof(double) * q.nb,
+                            hipMemcpyDeviceToHost, c->stream));
+  NLM_HIP(c, hipStreamSynchronize(c->stream));
+  double s = 0.;
+  for (double v : h) s += v;
+  *out = s;
+  return NLM_OK;
+}
+
+int assemble(nlm_ctx* c, int tangent) {
+  QsState& q = c->qs;
+  int rc;
+  if ((rc = qs_alloc(c, 8))) return rc;
+  hipLaunchKernelGGL(k_qs_tangent, dim3(slice_grid(c->fd)), dim3(256), 0,
+                     c->stream, qs_args(c), tangent, q.blk.as<double4>(),
+                     q.diag.as<double4>(), q.dinv.as<double2>());
+  NLM_HIP(c, hipGetLastError());
+  q.assembled = true;
+  q.tangent = tangent;
+  return NLM_OK;
+}
+
+template <int DOT>
+void spmv(nlm_ctx* c, const DevBuf& blk, const DevBuf& dg, const double2* x,
+          double2* y, const double2* w, double* P0, double* P1,
+          const int* ctl) {
+  hipLaunchKernelGGL(k_qs_spmv<DOT>, dim3(slice_grid(c->fd)), dim3(256), 0,
+                     c->stream, qs_args(c), blk.as<double4>(),
+                     dg.as<double4>(), x, y, w, P0, P1, ctl);
+}
+
+// BiCGSTAB on A_ff x = b (q.b -> q.x).  Returns iterations, relative
+// residual and whether it converged.
+int bicgstab(nlm_ctx* c, int precond, double tol, int max_iter, int* iters,
+             double* relres, bool* ok) {
+  FdState& s = c->fd;
+  QsState& q = c->qs;
+  int rc;
+  if ((rc = qs_alloc(c, max_iter))) return rc;
+  if (precond == NLM_QS_PRECOND_BLOCKS && !q.has_m)
+    return fail(c, NLM_ERR_STATE, "BiCGSTAB: no preconditioner blocks set");
+  VecArgs a = vec_args(c);
+  const unsigned g = node_grid(s);
+  hipLaunchKernelGGL(k_bicg_init, dim3(g), dim3(256), 0, c->stream, a,
+                     q.b.as<double2>());
+  NLM_HIP(c, hipGetLastError());
+  double bb = 0.;
+  if ((rc = partial_sum(c, a.P1, &bb))) return rc;
+  const double tol2 = tol * tol * bb;
+  NLM_HIP(c, hipMemcpyAsync(q.scal.p, &tol2, sizeof(double),
+                            hipMemcpyHostToDevice, c->stream));
+  NLM_HIP(c, hipStreamSynchronize(c->stream));
+  if (bb == 0.) {
+    *iters = 0;
+    *relres = 0.;
+    *ok = true;
+    return NLM_OK;
+  }
+  const double2* rh = q.rh.as<double2>();
+  int ctl[2] = {0, 0};
+  int it = 1;
+  const int batch = 8;
+  while (it <= max_iter) {
+    const int end = std::min(max_iter, it + batch - 1);
+    for (; it <= end; it++) {
+      if (precond == NLM_QS_PRECOND_JACOBI)
+        hipLaunchKernelGGL(k_bicg_p<1>, dim3(g), dim3(256), 0, c->stream, a,
+                           it);
+      else if (precond == NLM_QS_PRECOND_NONE)
+        hipLaunchKernelGGL(k_bicg_p<0>, dim3(g), dim3(256), 0, c->stream, a,
+                           it);
+      else {
+        hipLaunchKernelGGL(k_bicg_p<2>, dim3(g), dim3(256), 0, c->stream, a,
+                           it);
+        spmv<0>(c, q.mblk, q.mdiag, a.p, a.ph, nullptr, nullptr, nullptr,
+                a.ctl);
+      }
+      spmv<1>(c, q.blk, q.diag, a.ph, a.v, rh, a.P2, nullptr, a.ctl);
+      if (precond == NLM_QS_PRECOND_JACOBI)
+        hipLaunchKernelGGL(k_bicg_s<1>, dim3(g), dim3(256), 0, c->stream, a,
+                           it);
+      else if (precond == NLM_QS_PRECOND_NONE)
+        hipLaunchKernelGGL(k_bicg_s<0>, dim3(g), dim3(256), 0, c->stream, a,
+                           it);
+      else {
+        hipLaunchKernelGGL(k_bicg_s<2>, dim3(g), dim3(256), 0, c->stream, a,
+                           it);
+        spmv<0>(c, q.mblk, q.mdiag, a.s, a.sh, nullptr, nullptr, nullptr,
+                a.ctl);
+      }
+      spmv<2>(c, q.blk, q.diag, a.sh, a.t, a.s, a.P2, a.P3, a.ctl);
+      hipLaunchKernelGGL(k_bicg_x, dim3(g), dim3(256), 0, c->stream, a, it);
+    }
+    NLM_HIP(c, hipGetLastError());
+    NLM_HIP(c, hipMemcpyAsync(ctl, q.ctl.p, sizeof(ctl), hipMemcpyDeviceToHost,
+                              c->stream));
+    NLM_HIP(c, hipStreamSynchronize(c->stream));
+    if (ctl[0]) break;
+  }
+  double rr = 0.;
+  int done_it = max_iter;
+  if (ctl[0]) {
+    done_it = ctl[1];
+    const size_t off = 8 + 3 * size_t(q.max_iter_alloc) + size_t(done_it);
+    NLM_HIP(c, hipMemcpy(&rr, q.scal.as<double>() + off, sizeof(double),
+                         hipMemcpyDeviceToHost));
+  } else {
+    if ((rc = partial_sum(c, a.P1, &rr))) return rc;
+  }
+  *iters = done_it;
+  *relres = std::sqrt(rr / bb);
+  *ok = ctl[0] == 1;
+  return NLM_OK;
+}
+
+// ||R||_2 of the residual of the current force (q.b <- R).
+int residual(nlm_ctx* c, double* norm) {
+  FdState& s = c->fd;
+  QsState& q = c->qs;
+  hipLaunchKernelGGL(k_qs_residual, dim3(node_grid(s)), dim3(256), 0,
+                     c->stream, s.n, s.flags.as<uint32_t>(), s.f.as<double2>(),
+                     q.b.as<double2>(), q.part.as<double>());
+  NLM_HIP(c, hipGetLastError());
+  double rr = 0.;
+  int rc = partial_sum(c, q.part.as<double>(), &rr);
+  *norm = std::sqrt(rr);
+  return rc;
+}
+
+double ms_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(
+             std::chrono::steady_clock::now() - t0)
+      .count();
+}
+
+}  // namespace
+
+// Preconditioner construction by the graph network (gnn.hip); returns
+// NLM_ERR_STATE when no network is attached to the context.
+int gnn_predict_precond(nlm_ctx* c);
+
+}  // namespace nlm
+
+using namespace nlm;
+
+extern "C" int nlm_qs_assemble(nlm_ctx* c, int tangent) {
+  int rc = qs_check(c, "nlm_qs_assemble");
+  if (rc) return rc;
+  if (tangent != NLM_QS_TANGENT_FULL && tangent != NLM_QS_TANGENT_COMPONENT)
+    return fail(c, NLM_ERR_INVALID, "nlm_qs_assemble: unknown tangent");
+  if ((rc = assemble(c, tangent))) return rc;
+  NLM_HIP(c, hipStreamSynchronize(c->stream));
+  return NLM_OK;
+}
+
+namespace {
+// Copies slice-layout blocks and per-node blocks to the reference layout.
+int blocks_to_host(nlm_ctx* c, const DevBuf& blk, const DevBuf& dg,
+                   double* blocks, double* diag) {
+  FdState& s = c->fd;
+  NLM_HIP(c, hipStreamSynchronize(c->stream));
+  const int64_t nent = s.sell.nentries;
+  std::vector<double4> hb(nent), hd(s.n);
+  std::vector<int64_t> slp(s.sell.nslices + 1);
+  std::vector<int64_t> roff(s.n);
+  std::vector<uint16_t> rl(s.n);
+  NLM_HIP(c, hipMemcpy(hb.data(), blk.p, sizeof(double4) * nent,
+                       hipMemcpyDeviceToHost));
+  NLM_HIP(c, hipMemcpy(hd.data(), dg.p, sizeof(double4) * s.n,
+                       hipMemcpyDeviceToHost));
+  NLM_HIP(c, hipMemcpy(slp.data(), s.sell.sl_ptr.p,
+                       sizeof(int64_t) * slp.size(), hipMemcpyDeviceToHost));
+  NLM_HIP(c, hipMemcpy(roff.data(), s.ref_off.p, sizeof(int64_t) * s.n,
+                       hipMemcpyDeviceToHost));
+  NLM_HIP(c, hipMemcpy(rl.data(), s.sell.rowlen.p, sizeof(uint16_t) * s.n,
+                       hipMemcpyDeviceToHost));
+  for (int64_t a = 0; a < s.n; a++) {
+    const int64_t sl = a / 64, lane = a % 64;
+    if (blocks)
+      for (int k = 0; k < rl[a]; k++) {
+        const double4 B = hb[slp[sl] + 64 * k + lane];
+        double* o = blocks + 4 * (roff[a] + k);
+        o[0] = B.x;
+        o[1] = B.y;
+        o[2] = B.z;
+        o[3] = B.w;
+      }
+    if (diag) {
+      const double4 D = hd[a];
+      double* o = diag + 4 * int64_t(s.perm_host[a]);
+      o[0] = D.x;
+      o[1] = D.y;
+      o[2] = D.z;
+      o[3] = D.w;
+    }
+  }
+  return NLM_OK;
+}
+}  // namespace
+
+extern "C" int nlm_qs_get_tangent(nlm_ctx* c, double* blocks, double* diag) {
+  int rc = qs_check(c, "nlm_qs_get_tangent");
+  if (rc) return rc;
+  if (!c->qs.assembled)
+    return fail(c, NLM_ERR_STATE, "nlm_qs_get_tangent: nothing assembled");
+  return blocks_to_host(c, c->qs.blk, c->qs.diag, blocks, diag);
+}
+
+extern "C" int nlm_qs_get_precond(nlm_ctx* c, double* blocks, double* diag) {
+  int rc = qs_check(c, "nlm_qs_get_precond");
+  if (rc) return rc;
+  if (!c->qs.has_m)
+    return fail(c, NLM_ERR_STATE, "nlm_qs_get_precond: no blocks");
+  return blocks_to_host(c, c->qs.mblk, c->qs.mdiag, blocks, diag);
+}
+
+extern "C" int nlm_qs_set_precond(nlm_ctx* c, const double* blocks,
+                                  const double* diag) {
+  int rc = qs_check(c, "nlm_qs_set_precond");
+  if (rc) return rc;
+  if (!blocks || !diag)
+    return fail(c, NLM_ERR_INVALID, "nlm_qs_set_precond: null blocks");
+  FdState& s = c->fd;
+  QsState& q = c->qs;
+  if ((rc = qs_alloc(c, 8))) return rc;
+  NLM_HIP(c, hipStreamSynchronize(c->stream));
+  const int64_t nent = s.sell.nentries;
+  if (!q.mblk.p || q.mblk.bytes < sizeof(double4) * nent) {
+    if ((rc = alloc(c, q.mblk, sizeof(double4) * nent + 256))) return rc;
+    if ((rc = alloc(c, q.mdiag, sizeof(double4) * s.n + 256))) return rc;
+  }
+  std::vector<double4> hb(nent, make_double4(0., 0., 0., 0.)), hd(s.n);
+  std::vector<int64_t> slp(s.sell.nslices + 1), roff(s.n);
+  std::vector<uint16_t> rl(s.n);
+  NLM_HIP(c, hipMemcpy(slp.data(), s.sell.sl_ptr.p,
+                       sizeof(int64_t) * slp.size(), hipMemcpyDeviceToHost));
+  NLM_HIP(c, hipMemcpy(roff.data(), s.ref_off.p, sizeof(int64_t) * s.n,
+                       hipMemcpyDeviceToHost));
+  NLM_HIP(c, hipMemcpy(rl.data(), s.sell.rowlen.p, sizeof(uint16_t) * s.n,
+                       hipMemcpyDeviceToHost));
+  for (int64_t a = 0; a < s.n; a++) {
+    const int64_t sl = a / 64, lane = a % 64;
+    for (int k = 0; k < rl[a]; k++) {
+      const double* b = blocks + 4 * (roff[a] + k);
+      hb[slp[sl] + 64 * k + lane] = make_double4(b[0], b[1], b[2], b[3]);
+    }
+    const double* d = diag + 4 * int64_t(s.perm_host[a]);
+    hd[a] = make_double4(d[0], d[1], d[2], d[3]);
+  }
+  NLM_HIP(c, hipMemcpy(q.mblk.p, hb.data(), sizeof(double4) * nent,
+                       hipMemcpyHostToDevice));
+  NLM_HIP(c, hipMemcpy(q.mdiag.p, hd.data(), sizeof(double4) * s.n,
+                       hipMemcpyHostToDevice));
+  q.has_m = true;
+  return NLM_OK;
+}
+
+namespace {
+// AoS double[3n] (reference order) <-> double2 per internal node.
+int aos_to_dev(nlm_ctx* c, const double* src, DevBuf& dst) {
+  FdState& s = c->fd;
+  NLM_HIP(c, hipStreamSynchronize(c->stream));
+  std::vector<double2> h(s.n);
+  for (int64_t a = 0; a < s.n; a++) {
+    const int64_t i = s.perm_host[a];
+    h[a] = make_double2(src[3 * i], src[3 * i + 1]);
+  }
+  NLM_HIP(c, hipMemcpy(dst.p, h.data(), sizeof(double2) * s.n,
+                       hipMemcpyHostToDevice));
+  return NLM_OK;
+}
+int dev_to_aos(nlm_ctx* c, const DevBuf& src, double* dst) {
+  FdState& s = c->fd;
+  NLM_HIP(c, hipStreamSynchronize(c->stream));
+  std::vector<double2> h(s.n);
+  NLM_HIP(c, hipMemcpy(h.data(), src.p, sizeof(double2) * s.n,
+                       hipMemcpyDeviceToHost));
+  for (int64_t a = 0; a < s.n; a++) {
+    const int64_t i = s.perm_host[a];
+    dst[3 * i] = h[a].x;
+    dst[3 * i + 1] = h[a].y;
+    dst[3 * i + 2] = 0.;
+  }
+  return NLM_OK;
+}
+}  // namespace
+
+extern "C" int nlm_qs_apply(nlm_ctx* c, const double* x, double* y) {
+  int rc = qs_check(c, "nlm_qs_apply");
+  if (rc) return rc;
+  QsState& q = c->qs;
+  if (!q.assembled)
+    return fail(c, NLM_ERR_STATE, "nlm_qs_apply: nothing assembled");
+  if (!x || !y) return fail(c, NLM_ERR_INVALID, "nlm_qs_apply: null vector");
+  if ((rc = aos_to_dev(c, x, q.p))) return rc;
+  // prescribed entries of x are ignored: zero them through the init kernel's
+  // masking (b -> r)
+  VecArgs a = vec_args(c);
+  NLM_HIP(c, hipMemcpyAsync(q.b.p, q.p.p, sizeof(double2) * c->fd.n,
+                            hipMemcpyDeviceToDevice, c->stream));
+  hipLaunchKernelGGL(k_bicg_init, dim3(node_grid(c->fd)), dim3(256), 0,
+                     c->stream, a, q.b.as<double2>());
+  spmv<0>(c, q.blk, q.diag, a.r, q.y.as<double2>(), nullptr, nullptr, nullptr,
+          nullptr);
+  NLM_HIP(c, hipGetLastError());
+  return dev_to_aos(c, q.y, y);
+}
+
+extern "C" int nlm_qs_linear_solve(nlm_ctx* c, const double* b, double* x,
+                                   int precond, double tol, int max_iter,
+                                   int* iters, double* relres) {
+  int rc = qs_check(c, "nlm_qs_linear_solve");
+  if (rc) return rc;
+  QsState& q = c->qs;
+  if (!q.assembled)
+    return fail(c, NLM_ERR_STATE, "nlm_qs_linear_solve: nothing assembled");
+  if (!b || !x || max_iter < 1 || !(tol > 0.))
+    return fail(c, NLM_ERR_INVALID, "nlm_qs_linear_solve: bad arguments");
+  if (precond < NLM_QS_PRECOND_NONE || precond > NLM_QS_PRECOND_BLOCKS)
+    return fail(c, NLM_ERR_INVALID, "nlm_qs_linear_solve: unknown precond");
+  if ((rc = qs_alloc(c, max_iter))) return rc;
+  if ((rc = aos_to_dev(c, b, q.b))) return rc;
+  int it = 0;
+  double rr = 0.;
+  bool ok = false;
+  if ((rc = bicgstab(c, precond, tol, max_iter, &it, &rr, &ok))) return rc;
+  if (iters) *iters = it;
+  if (relres) *relres = rr;
+  return dev_to_aos(c, q.x, x);
+}
+
+extern "C" int nlm_qs_solve(nlm_ctx* c, const nlm_qs_options* o,
+                            nlm_qs_stats* st) {
+  int rc = qs_check(c, "nlm_qs_solve");
+  if (rc) return rc;
+  if (!o || o->nsteps < 0 || !(o->dt > 0.) || o->max_newton < 0 ||
+      o->lin_max_iter < 1 || !(o->lin_tol > 0.))
+    return fail(c, NLM_ERR_INVALID, "nlm_qs_solve: bad options");
+  if (o->tangent != NLM_QS_TANGENT_FULL &&
+      o->tangent != NLM_QS_TANGENT_COMPONENT)
+    return fail(c, NLM_ERR_INVALID, "nlm_qs_solve: unknown tangent");
+  if (o->precond < NLM_QS_PRECOND_NONE || o->precond > NLM_QS_PRECOND_BLOCKS)
+    return fail(c, NLM_ERR_INVALID, "nlm_qs_solve: unknown precond");
+  FdState& s = c->fd;
+  QsState& q = c->qs;
+  if ((rc = qs_alloc(c, o->lin_max_iter))) return rc;
+  nlm_qs_stats stats;
+  std::memset(&stats, 0, sizeof(stats));
+  const auto t_all = std::chrono::steady_clock::now();
+  const unsigned g = node_grid(s);
+  auto u_cur = [&]() { return s.u[s.cur].as<double2>(); };
+  auto precond_setup = [&]() -> int {
+    if (o->precond != NLM_QS_PRECOND_BLOCKS) return NLM_OK;
+    const auto t0 = std::chrono::steady_clock::now();
+    int r = gnn_predict_precond(c);
+    if (r == NLM_ERR_STATE && q.has_m) r = NLM_OK;  // blocks set by hand
+    NLM_HIP(c, hipStreamSynchronize(c->stream));
+    stats.ms_precond += ms_since(t0);
+    return r;
+  };
+  auto linear = [&](int* it_out) -> int {
+    const auto t0 = std::chrono::steady_clock::now();
+    int it = 0;
+    double rr = 0.;
+    bool ok = false;
+    int r = bicgstab(c, o->precond, o->lin_tol, o->lin_max_iter, &it, &rr,
+                     &ok);
+    stats.ms_linear += ms_since(t0);
+    stats.linear_iters += it;
+    if (!ok) stats.linear_failures++;
+    *it_out = it;
+    return r;
+  };
+  auto tangent = [&]() -> int {
+    const auto t0 = std::chrono::steady_clock::now();
+    int r = assemble(c, o->tangent);
+    NLM_HIP(c, hipStreamSynchronize(c->stream));
+    stats.ms_tangent += ms_since(t0);
+    return r;
+  };
+  for (int step = 0; step < o->nsteps; step++) {
+    const double t = s.time + o->dt;
+    double rnorm = 0., r0 = 0.;
+    int newton = 0, lit = 0;
+    if (o->predictor) {
+      // linearized first iteration at the previous equilibrium
+      hipLaunchKernelGGL(k_qs_dup, dim3(g), dim3(256), 0, c->stream, s.n,
+                         s.flags.as<uint32_t>(), s.x.as<double2>(), u_cur(),
+                         s.bctab.as<BcTable>(), t, q.dup.as<double2>());
+      fd_launch_force(c, false, t);
+      if ((rc = residual(c, &rnorm))) return rc;
+      if ((rc = tangent())) return rc;
+      spmv<0>(c, q.blk, q.diag, q.dup.as<double2>(), q.y.as<double2>(),
+              nullptr, nullptr, nullptr, nullptr);
+      hipLaunchKernelGGL(k_qs_sub, dim3(g), dim3(256), 0, c->stream, s.n,
+                         s.flags.as<uint32_t>(), q.b.as<double2>(),
+                         q.y.as<double2>(), q.part.as<double>());
+      double bb = 0.;
+      if ((rc = partial_sum(c, q.part.as<double>(), &bb))) return rc;
+      r0 = std::sqrt(bb);
+      if ((rc = precond_setup())) return rc;
+      if ((rc = linear(&lit))) return rc;
+      hipLaunchKernelGGL(k_qs_update_u, dim3(g), dim3(256), 0, c->stream, s.n,
+                         s.flags.as<uint32_t>(), u_cur(), q.x.as<double2>(),
+                         q.dup.as<double2>());
+      newton = 1;
+      fd_launch_force(c, false, t);
+      if ((rc = residual(c, &rnorm))) return rc;
+    } else {
+      // ULoading at t on the prescribed dofs, then the residual
+      hipLaunchKernelGGL(k_qs_dup, dim3(g), dim3(256), 0, c->stream, s.n,
+                         s.flags.as<uint32_t>(), s.x.as<double2>(), u_cur(),
+                         s.bctab.as<BcTable>(), t, q.dup.as<double2>());
+      NLM_HIP(c, hipMemsetAsync(q.x.p, 0, sizeof(double2) * s.n, c->stream));
+      hipLaunchKernelGGL(k_qs_update_u, dim3(g), dim3(256), 0, c->stream, s.n,
+                         s.flags.as<uint32_t>(), u_cur(), q.x.as<double2>(),
+                         q.dup.as<double2>());
+      fd_launch_force(c, false, t);
+      if ((rc = residual(c, &rnorm))) return rc;
+      r0 = rnorm;
+    }
+    const double target = std::max(o->tol_abs, o->tol_rel * r0);
+    bool refreshed = o->predictor;
+    while (rnorm > target && newton < o->max_newton) {
+      if ((rc = tangent())) return rc;
+      if (o->gnn_refresh || !refreshed) {
+        if ((rc = precond_setup())) return rc;
+        refreshed = true;
+      }
+      if ((rc = linear(&lit))) return rc;
+      hipLaunchKernelGGL(k_qs_update_u, dim3(g), dim3(256), 0, c->stream, s.n,
+                         s.flags.as<uint32_t>(), u_cur(), q.x.as<double2>(),
+                         nullptr);
+      newton++;
+      fd_launch_force(c, false, t);
+      if ((rc = residual(c, &rnorm))) return rc;
+    }
+    s.time = t;
+    s.step++;
+    stats.load_steps++;
+    stats.newton_iters += newton;
+    stats.residual = rnorm;
+    stats.residual0 = r0;
+  }
+  NLM_HIP(c, hipGetLastError());
+  NLM_HIP(c, hipStreamSynchronize(c->stream));
+  stats.ms_total = ms_since(t_all);
+  if (st) *st = stats;
+  return NLM_OK;
+}

@@ -277,6 +277,91 @@ class GpuFD:
         self._call("nlm_fd_get_time", C.byref(t), C.byref(s))
         return t.value, s.value
 
+    # -- quasi-static model (2D bond-based) ---------------------------------
+    def qs_solve(self, nsteps, dt=1.0, tol_abs=0.0, tol_rel=1e-8,
+                 max_newton=50, tangent=0, precond=1, lin_tol=1e-8,
+                 lin_max_iter=5000, predictor=True, gnn_refresh=True):
+        """nsteps load steps of the quasi-static Newton solver; returns the
+        run statistics as a dict."""
+        o = capi.QsOptions(nsteps=int(nsteps), dt=float(dt),
+                           tol_abs=float(tol_abs), tol_rel=float(tol_rel),
+                           max_newton=int(max_newton), tangent=int(tangent),
+                           precond=int(precond), lin_tol=float(lin_tol),
+                           lin_max_iter=int(lin_max_iter),
+                           predictor=int(bool(predictor)),
+                           gnn_refresh=int(bool(gnn_refresh)))
+        st = capi.QsStats()
+        self._call("nlm_qs_solve", C.byref(o), C.byref(st))
+        return {f: getattr(st, f) for f, _ in capi.QsStats._fields_}
+
+    def qs_assemble(self, tangent=0):
+        self._call("nlm_qs_assemble", int(tangent))
+
+    def qs_tangent(self):
+        """(blocks[nnz, 2, 2] in reference CSR order, diag[n, 2, 2])."""
+        b = np.zeros((max(self.nnz, 1), 4))
+        d = np.zeros((self.p.n, 4))
+        self._call("nlm_qs_get_tangent", ptr(b, C.c_double), ptr(d, C.c_double))
+        return b[: self.nnz].reshape(-1, 2, 2), d.reshape(-1, 2, 2)
+
+    def qs_apply(self, x):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros((self.p.n, 3))
+        self._call("nlm_qs_apply", ptr(x, C.c_double), ptr(y, C.c_double))
+        return y
+
+    def qs_set_precond(self, blocks, diag):
+        b = np.ascontiguousarray(np.asarray(blocks).reshape(-1, 4), np.float64)
+        d = np.ascontiguousarray(np.asarray(diag).reshape(-1, 4), np.float64)
+        self._call("nlm_qs_set_precond", ptr(b, C.c_double), ptr(d, C.c_double))
+
+    def qs_precond(self):
+        """(blocks[nnz, 2, 2], diag[n, 2, 2]) of the preconditioner."""
+        b = np.zeros((max(self.nnz, 1), 4))
+        d = np.zeros((self.p.n, 4))
+        self._call("nlm_qs_get_precond", ptr(b, C.c_double), ptr(d, C.c_double))
+        return b[: self.nnz].reshape(-1, 2, 2), d.reshape(-1, 2, 2)
+
+    # -- learned preconditioner -------------------------------------------
+    def gnn_create(self, seed=1):
+        self._call("nlm_gnn_create", C.c_uint64(seed))
+
+    def gnn_weights(self):
+        w = np.zeros(self.lib.nlm_gnn_num_params(), np.float32)
+        self._call("nlm_gnn_get_weights", ptr(w, C.c_float))
+        return w
+
+    def gnn_set_weights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        self._call("nlm_gnn_set_weights", ptr(w, C.c_float))
+
+    def gnn_predict(self):
+        self._call("nlm_gnn_predict")
+
+    def gnn_loss_grad(self, r, e, want_grad=False):
+        """r, e: [K, n, 3] residual / error pairs; returns (loss, grad)."""
+        r = np.ascontiguousarray(r, np.float64)
+        e = np.ascontiguousarray(e, np.float64)
+        K = r.shape[0]
+        loss = C.c_double()
+        g = np.zeros(self.lib.nlm_gnn_num_params(), np.float32) if want_grad else None
+        self._call("nlm_gnn_loss_grad", int(K), ptr(r, C.c_double),
+                   ptr(e, C.c_double), C.byref(loss), ptr(g, C.c_float))
+        return loss.value, g
+
+    def gnn_adam_step(self, lr):
+        self._call("nlm_gnn_adam_step", float(lr))
+
+    def qs_linear_solve(self, b, precond=1, tol=1e-8, max_iter=5000):
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros((self.p.n, 3))
+        it = C.c_int()
+        rr = C.c_double()
+        self._call("nlm_qs_linear_solve", ptr(b, C.c_double), ptr(x, C.c_double),
+                   int(precond), float(tol), int(max_iter), C.byref(it),
+                   C.byref(rr))
+        return x, it.value, rr.value
+
     def node_fields(self):
         """State-based material: (dilatation, weighted volume) per node."""
         th = np.zeros(self.p.n)

@@ -226,6 +226,45 @@ def cube3d(nx: int, h: float, ratio: int, dt: float, nsteps: int,
                    ubc=ubc, shape=(nx, ny, nz), v0=v0)
 
 
+def qs_plate(nx: int, h: float = 1e-3, ratio: int = 3, pull: float = 1e-5,
+             layers: int = 3, nsteps: int = 20, jitter: float = 1e-6,
+             seed: int = 7, notch_len: float = 0.0, notch_angle: float = 0.0,
+             name: str | None = None, **mat) -> Problem:
+    """Quasi-static specimen: square plate of nx x nx nodes, the bottom
+    `layers` node rows clamped (both dofs), the top rows held at u_x = 0 and
+    displaced by u_y = pull * t (one load step = pull with dt = 1); optional
+    pre-notch from the left edge at mid-height (angle in radians)."""
+    x, vol = grid2d(nx, nx, h, jitter, seed)
+    xmax = ytop = (nx - 1) * h
+    ylo = ytop - (layers - 1) * h - 0.5 * h
+    ubc = [BC("rectangle", [1, 2], "constant", [0.0], x1=-h, x2=xmax + h,
+              y1=-0.5 * h, y2=(layers - 1) * h + 0.5 * h),
+           BC("rectangle", [1], "constant", [0.0], x1=-h, x2=xmax + h,
+              y1=ylo, y2=ytop + 0.5 * h),
+           BC("rectangle", [2], "linear", [pull], x1=-h, x2=xmax + h,
+              y1=ylo, y2=ytop + 0.5 * h)]
+    cracks = []
+    if notch_len > 0:
+        ym = 0.5 * ytop
+        if notch_angle == 0.0:
+            cracks.append(CrackSpec(1, (-0.5 * h, ym),
+                                    (notch_len - 0.5 * h, ym)))
+        else:
+            ca, sa = np.cos(notch_angle), np.sin(notch_angle)
+            pb = (-0.5 * h, ym)
+            pt = (pb[0] + notch_len * ca, pb[1] + notch_len * sa)
+            cracks.append(CrackSpec(0, pb, pt, theta=notch_angle))
+    m = rnp_material(dim=2, **mat)
+    return Problem(name=name or f"qs_plate{nx}", dim=2, x=x, vol=vol,
+                   horizon=ratio * h, h=h, dt=1.0, nsteps=nsteps, material=m,
+                   ubc=ubc, cracks=cracks, shape=(nx, nx))
+
+
+# Quasi-static workload: 300 x 300 nodes, h = 1e-3 m, horizon 3h, 20 load
+# steps of 1e-5 m on the top rows, bottom rows clamped.
+QS_2D = {"qs_plate300": dict(nx=300, h=1e-3, ratio=3, pull=1e-5, nsteps=20)}
+
+
 # 3D ElasticState block: side in nodes, mesh size, horizon/h, step, steps.
 CUBE_3D = {
     # 160^3 nodes, h = 6.25e-4 m, horizon 3h, 500 steps
@@ -234,6 +273,10 @@ CUBE_3D = {
 
 
 def workload(key: str, **over) -> Problem:
+    if key in QS_2D:
+        spec = dict(QS_2D[key])
+        spec.update(over)
+        return qs_plate(name=key, **spec)
     if key in CUBE_3D:
         spec = dict(CUBE_3D[key])
         spec.update(over)

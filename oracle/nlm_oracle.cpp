@@ -203,6 +203,34 @@ inline void bond_ef(const ora_rnp& m, const Influence& J, const P3& xi,
   force = (influence * 4. * Sji * m.C * m.beta / m.inv_factor) * volj * eij;
 }
 
+// Derivative of getBondEF's force with respect to u_j at fixed bond state:
+// dF_ij/du_j = k e e^T, dF_ij/du_i = -k e e^T, with S = dx.du/dx.dx and the
+// reference direction e = dx/|dx| (rnpBond.cpp:243-300).  The quasi-static
+// tangent of the bond-based material is assembled from these scalars; the
+// contact force of broken bonds is not differentiated (k = 0).
+inline double bond_tangent(const ora_rnp& m, const Influence& J, const P3& xi,
+                           const P3& ui, const P3& xj, const P3& uj,
+                           double volj, bool interior_i, bool interior_j,
+                           bool fs) {
+  bool break_bonds = interior_i && interior_j;
+  double rji = dist(xj, xi);
+  P3 dx = xj - xi;
+  P3 du = uj - ui;
+  double Sji = dot(dx, du) / dot(dx, dx);
+  double h = m.h;
+  double check_up = m.horizon + 0.5 * h;
+  double check_low = m.horizon - 0.5 * h;
+  if (dgt(rji, check_low)) volj *= (check_up - rji) / h;
+  double influence = J.value(rji / m.horizon);
+  double a = influence * 4. * m.C * m.beta / m.inv_factor * volj;
+  if (break_bonds) {
+    if (fs) return 0.;
+    double q = m.beta * rji * Sji * Sji;
+    return a * std::exp(-q) * (1. - 2. * q) / rji;
+  }
+  return a / rji;
+}
+
 // ---------------------------------------------------------------------------
 // Geometry predicates, src/util/utilGeom.cpp
 // ---------------------------------------------------------------------------
@@ -721,6 +749,19 @@ struct FD {
     return force_i;
   }
 
+  // per-bond tangent scalars k (CSR order) at the current u and bond state
+  void tangent(double* kout) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++)
+      for (int64_t k = row_ptr[i]; k < row_ptr[i + 1]; k++) {
+        int64_t jl = k - row_ptr[i];
+        int64_t j = col[k];
+        kout[k] = bond_tangent(m, J, x[i], u[i], x[j], u[j], vol[j],
+                               interior[i] != 0, interior[j] != 0,
+                               get_bond(i, jl));
+      }
+  }
+
   // FDModel::computeForces, fDModel.cpp:528-541
   void compute_forces() {
 #pragma omp parallel for schedule(static)
@@ -1192,6 +1233,7 @@ void ora_es_get_node_fields(void* h, double* theta, double* mvol) {
   }
 }
 double ora_es_time(void* h) { return static_cast<ES*>(h)->time; }
+void ora_fd_tangent(void* h, double* k) { static_cast<FD*>(h)->tangent(k); }
 void ora_es_force_abs_sum(void* h, double* out) {
   static_cast<ES*>(h)->force_abs_sum(out);
 }

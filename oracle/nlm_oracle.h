@@ -172,6 +172,10 @@ void ora_fd_postproc(void* h, float* Z, float* phi, double* energy);
 /* per node sum_j |F_ij| at the current state (bond bits untouched) */
 void ora_fd_force_abs_sum(void* h, double* out);
 void ora_fd_set_time(void* h, double t, int64_t step);
+/* Bond-based tangent: per directed bond (CSR order) the scalar k with
+ * dF_ij/du_j = k e e^T, e = (xj - xi)/|xj - xi|, at the current u and bond
+ * state (derivative of getBondEF, rnpBond.cpp:216-301). */
+void ora_fd_tangent(void* h, double* k);
 /* ComputeInterior::getInteriorFlag (interiorFlags.cpp:57-100) for rectangle
  * and circle no-fail regions; regions[k] = {type(0 rect,1 circle), p0..p3}. */
 void ora_interior_flags(int64_t n, const double* x, const double* bbox_lo,

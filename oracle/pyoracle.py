@@ -146,6 +146,7 @@ SIGNATURES = {
     "ora_es_get_node_fields": (None, [_P, _D, _D]),
     "ora_es_time": (C.c_double, [_P]),
     "ora_es_force_abs_sum": (None, [_P, _D]),
+    "ora_fd_tangent": (None, [_P, _D]),
 }
 
 _lib = None
@@ -341,6 +342,20 @@ class OracleFD:
         out = np.zeros(self.p.n)
         self.lib.ora_fd_force_abs_sum(self.h, _p(out, C.c_double))
         return out
+
+    def tangent(self):
+        """Per-bond k of dF_ij/du_j = k e e^T at the current state."""
+        k = np.zeros(max(self.nnz, 1))
+        self.lib.ora_fd_tangent(self.h, _p(k, C.c_double))
+        return k[: self.nnz]
+
+    def internal_force(self, u):
+        """f_int(u) with the current bond bits (which the evaluation may
+        update, as computeForces does)."""
+        z = np.zeros((self.p.n, 3))
+        self.set_state(u=u, v=z, f=z)
+        self.compute_forces()
+        return self.get_state()[2]
 
     def postproc(self):
         n = self.p.n
