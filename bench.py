@@ -66,6 +66,9 @@ def parse():
                     help="state-based workload reported beside the plate")
     ap.add_argument("--es-steps", type=int, default=500)
     ap.add_argument("--no-es", action="store_true")
+    ap.add_argument("--qs-workload", default="qs_plate300")
+    ap.add_argument("--qs-steps", type=int, default=20)
+    ap.add_argument("--no-qs", action="store_true")
     return ap.parse_args()
 
 
@@ -155,6 +158,56 @@ def state_based(a):
             for name, r in res.items() if name != "exact"}
     out["fp32_over_fp64"] = (out["modes"]["fp32"]["value"] /
                              out["modes"]["fp64"]["value"])
+    return out
+
+
+def quasi_static(a):
+    """Quasi-static Newton on the 300 x 300 plate: time to reach the 1e-8
+    relative Newton residual in every load step with point-Jacobi BiCGSTAB
+    and with the graph-network preconditioner (trained first, on randomized
+    48 x 48 specimens, by the same library)."""
+    from nlmech_amd import gnn_train
+    from nlmech_amd import problems as pr
+    from nlmech_amd.capi import (NLM_PREC_FP64_EXACT, NLM_QS_PRECOND_BLOCKS,
+                                 NLM_QS_PRECOND_JACOBI)
+    from nlmech_amd.engine import GpuFD
+    t0 = time.perf_counter()
+    net = gnn_train.train(nx=48, specimens=100, load_steps=5, epochs=6,
+                          seed=1, log=lambda m: print(m, file=sys.stderr,
+                                                      flush=True)
+                          if m.endswith("samples") is False else None)
+    t_train = time.perf_counter() - t0
+    p = pr.workload(a.qs_workload, nsteps=a.qs_steps)
+    out = {"workload": a.qs_workload, "nodes": p.n, "load_steps": a.qs_steps,
+           "newton_tol_rel": 1e-8, "linear_tol_rel": 1e-8,
+           "training": {"pairs": net["pairs"], "specimens": 100,
+                        "loss_first_epoch": net["loss"][0],
+                        "loss_last_epoch": net["loss"][-1],
+                        "s_data": net["s_data"], "s_train": net["s_train"],
+                        "s_total": t_train}}
+    C = 16  # two-level coarse correction flag
+    for name, pre in (("jacobi", NLM_QS_PRECOND_JACOBI),
+                      ("learned", NLM_QS_PRECOND_BLOCKS),
+                      ("jacobi_two_level", NLM_QS_PRECOND_JACOBI | C),
+                      ("learned_two_level", NLM_QS_PRECOND_BLOCKS | C)):
+        g = GpuFD(p, precision=NLM_PREC_FP64_EXACT)
+        if pre & 15 == NLM_QS_PRECOND_BLOCKS:
+            g.gnn_create(1)
+            g.gnn_set_weights(net["weights"])
+        st = g.qs_solve(a.qs_steps, tol_rel=1e-8, precond=pre, lin_tol=1e-8,
+                        lin_max_iter=50000, gnn_refresh=False)
+        out[name] = st
+        g.close()
+        print(f"[bench] quasi-static {name}: {st['ms_total']:.1f} ms, "
+              f"{st['newton_iters']} Newton, {st['linear_iters']} BiCGSTAB "
+              f"iterations", file=sys.stderr, flush=True)
+    out["time_ratio_jacobi_over_learned"] = (
+        out["jacobi"]["ms_total"] / out["learned_two_level"]["ms_total"])
+    out["time_ratio_network_gain_two_level"] = (
+        out["jacobi_two_level"]["ms_total"] /
+        out["learned_two_level"]["ms_total"])
+    out["time_ratio_network_gain_one_level"] = (
+        out["jacobi"]["ms_total"] / out["learned"]["ms_total"])
     return out
 
 
@@ -282,6 +335,8 @@ def main():
                                  out["modes"]["fp64"]["value"])
     if not a.no_es:
         out["state_based_3d"] = state_based(a)
+    if not a.no_qs:
+        out["quasi_static"] = quasi_static(a)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, csr, a.cpu_seconds)
     else:

@@ -325,6 +325,11 @@ const char* nlm_fd_kernel_name(nlm_ctx* ctx);
                                     blocks on the neighbor sparsity, set by
                                     nlm_qs_set_precond or predicted by the
                                     graph network (nlm_gnn_*) */
+#define NLM_QS_PRECOND_COARSE 16 /* flag: add the two-level coarse correction
+                                    R A_c^-1 R^T (aggregates of 256 nodes,
+                                    piecewise constant; A_c = R^T A_ff R
+                                    inverted densely, at most 4096
+                                    aggregates) to JACOBI or BLOCKS */
 
 typedef struct {
   int nsteps;         /* load steps (ModelDeck Time_Steps) */
@@ -368,7 +373,8 @@ int nlm_qs_get_tangent(nlm_ctx* ctx, double* blocks, double* diag);
 /* y = A_ff x on the free dofs (x, y AoS double[3n]; entries of prescribed
  * dofs of y are 0 and those of x are ignored). */
 int nlm_qs_apply(nlm_ctx* ctx, const double* x, double* y);
-/* Preconditioner blocks in the same layout as nlm_qs_get_tangent. */
+/* Preconditioner blocks in the same layout as nlm_qs_get_tangent (stored
+ * in FP32 on the device). */
 int nlm_qs_set_precond(nlm_ctx* ctx, const double* blocks, const double* diag);
 int nlm_qs_get_precond(nlm_ctx* ctx, double* blocks, double* diag);
 /* Solves A_ff x = b with the assembled tangent (b, x AoS double[3n]). */

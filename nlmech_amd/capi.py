@@ -26,6 +26,7 @@ NLM_QS_TANGENT_COMPONENT = 1
 NLM_QS_PRECOND_NONE = 0
 NLM_QS_PRECOND_JACOBI = 1
 NLM_QS_PRECOND_BLOCKS = 2
+NLM_QS_PRECOND_COARSE = 16
 
 REGION = {"rectangle": 0, "angled_rectangle": 1, "circle": 2, "torus": 3,
           "line": 4, "cuboid": 5}

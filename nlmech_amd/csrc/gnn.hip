@@ -367,8 +367,8 @@ struct DecArgs {
   const uint16_t* gL;    // [E][64]
   const uint16_t* hL;    // [N][64]
   const float* isq;
-  double4* mblk;
-  double4* mdiag;
+  float4* mblk;
+  float4* mdiag;
 };
 
 // 2x2 mask of the preconditioner block (i rows, j columns): prescribed dofs
@@ -400,7 +400,7 @@ __global__ __launch_bounds__(256) void k_gnn_decode(DecArgs a) {
   const float isqi = ok ? a.isq[i] : 0.f;
   for (int k = 0; k < width; k++) {
     const int64_t e = cb + int64_t(k) * 64 + lane;
-    double4 out = make_double4(0., 0., 0., 0.);
+    float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
     if (k < len) {
       const int32_t j = a.g.col[e];
       float o[4] = {boe[0], boe[1], boe[2], boe[3]};
@@ -412,8 +412,8 @@ __global__ __launch_bounds__(256) void k_gnn_decode(DecArgs a) {
       float mk[4];
       block_mask(fi, a.g.flags[j], mk);
       const double w = double(isqi) * double(a.isq[j]);
-      out = make_double4(o[0] * mk[0] * w, o[1] * mk[1] * w, o[2] * mk[2] * w,
-                         o[3] * mk[3] * w);
+      out = make_float4(float(o[0] * mk[0] * w), float(o[1] * mk[1] * w),
+                        float(o[2] * mk[2] * w), float(o[3] * mk[3] * w));
     }
     a.mblk[e] = out;
   }
@@ -429,16 +429,16 @@ __global__ __launch_bounds__(256) void k_gnn_decode(DecArgs a) {
   float mk[4];
   block_mask(fi, fi, mk);
   const double w = double(isqi) * double(isqi);
-  a.mdiag[i] = make_double4(o[0] * mk[0] * w, o[1] * mk[1] * w,
-                            o[2] * mk[2] * w, o[3] * mk[3] * w);
+  a.mdiag[i] = make_float4(float(o[0] * mk[0] * w), float(o[1] * mk[1] * w),
+                           float(o[2] * mk[2] * w), float(o[3] * mk[3] * w));
 }
 
 // ---- loss ------------------------------------------------------------------
 
 // z_k = M r_k for K vectors (double2 per node, K blocks of n).
 __global__ __launch_bounds__(256) void k_gnn_spmm(
-    GraphArgs g, const double4* __restrict__ mblk,
-    const double4* __restrict__ mdiag, int K, const double2* __restrict__ r,
+    GraphArgs g, const float4* __restrict__ mblk,
+    const float4* __restrict__ mdiag, int K, const double2* __restrict__ r,
     double2* __restrict__ z) {
   int64_t s;
   int lane;
@@ -449,12 +449,12 @@ __global__ __launch_bounds__(256) void k_gnn_spmm(
   const int len = int(g.rowlen[i]);
   for (int kv = 0; kv < K; kv++) {
     const double2* rv = r + int64_t(kv) * g.n;
-    const double4 D = mdiag[i];
+    const float4 D = mdiag[i];
     const double2 ri = rv[i];
     double ax = D.x * ri.x + D.y * ri.y, ay = D.z * ri.x + D.w * ri.y;
     for (int k = 0; k < len; k++) {
       const int64_t e = cb + int64_t(k) * 64 + lane;
-      const double4 B = mblk[e];
+      const float4 B = mblk[e];
       const double2 rj = rv[g.col[e]];
       ax += B.x * rj.x + B.y * rj.y;
       ay += B.z * rj.x + B.w * rj.y;
@@ -918,9 +918,9 @@ int gnn_forward(nlm_ctx* c) {
   int rc;
   if ((rc = gnn_size(c))) return rc;
   const int64_t n = s.n, ne = s.sell.nentries;
-  if (!q.mblk.p || q.mblk.bytes < sizeof(double4) * ne) {
-    if ((rc = alloc(c, q.mblk, sizeof(double4) * ne + 256))) return rc;
-    if ((rc = alloc(c, q.mdiag, sizeof(double4) * n + 256))) return rc;
+  if (!q.mblk.p || q.mblk.bytes < sizeof(float4) * ne) {
+    if ((rc = alloc(c, q.mblk, sizeof(float4) * ne + 256))) return rc;
+    if ((rc = alloc(c, q.mdiag, sizeof(float4) * n + 256))) return rc;
   }
   const GraphArgs ga = graph_args(c);
   const unsigned gn = unsigned((n + 255) / 256);
@@ -978,8 +978,8 @@ int gnn_forward(nlm_ctx* c) {
   d.gL = gbuf(G, kL);
   d.hL = hbuf(G, kL);
   d.isq = G.isq.as<float>();
-  d.mblk = q.mblk.as<double4>();
-  d.mdiag = q.mdiag.as<double4>();
+  d.mblk = q.mblk.as<float4>();
+  d.mdiag = q.mdiag.as<float4>();
   hipLaunchKernelGGL(k_gnn_decode, dim3(gs), dim3(256), 0, c->stream, d);
   NLM_HIP(c, hipGetLastError());
   q.has_m = true;
@@ -1209,7 +1209,7 @@ extern "C" int nlm_gnn_loss_grad(nlm_ctx* c, int K, const double* r,
   const unsigned gs = unsigned((s.sell.nslices + 3) / 4);
   const unsigned gn = unsigned((n + 255) / 256);
   hipLaunchKernelGGL(k_gnn_spmm, dim3(gs), dim3(256), 0, c->stream, ga,
-                     c->qs.mblk.as<double4>(), c->qs.mdiag.as<double4>(), K,
+                     c->qs.mblk.as<float4>(), c->qs.mdiag.as<float4>(), K,
                      G.vr.as<double2>(), G.vz.as<double2>());
   hipLaunchKernelGGL(k_gnn_resid, dim3(gn), dim3(256), 0, c->stream, n, K,
                      dck.as<double>(), G.vz.as<double2>(), G.ve.as<double2>(),

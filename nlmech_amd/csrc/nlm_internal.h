@@ -173,6 +173,11 @@ struct QsState {
   DevBuf ctl;                // int control words (done flag, iteration)
   int64_t nb = 0;            // 256-node blocks
   int max_iter_alloc = 0;
+  // two-level: aggregate adjacency, dense coarse matrix and inverse, coarse
+  // restriction / correction vectors
+  DevBuf adj_ptr, adj, cA, cB, rc, zcp, zcs;
+  double* cinv = nullptr;
+  int m = 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
 };
 

@@ -149,12 +149,13 @@ __global__ __launch_bounds__(256) void k_qs_tangent(
 
 // y = (D x_i + sum_j B_ij x_j) on free dofs, 0 on prescribed dofs.
 // DOT 1: partial w.y into P0; DOT 2: partials w.y into P0 and y.y into P1.
-template <int DOT>
+template <int DOT, class BT>
 __global__ __launch_bounds__(256) void k_qs_spmv(
-    QsArgs a, const double4* __restrict__ blk, const double4* __restrict__ dg,
+    QsArgs a, const BT* __restrict__ blk, const BT* __restrict__ dg,
     const double2* __restrict__ xv, double2* __restrict__ y,
     const double2* __restrict__ wv, double* __restrict__ P0,
-    double* __restrict__ P1, const int* __restrict__ ctl) {
+    double* __restrict__ P1, const int* __restrict__ ctl,
+    const double* __restrict__ zc) {
   if (ctl && ctl[0]) return;
   const int64_t nblk = (a.nslices * 64 + 255) / 256;
   const int64_t lb = xcd_block(nblk);
@@ -171,16 +172,20 @@ __global__ __launch_bounds__(256) void k_qs_spmv(
     double2 acc = make_double2(0., 0.);
     for (int k = 0; k < width; k++) {
       const int64_t e = cb + int64_t(k) * 64 + lane;
-      const double4 B = blk[e];
+      const BT B = blk[e];
       const double2 xj = xv[a.col[e]];
-      acc.x += B.x * xj.x + B.y * xj.y;
-      acc.y += B.z * xj.x + B.w * xj.y;
+      acc.x += double(B.x) * xj.x + double(B.y) * xj.y;
+      acc.y += double(B.z) * xj.x + double(B.w) * xj.y;
     }
     if (valid) {
-      const double4 D = dg[i];
+      const BT D = dg[i];
       const double2 xi = xv[i];
-      acc.x += D.x * xi.x + D.y * xi.y;
-      acc.y += D.z * xi.x + D.w * xi.y;
+      acc.x += double(D.x) * xi.x + double(D.y) * xi.y;
+      acc.y += double(D.z) * xi.x + double(D.w) * xi.y;
+      if (zc) {
+        acc.x += zc[2 * (i >> 8)];
+        acc.y += zc[2 * (i >> 8) + 1];
+      }
       const uint32_t fl = a.flags[i];
       if (dof_fixed(fl, 0)) acc.x = 0.;
       if (dof_fixed(fl, 1)) acc.y = 0.;
@@ -214,9 +219,18 @@ struct VecArgs {
   double2 *r, *rh, *p, *v, *s, *t, *x, *ph, *sh;
   const double2* __restrict__ dinv;
   double *P0, *P1, *P2, *P3;
+  double *rc;  // restriction of p (then s): per 256-node aggregate, 2 dofs
   int* ctl;
   Scal sc;
 };
+
+// Restriction R^T v of the workgroup's 256 nodes (one aggregate of the
+// coarse level): rc[2 b + d] = sum over the aggregate of v_d.
+__device__ void block_restrict(double vx, double vy, double* __restrict__ rc,
+                               int64_t b) {
+  block_partial(vx, rc, 2 * b);
+  block_partial(vy, rc, 2 * b + 1);
+}
 
 // r = rh = b (masked), p = v = x = 0; partials (rh, r) and (r, r).
 __global__ __launch_bounds__(256) void k_bicg_init(VecArgs a,
@@ -249,7 +263,7 @@ __global__ __launch_bounds__(256) void k_bicg_init(VecArgs a,
 // Iteration it, step 1: convergence test on r_{it-1}; p = r + beta (p -
 // omega v); ph = M p for the diagonal preconditioners (PRE 0 none, 1 Jacobi;
 // PRE 2 leaves ph to a block product).
-template <int PRE>
+template <int PRE, bool RESTRICT = false>
 __global__ __launch_bounds__(256) void k_bicg_p(VecArgs a, int it) {
   if (a.ctl[0]) return;
   const double rho_i = block_sum(a.P0, a.nb);
@@ -271,21 +285,23 @@ __global__ __launch_bounds__(256) void k_bicg_p(VecArgs a, int it) {
     beta = (rho_i / a.sc.rho(it - 2)) * (a.sc.alpha(it - 1) / om);
   }
   const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (i >= a.n) return;
-  const double2 r = a.r[i], p = a.p[i], v = a.v[i];
-  double2 pn;
-  pn.x = r.x + beta * (p.x - om * v.x);
-  pn.y = r.y + beta * (p.y - om * v.y);
-  a.p[i] = pn;
-  if (PRE == 0) a.ph[i] = pn;
-  if (PRE == 1) {
-    const double2 di = a.dinv[i];
-    a.ph[i] = make_double2(di.x * pn.x, di.y * pn.y);
+  double2 pn = make_double2(0., 0.);
+  if (i < a.n) {
+    const double2 r = a.r[i], p = a.p[i], v = a.v[i];
+    pn.x = r.x + beta * (p.x - om * v.x);
+    pn.y = r.y + beta * (p.y - om * v.y);
+    a.p[i] = pn;
+    if (PRE == 0) a.ph[i] = pn;
+    if (PRE == 1) {
+      const double2 di = a.dinv[i];
+      a.ph[i] = make_double2(di.x * pn.x, di.y * pn.y);
+    }
   }
+  if (RESTRICT) block_restrict(pn.x, pn.y, a.rc, blockIdx.x);
 }
 
 // Step 2 (after v = A ph with partial (rh, v) in P2): alpha, s = r - alpha v.
-template <int PRE>
+template <int PRE, bool RESTRICT = false>
 __global__ __launch_bounds__(256) void k_bicg_s(VecArgs a, int it) {
   if (a.ctl[0]) return;
   const double rv = block_sum(a.P2, a.nb);
@@ -301,15 +317,150 @@ __global__ __launch_bounds__(256) void k_bicg_s(VecArgs a, int it) {
   }
   if (brk) return;
   const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (i >= a.n) return;
-  const double2 r = a.r[i], v = a.v[i];
-  const double2 sn = make_double2(r.x - alpha * v.x, r.y - alpha * v.y);
-  a.s[i] = sn;
-  if (PRE == 0) a.sh[i] = sn;
-  if (PRE == 1) {
-    const double2 di = a.dinv[i];
-    a.sh[i] = make_double2(di.x * sn.x, di.y * sn.y);
+  double2 sn = make_double2(0., 0.);
+  if (i < a.n) {
+    const double2 r = a.r[i], v = a.v[i];
+    sn = make_double2(r.x - alpha * v.x, r.y - alpha * v.y);
+    a.s[i] = sn;
+    if (PRE == 0) a.sh[i] = sn;
+    if (PRE == 1) {
+      const double2 di = a.dinv[i];
+      a.sh[i] = make_double2(di.x * sn.x, di.y * sn.y);
+    }
   }
+  if (RESTRICT) block_restrict(sn.x, sn.y, a.rc, blockIdx.x);
+}
+
+// ---- coarse level ------------------------------------------------------------
+// Aggregates are the layout's 256-node blocks (compact Morton patches); the
+// coarse space is piecewise constant per aggregate and dof.  A_c = R^T A R
+// on the free dofs, inverted densely once per refresh (Gauss-Jordan sweeps,
+// the matrix being symmetric positive definite up to the volume asymmetry of
+// the bond-based tangent); the preconditioner is S + R A_c^-1 R^T.
+
+// zc = Ainv rc (dense m x m, one wavefront per row).
+__global__ __launch_bounds__(256) void k_coarse_mv(const double* __restrict__ Ainv,
+                                                   int m,
+                                                   const double* __restrict__ rc,
+                                                   double* __restrict__ zc,
+                                                   const int* __restrict__ ctl) {
+  if (ctl && ctl[0]) return;
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= m) return;
+  const double* ar = Ainv + int64_t(row) * m;
+  double s = 0.;
+  for (int k = lane; k < m; k += 64) s += ar[k] * rc[k];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if (lane == 0) zc[row] = s;
+}
+
+// dst = dinv * src + zc[aggregate] on free dofs (point-Jacobi smoother plus
+// the prolongated coarse correction).
+__global__ __launch_bounds__(256) void k_smooth_jacobi(
+    VecArgs a, const double2* __restrict__ src, double2* __restrict__ dst,
+    const double* __restrict__ zc) {
+  if (a.ctl[0]) return;
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= a.n) return;
+  const double2 di = a.dinv[i], v = src[i];
+  const uint32_t fl = a.flags[i];
+  const int64_t b = i >> 8;
+  double2 o = make_double2(di.x * v.x, di.y * v.y);
+  if (zc) {
+    o.x += zc[2 * b];
+    o.y += zc[2 * b + 1];
+  }
+  if (dof_fixed(fl, 0)) o.x = 0.;
+  if (dof_fixed(fl, 1)) o.y = 0.;
+  dst[i] = o;
+}
+
+// Coarse matrix rows of aggregate b: A_c[(b, d)][(J, e)] for the aggregates J
+// adjacent to b (adj list), free rows and columns only; dense row-major.
+__global__ __launch_bounds__(256) void k_coarse_assemble(
+    QsArgs a, const double4* __restrict__ blk, const double4* __restrict__ dg,
+    const int64_t* __restrict__ adj_ptr, const int32_t* __restrict__ adj,
+    int m, double* __restrict__ Ac) {
+  __shared__ double red[4][256];
+  const int64_t b = blockIdx.x;
+  const int64_t i = b * 256 + threadIdx.x;
+  const bool valid = i < a.n;
+  const int64_t s = i >> 6;
+  const int lane = int(i & 63);
+  const uint32_t fi = valid ? a.flags[i] : 0u;
+  const double ri0 = dof_fixed(fi, 0) ? 0. : 1., ri1 = dof_fixed(fi, 1) ? 0. : 1.;
+  const int64_t cb = valid ? a.sl_ptr[s] : 0;
+  const int len = valid ? int(a.rowlen[i]) : 0;
+  for (int64_t q = adj_ptr[b]; q < adj_ptr[b + 1]; q++) {
+    const int32_t J = adj[q];
+    double acc[4] = {0., 0., 0., 0.};
+    for (int k = 0; k < len; k++) {
+      const int64_t e = cb + int64_t(k) * 64 + lane;
+      const int32_t j = a.col[e];
+      if ((j >> 8) != J) continue;
+      const uint32_t fj = a.flags[j];
+      const double cj0 = dof_fixed(fj, 0) ? 0. : 1.;
+      const double cj1 = dof_fixed(fj, 1) ? 0. : 1.;
+      const double4 B = blk[e];
+      acc[0] += B.x * ri0 * cj0;
+      acc[1] += B.y * ri0 * cj1;
+      acc[2] += B.z * ri1 * cj0;
+      acc[3] += B.w * ri1 * cj1;
+    }
+    if (valid && J == b) {
+      const double4 D = dg[i];
+      acc[0] += D.x * ri0 * ri0;
+      acc[1] += D.y * ri0 * ri1;
+      acc[2] += D.z * ri1 * ri0;
+      acc[3] += D.w * ri1 * ri1;
+    }
+    for (int t = 0; t < 4; t++) red[t][threadIdx.x] = acc[t];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (threadIdx.x < w)
+        for (int t = 0; t < 4; t++)
+          red[t][threadIdx.x] += red[t][threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      Ac[(2 * b) * m + 2 * J] = red[0][0];
+      Ac[(2 * b) * m + 2 * J + 1] = red[1][0];
+      Ac[(2 * b + 1) * m + 2 * J] = red[2][0];
+      Ac[(2 * b + 1) * m + 2 * J + 1] = red[3][0];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_fill(double* __restrict__ p, int64_t cnt, double v) {
+  const int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (t < cnt) p[t] = v;
+}
+
+// Coarse dofs without a free fine dof get an identity row and column.
+__global__ void k_coarse_fixdiag(double* __restrict__ Ac, int m) {
+  const int d = blockIdx.x * 256 + threadIdx.x;
+  if (d < m && Ac[int64_t(d) * m + d] == 0.) Ac[int64_t(d) * m + d] = 1.;
+}
+
+// One Gauss-Jordan sweep on pivot k, A -> B (after all m sweeps, A^-1).
+__global__ void k_gj_sweep(const double* __restrict__ A, double* __restrict__ B,
+                           int m, int k) {
+  const int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (t >= int64_t(m) * m) return;
+  const int i = int(t / m), j = int(t % m);
+  const double p = A[int64_t(k) * m + k];
+  double v;
+  if (i == k && j == k)
+    v = 1. / p;
+  else if (i == k)
+    v = A[int64_t(k) * m + j] / p;
+  else if (j == k)
+    v = -A[int64_t(i) * m + k] / p;
+  else
+    v = A[t] - A[int64_t(i) * m + k] * A[int64_t(k) * m + j] / p;
+  B[t] = v;
 }
 
 // Step 3 (after t = A sh with partials (t, s) in P2 and (t, t) in P3):
@@ -499,6 +650,7 @@ VecArgs vec_args(nlm_ctx* c) {
   a.P1 = P + q.nb;
   a.P2 = P + 2 * q.nb;
   a.P3 = P + 3 * q.nb;
+  a.rc = nullptr;
   a.ctl = q.ctl.as<int>();
   a.sc.base = q.scal.as<double>();
   a.sc.L = q.max_iter_alloc;
@@ -536,13 +688,90 @@ int assemble(nlm_ctx* c, int tangent) {
   return NLM_OK;
 }
 
-template <int DOT>
+template <int DOT, class BT = double4>
 void spmv(nlm_ctx* c, const DevBuf& blk, const DevBuf& dg, const double2* x,
           double2* y, const double2* w, double* P0, double* P1,
-          const int* ctl) {
-  hipLaunchKernelGGL(k_qs_spmv<DOT>, dim3(slice_grid(c->fd)), dim3(256), 0,
-                     c->stream, qs_args(c), blk.as<double4>(),
-                     dg.as<double4>(), x, y, w, P0, P1, ctl);
+          const int* ctl, const double* zc = nullptr) {
+  hipLaunchKernelGGL((k_qs_spmv<DOT, BT>), dim3(slice_grid(c->fd)), dim3(256),
+                     0, c->stream, qs_args(c), blk.as<BT>(), dg.as<BT>(), x, y,
+                     w, P0, P1, ctl, zc);
+}
+// The preconditioner's blocks are FP32 (their rounding only perturbs the
+// preconditioner, never the operator).
+void mapply(nlm_ctx* c, const double2* x, double2* y, const int* ctl,
+            const double* zc) {
+  spmv<0, float4>(c, c->qs.mblk, c->qs.mdiag, x, y, nullptr, nullptr, nullptr,
+                  ctl, zc);
+}
+
+// Coarse operator of the assembled tangent: adjacency of the aggregates
+// (once per layout), A_c assembly and its inverse in q.cinv.
+int coarse_setup(nlm_ctx* c) {
+  FdState& s = c->fd;
+  QsState& q = c->qs;
+  int rc;
+  const int64_t nagg = q.nb;
+  const int m = int(2 * nagg);
+  if (m > 8192)
+    return fail(c, NLM_ERR_INVALID,
+                "two-level preconditioner: more than 4096 aggregates");
+  if (!q.adj_ptr.p) {
+    std::vector<int64_t> slp(s.sell.nslices + 1);
+    std::vector<int32_t> col(s.sell.nentries);
+    std::vector<uint16_t> rl(s.n);
+    NLM_HIP(c, hipStreamSynchronize(c->stream));
+    NLM_HIP(c, hipMemcpy(slp.data(), s.sell.sl_ptr.p,
+                         sizeof(int64_t) * slp.size(), hipMemcpyDeviceToHost));
+    NLM_HIP(c, hipMemcpy(col.data(), s.sell.col.p,
+                         sizeof(int32_t) * col.size(), hipMemcpyDeviceToHost));
+    NLM_HIP(c, hipMemcpy(rl.data(), s.sell.rowlen.p, sizeof(uint16_t) * s.n,
+                         hipMemcpyDeviceToHost));
+    std::vector<int64_t> ap(nagg + 1, 0);
+    std::vector<int32_t> adj;
+    for (int64_t b = 0; b < nagg; b++) {
+      std::vector<int32_t> J{int32_t(b)};
+      for (int64_t i = b * 256; i < std::min(s.n, b * 256 + 256); i++)
+        for (int k = 0; k < rl[i]; k++)
+          J.push_back(col[slp[i / 64] + 64 * k + i % 64] >> 8);
+      std::sort(J.begin(), J.end());
+      J.erase(std::unique(J.begin(), J.end()), J.end());
+      adj.insert(adj.end(), J.begin(), J.end());
+      ap[b + 1] = int64_t(adj.size());
+    }
+    if ((rc = alloc(c, q.adj_ptr, sizeof(int64_t) * ap.size()))) return rc;
+    if ((rc = alloc(c, q.adj, sizeof(int32_t) * adj.size()))) return rc;
+    NLM_HIP(c, hipMemcpy(q.adj_ptr.p, ap.data(), sizeof(int64_t) * ap.size(),
+                         hipMemcpyHostToDevice));
+    NLM_HIP(c, hipMemcpy(q.adj.p, adj.data(), sizeof(int32_t) * adj.size(),
+                         hipMemcpyHostToDevice));
+    const size_t mb = sizeof(double) * size_t(m) * m + 256;
+    if ((rc = alloc(c, q.cA, mb)) || (rc = alloc(c, q.cB, mb)) ||
+        (rc = alloc(c, q.rc, sizeof(double) * (m + 64))) ||
+        (rc = alloc(c, q.zcp, sizeof(double) * (m + 64))) ||
+        (rc = alloc(c, q.zcs, sizeof(double) * (m + 64))))
+      return rc;
+  }
+  const int64_t mm2 = int64_t(m) * m;
+  const unsigned gm = unsigned((mm2 + 255) / 256);
+  hipLaunchKernelGGL(k_fill, dim3(gm), dim3(256), 0, c->stream,
+                     q.cA.as<double>(), mm2, 0.);
+  hipLaunchKernelGGL(k_coarse_assemble, dim3(unsigned(nagg)), dim3(256), 0,
+                     c->stream, qs_args(c), q.blk.as<double4>(),
+                     q.diag.as<double4>(), q.adj_ptr.as<int64_t>(),
+                     q.adj.as<int32_t>(), m, q.cA.as<double>());
+  hipLaunchKernelGGL(k_coarse_fixdiag, dim3(unsigned((m + 255) / 256)),
+                     dim3(256), 0, c->stream, q.cA.as<double>(), m);
+  double* A = q.cA.as<double>();
+  double* B = q.cB.as<double>();
+  for (int k = 0; k < m; k++) {
+    hipLaunchKernelGGL(k_gj_sweep, dim3(gm), dim3(256), 0, c->stream, A, B, m,
+                       k);
+    std::swap(A, B);
+  }
+  q.cinv = A;
+  q.m = m;
+  NLM_HIP(c, hipGetLastError());
+  return NLM_OK;
 }
 
 // BiCGSTAB on A_ff x = b (q.b -> q.x).  Returns iterations, relative
@@ -553,9 +782,15 @@ int bicgstab(nlm_ctx* c, int precond, double tol, int max_iter, int* iters,
   QsState& q = c->qs;
   int rc;
   if ((rc = qs_alloc(c, max_iter))) return rc;
+  const bool coarse = (precond & NLM_QS_PRECOND_COARSE) != 0;
+  precond &= ~NLM_QS_PRECOND_COARSE;
   if (precond == NLM_QS_PRECOND_BLOCKS && !q.has_m)
     return fail(c, NLM_ERR_STATE, "BiCGSTAB: no preconditioner blocks set");
+  if (coarse && (!q.cinv || precond == NLM_QS_PRECOND_NONE))
+    return fail(c, NLM_ERR_STATE, "BiCGSTAB: no coarse operator set up");
   VecArgs a = vec_args(c);
+  a.rc = coarse ? q.rc.as<double>() : nullptr;
+  const unsigned gmv = unsigned((q.m + 3) / 4);
   const unsigned g = node_grid(s);
   hipLaunchKernelGGL(k_bicg_init, dim3(g), dim3(256), 0, c->stream, a,
                      q.b.as<double2>());
@@ -578,6 +813,30 @@ int bicgstab(nlm_ctx* c, int precond, double tol, int max_iter, int* iters,
   const int batch = 8;
   while (it <= max_iter) {
     const int end = std::min(max_iter, it + batch - 1);
+    for (; it <= end && coarse; it++) {
+      // two-level: smoother + R A_c^-1 R^T, for p and then for s
+      hipLaunchKernelGGL((k_bicg_p<2, true>), dim3(g), dim3(256), 0, c->stream,
+                         a, it);
+      hipLaunchKernelGGL(k_coarse_mv, dim3(gmv), dim3(256), 0, c->stream,
+                         q.cinv, q.m, a.rc, q.zcp.as<double>(), a.ctl);
+      if (precond == NLM_QS_PRECOND_JACOBI)
+        hipLaunchKernelGGL(k_smooth_jacobi, dim3(g), dim3(256), 0, c->stream,
+                           a, a.p, a.ph, q.zcp.as<double>());
+      else
+        mapply(c, a.p, a.ph, a.ctl, q.zcp.as<double>());
+      spmv<1>(c, q.blk, q.diag, a.ph, a.v, rh, a.P2, nullptr, a.ctl);
+      hipLaunchKernelGGL((k_bicg_s<2, true>), dim3(g), dim3(256), 0, c->stream,
+                         a, it);
+      hipLaunchKernelGGL(k_coarse_mv, dim3(gmv), dim3(256), 0, c->stream,
+                         q.cinv, q.m, a.rc, q.zcs.as<double>(), a.ctl);
+      if (precond == NLM_QS_PRECOND_JACOBI)
+        hipLaunchKernelGGL(k_smooth_jacobi, dim3(g), dim3(256), 0, c->stream,
+                           a, a.s, a.sh, q.zcs.as<double>());
+      else
+        mapply(c, a.s, a.sh, a.ctl, q.zcs.as<double>());
+      spmv<2>(c, q.blk, q.diag, a.sh, a.t, a.s, a.P2, a.P3, a.ctl);
+      hipLaunchKernelGGL(k_bicg_x, dim3(g), dim3(256), 0, c->stream, a, it);
+    }
     for (; it <= end; it++) {
       if (precond == NLM_QS_PRECOND_JACOBI)
         hipLaunchKernelGGL(k_bicg_p<1>, dim3(g), dim3(256), 0, c->stream, a,
@@ -588,8 +847,7 @@ int bicgstab(nlm_ctx* c, int precond, double tol, int max_iter, int* iters,
       else {
         hipLaunchKernelGGL(k_bicg_p<2>, dim3(g), dim3(256), 0, c->stream, a,
                            it);
-        spmv<0>(c, q.mblk, q.mdiag, a.p, a.ph, nullptr, nullptr, nullptr,
-                a.ctl);
+        mapply(c, a.p, a.ph, a.ctl, nullptr);
       }
       spmv<1>(c, q.blk, q.diag, a.ph, a.v, rh, a.P2, nullptr, a.ctl);
       if (precond == NLM_QS_PRECOND_JACOBI)
@@ -601,8 +859,7 @@ int bicgstab(nlm_ctx* c, int precond, double tol, int max_iter, int* iters,
       else {
         hipLaunchKernelGGL(k_bicg_s<2>, dim3(g), dim3(256), 0, c->stream, a,
                            it);
-        spmv<0>(c, q.mblk, q.mdiag, a.s, a.sh, nullptr, nullptr, nullptr,
-                a.ctl);
+        mapply(c, a.s, a.sh, a.ctl, nullptr);
       }
       spmv<2>(c, q.blk, q.diag, a.sh, a.t, a.s, a.P2, a.P3, a.ctl);
       hipLaunchKernelGGL(k_bicg_x, dim3(g), dim3(256), 0, c->stream, a, it);
@@ -671,18 +928,19 @@ extern "C" int nlm_qs_assemble(nlm_ctx* c, int tangent) {
 
 namespace {
 // Copies slice-layout blocks and per-node blocks to the reference layout.
+template <class BT>
 int blocks_to_host(nlm_ctx* c, const DevBuf& blk, const DevBuf& dg,
                    double* blocks, double* diag) {
   FdState& s = c->fd;
   NLM_HIP(c, hipStreamSynchronize(c->stream));
   const int64_t nent = s.sell.nentries;
-  std::vector<double4> hb(nent), hd(s.n);
+  std::vector<BT> hb(nent), hd(s.n);
   std::vector<int64_t> slp(s.sell.nslices + 1);
   std::vector<int64_t> roff(s.n);
   std::vector<uint16_t> rl(s.n);
-  NLM_HIP(c, hipMemcpy(hb.data(), blk.p, sizeof(double4) * nent,
+  NLM_HIP(c, hipMemcpy(hb.data(), blk.p, sizeof(BT) * nent,
                        hipMemcpyDeviceToHost));
-  NLM_HIP(c, hipMemcpy(hd.data(), dg.p, sizeof(double4) * s.n,
+  NLM_HIP(c, hipMemcpy(hd.data(), dg.p, sizeof(BT) * s.n,
                        hipMemcpyDeviceToHost));
   NLM_HIP(c, hipMemcpy(slp.data(), s.sell.sl_ptr.p,
                        sizeof(int64_t) * slp.size(), hipMemcpyDeviceToHost));
@@ -694,7 +952,7 @@ int blocks_to_host(nlm_ctx* c, const DevBuf& blk, const DevBuf& dg,
     const int64_t sl = a / 64, lane = a % 64;
     if (blocks)
       for (int k = 0; k < rl[a]; k++) {
-        const double4 B = hb[slp[sl] + 64 * k + lane];
+        const BT B = hb[slp[sl] + 64 * k + lane];
         double* o = blocks + 4 * (roff[a] + k);
         o[0] = B.x;
         o[1] = B.y;
@@ -702,7 +960,7 @@ int blocks_to_host(nlm_ctx* c, const DevBuf& blk, const DevBuf& dg,
         o[3] = B.w;
       }
     if (diag) {
-      const double4 D = hd[a];
+      const BT D = hd[a];
       double* o = diag + 4 * int64_t(s.perm_host[a]);
       o[0] = D.x;
       o[1] = D.y;
@@ -719,7 +977,7 @@ extern "C" int nlm_qs_get_tangent(nlm_ctx* c, double* blocks, double* diag) {
   if (rc) return rc;
   if (!c->qs.assembled)
     return fail(c, NLM_ERR_STATE, "nlm_qs_get_tangent: nothing assembled");
-  return blocks_to_host(c, c->qs.blk, c->qs.diag, blocks, diag);
+  return blocks_to_host<double4>(c, c->qs.blk, c->qs.diag, blocks, diag);
 }
 
 extern "C" int nlm_qs_get_precond(nlm_ctx* c, double* blocks, double* diag) {
@@ -727,7 +985,7 @@ extern "C" int nlm_qs_get_precond(nlm_ctx* c, double* blocks, double* diag) {
   if (rc) return rc;
   if (!c->qs.has_m)
     return fail(c, NLM_ERR_STATE, "nlm_qs_get_precond: no blocks");
-  return blocks_to_host(c, c->qs.mblk, c->qs.mdiag, blocks, diag);
+  return blocks_to_host<float4>(c, c->qs.mblk, c->qs.mdiag, blocks, diag);
 }
 
 extern "C" int nlm_qs_set_precond(nlm_ctx* c, const double* blocks,
@@ -741,11 +999,11 @@ extern "C" int nlm_qs_set_precond(nlm_ctx* c, const double* blocks,
   if ((rc = qs_alloc(c, 8))) return rc;
   NLM_HIP(c, hipStreamSynchronize(c->stream));
   const int64_t nent = s.sell.nentries;
-  if (!q.mblk.p || q.mblk.bytes < sizeof(double4) * nent) {
-    if ((rc = alloc(c, q.mblk, sizeof(double4) * nent + 256))) return rc;
-    if ((rc = alloc(c, q.mdiag, sizeof(double4) * s.n + 256))) return rc;
+  if (!q.mblk.p || q.mblk.bytes < sizeof(float4) * nent) {
+    if ((rc = alloc(c, q.mblk, sizeof(float4) * nent + 256))) return rc;
+    if ((rc = alloc(c, q.mdiag, sizeof(float4) * s.n + 256))) return rc;
   }
-  std::vector<double4> hb(nent, make_double4(0., 0., 0., 0.)), hd(s.n);
+  std::vector<float4> hb(nent, make_float4(0.f, 0.f, 0.f, 0.f)), hd(s.n);
   std::vector<int64_t> slp(s.sell.nslices + 1), roff(s.n);
   std::vector<uint16_t> rl(s.n);
   NLM_HIP(c, hipMemcpy(slp.data(), s.sell.sl_ptr.p,
@@ -758,14 +1016,15 @@ extern "C" int nlm_qs_set_precond(nlm_ctx* c, const double* blocks,
     const int64_t sl = a / 64, lane = a % 64;
     for (int k = 0; k < rl[a]; k++) {
       const double* b = blocks + 4 * (roff[a] + k);
-      hb[slp[sl] + 64 * k + lane] = make_double4(b[0], b[1], b[2], b[3]);
+      hb[slp[sl] + 64 * k + lane] =
+          make_float4(float(b[0]), float(b[1]), float(b[2]), float(b[3]));
     }
     const double* d = diag + 4 * int64_t(s.perm_host[a]);
-    hd[a] = make_double4(d[0], d[1], d[2], d[3]);
+    hd[a] = make_float4(float(d[0]), float(d[1]), float(d[2]), float(d[3]));
   }
-  NLM_HIP(c, hipMemcpy(q.mblk.p, hb.data(), sizeof(double4) * nent,
+  NLM_HIP(c, hipMemcpy(q.mblk.p, hb.data(), sizeof(float4) * nent,
                        hipMemcpyHostToDevice));
-  NLM_HIP(c, hipMemcpy(q.mdiag.p, hd.data(), sizeof(double4) * s.n,
+  NLM_HIP(c, hipMemcpy(q.mdiag.p, hd.data(), sizeof(float4) * s.n,
                        hipMemcpyHostToDevice));
   q.has_m = true;
   return NLM_OK;
@@ -832,9 +1091,12 @@ extern "C" int nlm_qs_linear_solve(nlm_ctx* c, const double* b, double* x,
     return fail(c, NLM_ERR_STATE, "nlm_qs_linear_solve: nothing assembled");
   if (!b || !x || max_iter < 1 || !(tol > 0.))
     return fail(c, NLM_ERR_INVALID, "nlm_qs_linear_solve: bad arguments");
-  if (precond < NLM_QS_PRECOND_NONE || precond > NLM_QS_PRECOND_BLOCKS)
+  const int base = precond & ~NLM_QS_PRECOND_COARSE;
+  if (base < NLM_QS_PRECOND_NONE || base > NLM_QS_PRECOND_BLOCKS ||
+      (precond != base && base == NLM_QS_PRECOND_NONE))
     return fail(c, NLM_ERR_INVALID, "nlm_qs_linear_solve: unknown precond");
   if ((rc = qs_alloc(c, max_iter))) return rc;
+  if (precond != base && (rc = coarse_setup(c))) return rc;
   if ((rc = aos_to_dev(c, b, q.b))) return rc;
   int it = 0;
   double rr = 0.;
@@ -855,7 +1117,10 @@ extern "C" int nlm_qs_solve(nlm_ctx* c, const nlm_qs_options* o,
   if (o->tangent != NLM_QS_TANGENT_FULL &&
       o->tangent != NLM_QS_TANGENT_COMPONENT)
     return fail(c, NLM_ERR_INVALID, "nlm_qs_solve: unknown tangent");
-  if (o->precond < NLM_QS_PRECOND_NONE || o->precond > NLM_QS_PRECOND_BLOCKS)
+  const int pbase = o->precond & ~NLM_QS_PRECOND_COARSE;
+  const bool pcoarse = o->precond != pbase;
+  if (pbase < NLM_QS_PRECOND_NONE || pbase > NLM_QS_PRECOND_BLOCKS ||
+      (pcoarse && pbase == NLM_QS_PRECOND_NONE))
     return fail(c, NLM_ERR_INVALID, "nlm_qs_solve: unknown precond");
   FdState& s = c->fd;
   QsState& q = c->qs;
@@ -866,10 +1131,14 @@ extern "C" int nlm_qs_solve(nlm_ctx* c, const nlm_qs_options* o,
   const unsigned g = node_grid(s);
   auto u_cur = [&]() { return s.u[s.cur].as<double2>(); };
   auto precond_setup = [&]() -> int {
-    if (o->precond != NLM_QS_PRECOND_BLOCKS) return NLM_OK;
+    if (pbase != NLM_QS_PRECOND_BLOCKS && !pcoarse) return NLM_OK;
     const auto t0 = std::chrono::steady_clock::now();
-    int r = gnn_predict_precond(c);
-    if (r == NLM_ERR_STATE && q.has_m) r = NLM_OK;  // blocks set by hand
+    int r = NLM_OK;
+    if (pbase == NLM_QS_PRECOND_BLOCKS) {
+      r = gnn_predict_precond(c);
+      if (r == NLM_ERR_STATE && q.has_m) r = NLM_OK;  // blocks set by hand
+    }
+    if (r == NLM_OK && pcoarse) r = coarse_setup(c);
     NLM_HIP(c, hipStreamSynchronize(c->stream));
     stats.ms_precond += ms_since(t0);
     return r;

@@ -82,10 +82,12 @@ def test_qs_tangent_blocks(tangent):
 
 @pytest.mark.parametrize("precond", [NLM_QS_PRECOND_NONE,
                                      NLM_QS_PRECOND_JACOBI,
-                                     NLM_QS_PRECOND_BLOCKS])
+                                     NLM_QS_PRECOND_BLOCKS,
+                                     NLM_QS_PRECOND_JACOBI | 16,
+                                     NLM_QS_PRECOND_BLOCKS | 16])
 def test_qs_bicgstab(precond):
     progress(f"[gpu] qs bicgstab precond {precond}")
-    p = pr.qs_plate(24)
+    p = pr.qs_plate(40)
     q = qo.OracleQS(p)
     u = strained_state(p, 2)
     A = -q.jacobian_analytic(u)
@@ -96,7 +98,7 @@ def test_qs_bicgstab(precond):
     g.set_state(u=u, v=z, f=z)
     g.qs_assemble(NLM_QS_TANGENT_FULL)
     fr = q.free
-    if precond == NLM_QS_PRECOND_BLOCKS:
+    if precond & 15 == NLM_QS_PRECOND_BLOCKS:
         # block Jacobi written as explicit blocks: inverse diagonal blocks,
         # zero off-diagonal blocks
         blk, dg = g.qs_tangent()
@@ -116,6 +118,7 @@ def test_qs_bicgstab(precond):
     b[:, :2] = rng.standard_normal((p.n, 2))
     x, it, rr = g.qs_linear_solve(b, precond=precond, tol=1e-8,
                                   max_iter=4000)
+    progress(f"[gpu]   iterations {it}")
     assert 0 < it < 4000 and rr <= 1e-8
     Af = A[np.ix_(fr, fr)]
     bf = b[:, :2].reshape(-1)[fr]
@@ -155,6 +158,22 @@ def test_qs_newton_matches_oracle(case):
     for (u_ref, _, _), u in zip(ref, us):
         err = np.abs(u - u_ref).max() / np.abs(u_ref).max()
         assert err <= 1e-9, err
+
+
+def test_qs_newton_two_level_matches_jacobi():
+    progress("[gpu] qs newton two-level")
+    from nlmech_amd.engine import GpuFD
+    p = pr.qs_plate(60, pull=2e-6)
+    us = []
+    for pre in (NLM_QS_PRECOND_JACOBI, NLM_QS_PRECOND_JACOBI | 16):
+        g = GpuFD(p, precision=NLM_PREC_FP64_EXACT)
+        st = g.qs_solve(3, tol_rel=1e-10, precond=pre, lin_tol=1e-10,
+                        lin_max_iter=20000)
+        assert st["linear_failures"] == 0
+        progress(f"[gpu]   precond {pre}: {st['linear_iters']} iterations")
+        us.append(g.get_state()[0])
+        g.close()
+    assert np.abs(us[0] - us[1]).max() <= 1e-8 * np.abs(us[0]).max()
 
 
 def test_qs_plate300_jacobi_runs():
