@@ -69,6 +69,9 @@ def parse():
     ap.add_argument("--qs-workload", default="qs_plate300")
     ap.add_argument("--qs-steps", type=int, default=20)
     ap.add_argument("--no-qs", action="store_true")
+    ap.add_argument("--ens-steps", type=int, default=1000)
+    ap.add_argument("--ens-alone-steps", type=int, default=200)
+    ap.add_argument("--no-ens", action="store_true")
     return ap.parse_args()
 
 
@@ -159,6 +162,59 @@ def state_based(a):
     out["fp32_over_fp64"] = (out["modes"]["fp32"]["value"] /
                              out["modes"]["fp64"]["value"])
     return out
+
+
+def ensemble(a):
+    """256 uneven plates (sides U{50..400} nodes, random angled notches)
+    packed into one specimen and advanced together (FP32-force mode): one
+    fused launch per step for all plates, the steps replayed from a captured
+    HIP graph, the chunk width autotuned; against the same plates stepped one
+    at a time (one context each, direct launches)."""
+    from nlmech_amd import problems as pr
+    from nlmech_amd.capi import NLM_PREC_FP32_FORCE
+    from nlmech_amd.engine import GpuFD
+    p = pr.workload("ensemble256")
+    t0 = time.perf_counter()
+    g = GpuFD(p, precision=NLM_PREC_FP32_FORCE)
+    t_setup = time.perf_counter() - t0
+    g.init()
+    unroll, _ = g.autotune()
+    g.step_graph(10, 10)
+    steps = a.ens_steps
+    ms_total, ms_step = g.timed_steps_graph(steps, 50)
+    nnz = g.nnz
+    broken = int(g.get_bonds().sum())
+    g.close()
+    value = nnz * steps / (ms_total * 1e-3)
+    print(f"[bench] ensemble packed: {ms_total / steps * 1e3:.1f} us per step "
+          f"({p.n} nodes, {nnz} bonds)", file=sys.stderr, flush=True)
+    # one at a time
+    tot_ms, tot_upd, n_alone = 0.0, 0.0, 0
+    k_alone = a.ens_alone_steps
+    for k in range(len(p.specimens)):
+        m = pr.ensemble_member(p, k)
+        gm = GpuFD(m, precision=NLM_PREC_FP32_FORCE)
+        gm.init()
+        gm.step(2)
+        ms, _ = gm.timed_steps(k_alone)
+        tot_ms += ms
+        tot_upd += gm.nnz * k_alone
+        n_alone += 1
+        gm.close()
+    alone = tot_upd / (tot_ms * 1e-3)
+    print(f"[bench] ensemble one at a time: {alone:.3e} bond updates/s",
+          file=sys.stderr, flush=True)
+    return {"workload": "ensemble256", "plates": len(p.specimens),
+            "nodes": p.n, "bonds": nnz, "steps": steps, "dt": p.dt,
+            "mode": "fp32", "autotuned_unroll": unroll,
+            "graph_chunk": 50, "setup_s": t_setup,
+            "packed": {"value": value, "unit": "bond updates/s",
+                       "ms_per_step": ms_total / steps,
+                       "graph_step_us": ms_step * 1e3,
+                       "broken_bonds": broken},
+            "one_at_a_time": {"value": alone, "unit": "bond updates/s",
+                              "steps_each": k_alone, "plates": n_alone},
+            "packed_over_one_at_a_time": value / alone}
 
 
 def quasi_static(a):
@@ -335,6 +391,8 @@ def main():
                                  out["modes"]["fp64"]["value"])
     if not a.no_es:
         out["state_based_3d"] = state_based(a)
+    if not a.no_ens:
+        out["ensemble"] = ensemble(a)
     if not a.no_qs:
         out["quasi_static"] = quasi_static(a)
     if not a.no_cpu:

@@ -237,8 +237,15 @@ typedef struct {
 int nlm_fd_setup(nlm_ctx* ctx, const nlm_fd_problem* p);
 /* Displacement BC sets: ULoading::ULoading node selection and fixity
  * (uLoading.cpp:18-164); applied by nlm_fd_init / every step
- * (ULoading::apply, :166-253).  At most 15 sets. */
+ * (ULoading::apply, :166-253).  At most 14 sets.  Clears node-list BCs. */
 int nlm_fd_set_ubc(nlm_ctx* ctx, int nsets, const nlm_bc* sets);
+/* Node-list displacement BCs (no counterpart in the reference's loading
+ * deck; used for batched specimens): dof d of node nodes[k] is prescribed
+ * when bit d of dofs[k] is set, with u = rates[3 k + d] t and v =
+ * rates[3 k + d] (ULoading's "linear" time function per node).  Replaces
+ * earlier node-list BCs; call after nlm_fd_set_ubc. */
+int nlm_fd_set_node_ubc(nlm_ctx* ctx, int64_t count, const int64_t* nodes,
+                        const uint8_t* dofs, const double* rates);
 /* Force BC sets: FLoading (fLoading.cpp:18-279).  At most 8 sets. */
 int nlm_fd_set_fbc(nlm_ctx* ctx, int nsets, const nlm_bc* sets);
 /* Pre-cracks: cracks with activation_time < 0 break their crossing bonds now
@@ -290,6 +297,16 @@ int nlm_fd_get_node_fields(nlm_ctx* ctx, double* theta, double* mw);
  * and the mean duration of the fused step kernel. */
 int nlm_fd_timed_steps(nlm_ctx* ctx, int nsteps, double* ms_total,
                        double* ms_per_step);
+/* nlm_fd_step with the fused steps launched from a captured HIP graph of
+ * `chunk` steps (even) that is replayed; the step times advance on the
+ * device with the host's accumulation (t += dt).  Central difference, no
+ * pending timed cracks.  The graph is re-captured when chunk or the buffer
+ * parity changes. */
+int nlm_fd_step_graph(nlm_ctx* ctx, int nsteps, int chunk);
+/* Timed variant: elapsed milliseconds of the call and the mean time of one
+ * graph-launched step. */
+int nlm_fd_timed_steps_graph(nlm_ctx* ctx, int nsteps, int chunk,
+                             double* ms_total, double* ms_per_step);
 /* Autotuner (no counterpart in the reference): times the fused step kernel
  * for each neighbor-slot chunk width (2, 4, 8 slots per wavefront iteration)
  * on the current state without changing it, and keeps the fastest.  Returns

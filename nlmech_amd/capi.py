@@ -175,6 +175,9 @@ SIGNATURES = {
     "nlm_es_params_from_deck": (C.c_int, [C.POINTER(EsDeck),
                                           C.POINTER(EsParams)]),
     "nlm_fd_get_node_fields": (C.c_int, [_P, _D, _D]),
+    "nlm_fd_set_node_ubc": (C.c_int, [_P, C.c_int64, _I64, _U8, _D]),
+    "nlm_fd_step_graph": (C.c_int, [_P, C.c_int, C.c_int]),
+    "nlm_fd_timed_steps_graph": (C.c_int, [_P, C.c_int, C.c_int, _D, _D]),
     "nlm_qs_solve": (C.c_int, [_P, C.POINTER(QsOptions), C.POINTER(QsStats)]),
     "nlm_qs_assemble": (C.c_int, [_P, C.c_int]),
     "nlm_qs_get_tangent": (C.c_int, [_P, _D, _D]),

@@ -235,6 +235,10 @@ template <int DIM, int MODE, int KIND, bool BASE_F, int U, bool TILED>
 __global__ __launch_bounds__(256) void k_es_force(
     EsArgs<DIM> e, const unsigned long long* __restrict__ bits_ro,
     double t_force, double t_next) {
+  if (e.f.tdev) {
+    t_force = e.f.tdev[0];
+    t_next = e.f.tdev[1];
+  }
   static_assert(!TILED || (DIM == 3 && MODE != kModeExact),
                 "the LDS-staged force pass is 3D, fast modes");
   const FdArgs<DIM>& a = e.f;
@@ -480,8 +484,9 @@ void launch_force_k(nlm_ctx* c, const EsArgs<DIM>& e, double tf, double tn) {
 
 template <int DIM, int MODE>
 void launch_pair(nlm_ctx* c, int kind, bool base_f, double tf, double tn,
-                 int U) {
+                 int U, const double* tdev = nullptr) {
   EsArgs<DIM> e = make_es_args<DIM>(c);
+  e.f.tdev = tdev;
   launch_dil<DIM, MODE>(c, e);
   if (kind == KIND_FORCE) {
     if (base_f)
@@ -499,26 +504,26 @@ void launch_pair(nlm_ctx* c, int kind, bool base_f, double tf, double tn,
 
 template <int DIM>
 void launch_dim(nlm_ctx* c, int kind, bool base_f, double tf, double tn,
-                int U) {
+                int U, const double* tdev) {
   switch (c->fd.precision) {
     case kModeExact:
-      launch_pair<DIM, kModeExact>(c, kind, base_f, tf, tn, U);
+      launch_pair<DIM, kModeExact>(c, kind, base_f, tf, tn, U, tdev);
       break;
     case kModeFp64:
-      launch_pair<DIM, kModeFp64>(c, kind, base_f, tf, tn, U);
+      launch_pair<DIM, kModeFp64>(c, kind, base_f, tf, tn, U, tdev);
       break;
     default:
-      launch_pair<DIM, kModeFp32>(c, kind, base_f, tf, tn, U);
+      launch_pair<DIM, kModeFp32>(c, kind, base_f, tf, tn, U, tdev);
       break;
   }
 }
 
 void launch_any(nlm_ctx* c, int kind, bool base_f, double tf, double tn,
-                int U) {
+                int U, const double* tdev = nullptr) {
   if (c->fd.dim == 2)
-    launch_dim<2>(c, kind, base_f, tf, tn, U);
+    launch_dim<2>(c, kind, base_f, tf, tn, U, tdev);
   else
-    launch_dim<3>(c, kind, base_f, tf, tn, U);
+    launch_dim<3>(c, kind, base_f, tf, tn, U, tdev);
 }
 
 }  // namespace
@@ -563,6 +568,10 @@ void es_launch_force(nlm_ctx* c, bool base_f, double t) {
 
 void es_launch_fused(nlm_ctx* c, double tf, double tn) {
   launch_any(c, KIND_CD, false, tf, tn, c->fd.unroll);
+}
+
+void es_launch_fused_dev(nlm_ctx* c, const double* tdev) {
+  launch_any(c, KIND_CD, false, 0., 0., c->fd.unroll, tdev);
 }
 
 // Times the fused pass pair for each chunk width of the force pass on the

@@ -65,6 +65,10 @@ template <int DIM, int MODE, int KIND, bool BASE_F, int INF, int U,
 __global__ __launch_bounds__(256) void k_fd(
     FdArgs<DIM> a, const unsigned long long* __restrict__ bits_ro,
     double t_force, double t_next) {
+  if (a.tdev) {  // graph-captured steps read the step's times
+    t_force = a.tdev[0];
+    t_next = a.tdev[1];
+  }
   int64_t nblk = (a.nslices * 64 + 255) / 256;
   int64_t lb = xcd_block(nblk);
   if (lb >= nblk) return;
@@ -240,7 +244,7 @@ __global__ __launch_bounds__(256) void k_fd(
       }
     } else {
       double du, dv;
-      ubc_eval(a.bc->u[id - 1], xi[0], xi[1], xi[2], t_next, du, dv);
+      ubc_dof(a, id, i, d, xi, t_next, du, dv);
       un[d] = du;
       vn[d] = dv;
     }
@@ -271,7 +275,7 @@ __global__ __launch_bounds__(256) void k_update_cd(FdArgs<DIM> a,
       vn[d] = (un[d] - uo) / a.dt;
     } else {
       double du, dv;
-      ubc_eval(a.bc->u[id - 1], xi[0], xi[1], xi[2], t_next, du, dv);
+      ubc_dof(a, id, i, d, xi, t_next, du, dv);
       un[d] = du;
       vn[d] = dv;
     }
@@ -300,7 +304,7 @@ __global__ __launch_bounds__(256) void k_verlet_a(FdArgs<DIM> a,
       ui[d] += a.dt * vi[d];
     } else {
       double du, dv;
-      ubc_eval(a.bc->u[id - 1], xi[0], xi[1], xi[2], t_next, du, dv);
+      ubc_dof(a, id, i, d, xi, t_next, du, dv);
       ui[d] = du;
       vi[d] = dv;
     }
@@ -337,7 +341,7 @@ __global__ __launch_bounds__(256) void k_apply_ubc(FdArgs<DIM> a, double t) {
   load3<DIM>(a.v[i], vi);
   for (int d = 0; d < DIM; d++) {
     uint32_t id = (fl >> (kFlagUbcShift + 4 * d)) & 0xfu;
-    if (id) ubc_eval(a.bc->u[id - 1], xi[0], xi[1], xi[2], t, ui[d], vi[d]);
+    if (id) ubc_dof(a, id, i, d, xi, t, ui[d], vi[d]);
   }
   a.u_out[i] = make_v<DIM>(ui);
   a.v_out[i] = make_v<DIM>(vi);
@@ -909,6 +913,15 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
       }
   NLM_HIP(c, hipMemcpyAsync(s.flags.p, fl.data(), sizeof(uint32_t) * n,
                             hipMemcpyHostToDevice, st));
+  if ((rc = alloc(c, s.node_rate, sizeof(double) * 3 * size_t(n) + 256)))
+    return rc;
+  NLM_HIP(c, hipMemsetAsync(s.node_rate.p, 0, s.node_rate.bytes, st));
+  if ((rc = alloc(c, s.tdev, 64))) return rc;
+  if (s.graph) {
+    (void)hipGraphExecDestroy(s.graph);
+    s.graph = nullptr;
+  }
+  s.graph_chunk = 0;
   if ((rc = alloc(c, s.bctab, sizeof(BcTable)))) return rc;
   std::memset(&s.bc_host, 0, sizeof(BcTable));
   NLM_HIP(c, hipMemcpyAsync(s.bctab.p, &s.bc_host, sizeof(BcTable),
@@ -1010,7 +1023,7 @@ extern "C" int nlm_fd_set_ubc(nlm_ctx* c, int nsets, const nlm_bc* sets) {
   if (rc) return rc;
   FdState& s = c->fd;
   if (nsets < 0 || nsets > kMaxUbcSets || (nsets > 0 && !sets))
-    return fail(c, NLM_ERR_INVALID, "nlm_fd_set_ubc: 0..15 sets");
+    return fail(c, NLM_ERR_INVALID, "nlm_fd_set_ubc: 0..14 sets");
   std::vector<double> hx(size_t(s.n) * 3);
   {
     const int stride = s.dim == 2 ? 2 : 4;
@@ -1515,6 +1528,7 @@ int autotune(nlm_ctx* c, int* chosen, double* best_us) {
 extern "C" int nlm_fd_autotune(nlm_ctx* c, int* chosen, double* best_us) {
   int rc = check_ready(c, "nlm_fd_autotune");
   if (rc) return rc;
+  c->fd.graph_chunk = 0;  // a captured step graph is stale
   if (c->fd.precision == NLM_PREC_FP64_EXACT) {
     c->fd.unroll = 4;
     if (chosen) *chosen = 4;
@@ -1533,6 +1547,7 @@ extern "C" int nlm_fd_set_unroll(nlm_ctx* c, int unroll) {
   if (unroll != 2 && unroll != 4 && unroll != 8)
     return fail(c, NLM_ERR_INVALID, "nlm_fd_set_unroll: 2, 4 or 8");
   c->fd.unroll = unroll;
+  c->fd.graph_chunk = 0;  // a captured step graph is stale
   return NLM_OK;
 }
 
@@ -1547,4 +1562,204 @@ extern "C" const char* nlm_fd_kernel_name(nlm_ctx* c) {
   if (!c || !c->fd.ready) return "";
   if (c->fd.material == NLM_MATERIAL_ELASTIC_STATE) return "k_es_force";
   return "k_fd";
+}
+
+// Node-list displacement BCs: flags field kNodeListBc and per-node rates.
+extern "C" int nlm_fd_set_node_ubc(nlm_ctx* c, int64_t count,
+                                   const int64_t* nodes, const uint8_t* dofs,
+                                   const double* rates) {
+  int rc = check_ready(c, "nlm_fd_set_node_ubc");
+  if (rc) return rc;
+  FdState& s = c->fd;
+  if (count < 0 || (count > 0 && (!nodes || !dofs || !rates)))
+    return fail(c, NLM_ERR_INVALID, "nlm_fd_set_node_ubc: bad arguments");
+  NLM_HIP(c, hipStreamSynchronize(c->stream));
+  std::vector<uint32_t> fl(s.n);
+  NLM_HIP(c, hipMemcpy(fl.data(), s.flags.p, sizeof(uint32_t) * s.n,
+                       hipMemcpyDeviceToHost));
+  std::vector<double> rate(3 * size_t(s.n), 0.);
+  // drop earlier node-list BCs
+  for (int64_t a = 0; a < s.n; a++)
+    for (int d = 0; d < 3; d++)
+      if (((fl[a] >> (kFlagUbcShift + 4 * d)) & 0xfu) == kNodeListBc) {
+        fl[a] &= ~(0xfu << (kFlagUbcShift + 4 * d));
+        s.fix_host[s.perm_host[a]] &= uint8_t(~(1u << d));
+      }
+  for (int64_t k = 0; k < count; k++) {
+    const int64_t i = nodes[k];
+    if (i < 0 || i >= s.n)
+      return fail(c, NLM_ERR_INVALID, "nlm_fd_set_node_ubc: node out of range");
+    if (dofs[k] >> s.dim)
+      return fail(c, NLM_ERR_INVALID, "nlm_fd_set_node_ubc: bad dof mask");
+    const int64_t a = s.inv_host[i];
+    for (int d = 0; d < s.dim; d++) {
+      if (!((dofs[k] >> d) & 1)) continue;
+      fl[a] = (fl[a] & ~(0xfu << (kFlagUbcShift + 4 * d))) |
+              (kNodeListBc << (kFlagUbcShift + 4 * d));
+      rate[3 * a + d] = rates[3 * k + d];
+      s.fix_host[i] |= uint8_t(1u << d);
+    }
+  }
+  NLM_HIP(c, hipMemcpy(s.flags.p, fl.data(), sizeof(uint32_t) * s.n,
+                       hipMemcpyHostToDevice));
+  NLM_HIP(c, hipMemcpy(s.node_rate.p, rate.data(), sizeof(double) * rate.size(),
+                       hipMemcpyHostToDevice));
+  return NLM_OK;
+}
+
+namespace {
+// t_force <- t_next; t_next <- t_next + dt (the host's accumulation, on the
+// device, between graph-captured steps).
+__global__ void k_time_advance(double* __restrict__ t, double dt) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    t[0] = t[1];
+    t[1] = t[1] + dt;
+  }
+}
+
+template <int DIM>
+void launch_fused_graph(nlm_ctx* c) {
+  FdState& s = c->fd;
+  if (s.material == NLM_MATERIAL_ELASTIC_STATE) {
+    es_launch_fused_dev(c, s.tdev.as<double>());
+    return;
+  }
+  FdArgs<DIM> a = make_args<DIM>(c);
+  a.tdev = s.tdev.as<double>();
+  launch_mode<DIM, KIND_CD, false>(c, a, 0., 0., s.unroll);
+}
+
+// Captures `chunk` fused steps (even, ping-pong buffers return to the
+// starting parity) into an executable graph.
+template <int DIM>
+int capture_steps(nlm_ctx* c, int chunk) {
+  FdState& s = c->fd;
+  if (s.graph) {
+    (void)hipGraphExecDestroy(s.graph);
+    s.graph = nullptr;
+  }
+  hipGraph_t gr = nullptr;
+  NLM_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  const int cur0 = s.cur;
+  for (int k = 0; k < chunk; k++) {
+    launch_fused_graph<DIM>(c);
+    hipLaunchKernelGGL(k_time_advance, dim3(1), dim3(64), 0, c->stream,
+                       s.tdev.as<double>(), s.dt);
+    s.cur ^= 1;
+  }
+  s.cur = cur0;
+  NLM_HIP(c, hipStreamEndCapture(c->stream, &gr));
+  hipError_t e = hipGraphInstantiate(&s.graph, gr, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(gr);
+  if (e != hipSuccess) {
+    s.graph = nullptr;
+    return fail(c, NLM_ERR_HIP,
+                std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  s.graph_chunk = chunk;
+  s.graph_cur = cur0;
+  return NLM_OK;
+}
+
+// nsteps of integrate() with the fused steps replayed from a graph: the first
+// update from the stored force, (nsteps - 1) fused steps (graph chunks, the
+// remainder launched directly), the trailing force.
+template <int DIM>
+int do_steps_graph(nlm_ctx* c, int nsteps, int chunk, double* ms_fused) {
+  FdState& s = c->fd;
+  const unsigned nb = node_blocks(s);
+  {
+    double tn = s.time + s.dt;
+    FdArgs<DIM> a = make_args<DIM>(c);
+    hipLaunchKernelGGL(k_update_cd<DIM>, dim3(nb), dim3(256), 0, c->stream, a,
+                       tn);
+    s.cur ^= 1;
+    s.time = tn;
+    s.step++;
+  }
+  int rc;
+  const int fused = nsteps - 1;
+  const int reps = fused / chunk;
+  if (reps > 0 && (!s.graph || s.graph_chunk != chunk || s.graph_cur != s.cur))
+    if ((rc = capture_steps<DIM>(c, chunk))) return rc;
+  // device times for the first graph step
+  double t2[2] = {s.time, s.time + s.dt};
+  NLM_HIP(c, hipMemcpyAsync(s.tdev.p, t2, sizeof(t2), hipMemcpyHostToDevice,
+                            c->stream));
+  NLM_HIP(c, hipStreamSynchronize(c->stream));
+  if (ms_fused) NLM_HIP(c, hipEventRecord(s.ev0, c->stream));
+  for (int r = 0; r < reps; r++) {
+    NLM_HIP(c, hipGraphLaunch(s.graph, c->stream));
+    for (int k = 0; k < chunk; k++) {  // the host keeps the same clock
+      s.time = s.time + s.dt;
+      s.step++;
+    }
+  }
+  if (ms_fused) NLM_HIP(c, hipEventRecord(s.ev1, c->stream));
+  for (int k = reps * chunk; k < fused; k++) {
+    double tn = s.time + s.dt;
+    launch_fused_dispatch<DIM>(c, s.time, tn);
+    s.cur ^= 1;
+    s.time = tn;
+    s.step++;
+  }
+  launch_force_dispatch<DIM>(c, false, s.time);
+  NLM_HIP(c, hipGetLastError());
+  if (ms_fused) {
+    NLM_HIP(c, hipEventSynchronize(s.ev1));
+    float ms = 0.f;
+    NLM_HIP(c, hipEventElapsedTime(&ms, s.ev0, s.ev1));
+    *ms_fused = reps > 0 ? double(ms) / double(reps * chunk) : 0.;
+  }
+  return NLM_OK;
+}
+
+int graph_check(nlm_ctx* c, int nsteps, int chunk, const char* fn) {
+  int rc = check_ready(c, fn);
+  if (rc) return rc;
+  FdState& s = c->fd;
+  if (nsteps < 1 || chunk < 2 || (chunk & 1))
+    return fail(c, NLM_ERR_INVALID,
+                std::string(fn) + ": nsteps >= 1, even chunk >= 2");
+  if (s.scheme != NLM_TIME_CENTRAL_DIFFERENCE)
+    return fail(c, NLM_ERR_INVALID, std::string(fn) + ": central difference");
+  for (size_t k = 0; k < s.crack_done.size(); k++)
+    if (!s.crack_done[k])
+      return fail(c, NLM_ERR_STATE,
+                  std::string(fn) + ": pending timed cracks need nlm_fd_step");
+  return NLM_OK;
+}
+}  // namespace
+
+extern "C" int nlm_fd_step_graph(nlm_ctx* c, int nsteps, int chunk) {
+  int rc = graph_check(c, nsteps, chunk, "nlm_fd_step_graph");
+  if (rc) return rc;
+  rc = c->fd.dim == 2 ? do_steps_graph<2>(c, nsteps, chunk, nullptr)
+                      : do_steps_graph<3>(c, nsteps, chunk, nullptr);
+  if (rc) return rc;
+  NLM_HIP(c, hipStreamSynchronize(c->stream));
+  return NLM_OK;
+}
+
+extern "C" int nlm_fd_timed_steps_graph(nlm_ctx* c, int nsteps, int chunk,
+                                        double* ms_total, double* ms_per_step) {
+  int rc = graph_check(c, nsteps, chunk, "nlm_fd_timed_steps_graph");
+  if (rc) return rc;
+  hipEvent_t a0, a1;
+  NLM_HIP(c, hipEventCreate(&a0));
+  NLM_HIP(c, hipEventCreate(&a1));
+  NLM_HIP(c, hipEventRecord(a0, c->stream));
+  double per = 0.;
+  rc = c->fd.dim == 2 ? do_steps_graph<2>(c, nsteps, chunk, &per)
+                      : do_steps_graph<3>(c, nsteps, chunk, &per);
+  if (rc) return rc;
+  NLM_HIP(c, hipEventRecord(a1, c->stream));
+  NLM_HIP(c, hipEventSynchronize(a1));
+  float ms = 0.f;
+  NLM_HIP(c, hipEventElapsedTime(&ms, a0, a1));
+  (void)hipEventDestroy(a0);
+  (void)hipEventDestroy(a1);
+  if (ms_total) *ms_total = ms;
+  if (ms_per_step) *ms_per_step = per;
+  return NLM_OK;
 }

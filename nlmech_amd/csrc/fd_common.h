@@ -37,7 +37,24 @@ struct FdArgs {
   double fact;
   int has_fbc;
   int has_nofail;
+  const double* __restrict__ node_rate;  // node-list BC rates (or null)
+  const double* __restrict__ tdev;       // step times from device memory
 };
+
+// Displacement and velocity of a prescribed dof at time t: ULoading::apply
+// for BC sets, u = rate t for node-list BCs.
+template <int DIM>
+__device__ __forceinline__ void ubc_dof(const FdArgs<DIM>& a, uint32_t id,
+                                        int64_t i, int d, const double* xi,
+                                        double t, double& du, double& dv) {
+  if (id == kNodeListBc) {
+    const double r = a.node_rate[3 * i + d];
+    du = r * t;
+    dv = r;
+  } else {
+    ubc_eval(a.bc->u[id - 1], xi[0], xi[1], xi[2], t, du, dv);
+  }
+}
 
 template <int DIM>
 __device__ __forceinline__ void load3(const typename VT<DIM>::T& a,
@@ -116,7 +133,7 @@ __device__ __forceinline__ void node_tail(const FdArgs<DIM>& a, int64_t i,
       vn[d] = (un[d] - uo) / a.dt;
     } else {
       double du, dv;
-      ubc_eval(a.bc->u[id - 1], xi[0], xi[1], xi[2], t_next, du, dv);
+      ubc_dof(a, id, i, d, xi, t_next, du, dv);
       un[d] = du;
       vn[d] = dv;
     }
@@ -158,6 +175,8 @@ inline FdArgs<DIM> make_args(nlm_ctx* c) {
                : 0.5 * s.dt / s.density;
   a.has_fbc = s.has_fbc ? 1 : 0;
   a.has_nofail = s.has_nofail ? 1 : 0;
+  a.node_rate = s.node_rate.as<double>();
+  a.tdev = nullptr;
   return a;
 }
 
@@ -172,6 +191,8 @@ void fd_launch_force(nlm_ctx* c, bool base_f, double t);
 int es_prepare(nlm_ctx* c);
 void es_launch_force(nlm_ctx* c, bool base_f, double t);
 void es_launch_fused(nlm_ctx* c, double tf, double tn);
+// fused pass pair reading the step times from tdev (graph capture)
+void es_launch_fused_dev(nlm_ctx* c, const double* tdev);
 int es_autotune(nlm_ctx* c, int* chosen, double* best_us);
 
 }  // namespace nlm

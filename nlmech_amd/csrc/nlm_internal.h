@@ -35,7 +35,8 @@ struct DevBuf {
 constexpr uint32_t kFlagUbcShift = 0;         // 4 bits per dof: u-BC set id + 1
 constexpr uint32_t kFlagNonInterior = 1u << 12;  // node never breaks bonds
 constexpr uint32_t kFlagFbcShift = 16;        // 8 bits: force-BC set membership
-constexpr int kMaxUbcSets = 15;
+constexpr int kMaxUbcSets = 14;
+constexpr uint32_t kNodeListBc = 15;  // u-BC id of node-list BCs (per-node rate)
 constexpr int kMaxFbcSets = 8;
 
 // One BC set as evaluated on the device (inp::BCData with codes).
@@ -148,6 +149,11 @@ struct FdState {
   bool tiled = false;   // LDS-staged step kernel usable
   bool es_tiled = false;  // LDS-staged state-based passes usable
   bool use_tiled = true;
+  DevBuf node_rate;  // double[3n] internal order: node-list BC rates
+  DevBuf tdev;       // (t_force, t_next) read by graph-captured steps
+  hipGraphExec_t graph = nullptr;
+  int graph_chunk = 0;
+  int graph_cur = -1;  // value of cur when the graph was captured
   DevBuf bctab;  // BcTable
   BcTable bc_host{};
   std::vector<uint8_t> fix_host;

@@ -514,7 +514,8 @@ __global__ __launch_bounds__(256) void k_qs_residual(
 __global__ __launch_bounds__(256) void k_qs_dup(
     int64_t n, const uint32_t* __restrict__ flags,
     const double2* __restrict__ x, const double2* __restrict__ u,
-    const BcTable* __restrict__ bc, double t, double2* __restrict__ dup) {
+    const BcTable* __restrict__ bc, const double* __restrict__ node_rate,
+    double t, double2* __restrict__ dup) {
   const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
   if (i >= n) return;
   const uint32_t fl = flags[i];
@@ -525,7 +526,11 @@ __global__ __launch_bounds__(256) void k_qs_dup(
     const uint32_t id = (fl >> (kFlagUbcShift + 4 * c)) & 0xfu;
     if (id) {
       double du, dv;
-      ubc_eval(bc->u[id - 1], xi.x, xi.y, 0., t, du, dv);
+      if (id == kNodeListBc) {
+        du = node_rate[3 * i + c] * t;
+      } else {
+        ubc_eval(bc->u[id - 1], xi.x, xi.y, 0., t, du, dv);
+      }
       d[c] = du - uu[c];
     }
   }
@@ -1171,7 +1176,8 @@ extern "C" int nlm_qs_solve(nlm_ctx* c, const nlm_qs_options* o,
       // linearized first iteration at the previous equilibrium
       hipLaunchKernelGGL(k_qs_dup, dim3(g), dim3(256), 0, c->stream, s.n,
                          s.flags.as<uint32_t>(), s.x.as<double2>(), u_cur(),
-                         s.bctab.as<BcTable>(), t, q.dup.as<double2>());
+                         s.bctab.as<BcTable>(), s.node_rate.as<double>(), t,
+                         q.dup.as<double2>());
       fd_launch_force(c, false, t);
       if ((rc = residual(c, &rnorm))) return rc;
       if ((rc = tangent())) return rc;
@@ -1195,7 +1201,8 @@ extern "C" int nlm_qs_solve(nlm_ctx* c, const nlm_qs_options* o,
       // ULoading at t on the prescribed dofs, then the residual
       hipLaunchKernelGGL(k_qs_dup, dim3(g), dim3(256), 0, c->stream, s.n,
                          s.flags.as<uint32_t>(), s.x.as<double2>(), u_cur(),
-                         s.bctab.as<BcTable>(), t, q.dup.as<double2>());
+                         s.bctab.as<BcTable>(), s.node_rate.as<double>(), t,
+                         q.dup.as<double2>());
       NLM_HIP(c, hipMemsetAsync(q.x.p, 0, sizeof(double2) * s.n, c->stream));
       hipLaunchKernelGGL(k_qs_update_u, dim3(g), dim3(256), 0, c->stream, s.n,
                          s.flags.as<uint32_t>(), u_cur(), q.x.as<double2>(),

@@ -178,6 +178,8 @@ class GpuFD:
         if p.cracks:
             arr = (Crack * len(p.cracks))(*[crack_struct(c) for c in p.cracks])
             self._call("nlm_fd_add_cracks", len(p.cracks), arr)
+        if getattr(p, "node_bc", None) is not None:
+            self.set_node_ubc(*p.node_bc)
         if getattr(p, "v0", None) is not None:
             self.set_state(v=p.v0)
 
@@ -226,6 +228,24 @@ class GpuFD:
         per = C.c_double()
         self._call("nlm_fd_timed_steps", int(nsteps), C.byref(tot),
                    C.byref(per))
+        return tot.value, per.value
+
+    def set_node_ubc(self, nodes, dofs, rates):
+        """Node-list displacement BCs: u = rate t on the masked dofs."""
+        nodes = np.ascontiguousarray(nodes, np.int64)
+        dofs = np.ascontiguousarray(dofs, np.uint8)
+        rates = np.ascontiguousarray(rates, np.float64).reshape(-1, 3)
+        self._call("nlm_fd_set_node_ubc", int(nodes.size), ptr(nodes, C.c_int64),
+                   ptr(dofs, C.c_uint8), ptr(rates, C.c_double))
+
+    def step_graph(self, nsteps: int, chunk: int = 50):
+        self._call("nlm_fd_step_graph", int(nsteps), int(chunk))
+
+    def timed_steps_graph(self, nsteps: int, chunk: int = 50):
+        tot = C.c_double()
+        per = C.c_double()
+        self._call("nlm_fd_timed_steps_graph", int(nsteps), int(chunk),
+                   C.byref(tot), C.byref(per))
         return tot.value, per.value
 
     def neighbors(self):

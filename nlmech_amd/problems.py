@@ -71,6 +71,9 @@ class Problem:
     interior: np.ndarray | None = None
     shape: tuple = ()
     v0: np.ndarray | None = None   # (n, 3) initial velocity, None: zero
+    # node-list displacement BCs (nodes, dof masks, rates[k, 3])
+    node_bc: tuple | None = None
+    specimens: list = field(default_factory=list)  # ensemble members
 
     @property
     def n(self) -> int:
@@ -260,6 +263,84 @@ def qs_plate(nx: int, h: float = 1e-3, ratio: int = 3, pull: float = 1e-5,
                    ubc=ubc, cracks=cracks, shape=(nx, nx))
 
 
+def ensemble(count: int = 256, lo: int = 50, hi: int = 400, h: float = 1e-3,
+             ratio: int = 3, dt: float = 1e-7, nsteps: int = 1000,
+             pull: float = 0.5, layers: int = 3, seed: int = 3,
+             as_sets: bool = False, name: str | None = None) -> Problem:
+    """Independent square RNP plates of uneven sides (U{lo..hi} nodes), each
+    with an angled pre-notch from its left edge at mid-height (length
+    U[0.1, 0.3] L, angle U[-30, 30] degrees) and its top / bottom `layers`
+    rows pulled at +/- pull m/s (dof 2), packed into one specimen: the plates
+    sit on a square grid of cells (hi + 8) h wide, far apart compared to the
+    horizon, so no bond joins two plates.  The pulls are node-list BCs
+    (as_sets: one rectangle BC set per pulled layer, at most 7 plates)."""
+    rng = np.random.default_rng(seed)
+    sides = rng.integers(lo, hi + 1, size=count)
+    lens = rng.uniform(0.1, 0.3, size=count)
+    angles = np.radians(rng.uniform(-30.0, 30.0, size=count))
+    per_row = int(np.ceil(np.sqrt(count)))
+    cell = (hi + 8) * h
+    xs, vols, cracks, ubc, specs = [], [], [], [], []
+    bc_nodes, bc_rates = [], []
+    base = 0
+    for k in range(count):
+        L = int(sides[k])
+        ox, oy = (k % per_row) * cell, (k // per_row) * cell
+        x, vol = grid2d(L, L, h, 1e-6, seed * 100003 + k, x0=ox, y0=oy)
+        ytop = oy + (L - 1) * h
+        ym = oy + 0.5 * (L - 1) * h
+        pb = (ox - 0.5 * h, ym)
+        ln = lens[k] * (L - 1) * h
+        pt = (pb[0] + ln * np.cos(angles[k]), pb[1] + ln * np.sin(angles[k]))
+        cracks.append(CrackSpec(0, pb, pt, theta=float(angles[k])))
+        jj = np.arange(L * L) // L
+        top = np.nonzero(jj >= L - layers)[0]
+        bot = np.nonzero(jj < layers)[0]
+        if as_sets:
+            xmax = ox + (L - 1) * h
+            ubc.append(BC("rectangle", [2], "linear", [pull], x1=ox - h,
+                          x2=xmax + h, y1=ytop - (layers - 1) * h - 0.5 * h,
+                          y2=ytop + 0.5 * h))
+            ubc.append(BC("rectangle", [2], "linear", [-pull], x1=ox - h,
+                          x2=xmax + h, y1=oy - 0.5 * h,
+                          y2=oy + (layers - 1) * h + 0.5 * h))
+        bc_nodes += [base + top, base + bot]
+        bc_rates += [np.full(top.size, pull), np.full(bot.size, -pull)]
+        specs.append(dict(offset=base, side=L, origin=(ox, oy),
+                          notch=(float(ln), float(angles[k]))))
+        xs.append(x)
+        vols.append(vol)
+        base += L * L
+    x = np.concatenate(xs)
+    vol = np.concatenate(vols)
+    node_bc = None
+    if not as_sets:
+        nodes = np.concatenate(bc_nodes).astype(np.int64)
+        rates = np.zeros((nodes.size, 3))
+        rates[:, 1] = np.concatenate(bc_rates)
+        node_bc = (nodes, np.full(nodes.size, 2, np.uint8), rates)
+    return Problem(name=name or f"ensemble{count}", dim=2, x=x, vol=vol,
+                   horizon=ratio * h, h=h, dt=dt, nsteps=nsteps,
+                   material=rnp_material(dim=2), ubc=ubc, cracks=cracks,
+                   node_bc=node_bc, specimens=specs)
+
+
+def ensemble_member(p: Problem, k: int) -> Problem:
+    """Plate k of an ensemble as a specimen of its own (same nodes, notch and
+    pulls)."""
+    sp = p.specimens[k]
+    a, L = sp["offset"], sp["side"]
+    sl = slice(a, a + L * L)
+    nb = None
+    if p.node_bc is not None:
+        nodes, dofs, rates = p.node_bc
+        m = (nodes >= a) & (nodes < a + L * L)
+        nb = (nodes[m] - a, dofs[m], rates[m])
+    return dataclasses.replace(p, name=f"{p.name}_m{k}", x=p.x[sl].copy(),
+                               vol=p.vol[sl].copy(), cracks=[p.cracks[k]],
+                               ubc=[], node_bc=nb, specimens=[sp])
+
+
 # Quasi-static workload: 300 x 300 nodes, h = 1e-3 m, horizon 3h, 20 load
 # steps of 1e-5 m on the top rows, bottom rows clamped.
 QS_2D = {"qs_plate300": dict(nx=300, h=1e-3, ratio=3, pull=1e-5, nsteps=20)}
@@ -273,6 +354,8 @@ CUBE_3D = {
 
 
 def workload(key: str, **over) -> Problem:
+    if key == "ensemble256":
+        return ensemble(name=key, **over)
     if key in QS_2D:
         spec = dict(QS_2D[key])
         spec.update(over)
