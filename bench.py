@@ -233,6 +233,10 @@ def quasi_static(a):
                                                       flush=True)
                           if m.endswith("samples") is False else None)
     t_train = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    net2 = gnn_train.train(nx=64, specimens=60, load_steps=5, epochs=6,
+                           seed=2, coarse=True, log=None)
+    t_train2 = time.perf_counter() - t0
     p = pr.workload(a.qs_workload, nsteps=a.qs_steps)
     out = {"workload": a.qs_workload, "nodes": p.n, "load_steps": a.qs_steps,
            "newton_tol_rel": 1e-8, "linear_tol_rel": 1e-8,
@@ -240,7 +244,11 @@ def quasi_static(a):
                         "loss_first_epoch": net["loss"][0],
                         "loss_last_epoch": net["loss"][-1],
                         "s_data": net["s_data"], "s_train": net["s_train"],
-                        "s_total": t_train}}
+                        "s_total": t_train},
+           "training_two_level": {"pairs": net2["pairs"], "specimens": 60,
+                                  "loss_first_epoch": net2["loss"][0],
+                                  "loss_last_epoch": net2["loss"][-1],
+                                  "s_total": t_train2}}
     C = 16  # two-level coarse correction flag
     for name, pre in (("jacobi", NLM_QS_PRECOND_JACOBI),
                       ("learned", NLM_QS_PRECOND_BLOCKS),
@@ -249,9 +257,11 @@ def quasi_static(a):
         g = GpuFD(p, precision=NLM_PREC_FP64_EXACT)
         if pre & 15 == NLM_QS_PRECOND_BLOCKS:
             g.gnn_create(1)
-            g.gnn_set_weights(net["weights"])
+            g.gnn_set_weights(net2["weights"] if pre & C else net["weights"])
+        # the preconditioner (network prediction, coarse operator) is built
+        # from the first load step's tangent and kept for the call
         st = g.qs_solve(a.qs_steps, tol_rel=1e-8, precond=pre, lin_tol=1e-8,
-                        lin_max_iter=50000, gnn_refresh=False)
+                        lin_max_iter=50000, gnn_refresh=2)
         out[name] = st
         g.close()
         print(f"[bench] quasi-static {name}: {st['ms_total']:.1f} ms, "

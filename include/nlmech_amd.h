@@ -361,9 +361,10 @@ typedef struct {
   int predictor;      /* 1: linearized first iteration with the prescribed
                          increment as Dirichlet data (no force evaluation at
                          the jumped state); 0: the reference's order */
-  int gnn_refresh;    /* NLM_QS_PRECOND_BLOCKS with a graph network: 1
-                         predict the preconditioner at every Newton
-                         iteration, 0 once per load step */
+  int gnn_refresh;    /* preconditioner construction (network prediction,
+                         coarse operator): 1 at every Newton iteration, 0 once
+                         per load step, 2 once per call (the first load
+                         step's tangent serves the whole call) */
 } nlm_qs_options;
 
 typedef struct {
@@ -394,6 +395,9 @@ int nlm_qs_apply(nlm_ctx* ctx, const double* x, double* y);
  * in FP32 on the device). */
 int nlm_qs_set_precond(nlm_ctx* ctx, const double* blocks, const double* diag);
 int nlm_qs_get_precond(nlm_ctx* ctx, double* blocks, double* diag);
+/* z = R A_c^-1 R^T r: the coarse correction of the two-level preconditioner
+ * on the assembled tangent (r, z AoS double[3n]). */
+int nlm_qs_coarse_apply(nlm_ctx* ctx, const double* r, double* z);
 /* Solves A_ff x = b with the assembled tangent (b, x AoS double[3n]). */
 int nlm_qs_linear_solve(nlm_ctx* ctx, const double* b, double* x,
                         int precond, double tol, int max_iter, int* iters,

@@ -184,6 +184,7 @@ SIGNATURES = {
     "nlm_qs_apply": (C.c_int, [_P, _D, _D]),
     "nlm_qs_set_precond": (C.c_int, [_P, _D, _D]),
     "nlm_qs_get_precond": (C.c_int, [_P, _D, _D]),
+    "nlm_qs_coarse_apply": (C.c_int, [_P, _D, _D]),
     "nlm_gnn_num_params": (C.c_int64, []),
     "nlm_gnn_create": (C.c_int, [_P, C.c_uint64]),
     "nlm_gnn_get_weights": (C.c_int, [_P, _F]),

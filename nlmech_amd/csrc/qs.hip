@@ -201,6 +201,72 @@ __global__ __launch_bounds__(256) void k_qs_spmv(
   if (DOT == 2) block_partial(p1, P1, lb);
 }
 
+// Same product with one workgroup per slice: wavefront q of the four takes
+// slots q, q+4, ..., their sums are added in a fixed order by wavefront 0.
+// Partials (DOT) are per slice.  Four times the parallelism of k_qs_spmv, for
+// the latency-bound Krylov products of small systems.
+template <int DOT, class BT>
+__global__ __launch_bounds__(256) void k_qs_spmv4(
+    QsArgs a, const BT* __restrict__ blk, const BT* __restrict__ dg,
+    const double2* __restrict__ xv, double2* __restrict__ y,
+    const double2* __restrict__ wv, double* __restrict__ P0,
+    double* __restrict__ P1, const int* __restrict__ ctl,
+    const double* __restrict__ zc) {
+  if (ctl && ctl[0]) return;
+  const int64_t s = xcd_block(a.nslices);
+  if (s >= a.nslices) return;
+  __shared__ double2 red[3][64];
+  const int lane = threadIdx.x & 63;
+  const int wq = threadIdx.x >> 6;
+  const int64_t i = s * 64 + lane;
+  const bool valid = i < a.n;
+  const int64_t cb = a.sl_ptr[s];
+  const int width = int((a.sl_ptr[s + 1] - cb) >> 6);
+  double2 acc = make_double2(0., 0.);
+  for (int k = wq; k < width; k += 4) {
+    const int64_t e = cb + int64_t(k) * 64 + lane;
+    const BT B = blk[e];
+    const double2 xj = xv[a.col[e]];
+    acc.x += double(B.x) * xj.x + double(B.y) * xj.y;
+    acc.y += double(B.z) * xj.x + double(B.w) * xj.y;
+  }
+  if (wq > 0) red[wq - 1][lane] = acc;
+  __syncthreads();
+  if (wq != 0) return;
+  acc.x += red[0][lane].x + red[1][lane].x + red[2][lane].x;
+  acc.y += red[0][lane].y + red[1][lane].y + red[2][lane].y;
+  double p0 = 0., p1 = 0.;
+  if (valid) {
+    const BT D = dg[i];
+    const double2 xi = xv[i];
+    acc.x += double(D.x) * xi.x + double(D.y) * xi.y;
+    acc.y += double(D.z) * xi.x + double(D.w) * xi.y;
+    if (zc) {
+      acc.x += zc[2 * (i >> 8)];
+      acc.y += zc[2 * (i >> 8) + 1];
+    }
+    const uint32_t fl = a.flags[i];
+    if (dof_fixed(fl, 0)) acc.x = 0.;
+    if (dof_fixed(fl, 1)) acc.y = 0.;
+    y[i] = acc;
+    if (DOT >= 1) {
+      const double2 w = wv[i];
+      p0 = w.x * acc.x + w.y * acc.y;
+    }
+    if (DOT == 2) p1 = acc.x * acc.x + acc.y * acc.y;
+  }
+  if (DOT >= 1) {
+    for (int off = 32; off > 0; off >>= 1) {
+      p0 += __shfl_xor(p0, off);
+      if (DOT == 2) p1 += __shfl_xor(p1, off);
+    }
+    if (lane == 0) {
+      P0[s] = p0;
+      if (DOT == 2) P1[s] = p1;
+    }
+  }
+}
+
 // Scalars of iteration it: rho[it] = (rh, r_it), alpha[it], omega[it],
 // rr[it] = (r_it, r_it), at offsets 8 + {0, 1, 2, 3} * L + it.
 struct Scal {
@@ -215,6 +281,7 @@ struct Scal {
 
 struct VecArgs {
   int64_t n, nb;
+  int64_t nbs;  // partials of the Krylov products (P2, P3): one per slice
   const uint32_t* __restrict__ flags;
   double2 *r, *rh, *p, *v, *s, *t, *x, *ph, *sh;
   const double2* __restrict__ dinv;
@@ -304,7 +371,7 @@ __global__ __launch_bounds__(256) void k_bicg_p(VecArgs a, int it) {
 template <int PRE, bool RESTRICT = false>
 __global__ __launch_bounds__(256) void k_bicg_s(VecArgs a, int it) {
   if (a.ctl[0]) return;
-  const double rv = block_sum(a.P2, a.nb);
+  const double rv = block_sum(a.P2, a.nbs);
   const double rho_i = a.sc.rho(it - 1);
   const bool brk = rv == 0.;
   const double alpha = brk ? 0. : rho_i / rv;
@@ -438,6 +505,32 @@ __global__ void k_fill(double* __restrict__ p, int64_t cnt, double v) {
   if (t < cnt) p[t] = v;
 }
 
+// rc = R^T v (free dofs) per aggregate.
+__global__ __launch_bounds__(256) void k_restrict(VecArgs a,
+                                                  const double2* __restrict__ v) {
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  double2 w = make_double2(0., 0.);
+  if (i < a.n) {
+    w = v[i];
+    const uint32_t fl = a.flags[i];
+    if (dof_fixed(fl, 0)) w.x = 0.;
+    if (dof_fixed(fl, 1)) w.y = 0.;
+  }
+  block_restrict(w.x, w.y, a.rc, blockIdx.x);
+}
+
+// z = R zc on free dofs.
+__global__ __launch_bounds__(256) void k_prolong(VecArgs a,
+                                                 const double* __restrict__ zc,
+                                                 double2* __restrict__ z) {
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= a.n) return;
+  const uint32_t fl = a.flags[i];
+  const int64_t b = i >> 8;
+  z[i] = make_double2(dof_fixed(fl, 0) ? 0. : zc[2 * b],
+                      dof_fixed(fl, 1) ? 0. : zc[2 * b + 1]);
+}
+
 // Coarse dofs without a free fine dof get an identity row and column.
 __global__ void k_coarse_fixdiag(double* __restrict__ Ac, int m) {
   const int d = blockIdx.x * 256 + threadIdx.x;
@@ -467,8 +560,8 @@ __global__ void k_gj_sweep(const double* __restrict__ A, double* __restrict__ B,
 // omega, x += alpha ph + omega sh, r = s - omega t; partials (rh, r), (r, r).
 __global__ __launch_bounds__(256) void k_bicg_x(VecArgs a, int it) {
   if (a.ctl[0]) return;
-  const double ts = block_sum(a.P2, a.nb);
-  const double tt = block_sum(a.P3, a.nb);
+  const double ts = block_sum(a.P2, a.nbs);
+  const double tt = block_sum(a.P3, a.nbs);
   const double omega = tt > 0. ? ts / tt : 0.;
   const double alpha = a.sc.alpha(it);
   if (blockIdx.x == 0 && threadIdx.x == 0) a.sc.omega(it) = omega;
@@ -620,7 +713,9 @@ int qs_alloc(nlm_ctx* c, int max_iter) {
     for (DevBuf* d : vec)
       if ((rc = alloc(c, *d, n * sizeof(double2) + 256))) return rc;
     q.nb = (n + 255) / 256;
-    if ((rc = alloc(c, q.part, 4 * q.nb * sizeof(double) + 256))) return rc;
+    if ((rc = alloc(c, q.part, (2 * q.nb + 2 * s.sell.nslices) *
+                                   sizeof(double) + 256)))
+      return rc;
     if ((rc = alloc(c, q.ctl, 64))) return rc;
     if (!q.e0) NLM_HIP(c, hipEventCreate(&q.e0));
     if (!q.e1) NLM_HIP(c, hipEventCreate(&q.e1));
@@ -651,10 +746,11 @@ VecArgs vec_args(nlm_ctx* c) {
   a.sh = q.sh.as<double2>();
   a.dinv = q.dinv.as<double2>();
   double* P = q.part.as<double>();
+  a.nbs = c->fd.sell.nslices;
   a.P0 = P;
   a.P1 = P + q.nb;
   a.P2 = P + 2 * q.nb;
-  a.P3 = P + 3 * q.nb;
+  a.P3 = P + 2 * q.nb + a.nbs;
   a.rc = nullptr;
   a.ctl = q.ctl.as<int>();
   a.sc.base = q.scal.as<double>();
@@ -697,9 +793,10 @@ template <int DOT, class BT = double4>
 void spmv(nlm_ctx* c, const DevBuf& blk, const DevBuf& dg, const double2* x,
           double2* y, const double2* w, double* P0, double* P1,
           const int* ctl, const double* zc = nullptr) {
-  hipLaunchKernelGGL((k_qs_spmv<DOT, BT>), dim3(slice_grid(c->fd)), dim3(256),
-                     0, c->stream, qs_args(c), blk.as<BT>(), dg.as<BT>(), x, y,
-                     w, P0, P1, ctl, zc);
+  const unsigned g = unsigned((c->fd.sell.nslices + 7) / 8 * 8);
+  hipLaunchKernelGGL((k_qs_spmv4<DOT, BT>), dim3(g), dim3(256), 0, c->stream,
+                     qs_args(c), blk.as<BT>(), dg.as<BT>(), x, y, w, P0, P1,
+                     ctl, zc);
 }
 // The preconditioner's blocks are FP32 (their rounding only perturbs the
 // preconditioner, never the operator).
@@ -1086,6 +1183,28 @@ extern "C" int nlm_qs_apply(nlm_ctx* c, const double* x, double* y) {
   return dev_to_aos(c, q.y, y);
 }
 
+extern "C" int nlm_qs_coarse_apply(nlm_ctx* c, const double* r, double* z) {
+  int rc = qs_check(c, "nlm_qs_coarse_apply");
+  if (rc) return rc;
+  QsState& q = c->qs;
+  if (!q.assembled)
+    return fail(c, NLM_ERR_STATE, "nlm_qs_coarse_apply: nothing assembled");
+  if (!r || !z) return fail(c, NLM_ERR_INVALID, "nlm_qs_coarse_apply: null");
+  if ((rc = coarse_setup(c))) return rc;
+  if ((rc = aos_to_dev(c, r, q.b))) return rc;
+  VecArgs a = vec_args(c);
+  a.rc = q.rc.as<double>();
+  const unsigned g = node_grid(c->fd);
+  hipLaunchKernelGGL(k_restrict, dim3(g), dim3(256), 0, c->stream, a,
+                     q.b.as<double2>());
+  hipLaunchKernelGGL(k_coarse_mv, dim3(unsigned((q.m + 3) / 4)), dim3(256), 0,
+                     c->stream, q.cinv, q.m, a.rc, q.zcp.as<double>(), nullptr);
+  hipLaunchKernelGGL(k_prolong, dim3(g), dim3(256), 0, c->stream, a,
+                     q.zcp.as<double>(), q.y.as<double2>());
+  NLM_HIP(c, hipGetLastError());
+  return dev_to_aos(c, q.y, z);
+}
+
 extern "C" int nlm_qs_linear_solve(nlm_ctx* c, const double* b, double* x,
                                    int precond, double tol, int max_iter,
                                    int* iters, double* relres) {
@@ -1168,6 +1287,7 @@ extern "C" int nlm_qs_solve(nlm_ctx* c, const nlm_qs_options* o,
     stats.ms_tangent += ms_since(t0);
     return r;
   };
+  bool have_precond = false;
   for (int step = 0; step < o->nsteps; step++) {
     const double t = s.time + o->dt;
     double rnorm = 0., r0 = 0.;
@@ -1189,7 +1309,10 @@ extern "C" int nlm_qs_solve(nlm_ctx* c, const nlm_qs_options* o,
       double bb = 0.;
       if ((rc = partial_sum(c, q.part.as<double>(), &bb))) return rc;
       r0 = std::sqrt(bb);
-      if ((rc = precond_setup())) return rc;
+      if (!(o->gnn_refresh == 2 && have_precond)) {
+        if ((rc = precond_setup())) return rc;
+        have_precond = true;
+      }
       if ((rc = linear(&lit))) return rc;
       hipLaunchKernelGGL(k_qs_update_u, dim3(g), dim3(256), 0, c->stream, s.n,
                          s.flags.as<uint32_t>(), u_cur(), q.x.as<double2>(),
@@ -1212,12 +1335,13 @@ extern "C" int nlm_qs_solve(nlm_ctx* c, const nlm_qs_options* o,
       r0 = rnorm;
     }
     const double target = std::max(o->tol_abs, o->tol_rel * r0);
-    bool refreshed = o->predictor;
+    bool refreshed = o->predictor || (o->gnn_refresh == 2 && have_precond);
     while (rnorm > target && newton < o->max_newton) {
       if ((rc = tangent())) return rc;
-      if (o->gnn_refresh || !refreshed) {
+      if (o->gnn_refresh == 1 || !refreshed) {
         if ((rc = precond_setup())) return rc;
         refreshed = true;
+        have_precond = true;
       }
       if ((rc = linear(&lit))) return rc;
       hipLaunchKernelGGL(k_qs_update_u, dim3(g), dim3(256), 0, c->stream, s.n,

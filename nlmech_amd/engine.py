@@ -300,7 +300,7 @@ class GpuFD:
     # -- quasi-static model (2D bond-based) ---------------------------------
     def qs_solve(self, nsteps, dt=1.0, tol_abs=0.0, tol_rel=1e-8,
                  max_newton=50, tangent=0, precond=1, lin_tol=1e-8,
-                 lin_max_iter=5000, predictor=True, gnn_refresh=True):
+                 lin_max_iter=5000, predictor=True, gnn_refresh=1):
         """nsteps load steps of the quasi-static Newton solver; returns the
         run statistics as a dict."""
         o = capi.QsOptions(nsteps=int(nsteps), dt=float(dt),
@@ -309,7 +309,7 @@ class GpuFD:
                            precond=int(precond), lin_tol=float(lin_tol),
                            lin_max_iter=int(lin_max_iter),
                            predictor=int(bool(predictor)),
-                           gnn_refresh=int(bool(gnn_refresh)))
+                           gnn_refresh=int(gnn_refresh))
         st = capi.QsStats()
         self._call("nlm_qs_solve", C.byref(o), C.byref(st))
         return {f: getattr(st, f) for f, _ in capi.QsStats._fields_}
@@ -334,6 +334,12 @@ class GpuFD:
         b = np.ascontiguousarray(np.asarray(blocks).reshape(-1, 4), np.float64)
         d = np.ascontiguousarray(np.asarray(diag).reshape(-1, 4), np.float64)
         self._call("nlm_qs_set_precond", ptr(b, C.c_double), ptr(d, C.c_double))
+
+    def qs_coarse_apply(self, r):
+        r = np.ascontiguousarray(r, np.float64)
+        z = np.zeros((self.p.n, 3))
+        self._call("nlm_qs_coarse_apply", ptr(r, C.c_double), ptr(z, C.c_double))
+        return z
 
     def qs_precond(self):
         """(blocks[nnz, 2, 2], diag[n, 2, 2]) of the preconditioner."""

@@ -36,8 +36,10 @@ def specimen(nx: int, seed: int, pull: float = 1e-5) -> pr.Problem:
 
 
 def collect(nx: int, specimens: int, load_steps: int, seed: int = 0,
-            probes: int = PROBES, log=None):
-    """List of samples (bonds, u, r[K, n, 3], e[K, n, 3])."""
+            probes: int = PROBES, log=None, coarse: bool = False):
+    """List of samples (bonds, u, r[K, n, 3], e[K, n, 3]).  coarse: the
+    targets are e_k - C r_k with C the two-level coarse correction, so the
+    network learns the part of A^-1 the coarse level leaves (P = M + C)."""
     from .engine import GpuFD
     out = []
     rng = np.random.default_rng(seed + 4242)
@@ -66,6 +68,8 @@ def collect(nx: int, specimens: int, load_steps: int, seed: int = 0,
                 v[fix & 2 != 0, 1] = 0.0
                 es.append(v)
             rs = [g.qs_apply(e) for e in es]
+            if coarse:
+                es = [e - g.qs_coarse_apply(r) for e, r in zip(es, rs)]
             r_, e_ = np.stack(rs), np.stack(es)
             if np.isfinite(r_).all() and np.isfinite(e_).all():
                 out.append((g.get_bonds(), u, r_, e_))
@@ -77,13 +81,14 @@ def collect(nx: int, specimens: int, load_steps: int, seed: int = 0,
 
 def train(nx: int = 48, specimens: int = 20, load_steps: int = 4,
           epochs: int = 10, lr: float = 1e-3, seed: int = 0, log=None,
-          samples=None):
+          samples=None, coarse: bool = False):
     """Trains a network on freshly collected samples; returns its FP32
     weights and the loss history."""
     from .engine import GpuFD
     t0 = time.perf_counter()
     if samples is None:
-        samples = collect(nx, specimens, load_steps, seed, log=log)
+        samples = collect(nx, specimens, load_steps, seed, log=log,
+                          coarse=coarse)
     t_data = time.perf_counter() - t0
     base = pr.qs_plate(nx, notch_len=0.0)
     g = GpuFD(base, precision=NLM_PREC_FP64_EXACT)
