@@ -324,9 +324,11 @@ const char* nlm_fd_kernel_name(nlm_ctx* ctx);
 /* Quasi-static model: QuasiStaticModel<T> (src/model/quasistatic/)          */
 /* ------------------------------------------------------------------------ */
 
-/* The quasi-static solver runs on a context set up with nlm_fd_setup (2D,
- * bond-based material) and its displacement / force BC sets.  Load step n
- * is t = t0 + n dt: FLoading gives the body force b(t), ULoading the
+/* The quasi-static solver runs on a context set up with nlm_fd_setup (2D;
+ * bond-based material, or the state-based material without bond breaking
+ * with the finite-difference tangent) and its displacement / force BC sets.  Load step n
+ * (the context's step count plus one, then advanced) is at t = n dt as
+ * QuasiStaticModel::solver sets it: FLoading gives the body force b(t), ULoading the
  * prescribed displacements, and Newton iterations solve R(u) = f_int(u) + b
  * = 0 on the free dofs (QuasiStaticModel::solver, QuasiStaticModel.cpp:
  * 527-592; computeResidual :597-646).  The tangent is the analytic derivative
@@ -336,6 +338,13 @@ const char* nlm_fd_kernel_name(nlm_ctx* ctx);
 #define NLM_QS_TANGENT_COMPONENT 1 /* same-component entries only, the
                                       entries the reference's central-
                                       difference Jacobian keeps (:436-440) */
+#define NLM_QS_TANGENT_FD 2        /* the reference's Jacobian itself:
+                                      central differences of the force with
+                                      eps = perturbation * h, same-component
+                                      entries on the neighbor sparsity
+                                      (assembly_jacobian_matrix_part,
+                                      :363-454); the state-based material's
+                                      tangent */
 #define NLM_QS_PRECOND_NONE 0
 #define NLM_QS_PRECOND_JACOBI 1  /* point Jacobi */
 #define NLM_QS_PRECOND_BLOCKS 2  /* sparse approximate inverse with 2x2
@@ -365,6 +374,9 @@ typedef struct {
                          coarse operator): 1 at every Newton iteration, 0 once
                          per load step, 2 once per call (the first load
                          step's tangent serves the whole call) */
+  double perturbation; /* NLM_QS_TANGENT_FD: eps = perturbation * h
+                          (SolverDeck Perturbation); <= 0 keeps the last
+                          value (initially 1e-7) */
 } nlm_qs_options;
 
 typedef struct {
@@ -398,6 +410,10 @@ int nlm_qs_get_precond(nlm_ctx* ctx, double* blocks, double* diag);
 /* z = R A_c^-1 R^T r: the coarse correction of the two-level preconditioner
  * on the assembled tangent (r, z AoS double[3n]). */
 int nlm_qs_coarse_apply(nlm_ctx* ctx, const double* r, double* z);
+/* BiCGSTAB launch mode (no counterpart in the reference): on (default),
+ * each batch of 8 iterations is one replay of a captured HIP graph; off,
+ * the same kernels are launched one by one.  Results are identical. */
+int nlm_qs_set_graph(nlm_ctx* ctx, int on);
 /* Solves A_ff x = b with the assembled tangent (b, x AoS double[3n]). */
 int nlm_qs_linear_solve(nlm_ctx* ctx, const double* b, double* x,
                         int precond, double tol, int max_iter, int* iters,

@@ -23,6 +23,7 @@ NLM_MATERIAL_RNP_BOND = 0
 NLM_MATERIAL_ELASTIC_STATE = 1
 NLM_QS_TANGENT_FULL = 0
 NLM_QS_TANGENT_COMPONENT = 1
+NLM_QS_TANGENT_FD = 2
 NLM_QS_PRECOND_NONE = 0
 NLM_QS_PRECOND_JACOBI = 1
 NLM_QS_PRECOND_BLOCKS = 2
@@ -118,7 +119,7 @@ class QsOptions(C.Structure):
         ("tol_rel", C.c_double), ("max_newton", C.c_int),
         ("tangent", C.c_int), ("precond", C.c_int), ("lin_tol", C.c_double),
         ("lin_max_iter", C.c_int), ("predictor", C.c_int),
-        ("gnn_refresh", C.c_int),
+        ("gnn_refresh", C.c_int), ("perturbation", C.c_double),
     ]
 
 
@@ -194,6 +195,7 @@ SIGNATURES = {
     "nlm_gnn_adam_step": (C.c_int, [_P, C.c_double]),
     "nlm_qs_linear_solve": (C.c_int, [_P, _D, _D, C.c_int, C.c_double,
                                       C.c_int, C.POINTER(C.c_int), _D]),
+    "nlm_qs_set_graph": (C.c_int, [_P, C.c_int]),
 }
 
 _lib = None

@@ -53,6 +53,7 @@ extern "C" void nlm_ctx_destroy(nlm_ctx* c) {
   if (c->fd.ev0) (void)hipEventDestroy(c->fd.ev0);
   if (c->fd.ev1) (void)hipEventDestroy(c->fd.ev1);
   if (c->fd.graph) (void)hipGraphExecDestroy(c->fd.graph);
+  if (c->qs.bicg_graph) (void)hipGraphExecDestroy(c->qs.bicg_graph);
   if (c->qs.e0) (void)hipEventDestroy(c->qs.e0);
   if (c->qs.e1) (void)hipEventDestroy(c->qs.e1);
   hipStream_t st = c->stream;

@@ -224,13 +224,25 @@ __global__ __launch_bounds__(256) void k_mm_tn(int64_t M, int K,
 }
 
 // grad[off + t] += sum_b part[b][t], t < size (fixed order).
-__global__ void k_reduce_add(const float* __restrict__ part, int nparts,
-                             int size, float* __restrict__ grad) {
-  const int t = blockIdx.x * 256 + threadIdx.x;
-  if (t >= size) return;
+// A workgroup owns 16 entries; its 16 thread groups sum every 16th part
+// and the groups' sums are added in group order.
+constexpr int kRedCols = 16;
+__global__ __launch_bounds__(256) void k_reduce_add(
+    const float* __restrict__ part, int nparts, int size,
+    float* __restrict__ grad) {
+  __shared__ float red[16][kRedCols + 1];
+  const int tc = threadIdx.x % kRedCols, gq = threadIdx.x / kRedCols;
+  const int t = blockIdx.x * kRedCols + tc;
   float s = 0.f;
-  for (int b = 0; b < nparts; b++) s += part[int64_t(b) * size + t];
-  grad[t] += s;
+  if (t < size)
+    for (int b = gq; b < nparts; b += 16) s += part[int64_t(b) * size + t];
+  red[gq][tc] = s;
+  __syncthreads();
+  if (gq == 0 && t < size) {
+    float r = 0.f;
+    for (int k = 0; k < 16; k++) r += red[k][tc];
+    grad[t] += r;
+  }
 }
 
 // Column sums of dY [M][64] (BF16 or FP32) into part[block][64].
@@ -878,7 +890,8 @@ int mm_tn(nlm_ctx* c, int64_t M, int K, const uint16_t* X1, const uint16_t* X2,
   }
   hipLaunchKernelGGL(k_mm_tn, dim3(nb), dim3(256), 0, c->stream, M, K, X1, X2,
                      dY, G.part.as<float>());
-  hipLaunchKernelGGL(k_reduce_add, dim3(unsigned((K * 64 + 255) / 256)),
+  hipLaunchKernelGGL(k_reduce_add,
+                     dim3(unsigned((K * 64 + kRedCols - 1) / kRedCols)),
                      dim3(256), 0, c->stream, G.part.as<float>(), nb, K * 64,
                      G.grad.as<float>() + off);
   return NLM_OK;
@@ -895,8 +908,9 @@ int colsum(nlm_ctx* c, int64_t M, const T* dY, int64_t off) {
   }
   hipLaunchKernelGGL(k_colsum<T>, dim3(nb), dim3(256), 0, c->stream, M, dY,
                      G.part.as<float>());
-  hipLaunchKernelGGL(k_reduce_add, dim3(1), dim3(256), 0, c->stream,
-                     G.part.as<float>(), nb, 64, G.grad.as<float>() + off);
+  hipLaunchKernelGGL(k_reduce_add, dim3(64 / kRedCols), dim3(256), 0,
+                     c->stream, G.part.as<float>(), nb, 64,
+                     G.grad.as<float>() + off);
   return NLM_OK;
 }
 
@@ -1008,16 +1022,16 @@ int gnn_backward(nlm_ctx* c) {
     const int nb1 = int(std::min<int64_t>(ne, nbw));
     hipLaunchKernelGGL(k_dec_dw, dim3(nb1), dim3(256), 0, c->stream, ne,
                        G.dme.as<float>(), gbuf(G, kL), G.part.as<float>());
-    hipLaunchKernelGGL(k_reduce_add, dim3(2), dim3(256), 0, c->stream,
-                       G.part.as<float>(), nb1, 260, grad + kOffWoe);
+    hipLaunchKernelGGL(k_reduce_add, dim3((260 + kRedCols - 1) / kRedCols),
+                       dim3(256), 0, c->stream, G.part.as<float>(), nb1, 260, grad + kOffWoe);
     hipLaunchKernelGGL(k_dec_dx<true>, dim3(unsigned((n * 64 + 255) / 256)),
                        dim3(256), 0, c->stream, n, G.dmn.as<float>(),
                        w + kOffWon, nullptr, G.dh.as<float>());
     const int nb2 = int(std::min<int64_t>(n, nbw));
     hipLaunchKernelGGL(k_dec_dw, dim3(nb2), dim3(256), 0, c->stream, n,
                        G.dmn.as<float>(), hbuf(G, kL), G.part.as<float>());
-    hipLaunchKernelGGL(k_reduce_add, dim3(2), dim3(256), 0, c->stream,
-                       G.part.as<float>(), nb2, 260, grad + kOffWon);
+    hipLaunchKernelGGL(k_reduce_add, dim3((260 + kRedCols - 1) / kRedCols),
+                       dim3(256), 0, c->stream, G.part.as<float>(), nb2, 260, grad + kOffWon);
   }
   uint16_t* dpre_n = G.dh16.as<uint16_t>();
   uint16_t* dP16 = G.dPQ16.as<uint16_t>();

@@ -185,6 +185,16 @@ struct QsState {
   double* cinv = nullptr;
   int m = 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
+  // finite-difference tangent: node colors (no two nodes of one color
+  // within 4 horizons), perturbation factor (eps = factor * h)
+  DevBuf color;
+  int ncolors = 0;
+  double fd_perturbation = 1e-7;
+  // BiCGSTAB iterations run in batches replayed from a captured HIP graph;
+  // the key holds the kernel arguments of the capture
+  bool use_graph = true;
+  hipGraphExec_t bicg_graph = nullptr;
+  std::vector<unsigned char> bicg_key;
 };
 
 // Graph-network preconditioner: FP32 master weights, gradient, Adam

@@ -147,6 +147,72 @@ __global__ __launch_bounds__(256) void k_qs_tangent(
                          !dof_fixed(fli, 1) && D.w != 0. ? 1. / D.w : 0.);
 }
 
+// Finite-difference tangent (assembly_jacobian_matrix_part,
+// QuasiStaticModel.cpp:363-454).  u <- u0 +/- eps on component r of the
+// nodes of color cl.
+__global__ __launch_bounds__(256) void k_fd_perturb(
+    int64_t n, const double2* __restrict__ u0, const int* __restrict__ color,
+    int cl, int r, double eps, double2* __restrict__ u) {
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  double2 v = u0[i];
+  if (color[i] == cl) {
+    if (r == 0)
+      v.x = v.x + eps;
+    else
+      v.y = v.y + eps;
+  }
+  u[i] = v;
+}
+
+// A_ij (component r, r) = -(f+_i - f-_i)_r / (2 eps) for every slice entry
+// whose neighbor j has color cl, and A_ii when i has color cl.  Every node
+// within the force's reach of row i (two horizons) carries a distinct color,
+// so f+_i - f-_i holds the perturbation of j alone.
+__global__ __launch_bounds__(256) void k_fd_collect(
+    QsArgs a, const double2* __restrict__ fp, const double2* __restrict__ fm,
+    const int* __restrict__ color, int cl, int r, double eps,
+    double4* __restrict__ blk, double4* __restrict__ diag) {
+  const int64_t nblk = (a.nslices * 64 + 255) / 256;
+  const int64_t lb = xcd_block(nblk);
+  if (lb >= nblk) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t s = lb * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (s >= a.nslices) return;
+  const int64_t i = s * 64 + lane;
+  if (i >= a.n) return;
+  const int64_t cb = a.sl_ptr[s];
+  const int len = int(a.rowlen[i]);
+  const double2 dp = fp[i], dm = fm[i];
+  const double df = r == 0 ? dp.x - dm.x : dp.y - dm.y;
+  const double val = -(df / (2. * eps));
+  for (int k = 0; k < len; k++) {
+    const int64_t e = cb + int64_t(k) * 64 + lane;
+    if (color[a.col[e]] != cl) continue;
+    if (r == 0)
+      blk[e].x = val;
+    else
+      blk[e].w = val;
+  }
+  if (color[i] == cl) {
+    if (r == 0)
+      diag[i].x = val;
+    else
+      diag[i].w = val;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_fd_dinv(
+    int64_t n, const uint32_t* __restrict__ flags,
+    const double4* __restrict__ diag, double2* __restrict__ dinv) {
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double4 D = diag[i];
+  const uint32_t fl = flags[i];
+  dinv[i] = make_double2(!dof_fixed(fl, 0) && D.x != 0. ? 1. / D.x : 0.,
+                         !dof_fixed(fl, 1) && D.w != 0. ? 1. / D.w : 0.);
+}
+
 // y = (D x_i + sum_j B_ij x_j) on free dofs, 0 on prescribed dofs.
 // DOT 1: partial w.y into P0; DOT 2: partials w.y into P0 and y.y into P1.
 template <int DOT, class BT>
@@ -287,7 +353,9 @@ struct VecArgs {
   const double2* __restrict__ dinv;
   double *P0, *P1, *P2, *P3;
   double *rc;  // restriction of p (then s): per 256-node aggregate, 2 dofs
-  int* ctl;
+  int* ctl;     // [0] done (1 converged, 2 breakdown), [1] its iteration,
+                // [2] first iteration of the current batch
+  int max_it;
   Scal sc;
 };
 
@@ -324,15 +392,23 @@ __global__ __launch_bounds__(256) void k_bicg_init(VecArgs a,
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     a.ctl[0] = 0;
     a.ctl[1] = 0;
+    a.ctl[2] = 1;
   }
+}
+
+// Moves the batch's first iteration on by n (one thread).
+__global__ void k_iter_advance(int* __restrict__ ctl, int n) {
+  if (threadIdx.x == 0) ctl[2] += n;
 }
 
 // Iteration it, step 1: convergence test on r_{it-1}; p = r + beta (p -
 // omega v); ph = M p for the diagonal preconditioners (PRE 0 none, 1 Jacobi;
 // PRE 2 leaves ph to a block product).
 template <int PRE, bool RESTRICT = false>
-__global__ __launch_bounds__(256) void k_bicg_p(VecArgs a, int it) {
+__global__ __launch_bounds__(256) void k_bicg_p(VecArgs a, int k) {
   if (a.ctl[0]) return;
+  const int it = a.ctl[2] + k;
+  if (it > a.max_it) return;
   const double rho_i = block_sum(a.P0, a.nb);
   const double rr = block_sum(a.P1, a.nb);
   const bool conv = !(rr > a.sc.tol2());
@@ -369,8 +445,10 @@ __global__ __launch_bounds__(256) void k_bicg_p(VecArgs a, int it) {
 
 // Step 2 (after v = A ph with partial (rh, v) in P2): alpha, s = r - alpha v.
 template <int PRE, bool RESTRICT = false>
-__global__ __launch_bounds__(256) void k_bicg_s(VecArgs a, int it) {
+__global__ __launch_bounds__(256) void k_bicg_s(VecArgs a, int k) {
   if (a.ctl[0]) return;
+  const int it = a.ctl[2] + k;
+  if (it > a.max_it) return;
   const double rv = block_sum(a.P2, a.nbs);
   const double rho_i = a.sc.rho(it - 1);
   const bool brk = rv == 0.;
@@ -558,8 +636,10 @@ __global__ void k_gj_sweep(const double* __restrict__ A, double* __restrict__ B,
 
 // Step 3 (after t = A sh with partials (t, s) in P2 and (t, t) in P3):
 // omega, x += alpha ph + omega sh, r = s - omega t; partials (rh, r), (r, r).
-__global__ __launch_bounds__(256) void k_bicg_x(VecArgs a, int it) {
+__global__ __launch_bounds__(256) void k_bicg_x(VecArgs a, int k) {
   if (a.ctl[0]) return;
+  const int it = a.ctl[2] + k;
+  if (it > a.max_it) return;
   const double ts = block_sum(a.P2, a.nbs);
   const double tt = block_sum(a.P3, a.nbs);
   const double omega = tt > 0. ? ts / tt : 0.;
@@ -673,6 +753,7 @@ __global__ __launch_bounds__(256) void k_qs_update_u(
 QsArgs qs_args(nlm_ctx* c) {
   FdState& s = c->fd;
   QsArgs a;
+  std::memset(&a, 0, sizeof(a));
   a.n = s.n;
   a.nslices = s.sell.nslices;
   a.nb = (s.n + 255) / 256;
@@ -692,9 +773,32 @@ int qs_check(nlm_ctx* c, const char* fn) {
   if (!c) return NLM_ERR_INVALID;
   if (!c->fd.ready)
     return fail(c, NLM_ERR_STATE, std::string(fn) + ": nlm_fd_setup not done");
-  if (c->fd.dim != 2 || c->fd.material != NLM_MATERIAL_RNP_BOND)
+  if (c->fd.dim != 2)
+    return fail(c, NLM_ERR_INVALID, std::string(fn) + ": 2D models only");
+  if (c->fd.material == NLM_MATERIAL_ELASTIC_STATE &&
+      c->fd.es_par.critical_stretch > 0.)
     return fail(c, NLM_ERR_INVALID,
-                std::string(fn) + ": 2D bond-based material only");
+                std::string(fn) + ": the state-based material must not break "
+                                  "bonds in a quasi-static solve");
+  return NLM_OK;
+}
+
+// Checks that the tangent kind fits the material: the analytic tangents
+// are the bond-based force's, the finite-difference tangent needs a force
+// without bond breaking (the state-based material here).
+int tangent_check(nlm_ctx* c, int tangent, const char* fn) {
+  const bool es = c->fd.material == NLM_MATERIAL_ELASTIC_STATE;
+  if (tangent == NLM_QS_TANGENT_FD && !es)
+    return fail(c, NLM_ERR_INVALID,
+                std::string(fn) + ": the finite-difference tangent needs the "
+                                  "state-based material");
+  if (tangent != NLM_QS_TANGENT_FD && es)
+    return fail(c, NLM_ERR_INVALID,
+                std::string(fn) + ": the state-based material needs "
+                                  "NLM_QS_TANGENT_FD");
+  if (tangent != NLM_QS_TANGENT_FULL && tangent != NLM_QS_TANGENT_COMPONENT &&
+      tangent != NLM_QS_TANGENT_FD)
+    return fail(c, NLM_ERR_INVALID, std::string(fn) + ": unknown tangent");
   return NLM_OK;
 }
 
@@ -732,6 +836,7 @@ int qs_alloc(nlm_ctx* c, int max_iter) {
 VecArgs vec_args(nlm_ctx* c) {
   QsState& q = c->qs;
   VecArgs a;
+  std::memset(&a, 0, sizeof(a));  // padding too: the graph key compares bytes
   a.n = c->fd.n;
   a.nb = q.nb;
   a.flags = c->fd.flags.as<uint32_t>();
@@ -758,6 +863,21 @@ VecArgs vec_args(nlm_ctx* c) {
   return a;
 }
 
+// BiCGSTAB iterations per host check (and per graph replay).
+constexpr int kBicgBatch = 8;
+
+// Everything a captured BiCGSTAB batch depends on.
+struct BicgKey {
+  VecArgs a;
+  QsArgs qa;
+  int precond;
+  int coarse;
+  int m;
+  unsigned g, gmv;
+  const double* cinv;
+  const void *blk, *diag, *mblk, *mdiag, *zcp, *zcs;
+};
+
 unsigned node_grid(const FdState& s) { return unsigned((s.n + 255) / 256); }
 unsigned slice_grid(const FdState& s) {
   return grid_xcd(force_blocks(s) * 256, 256);
@@ -776,10 +896,132 @@ int partial_sum(nlm_ctx* c, const double* P, double* out) {
   return NLM_OK;
 }
 
+// Greedy coloring of the nodes (internal order): two nodes closer than
+// 4 horizons (plus the neighbor test's tolerance) get different colors, so
+// no force f_i, which reads the displacements within two horizons of i,
+// sees two perturbed nodes.
+int fd_colors(nlm_ctx* c) {
+  FdState& s = c->fd;
+  QsState& q = c->qs;
+  if (q.ncolors > 0) return NLM_OK;
+  const int64_t n = s.n;
+  std::vector<double2> x(n);
+  NLM_HIP(c, hipMemcpy(x.data(), s.x.p, sizeof(double2) * n,
+                       hipMemcpyDeviceToHost));
+  const double horizon = s.material == NLM_MATERIAL_ELASTIC_STATE
+                             ? s.es_par.horizon
+                             : s.mat.horizon;
+  const double R = 4. * (horizon + 1e-10) * (1. + 2e-5);
+  const double R2 = R * R;
+  double lo0 = x[0].x, lo1 = x[0].y, hi0 = lo0, hi1 = lo1;
+  for (const auto& p : x) {
+    lo0 = std::min(lo0, p.x);
+    lo1 = std::min(lo1, p.y);
+    hi0 = std::max(hi0, p.x);
+    hi1 = std::max(hi1, p.y);
+  }
+  const int64_t nc0 = int64_t((hi0 - lo0) / R) + 1;
+  const int64_t nc1 = int64_t((hi1 - lo1) / R) + 1;
+  auto cell = [&](const double2& p, int64_t* a, int64_t* b) {
+    *a = std::min(nc0 - 1, int64_t((p.x - lo0) / R));
+    *b = std::min(nc1 - 1, int64_t((p.y - lo1) / R));
+  };
+  std::vector<int64_t> head(nc0 * nc1, -1), next(n, -1);
+  for (int64_t i = 0; i < n; i++) {
+    int64_t a, b;
+    cell(x[i], &a, &b);
+    next[i] = head[a * nc1 + b];
+    head[a * nc1 + b] = i;
+  }
+  std::vector<int> color(n, -1);
+  std::vector<int64_t> seen;  // seen[c] == i: color c taken near node i
+  int ncol = 0;
+  for (int64_t i = 0; i < n; i++) {
+    int64_t a, b;
+    cell(x[i], &a, &b);
+    for (int64_t da = -1; da <= 1; da++)
+      for (int64_t db = -1; db <= 1; db++) {
+        const int64_t ca = a + da, cb = b + db;
+        if (ca < 0 || cb < 0 || ca >= nc0 || cb >= nc1) continue;
+        for (int64_t j = head[ca * nc1 + cb]; j >= 0; j = next[j]) {
+          if (color[j] < 0) continue;
+          const double d0 = x[j].x - x[i].x, d1 = x[j].y - x[i].y;
+          if (d0 * d0 + d1 * d1 <= R2) seen[color[j]] = i;
+        }
+      }
+    int cl = 0;
+    while (cl < ncol && seen[cl] == i) cl++;
+    if (cl == ncol) {
+      ncol++;
+      seen.push_back(-1);
+    }
+    color[i] = cl;
+  }
+  int rc;
+  if ((rc = alloc(c, q.color, sizeof(int) * n))) return rc;
+  NLM_HIP(c, hipMemcpy(q.color.p, color.data(), sizeof(int) * n,
+                       hipMemcpyHostToDevice));
+  q.ncolors = ncol;
+  return NLM_OK;
+}
+
+// The reference's Jacobian by central differences of the force, colored:
+// 2 * dim force evaluations per color.  u is restored and the force of the
+// current state recomputed at the end.
+int assemble_fd(nlm_ctx* c) {
+  FdState& s = c->fd;
+  QsState& q = c->qs;
+  int rc;
+  if ((rc = fd_colors(c))) return rc;
+  const double h = s.material == NLM_MATERIAL_ELASTIC_STATE ? s.es_par.h
+                                                             : s.mat.h;
+  const double eps = q.fd_perturbation * h;
+  if (!(eps > 0.))
+    return fail(c, NLM_ERR_INVALID, "finite-difference tangent: eps <= 0");
+  const size_t nent = size_t(s.sell.nentries);
+  NLM_HIP(c, hipMemsetAsync(q.blk.p, 0, nent * sizeof(double4), c->stream));
+  NLM_HIP(c, hipMemsetAsync(q.diag.p, 0, s.n * sizeof(double4), c->stream));
+  double2* u = s.u[s.cur].as<double2>();
+  NLM_HIP(c, hipMemcpyAsync(q.u0.p, u, sizeof(double2) * s.n,
+                            hipMemcpyDeviceToDevice, c->stream));
+  const unsigned g = node_grid(s);
+  const double t = s.time;
+  for (int cl = 0; cl < q.ncolors; cl++)
+    for (int r = 0; r < 2; r++) {
+      hipLaunchKernelGGL(k_fd_perturb, dim3(g), dim3(256), 0, c->stream, s.n,
+                         q.u0.as<double2>(), q.color.as<int>(), cl, r, eps, u);
+      fd_launch_force(c, false, t);
+      NLM_HIP(c, hipMemcpyAsync(q.y.p, s.f.p, sizeof(double2) * s.n,
+                                hipMemcpyDeviceToDevice, c->stream));
+      hipLaunchKernelGGL(k_fd_perturb, dim3(g), dim3(256), 0, c->stream, s.n,
+                         q.u0.as<double2>(), q.color.as<int>(), cl, r, -eps,
+                         u);
+      fd_launch_force(c, false, t);
+      hipLaunchKernelGGL(k_fd_collect, dim3(slice_grid(s)), dim3(256), 0,
+                         c->stream, qs_args(c), q.y.as<double2>(),
+                         s.f.as<double2>(), q.color.as<int>(), cl, r, eps,
+                         q.blk.as<double4>(), q.diag.as<double4>());
+    }
+  NLM_HIP(c, hipMemcpyAsync(u, q.u0.p, sizeof(double2) * s.n,
+                            hipMemcpyDeviceToDevice, c->stream));
+  fd_launch_force(c, false, t);
+  hipLaunchKernelGGL(k_fd_dinv, dim3(g), dim3(256), 0, c->stream, s.n,
+                     s.flags.as<uint32_t>(), q.diag.as<double4>(),
+                     q.dinv.as<double2>());
+  NLM_HIP(c, hipGetLastError());
+  return NLM_OK;
+}
+
 int assemble(nlm_ctx* c, int tangent) {
   QsState& q = c->qs;
   int rc;
   if ((rc = qs_alloc(c, 8))) return rc;
+  if (tangent == NLM_QS_TANGENT_FD) {
+    if ((rc = assemble_fd(c))) return rc;
+    q.assembled = true;
+    q.tangent = tangent;
+    return NLM_OK;
+  }
   hipLaunchKernelGGL(k_qs_tangent, dim3(slice_grid(c->fd)), dim3(256), 0,
                      c->stream, qs_args(c), tangent, q.blk.as<double4>(),
                      q.diag.as<double4>(), q.dinv.as<double2>());
@@ -910,15 +1152,16 @@ int bicgstab(nlm_ctx* c, int precond, double tol, int max_iter, int* iters,
     return NLM_OK;
   }
   const double2* rh = q.rh.as<double2>();
+  a.max_it = max_iter;
   int ctl[2] = {0, 0};
-  int it = 1;
-  const int batch = 8;
-  while (it <= max_iter) {
-    const int end = std::min(max_iter, it + batch - 1);
-    for (; it <= end && coarse; it++) {
+  const int batch = kBicgBatch;
+  // one batch: iterations ctl[2] .. ctl[2] + batch - 1 (no-ops past
+  // max_iter or after convergence), then the batch start moves on
+  auto launch_batch = [&]() {
+    for (int k = 0; k < batch && coarse; k++) {
       // two-level: smoother + R A_c^-1 R^T, for p and then for s
       hipLaunchKernelGGL((k_bicg_p<2, true>), dim3(g), dim3(256), 0, c->stream,
-                         a, it);
+                         a, k);
       hipLaunchKernelGGL(k_coarse_mv, dim3(gmv), dim3(256), 0, c->stream,
                          q.cinv, q.m, a.rc, q.zcp.as<double>(), a.ctl);
       if (precond == NLM_QS_PRECOND_JACOBI)
@@ -928,7 +1171,7 @@ int bicgstab(nlm_ctx* c, int precond, double tol, int max_iter, int* iters,
         mapply(c, a.p, a.ph, a.ctl, q.zcp.as<double>());
       spmv<1>(c, q.blk, q.diag, a.ph, a.v, rh, a.P2, nullptr, a.ctl);
       hipLaunchKernelGGL((k_bicg_s<2, true>), dim3(g), dim3(256), 0, c->stream,
-                         a, it);
+                         a, k);
       hipLaunchKernelGGL(k_coarse_mv, dim3(gmv), dim3(256), 0, c->stream,
                          q.cinv, q.m, a.rc, q.zcs.as<double>(), a.ctl);
       if (precond == NLM_QS_PRECOND_JACOBI)
@@ -937,35 +1180,85 @@ int bicgstab(nlm_ctx* c, int precond, double tol, int max_iter, int* iters,
       else
         mapply(c, a.s, a.sh, a.ctl, q.zcs.as<double>());
       spmv<2>(c, q.blk, q.diag, a.sh, a.t, a.s, a.P2, a.P3, a.ctl);
-      hipLaunchKernelGGL(k_bicg_x, dim3(g), dim3(256), 0, c->stream, a, it);
+      hipLaunchKernelGGL(k_bicg_x, dim3(g), dim3(256), 0, c->stream, a, k);
     }
-    for (; it <= end; it++) {
+    for (int k = 0; k < batch && !coarse; k++) {
       if (precond == NLM_QS_PRECOND_JACOBI)
         hipLaunchKernelGGL(k_bicg_p<1>, dim3(g), dim3(256), 0, c->stream, a,
-                           it);
+                           k);
       else if (precond == NLM_QS_PRECOND_NONE)
         hipLaunchKernelGGL(k_bicg_p<0>, dim3(g), dim3(256), 0, c->stream, a,
-                           it);
+                           k);
       else {
         hipLaunchKernelGGL(k_bicg_p<2>, dim3(g), dim3(256), 0, c->stream, a,
-                           it);
+                           k);
         mapply(c, a.p, a.ph, a.ctl, nullptr);
       }
       spmv<1>(c, q.blk, q.diag, a.ph, a.v, rh, a.P2, nullptr, a.ctl);
       if (precond == NLM_QS_PRECOND_JACOBI)
         hipLaunchKernelGGL(k_bicg_s<1>, dim3(g), dim3(256), 0, c->stream, a,
-                           it);
+                           k);
       else if (precond == NLM_QS_PRECOND_NONE)
         hipLaunchKernelGGL(k_bicg_s<0>, dim3(g), dim3(256), 0, c->stream, a,
-                           it);
+                           k);
       else {
         hipLaunchKernelGGL(k_bicg_s<2>, dim3(g), dim3(256), 0, c->stream, a,
-                           it);
+                           k);
         mapply(c, a.s, a.sh, a.ctl, nullptr);
       }
       spmv<2>(c, q.blk, q.diag, a.sh, a.t, a.s, a.P2, a.P3, a.ctl);
-      hipLaunchKernelGGL(k_bicg_x, dim3(g), dim3(256), 0, c->stream, a, it);
+      hipLaunchKernelGGL(k_bicg_x, dim3(g), dim3(256), 0, c->stream, a, k);
     }
+    hipLaunchKernelGGL(k_iter_advance, dim3(1), dim3(64), 0, c->stream, a.ctl,
+                       batch);
+  };
+  hipGraphExec_t exec = nullptr;
+  if (q.use_graph) {
+    // the captured batch depends on every kernel argument: recapture when
+    // any of them changed since the last capture
+    BicgKey key;
+    std::memset(&key, 0, sizeof(key));
+    std::memcpy(&key.a, &a, sizeof(a));
+    const QsArgs qa = qs_args(c);
+    std::memcpy(&key.qa, &qa, sizeof(qa));
+    key.precond = precond;
+    key.coarse = coarse;
+    key.m = q.m;
+    key.cinv = q.cinv;
+    key.blk = q.blk.p;
+    key.diag = q.diag.p;
+    key.mblk = q.mblk.p;
+    key.mdiag = q.mdiag.p;
+    key.zcp = q.zcp.p;
+    key.zcs = q.zcs.p;
+    key.g = g;
+    key.gmv = gmv;
+    const unsigned char* kb = reinterpret_cast<const unsigned char*>(&key);
+    if (!q.bicg_graph || q.bicg_key.size() != sizeof(key) ||
+        std::memcmp(kb, q.bicg_key.data(), sizeof(key)) != 0) {
+      if (q.bicg_graph) (void)hipGraphExecDestroy(q.bicg_graph);
+      q.bicg_graph = nullptr;
+      hipGraph_t gr = nullptr;
+      NLM_HIP(c, hipStreamBeginCapture(c->stream,
+                                       hipStreamCaptureModeThreadLocal));
+      launch_batch();
+      NLM_HIP(c, hipStreamEndCapture(c->stream, &gr));
+      hipError_t e = hipGraphInstantiate(&q.bicg_graph, gr, nullptr, nullptr,
+                                         0);
+      (void)hipGraphDestroy(gr);
+      if (e != hipSuccess) {
+        q.bicg_graph = nullptr;
+        return fail(c, NLM_ERR_HIP, "BiCGSTAB: graph instantiation failed");
+      }
+      q.bicg_key.assign(kb, kb + sizeof(key));
+    }
+    exec = q.bicg_graph;
+  }
+  for (int it = 1; it <= max_iter; it += batch) {
+    if (exec)
+      NLM_HIP(c, hipGraphLaunch(exec, c->stream));
+    else
+      launch_batch();
     NLM_HIP(c, hipGetLastError());
     NLM_HIP(c, hipMemcpyAsync(ctl, q.ctl.p, sizeof(ctl), hipMemcpyDeviceToHost,
                               c->stream));
@@ -1018,11 +1311,16 @@ int gnn_predict_precond(nlm_ctx* c);
 
 using namespace nlm;
 
+extern "C" int nlm_qs_set_graph(nlm_ctx* c, int on) {
+  if (!c) return NLM_ERR_INVALID;
+  c->qs.use_graph = on != 0;
+  return NLM_OK;
+}
+
 extern "C" int nlm_qs_assemble(nlm_ctx* c, int tangent) {
   int rc = qs_check(c, "nlm_qs_assemble");
   if (rc) return rc;
-  if (tangent != NLM_QS_TANGENT_FULL && tangent != NLM_QS_TANGENT_COMPONENT)
-    return fail(c, NLM_ERR_INVALID, "nlm_qs_assemble: unknown tangent");
+  if ((rc = tangent_check(c, tangent, "nlm_qs_assemble"))) return rc;
   if ((rc = assemble(c, tangent))) return rc;
   NLM_HIP(c, hipStreamSynchronize(c->stream));
   return NLM_OK;
@@ -1238,9 +1536,8 @@ extern "C" int nlm_qs_solve(nlm_ctx* c, const nlm_qs_options* o,
   if (!o || o->nsteps < 0 || !(o->dt > 0.) || o->max_newton < 0 ||
       o->lin_max_iter < 1 || !(o->lin_tol > 0.))
     return fail(c, NLM_ERR_INVALID, "nlm_qs_solve: bad options");
-  if (o->tangent != NLM_QS_TANGENT_FULL &&
-      o->tangent != NLM_QS_TANGENT_COMPONENT)
-    return fail(c, NLM_ERR_INVALID, "nlm_qs_solve: unknown tangent");
+  if ((rc = tangent_check(c, o->tangent, "nlm_qs_solve"))) return rc;
+  if (o->perturbation > 0.) c->qs.fd_perturbation = o->perturbation;
   const int pbase = o->precond & ~NLM_QS_PRECOND_COARSE;
   const bool pcoarse = o->precond != pbase;
   if (pbase < NLM_QS_PRECOND_NONE || pbase > NLM_QS_PRECOND_BLOCKS ||
@@ -1289,7 +1586,8 @@ extern "C" int nlm_qs_solve(nlm_ctx* c, const nlm_qs_options* o,
   };
   bool have_precond = false;
   for (int step = 0; step < o->nsteps; step++) {
-    const double t = s.time + o->dt;
+    // QuasiStaticModel::solver: d_time = d_n * dt (QuasiStaticModel.cpp:539)
+    const double t = double(s.step + 1) * o->dt;
     double rnorm = 0., r0 = 0.;
     int newton = 0, lit = 0;
     if (o->predictor) {

@@ -300,7 +300,8 @@ class GpuFD:
     # -- quasi-static model (2D bond-based) ---------------------------------
     def qs_solve(self, nsteps, dt=1.0, tol_abs=0.0, tol_rel=1e-8,
                  max_newton=50, tangent=0, precond=1, lin_tol=1e-8,
-                 lin_max_iter=5000, predictor=True, gnn_refresh=1):
+                 lin_max_iter=5000, predictor=True, gnn_refresh=1,
+                 perturbation=0.0):
         """nsteps load steps of the quasi-static Newton solver; returns the
         run statistics as a dict."""
         o = capi.QsOptions(nsteps=int(nsteps), dt=float(dt),
@@ -309,10 +310,16 @@ class GpuFD:
                            precond=int(precond), lin_tol=float(lin_tol),
                            lin_max_iter=int(lin_max_iter),
                            predictor=int(bool(predictor)),
-                           gnn_refresh=int(gnn_refresh))
+                           gnn_refresh=int(gnn_refresh),
+                           perturbation=float(perturbation))
         st = capi.QsStats()
         self._call("nlm_qs_solve", C.byref(o), C.byref(st))
         return {f: getattr(st, f) for f, _ in capi.QsStats._fields_}
+
+    def qs_set_graph(self, on: bool):
+        """BiCGSTAB batches from a captured HIP graph (on) or launched
+        kernel by kernel (off)."""
+        self._call("nlm_qs_set_graph", int(bool(on)))
 
     def qs_assemble(self, tangent=0):
         self._call("nlm_qs_assemble", int(tangent))

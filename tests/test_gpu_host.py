@@ -1,0 +1,249 @@
+"""The C++ host layer on the GPU: nlm_fd runs input decks end to end (mesh
+file, deck, FDModel / QuasiStaticModel, .vtu output) as the reference's
+NLMech executable does.
+
+Bars:
+  - a bond-based plate deck (pre-crack, pulled layers, central difference)
+    matches the oracle given the same mesh, loading and crack: relative L2
+    displacement and force errors <= 1e-9 (FP64 exact mode), broken bonds in
+    the output's Bond_State identical to the oracle's, bit for bit;
+  - the same deck with velocity_verlet matches the oracle's Verlet run;
+  - restarting from an output file written with Bond_State continues
+    exactly: the restarted run's final output equals the uninterrupted
+    run's, bit for bit (displacement, velocity, force, bond bits);
+  - stepping through a captured HIP graph (GPU: Graph_Chunk) writes the same
+    output bit for bit;
+  - the reference's quasi-static example deck (examples/qsModel/2D, state-
+    based material, its mesh made by nlm_mesh) reproduces the reference-held
+    result output_res.vtu within the reference test's own dc tolerance
+    (displacement L2 < 1e-7, test/CMakeLists.txt:39-52); the internal force
+    too.
+"""
+import shutil
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, progress, run_tool
+from nlmech_amd import problems as pr
+from vtu_read import read_vtu
+
+pytestmark = pytest.mark.gpu
+
+PLATE_MESH = """Output:
+  Path: ./
+  Mesh: mesh
+Domain: [0.0, 0.0, 0.031, 0.031]
+Horizon: 0.003
+Horizon_h_Ratio: 3
+Mesh_Type: uniform_square
+Is_FD: true
+Compress_Type: zlib
+"""
+
+PLATE_DECK = """Model:
+  Dimension: 2
+  Discretization_Type:
+    Spatial: finite_difference
+    Time: {time}
+  Final_Time: 4.0e-6
+  Time_Steps: 40
+  Horizon: 0.003
+  Horizon_h_Ratio: 3
+Policy:
+  Enable_PostProcessing: true
+Mesh:
+  File: {mesh}
+Material:
+  Type: RNPBond
+  Density: 1200.0
+  Compute_From_Classical: true
+  E: 3.24e9
+  Gc: 500.0
+  Bond_Potential:
+    Type: 1
+    Check_Sc_Factor: 1.0
+    Irreversible_Bond_Fracture: true
+  Influence_Function:
+    Type: 0
+    Parameters: [1.0]
+Fracture:
+  Cracks:
+    Sets: 1
+    Set_1:
+      Orientation: 1
+      Line: [-0.0005, 0.0155, 0.0195, 0.0155]
+Displacement_BC:
+  Sets: 2
+  Set_1:
+    Location:
+      Rectangle: [-0.001, 0.0285, 0.032, 0.0315]
+    Direction: [2]
+    Time_Function:
+      Type: linear
+      Parameters:
+        - 20.0
+    Spatial_Function:
+      Type: constant
+      Parameters: [1.0]
+  Set_2:
+    Location:
+      Rectangle: [-0.001, -0.0005, 0.032, 0.0025]
+    Direction: [2]
+    Time_Function:
+      Type: linear
+      Parameters:
+        - -20.0
+    Spatial_Function:
+      Type: constant
+      Parameters: [1.0]
+Output:
+  Path: ./
+  Tags:
+    - Displacement
+    - Velocity
+    - Force
+    - Damage_Z
+    - Fixity
+    - Node_Volume
+    - Neighbors
+  Output_Interval: 20
+  Compress_Type: zlib
+{extra}"""
+
+
+def plate_dir(host_bin, root: Path, name: str, time="central_difference",
+              extra="") -> Path:
+    d = root / name
+    d.mkdir()
+    mesh = root / "mesh.vtu"
+    if not mesh.exists():
+        (root / "m.yaml").write_text(PLATE_MESH)
+        run_tool(host_bin / "nlm_mesh", "-i", "m.yaml", "-d", "2", cwd=root)
+    (d / "input.yaml").write_text(PLATE_DECK.format(time=time, mesh=mesh,
+                                                    extra=extra))
+    return d
+
+
+def run_deck(host_bin, d: Path):
+    progress(f"[gpu] nlm_fd {d.name}")
+    rc, out = run_tool(host_bin / "nlm_fd", "-i", "input.yaml", cwd=d)
+    return out
+
+
+def plate_problem(root: Path) -> pr.Problem:
+    """The deck's model as the oracle takes it (same mesh file, same BCs,
+    crack and material)."""
+    m = read_vtu(str(root / "mesh.vtu"))
+    x = m["Points"].astype(np.float64)
+    vol = m["Node_Volume"].astype(np.float64)
+    ubc = [pr.BC("rectangle", [2], "linear", [20.0], "constant", [1.0],
+                 x1=-0.001, x2=0.032, y1=0.0285, y2=0.0315),
+           pr.BC("rectangle", [2], "linear", [-20.0], "constant", [1.0],
+                 x1=-0.001, x2=0.032, y1=-0.0005, y2=0.0025)]
+    cracks = [pr.CrackSpec(1, (-0.0005, 0.0155), (0.0195, 0.0155))]
+    mat = pr.rnp_material(dim=2, E=3.24e9, Gc=500.0, density=1200.0,
+                          check_sc=1.0, inf_type=0, inf_params=(1.0,))
+    return pr.Problem(name="deck_plate", dim=2, x=x, vol=vol, horizon=0.003,
+                      h=0.003 / 3.0, dt=4.0e-6 / 40, nsteps=40, material=mat,
+                      ubc=ubc, cracks=cracks)
+
+
+def pack_bits(row_ptr, bits):
+    """Per-bond bits (CSR order) in the reference packing: ceil(n_i / 8)
+    bytes per node (Fracture::d_fracture)."""
+    out = []
+    for i in range(len(row_ptr) - 1):
+        b = bits[row_ptr[i]:row_ptr[i + 1]]
+        nb = (len(b) + 7) // 8
+        by = np.zeros(nb, np.uint8)
+        for j, v in enumerate(b):
+            if v:
+                by[j // 8] |= np.uint8(1 << (j % 8))
+        out.append(by)
+    return np.concatenate(out) if out else np.zeros(0, np.uint8)
+
+
+@pytest.mark.parametrize("scheme", ["central_difference", "velocity_verlet"])
+def test_plate_deck_matches_oracle(host_bin, oracle, tmp_path, scheme):
+    d = plate_dir(host_bin, tmp_path, "run", time=scheme)
+    run_deck(host_bin, d)
+    out = read_vtu(str(d / "output_2.vtu"))
+    p = plate_problem(tmp_path)
+    o = oracle.OracleFD(p)
+    o.init()
+    o.step(40, scheme="verlet" if scheme == "velocity_verlet" else "cd")
+    u_ref, v_ref, f_ref = o.get_state()
+    bits = o.get_bonds()
+    bytes_ref = o.get_bond_bytes()
+    rp = o.row_ptr
+    # damage of the GPU state (postproc parity itself is covered elsewhere)
+    o.set_state(u=out["Displacement"], v=out["Velocity"],
+                f=out["Force"] / p.vol[:, None])
+    Zr, _, _ = o.postproc()
+    o.close()
+    u = out["Displacement"]
+    assert np.linalg.norm(u - u_ref) <= 1e-9 * np.linalg.norm(u_ref)
+    F_ref = f_ref * p.vol[:, None]
+    assert np.linalg.norm(out["Force"] - F_ref) <= 1e-9 * np.linalg.norm(F_ref)
+    assert bits.sum() > 0
+    assert np.array_equal(out["Bond_State"], bytes_ref)
+    assert np.array_equal(bytes_ref, pack_bits(rp, bits))
+    Z = out["Damage_Z"]
+    assert np.max(np.abs(Z - Zr) / np.maximum(Zr, 1e-30)) <= 1e-6
+    assert np.array_equal(out["Node_Volume"], p.vol)
+    assert np.array_equal(out["Neighbors"], np.diff(rp).astype(np.float64))
+    assert out["TIME"][0] == pytest.approx(40 * p.dt, rel=1e-12)
+    assert (d / "output_0.vtu").exists() and (d / "output_1.vtu").exists()
+
+
+def test_restart_with_bond_state_is_exact(host_bin, tmp_path):
+    a = plate_dir(host_bin, tmp_path, "a")
+    run_deck(host_bin, a)
+    b = plate_dir(host_bin, tmp_path, "b",
+                  extra=f"Restart:\n  File: {a / 'output_1.vtu'}\n  Step: 20\n")
+    run_deck(host_bin, b)
+    assert not (b / "output_1.vtu").exists()
+    ra = read_vtu(str(a / "output_2.vtu"))
+    rb = read_vtu(str(b / "output_2.vtu"))
+    assert ra["Bond_State"].sum() > 0
+    for k in ("Displacement", "Velocity", "Force", "Bond_State", "Points",
+              "TIME"):
+        assert np.array_equal(ra[k], rb[k]), k
+
+
+def test_graph_stepping_deck_is_bitwise_equal(host_bin, tmp_path):
+    a = plate_dir(host_bin, tmp_path, "a")
+    run_deck(host_bin, a)
+    g = plate_dir(host_bin, tmp_path, "g", extra="GPU:\n  Graph_Chunk: 10\n")
+    run_deck(host_bin, g)
+    ra = read_vtu(str(a / "output_2.vtu"))
+    rg = read_vtu(str(g / "output_2.vtu"))
+    for k in ("Displacement", "Velocity", "Force", "Bond_State"):
+        assert np.array_equal(ra[k], rg[k]), k
+
+
+def test_reference_quasi_static_deck(host_bin, tmp_path):
+    src = GOLDEN / "qs2d"
+    shutil.copy(src / "input_mesh.yaml", tmp_path / "input_mesh.yaml")
+    shutil.copy(src / "input.yaml", tmp_path / "input.yaml")
+    run_tool(host_bin / "nlm_mesh", "-i", "input_mesh.yaml", "-d", "2",
+             cwd=tmp_path)
+    out = run_deck(host_bin, tmp_path)
+    progress("[gpu]   " + " | ".join(l for l in out.splitlines()
+                                     if l.startswith("Load step")))
+    ours = read_vtu(str(tmp_path / "output_1.vtu"))
+    ref = read_vtu(str(src / "output_res.vtu"))
+    du = np.linalg.norm(ours["Displacement"] - ref["Displacement"])
+    dF = np.linalg.norm(ours["Force"] - ref["Force"])
+    progress(f"[gpu]   displacement L2 {du:.3e}, force L2 {dF:.3e}")
+    assert du < 1e-7
+    assert dF < 1e-4 * np.linalg.norm(ref["Force"])
+    (tmp_path / "compare.yaml").write_text(
+        "Dimension: 2\nFilename_1: output_1.vtu\n"
+        f"Filename_2: {src / 'output_res.vtu'}\n"
+        "Out_Filename: compare.txt\nPrint_Screen: false\nTolerance: 1e-7\n"
+        "Compare_Tags:\n    - Displacement\n")
+    run_tool(host_bin / "nlm_dc", "-i", "compare.yaml", "-k", "fd_simple",
+             cwd=tmp_path)

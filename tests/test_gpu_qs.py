@@ -188,3 +188,37 @@ def test_qs_plate300_jacobi_runs():
     assert st["residual"] <= 1e-8 * st["residual0"]
     progress(f"[gpu]   {st}")
     g.close()
+
+
+@pytest.mark.parametrize("precond", [NLM_QS_PRECOND_JACOBI,
+                                     NLM_QS_PRECOND_JACOBI | 16,
+                                     NLM_QS_PRECOND_NONE])
+def test_qs_bicgstab_graph_equals_eager(precond):
+    """Batches replayed from the captured HIP graph give the eager launches'
+    result bit for bit, on every replay, with eager work in between; an
+    iteration cap inside a batch stops both at the same iteration."""
+    progress(f"[gpu] qs bicgstab graph vs eager, precond {precond}")
+    p = pr.qs_plate(40)
+    q = qo.OracleQS(p)
+    u = strained_state(p, 3)
+    g = gpu(p, (q.fd.row_ptr, q.fd.col))
+    z = np.zeros_like(u)
+    g.set_state(u=u, v=z, f=z)
+    g.qs_assemble(NLM_QS_TANGENT_FULL)
+    rng = np.random.default_rng(5)
+    b = np.zeros((p.n, 3))
+    b[:, :2] = rng.standard_normal((p.n, 2))
+    for max_iter, tol in ((4000, 1e-8), (13, 1e-30)):
+        g.qs_set_graph(False)
+        x0, it0, rr0 = g.qs_linear_solve(b, precond=precond, tol=tol,
+                                         max_iter=max_iter)
+        g.qs_set_graph(True)
+        for rep in range(3):
+            x, it, rr = g.qs_linear_solve(b, precond=precond, tol=tol,
+                                          max_iter=max_iter)
+            assert it == it0 and np.array_equal(x, x0) and rr == rr0, rep
+            g.qs_apply(x)          # eager work between replays
+        if max_iter == 13:
+            assert it0 == 13
+    g.close()
+    q.close()
