@@ -361,12 +361,13 @@ __device__ __forceinline__ void bond_fp64_bf(const RnpDev& m,
   F[2] = DIM == 3 ? sr * dx2 : 0.0;
 }
 
+// cvj = vol_j c4 (J) in FP32 (c4f_infl): the bond's constant factor.
 template <int DIM, int INF>
 __device__ __forceinline__ void bond_fp32_bf(const RnpDev& m,
                                              const double* xi,
                                              const double* ui,
                                              const double* xj,
-                                             const double* uj, double volj,
+                                             const double* uj, float cvj,
                                              bool act, bool brk, bool fs0,
                                              bool& fs, bool& exact_needed,
                                              double* F) {
@@ -382,19 +383,19 @@ __device__ __forceinline__ void bond_fp32_bf(const RnpDev& m,
   }
   float rinv = __builtin_amdgcn_rsqf(dd);
   float r = dd * rinv;
-  float S = num * (rinv * rinv);
-  float rS2 = r * (S * S);        // beta r S^2 / beta, shared by the
-  float rho2 = rS2 * m.rcp_krbar_sq_f;  // exponent and the break ratio
+  float q = num * rinv;           // S r
+  float S = q * rinv;
+  float rS2 = q * S;              // r S^2: exponent and break ratio
   bool vex = fabsf(r - m.low_band_f) <= 3e-5f * m.low_band_f;
   float fac = fminf((m.check_up_f - r) * m.rcp_h_f, 1.0f);
   const bool brk_irr = brk && m.irreversible;
-  bool bex = brk_irr && !fs0 &&
-             fabsf(rho2 - float(kBreakRho2)) <= kGuard32 * float(kBreakRho2);
-  fs = fs0 || (brk_irr && rho2 > float(kBreakRho2));
+  const float dbr = rS2 - m.break_rs2_f;
+  bool bex = brk_irr && !fs0 && fabsf(dbr) <= m.break_guard_f;
+  fs = fs0 || (brk_irr && dbr > 0.f);
   exact_needed = vex || bex || (m.contact && brk && fs);
-  float infl = influence_f32<INF>(m, r);
   float ex = brk ? __builtin_amdgcn_exp2f(rS2 * m.nbeta_log2e_f) : 1.0f;
-  float sr = (infl * m.c4f) * S * ex * (float(volj) * fac) * rinv;
+  float sr = (S * ex) * ((cvj * fac) * rinv);
+  if constexpr (INF != 0) sr *= influence_f32<INF>(m, r);
   const bool keep = act & !(brk & fs);
   sr = keep ? sr : 0.f;
   F[0] = double(sr * dx0);

@@ -57,7 +57,7 @@ struct alignas(16) HaloRec {
   double2 u;
   double vol;
   uint32_t flags;
-  uint32_t pad;
+  float cv;  // FP32-force mode: vol * c4 (* J when constant), in FP32
 };
 
 template <int DIM, int MODE, int KIND, bool BASE_F, int INF, int U,
@@ -87,7 +87,7 @@ __global__ __launch_bounds__(256) void k_fd(
       r.u = ld32(a.u, g);
       r.vol = ld32(a.vol, g);
       r.flags = NOFAIL ? ld32(a.flags, g) : 0u;
-      r.pad = 0u;
+      r.cv = float(r.vol) * a.m.c4f_infl;
       smem_halo[t] = r;
     }
     __syncthreads();
@@ -127,11 +127,12 @@ __global__ __launch_bounds__(256) void k_fd(
   // overlaps the previous chunk's arithmetic.  Every load is unconditional:
   // slots are clamped to the slice width (padding slots hold the node
   // itself), and slots at or past the row length are masked out of the
-  // arithmetic.
+  // arithmetic.  A slot's 64 entries start at a wavefront-uniform address;
+  // the lane offset is the only per-lane part.
   auto slot_index = [&](int k) -> uint32_t {
-    const uint32_t e = uint32_t(64 * (k < wlast ? k : wlast) + lane);
-    if constexpr (TILED) return uint32_t(ld32(colL, e));
-    else return uint32_t(ld32(colS, e));
+    const int kc = k < wlast ? k : wlast;
+    if constexpr (TILED) return uint32_t(ld32(colL + 64 * kc, uint32_t(lane)));
+    else return uint32_t(ld32(colS + 64 * kc, uint32_t(lane)));
   };
   uint32_t jn[U];
 #pragma unroll
@@ -144,27 +145,33 @@ __global__ __launch_bounds__(256) void k_fd(
       jn[q] = slot_index(k0 + U + q);
     }
     double xj[U][3], uj[U][3], vj[U];
+    float cvj[U];
     unsigned long long w[U];
     uint32_t flj[U];
 #pragma unroll
     for (int q = 0; q < U; ++q) {
       const uint32_t j = jc[q];
       if constexpr (TILED) {
-        const HaloRec& r = *reinterpret_cast<const HaloRec*>(
-            reinterpret_cast<const char*>(smem_halo) +
-            j * uint32_t(sizeof(HaloRec)));
-        xj[q][0] = r.x.x;
-        xj[q][1] = r.x.y;
+        // LDS addresses are 32-bit and halo indices 16-bit: the record
+        // offset is one 24-bit multiply-add on an address-space-3 pointer
+        typedef const __attribute__((address_space(3))) char LdsByte;
+        typedef const __attribute__((address_space(3))) HaloRec LdsRec;
+        LdsRec* r = (LdsRec*)((LdsByte*)smem_halo +
+                              __umul24(j, uint32_t(sizeof(HaloRec))));
+        xj[q][0] = r->x.x;
+        xj[q][1] = r->x.y;
         xj[q][2] = 0.;
-        uj[q][0] = r.u.x;
-        uj[q][1] = r.u.y;
+        uj[q][0] = r->u.x;
+        uj[q][1] = r->u.y;
         uj[q][2] = 0.;
-        vj[q] = r.vol;
-        if (NOFAIL) flj[q] = r.flags;
+        vj[q] = r->vol;
+        cvj[q] = r->cv;
+        if (NOFAIL) flj[q] = r->flags;
       } else {
         load3<DIM>(ld32(a.x, j), xj[q]);
         load3<DIM>(ld32(a.u, j), uj[q]);
         vj[q] = ld32(a.vol, j);
+        cvj[q] = float(vj[q]) * a.m.c4f_infl;
         if (NOFAIL) flj[q] = ld32(a.flags, j);
       }
       int kq = k0 + q;
@@ -190,16 +197,14 @@ __global__ __launch_bounds__(256) void k_fd(
           bond_fp64_bf<DIM, INF>(a.m, xi, ui, xj[q], uj[q], vj[q], act, brk,
                                  fs0, fs, exn, F);
         else
-          bond_fp32_bf<DIM, INF>(a.m, xi, ui, xj[q], uj[q], vj[q], act, brk,
+          bond_fp32_bf<DIM, INF>(a.m, xi, ui, xj[q], uj[q], cvj[q], act, brk,
                                  fs0, fs, exn, F);
         // rare: a decision inside its guard band, or a contact force
         exn = exn && act;
-        if (__ballot(exn) != 0ull) {
-          if (exn) {
-            fs = fs0;
-            bond_mode<DIM, MODE, INF>(a.m, xi, ui, xj[q], uj[q], voli, vj[q],
-                                      brk, fs, F);
-          }
+        if (exn) {  // skipped as a whole when no lane needs it
+          fs = fs0;
+          bond_mode<DIM, MODE, INF>(a.m, xi, ui, xj[q], uj[q], voli, vj[q],
+                                    brk, fs, F);
         }
       }
       bool newly = act && fs && !fs0;
@@ -213,7 +218,7 @@ __global__ __launch_bounds__(256) void k_fd(
         if (DIM == 3) acc[2] += F[2];
       }
       if (k0 + q < width) {
-        unsigned long long nb = __ballot(newly);
+        const unsigned long long nb = __builtin_amdgcn_ballot_w64(newly);
         if (nb != 0ull && lane == 0) a.bits[bb + k0 + q] = w[q] | nb;
       }
     }
@@ -981,6 +986,9 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
   m.rcp_inf_b_f = float(m.rcp_inf_b);
   m.rcp_krbar_sq_f = float(m.rcp_krbar_sq);
   m.nbeta_log2e_f = float(-m.beta * 1.4426950408889634);
+  m.c4f_infl = m.inf_type == 0 ? float(m.c4 * m.inf_a) : float(m.c4);
+  m.break_rs2_f = float(kBreakRho2 * m.krbar_sq);
+  m.break_guard_f = float(double(kGuard32) * kBreakRho2 * m.krbar_sq);
   s.unroll = 4;
   if (es_mat) {
     // ElasticState constants (ElasticState.cpp:47-158, volumeCorrection.cpp)

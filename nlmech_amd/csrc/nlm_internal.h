@@ -80,6 +80,10 @@ struct RnpDev {
   float c4f, beta_f, check_up_f, low_band_f, rcp_h_f, rcp_horizon_f;
   float inf_a_f, inf_b_f, rcp_inf_b_f, rcp_krbar_sq_f;
   float nbeta_log2e_f;     // -beta * log2(e): exp(-beta t) = exp2(t * this)
+  float c4f_infl;          // c4 * J for the constant influence, else c4
+  float break_rs2_f;       // r S^2 at which |S| / (k rbar / sqrt(r)) reaches
+                           // 1 / (1 - 1e-5): krbar_sq / (1 - 1e-5)^2
+  float break_guard_f;     // half-width of the exact-decision band around it
 };
 
 // ElasticState constants as used by the kernels.
