@@ -412,7 +412,9 @@ int nlm_qs_get_precond(nlm_ctx* ctx, double* blocks, double* diag);
 int nlm_qs_coarse_apply(nlm_ctx* ctx, const double* r, double* z);
 /* BiCGSTAB launch mode (no counterpart in the reference): on (default),
  * each batch of 8 iterations is one replay of a captured HIP graph; off,
- * the same kernels are launched one by one.  Results are identical. */
+ * the same kernels are launched one by one.  Results are identical.  The
+ * environment variable NLM_QS_GRAPH=0 makes off the default of new
+ * contexts. */
 int nlm_qs_set_graph(nlm_ctx* ctx, int on);
 /* Solves A_ff x = b with the assembled tangent (b, x AoS double[3n]). */
 int nlm_qs_linear_solve(nlm_ctx* ctx, const double* b, double* x,

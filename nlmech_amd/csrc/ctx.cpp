@@ -2,6 +2,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -42,6 +43,11 @@ extern "C" int nlm_ctx_create(nlm_ctx** out, int device) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess)
     c->arch = prop.gcnArchName;
+  // NLM_QS_GRAPH=0 starts contexts with the BiCGSTAB batches launched kernel
+  // by kernel (nlm_qs_set_graph); profilers that do not follow captured
+  // graphs need it
+  if (const char* g = std::getenv("NLM_QS_GRAPH"))
+    c->qs.use_graph = std::strcmp(g, "0") != 0;
   *out = c;
   return NLM_OK;
 }
