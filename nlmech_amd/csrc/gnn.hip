@@ -287,6 +287,34 @@ __device__ __forceinline__ bool slice_lane(int64_t nslices, int64_t& s,
   return s < nslices;
 }
 
+// Reverse slice entry of every bond: rev[e(i, k)] = e(j, k') with j the
+// neighbor in slot k of node i and i the neighbor in slot k' of node j
+// (a scan of row j: rows keep the reference's order, not the internal one).
+// A missing reverse bond sets *bad.
+__global__ __launch_bounds__(256) void k_gnn_rev(
+    int64_t n, int64_t nslices, const int64_t* __restrict__ sl_ptr,
+    const int32_t* __restrict__ col, const uint16_t* __restrict__ rowlen,
+    int32_t* __restrict__ rev, int* __restrict__ bad) {
+  int64_t sl;
+  int lane;
+  if (!slice_lane(nslices, sl, lane)) return;
+  const int64_t i = sl * 64 + lane;
+  if (i >= n) return;
+  const int64_t cb = sl_ptr[sl];
+  const int len = rowlen[i];
+  for (int k = 0; k < len; k++) {
+    const int64_t e = cb + 64 * int64_t(k) + lane;
+    const int32_t j = col[e];
+    const int64_t cj = sl_ptr[j >> 6] + (j & 63);
+    const int lj = rowlen[j];
+    int kk = 0;
+    while (kk < lj && col[cj + 64 * int64_t(kk)] != int32_t(i)) kk++;
+    const bool ok = kk < lj;
+    rev[e] = ok ? int32_t(cj + 64 * int64_t(kk)) : 0;
+    if (!ok) bad[0] = 1;
+  }
+}
+
 // Node scale s_i = (A_ii[0][0] + A_ii[1][1]) / 2, 1/sqrt(s_i), degree.
 __global__ __launch_bounds__(256) void k_gnn_scale(GraphArgs g,
                                                    double* __restrict__ sc,
@@ -821,38 +849,21 @@ int gnn_size(nlm_ctx* c) {
       (rc = A(G.lpart, ((n + 255) / 256) * 8)))
     return rc;
   // reverse entries: the slice entry of bond (j, i) for entry (i, k)
-  std::vector<int64_t> slp(s.sell.nslices + 1);
-  std::vector<int32_t> col(ne);
-  std::vector<uint16_t> rl(n);
-  NLM_HIP(c, hipMemcpy(slp.data(), s.sell.sl_ptr.p, sizeof(int64_t) * slp.size(),
-                       hipMemcpyDeviceToHost));
-  NLM_HIP(c, hipMemcpy(col.data(), s.sell.col.p, sizeof(int32_t) * ne,
-                       hipMemcpyDeviceToHost));
-  NLM_HIP(c, hipMemcpy(rl.data(), s.sell.rowlen.p, sizeof(uint16_t) * n,
-                       hipMemcpyDeviceToHost));
-  auto entry = [&](int64_t a, int k) {
-    return slp[a / 64] + 64 * int64_t(k) + a % 64;
-  };
-  std::vector<std::vector<std::pair<int32_t, int32_t>>> rows(n);
-  for (int64_t a = 0; a < n; a++) {
-    rows[a].reserve(rl[a]);
-    for (int k = 0; k < rl[a]; k++) rows[a].push_back({col[entry(a, k)], k});
-    std::sort(rows[a].begin(), rows[a].end());
-  }
-  std::vector<int32_t> rev(ne, 0);
-  for (int64_t a = 0; a < n; a++)
-    for (int k = 0; k < rl[a]; k++) {
-      const int32_t j = col[entry(a, k)];
-      auto& rj = rows[j];
-      auto it = std::lower_bound(rj.begin(), rj.end(),
-                                 std::make_pair(int32_t(a), int32_t(-1)));
-      if (it == rj.end() || it->first != int32_t(a))
-        return fail(c, NLM_ERR_INVALID,
-                    "nlm_gnn: neighbor lists are not symmetric");
-      rev[entry(a, k)] = int32_t(entry(j, it->second));
-    }
-  NLM_HIP(c, hipMemcpy(G.rev.p, rev.data(), sizeof(int32_t) * ne,
-                       hipMemcpyHostToDevice));
+  DevBuf bad;
+  if ((rc = alloc(c, bad, 256))) return rc;
+  NLM_HIP(c, hipMemsetAsync(bad.p, 0, sizeof(int), c->stream));
+  hipLaunchKernelGGL(k_gnn_rev, dim3(unsigned((s.sell.nslices + 3) / 4)),
+                     dim3(256), 0, c->stream, n, s.sell.nslices,
+                     s.sell.sl_ptr.as<int64_t>(), s.sell.col.as<int32_t>(),
+                     s.sell.rowlen.as<uint16_t>(), G.rev.as<int32_t>(),
+                     bad.as<int>());
+  NLM_HIP(c, hipGetLastError());
+  int hbad = 0;
+  NLM_HIP(c, hipMemcpyAsync(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost,
+                            c->stream));
+  NLM_HIP(c, hipStreamSynchronize(c->stream));
+  if (hbad)
+    return fail(c, NLM_ERR_INVALID, "nlm_gnn: neighbor lists are not symmetric");
   G.n = n;
   G.ne = ne;
   return NLM_OK;

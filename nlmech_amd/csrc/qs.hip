@@ -271,17 +271,49 @@ __global__ __launch_bounds__(256) void k_qs_spmv(
 // slots q, q+4, ..., their sums are added in a fixed order by wavefront 0.
 // Partials (DOT) are per slice.  Four times the parallelism of k_qs_spmv, for
 // the latency-bound Krylov products of small systems.
+// Coarse correction of aggregate b: rows 2b and 2b+1 of the dense m x m
+// inverse times the restricted vector rc, summed by the whole 256-thread
+// workgroup in a fixed order (every workgroup of the aggregate obtains the
+// same value).
+__device__ double2 coarse_rows(const double* __restrict__ Ainv, int m,
+                               const double* __restrict__ rc, int64_t b) {
+  __shared__ double2 cr[256];
+  const double* r0 = Ainv + 2 * b * int64_t(m);
+  const double* r1 = r0 + m;
+  double s0 = 0., s1 = 0.;
+  for (int k = threadIdx.x; k < m; k += 256) {
+    const double v = rc[k];
+    s0 += r0[k] * v;
+    s1 += r1[k] * v;
+  }
+  cr[threadIdx.x] = make_double2(s0, s1);
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      cr[threadIdx.x].x += cr[threadIdx.x + w].x;
+      cr[threadIdx.x].y += cr[threadIdx.x + w].y;
+    }
+    __syncthreads();
+  }
+  const double2 z = cr[0];
+  __syncthreads();
+  return z;
+}
+
 template <int DOT, class BT>
 __global__ __launch_bounds__(256) void k_qs_spmv4(
     QsArgs a, const BT* __restrict__ blk, const BT* __restrict__ dg,
     const double2* __restrict__ xv, double2* __restrict__ y,
     const double2* __restrict__ wv, double* __restrict__ P0,
     double* __restrict__ P1, const int* __restrict__ ctl,
-    const double* __restrict__ zc) {
+    const double* __restrict__ Ainv, int m, const double* __restrict__ rc) {
   if (ctl && ctl[0]) return;
   const int64_t s = xcd_block(a.nslices);
   if (s >= a.nslices) return;
   __shared__ double2 red[3][64];
+  // two-level: the slice's aggregate (4 slices) adds its coarse correction
+  double2 zc = make_double2(0., 0.);
+  if (Ainv) zc = coarse_rows(Ainv, m, rc, s >> 2);
   const int lane = threadIdx.x & 63;
   const int wq = threadIdx.x >> 6;
   const int64_t i = s * 64 + lane;
@@ -307,10 +339,8 @@ __global__ __launch_bounds__(256) void k_qs_spmv4(
     const double2 xi = xv[i];
     acc.x += double(D.x) * xi.x + double(D.y) * xi.y;
     acc.y += double(D.z) * xi.x + double(D.w) * xi.y;
-    if (zc) {
-      acc.x += zc[2 * (i >> 8)];
-      acc.y += zc[2 * (i >> 8) + 1];
-    }
+    acc.x += zc.x;
+    acc.y += zc.y;
     const uint32_t fl = a.flags[i];
     if (dof_fixed(fl, 0)) acc.x = 0.;
     if (dof_fixed(fl, 1)) acc.y = 0.;
@@ -504,18 +534,16 @@ __global__ __launch_bounds__(256) void k_coarse_mv(const double* __restrict__ Ai
 // the prolongated coarse correction).
 __global__ __launch_bounds__(256) void k_smooth_jacobi(
     VecArgs a, const double2* __restrict__ src, double2* __restrict__ dst,
-    const double* __restrict__ zc) {
-  if (a.ctl[0]) return;
+    const double* __restrict__ Ainv, int m) {
+  if (a.ctl && a.ctl[0]) return;
+  // the workgroup's 256 nodes are one aggregate
+  double2 zc = make_double2(0., 0.);
+  if (Ainv) zc = coarse_rows(Ainv, m, a.rc, blockIdx.x);
   const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
   if (i >= a.n) return;
   const double2 di = a.dinv[i], v = src[i];
   const uint32_t fl = a.flags[i];
-  const int64_t b = i >> 8;
-  double2 o = make_double2(di.x * v.x, di.y * v.y);
-  if (zc) {
-    o.x += zc[2 * b];
-    o.y += zc[2 * b + 1];
-  }
+  double2 o = make_double2(di.x * v.x + zc.x, di.y * v.y + zc.y);
   if (dof_fixed(fl, 0)) o.x = 0.;
   if (dof_fixed(fl, 1)) o.y = 0.;
   dst[i] = o;
@@ -1034,18 +1062,19 @@ int assemble(nlm_ctx* c, int tangent) {
 template <int DOT, class BT = double4>
 void spmv(nlm_ctx* c, const DevBuf& blk, const DevBuf& dg, const double2* x,
           double2* y, const double2* w, double* P0, double* P1,
-          const int* ctl, const double* zc = nullptr) {
+          const int* ctl, const double* rc = nullptr) {
   const unsigned g = unsigned((c->fd.sell.nslices + 7) / 8 * 8);
   hipLaunchKernelGGL((k_qs_spmv4<DOT, BT>), dim3(g), dim3(256), 0, c->stream,
                      qs_args(c), blk.as<BT>(), dg.as<BT>(), x, y, w, P0, P1,
-                     ctl, zc);
+                     ctl, rc ? c->qs.cinv : nullptr, c->qs.m, rc);
 }
 // The preconditioner's blocks are FP32 (their rounding only perturbs the
 // preconditioner, never the operator).
+// rc non-null: add the coarse correction of the restricted vector rc.
 void mapply(nlm_ctx* c, const double2* x, double2* y, const int* ctl,
-            const double* zc) {
+            const double* rc) {
   spmv<0, float4>(c, c->qs.mblk, c->qs.mdiag, x, y, nullptr, nullptr, nullptr,
-                  ctl, zc);
+                  ctl, rc);
 }
 
 // Coarse operator of the assembled tangent: adjacency of the aggregates
@@ -1162,23 +1191,19 @@ int bicgstab(nlm_ctx* c, int precond, double tol, int max_iter, int* iters,
       // two-level: smoother + R A_c^-1 R^T, for p and then for s
       hipLaunchKernelGGL((k_bicg_p<2, true>), dim3(g), dim3(256), 0, c->stream,
                          a, k);
-      hipLaunchKernelGGL(k_coarse_mv, dim3(gmv), dim3(256), 0, c->stream,
-                         q.cinv, q.m, a.rc, q.zcp.as<double>(), a.ctl);
       if (precond == NLM_QS_PRECOND_JACOBI)
         hipLaunchKernelGGL(k_smooth_jacobi, dim3(g), dim3(256), 0, c->stream,
-                           a, a.p, a.ph, q.zcp.as<double>());
+                           a, a.p, a.ph, q.cinv, q.m);
       else
-        mapply(c, a.p, a.ph, a.ctl, q.zcp.as<double>());
+        mapply(c, a.p, a.ph, a.ctl, a.rc);
       spmv<1>(c, q.blk, q.diag, a.ph, a.v, rh, a.P2, nullptr, a.ctl);
       hipLaunchKernelGGL((k_bicg_s<2, true>), dim3(g), dim3(256), 0, c->stream,
                          a, k);
-      hipLaunchKernelGGL(k_coarse_mv, dim3(gmv), dim3(256), 0, c->stream,
-                         q.cinv, q.m, a.rc, q.zcs.as<double>(), a.ctl);
       if (precond == NLM_QS_PRECOND_JACOBI)
         hipLaunchKernelGGL(k_smooth_jacobi, dim3(g), dim3(256), 0, c->stream,
-                           a, a.s, a.sh, q.zcs.as<double>());
+                           a, a.s, a.sh, q.cinv, q.m);
       else
-        mapply(c, a.s, a.sh, a.ctl, q.zcs.as<double>());
+        mapply(c, a.s, a.sh, a.ctl, a.rc);
       spmv<2>(c, q.blk, q.diag, a.sh, a.t, a.s, a.P2, a.P3, a.ctl);
       hipLaunchKernelGGL(k_bicg_x, dim3(g), dim3(256), 0, c->stream, a, k);
     }
