@@ -85,6 +85,7 @@ void dump(nlm::inp::Input& d) {
        ", \"bond_params\": " + list(mat->d_bondPotentialParams) +
        ", \"influence_type\": " + num(double(mat->d_influenceFnType)) +
        ", \"influence_params\": " + list(mat->d_influenceFnParams) +
+       ", \"critical_stretch\": " + num(mat->d_criticalStretch) +
        ", \"no_penetration\": " + (mat->d_applyContact ? "true" : "false") +
        "}";
   s += ", \"cracks\": [";

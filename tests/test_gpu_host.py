@@ -13,6 +13,9 @@ Bars:
     run's, bit for bit (displacement, velocity, force, bond bits);
   - stepping through a captured HIP graph (GPU: Graph_Chunk) writes the same
     output bit for bit;
+  - a 3D state-based deck (ElasticState with Critical_Stretch, .csv mesh,
+    cuboid displacement sets, explicit dynamics) matches the oracle: relative
+    L2 displacement error <= 1e-9 and identical bond bits;
   - the reference's quasi-static example deck (examples/qsModel/2D, state-
     based material, its mesh made by nlm_mesh) reproduces the reference-held
     result output_res.vtu within the reference test's own dc tolerance
@@ -247,3 +250,96 @@ def test_reference_quasi_static_deck(host_bin, tmp_path):
         "Compare_Tags:\n    - Displacement\n")
     run_tool(host_bin / "nlm_dc", "-i", "compare.yaml", "-k", "fd_simple",
              cwd=tmp_path)
+
+
+ES_DECK = """Model:
+  Dimension: 3
+  Discretization_Type:
+    Spatial: finite_difference
+    Time: central_difference
+  Final_Time: 2.0e-4
+  Time_Steps: 20
+  Horizon: 0.1875
+  Horizon_h_Ratio: 3
+Mesh:
+  File: mesh.csv
+Material:
+  Type: ElasticState
+  Density: 1200.0
+  Compute_From_Classical: true
+  K: 2.16e9
+  G: 1.30e9
+  Critical_Stretch: 1.0e-3
+Displacement_BC:
+  Sets: 2
+  Set_1:
+    Location:
+      Cuboid: [-0.0625, -0.0625, {ztop1}, {xmax}, {xmax}, {ztop2}]
+    Direction: [3]
+    Time_Function:
+      Type: linear
+      Parameters: [1.0]
+    Spatial_Function:
+      Type: constant
+      Parameters: [1.0]
+  Set_2:
+    Location:
+      Cuboid: [-0.0625, -0.0625, -0.03125, {xmax}, {xmax}, {zbot2}]
+    Direction: [3]
+    Time_Function:
+      Type: linear
+      Parameters: [-1.0]
+    Spatial_Function:
+      Type: constant
+      Parameters: [1.0]
+Output:
+  Path: ./
+  Tags:
+    - Displacement
+    - Velocity
+    - Force
+  Output_Interval: 20
+"""
+
+
+def test_state_based_3d_deck_matches_oracle(host_bin, oracle, tmp_path):
+    """FDModel<ElasticState> (explicit, with the critical-stretch breaking)
+    from a deck with a 3D .csv mesh, against the oracle."""
+    nx, h = 10, 0.0625
+    x, vol = pr.grid3d(nx, nx, nx, h, jitter=0.0)
+    with open(tmp_path / "mesh.csv", "w") as f:
+        f.write("id,x,y,z,volume\n")
+        for i in range(x.shape[0]):
+            f.write(f"{i},{x[i, 0]:.17g},{x[i, 1]:.17g},{x[i, 2]:.17g},"
+                    f"{vol[i]:.17g}\n")
+    zmax = (nx - 1) * h
+    xmax = zmax + h
+    ztop1, ztop2 = zmax - 2 * h - 0.5 * h, zmax + 0.5 * h
+    zbot2 = 2 * h + 0.5 * h
+    (tmp_path / "input.yaml").write_text(ES_DECK.format(
+        ztop1=f"{ztop1:.17g}", ztop2=f"{ztop2:.17g}", zbot2=f"{zbot2:.17g}",
+        xmax=f"{xmax:.17g}"))
+    run_deck(host_bin, tmp_path)
+    out = read_vtu(str(tmp_path / "output_1.vtu"))
+    box = dict(x1=-0.0625, x2=xmax, y1=-0.0625, y2=xmax)
+    ubc = [pr.BC("cuboid", [3], "linear", [1.0], "constant", [1.0],
+                 z1=ztop1, z2=ztop2, **box),
+           pr.BC("cuboid", [3], "linear", [-1.0], "constant", [1.0],
+                 z1=-0.03125, z2=zbot2, **box)]
+    p = pr.Problem(name="deck_cube", dim=3, x=x, vol=vol, horizon=0.1875,
+                   h=0.1875 / 3.0, dt=2.0e-4 / 20, nsteps=20,
+                   material=pr.es_material(dim=3, K=2.16e9, G=1.30e9,
+                                           density=1200.0, sc=1.0e-3),
+                   ubc=ubc)
+    o = oracle.OracleES(p)
+    o.init()
+    o.step(20)
+    u_ref, _, _ = o.get_state()
+    bits = o.get_bonds()
+    rp = o.row_ptr
+    o.close()
+    u = out["Displacement"]
+    progress(f"[gpu]   broken bonds {int(bits.sum())}")
+    assert np.abs(u_ref[:, 2]).max() > 0
+    assert np.linalg.norm(u - u_ref) <= 1e-9 * np.linalg.norm(u_ref)
+    assert np.array_equal(out["Bond_State"], pack_bits(rp, bits))

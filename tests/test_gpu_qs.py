@@ -192,7 +192,8 @@ def test_qs_plate300_jacobi_runs():
 
 @pytest.mark.parametrize("precond", [NLM_QS_PRECOND_JACOBI,
                                      NLM_QS_PRECOND_JACOBI | 16,
-                                     NLM_QS_PRECOND_NONE])
+                                     NLM_QS_PRECOND_NONE,
+                                     NLM_QS_PRECOND_BLOCKS | 16])
 def test_qs_bicgstab_graph_equals_eager(precond):
     """Batches replayed from the captured HIP graph give the eager launches'
     result bit for bit, on every replay, with eager work in between; an
@@ -205,6 +206,16 @@ def test_qs_bicgstab_graph_equals_eager(precond):
     z = np.zeros_like(u)
     g.set_state(u=u, v=z, f=z)
     g.qs_assemble(NLM_QS_TANGENT_FULL)
+    if precond & 15 == NLM_QS_PRECOND_BLOCKS:
+        # a sparse approximate inverse with off-diagonal blocks: scaled
+        # tangent blocks around the inverse diagonal
+        blk, dg = g.qs_tangent()
+        inv = np.zeros_like(dg)
+        for i in range(p.n):
+            if abs(np.linalg.det(dg[i])) > 0:
+                inv[i] = np.linalg.inv(dg[i])
+        sc = 0.5 / np.abs(dg[:, 0, 0]).max()
+        g.qs_set_precond(-sc * 0.1 * blk, inv)
     rng = np.random.default_rng(5)
     b = np.zeros((p.n, 3))
     b[:, :2] = rng.standard_normal((p.n, 2))
