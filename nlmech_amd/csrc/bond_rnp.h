@@ -311,21 +311,27 @@ __device__ __forceinline__ void bond_fp32(const RnpDev& m, const double* xi,
   if (DIM == 3) F[2] = double(sr * dx2);
 }
 
+// Wavefront masks (bit = lane) of one neighbor slot for the fast variants.
+struct SlotMasks {
+  unsigned long long act;  // the slot lies within the lane's row
+  unsigned long long brk;  // both ends breakable (no-fail regions)
+  unsigned long long w;    // broken before this step
+};
+
 // Branch-free common path of the fast variants.  Computes the force of a
 // bond whose two boundary decisions (volume correction, break test) are
-// clear of their guard bands; `exact_needed` flags the rare bonds inside a
-// band, which the caller recomputes with bond_fp64 / bond_fp32 (those take
-// the exact decision).  Broken bonds contribute zero (contact forces are also
-// left to the caller's slow path).  brk: both ends breakable.
-template <int DIM, int INF>
-__device__ __forceinline__ void bond_fp64_bf(const RnpDev& m,
-                                             const double* xi,
-                                             const double* ui,
-                                             const double* xj,
-                                             const double* uj, double volj,
-                                             bool act, bool brk, bool fs0,
-                                             bool& fs, bool& exact_needed,
-                                             double* F) {
+// clear of their guard bands; the decisions are wavefront votes on single
+// comparisons (a vote on a comparison is its lane mask), combined by scalar
+// mask arithmetic.  Outputs: nb, the bonds that break now outside the break
+// guard band; band, the breakable bonds inside it; ex, the active bonds the
+// caller must recompute with bond_fp64 / bond_fp32 (a decision inside a
+// band, or a contact force), whose force here is 0.  Broken bonds and
+// inactive slots contribute 0.
+template <int DIM, int INF, bool NOFAIL>
+__device__ __forceinline__ void bond_fp64_fast(
+    const RnpDev& m, const double* xi, const double* ui, const double* xj,
+    const double* uj, double volj, const SlotMasks& k, unsigned long long& nb,
+    unsigned long long& band, unsigned long long& ex, double* F) {
   double dx0 = xj[0] - xi[0], dx1 = xj[1] - xi[1];
   double dx2 = DIM == 3 ? xj[2] - xi[2] : 0.0;
   double du0 = uj[0] - ui[0], du1 = uj[1] - ui[1];
@@ -341,36 +347,40 @@ __device__ __forceinline__ void bond_fp64_bf(const RnpDev& m,
   double S = num * (rinv * rinv);
   // (check_up - r)/h >= 1 exactly when r <= check_low, so the clamp gives
   // the reference's factor away from the relative-epsilon edge
-  bool vex = fabs(r - m.low_band) <= 2e-9 * m.low_band;
+  const unsigned long long v_vex = __builtin_amdgcn_ballot_w64(
+      fabs(r - m.low_band) <= 2e-9 * m.low_band);
   double fac = fmin((m.check_up - r) * m.rcp_h, 1.0);
-  double d = (S * S) * r * m.rcp_krbar_sq - kBreakRho2;
-  bool bex = brk && m.irreversible && !fs0 &&
-             fabs(d) <= kGuard64 * kBreakRho2;
-  fs = fs0 || (brk && m.irreversible && d > 0.);
-  // contact forces of broken bonds are left to the slow path
-  exact_needed = vex || bex || (m.contact && brk && fs);
+  // (break_rho2 is +inf when bonds never break)
+  double d = (S * S) * r * m.rcp_krbar_sq - m.break_rho2;
+  const unsigned long long v_cross = __builtin_amdgcn_ballot_w64(d > 0.);
+  const unsigned long long v_band =
+      __builtin_amdgcn_ballot_w64(fabs(d) <= kGuard64 * kBreakRho2);
+  const unsigned long long cand = k.act & k.brk & ~k.w;
+  band = cand & v_band;
+  nb = cand & v_cross & ~band;
+  const unsigned long long fs = k.w | nb;
+  ex = (v_vex | band | (fs & k.brk & m.contact_mask)) & k.act;
+  const unsigned long long keep = k.act & ~(fs & k.brk) & ~ex;
   double infl = influence_f64<INF>(m, r);
+  double ex_ = exp(-m.beta * r * S * S);
   // bonds touching a no-fail node use the linearised force whatever their
   // bit (rnpBond.cpp:296-300)
-  double ex = brk ? exp(-m.beta * r * S * S) : 1.0;
-  double sr = (infl * m.c4) * S * ex * (volj * fac) * rinv;
-  // masked slots (neighbor = the node itself: dx = 0) and broken bonds add 0
-  sr = (act && !(brk && fs)) ? sr : 0.0;
+  if (NOFAIL) ex_ = __builtin_amdgcn_inverse_ballot_w64(k.brk) ? ex_ : 1.0;
+  double sr = (infl * m.c4) * S * ex_ * (volj * fac) * rinv;
+  sr = __builtin_amdgcn_inverse_ballot_w64(keep) ? sr : 0.0;
   F[0] = sr * dx0;
   F[1] = sr * dx1;
   F[2] = DIM == 3 ? sr * dx2 : 0.0;
 }
 
-// cvj = vol_j c4 (J) in FP32 (c4f_infl): the bond's constant factor.
-template <int DIM, int INF>
-__device__ __forceinline__ void bond_fp32_bf(const RnpDev& m,
-                                             const double* xi,
-                                             const double* ui,
-                                             const double* xj,
-                                             const double* uj, float cvj,
-                                             bool act, bool brk, bool fs0,
-                                             bool& fs, bool& exact_needed,
-                                             double* F) {
+// cvj = vol_j c4 (J) in FP32 (c4f_infl): the bond's constant factor.  The
+// force comes out in FP32; the caller sums a chunk of slots in FP32 and adds
+// the chunk to its FP64 accumulator.
+template <int DIM, int INF, bool NOFAIL>
+__device__ __forceinline__ void bond_fp32_fast(
+    const RnpDev& m, const double* xi, const double* ui, const double* xj,
+    const double* uj, float cvj, const SlotMasks& k, unsigned long long& nb,
+    unsigned long long& band, unsigned long long& ex, float* F) {
   float dx0 = float(xj[0] - xi[0]), dx1 = float(xj[1] - xi[1]);
   float dx2 = DIM == 3 ? float(xj[2] - xi[2]) : 0.f;
   float du0 = float(uj[0] - ui[0]), du1 = float(uj[1] - ui[1]);
@@ -386,21 +396,27 @@ __device__ __forceinline__ void bond_fp32_bf(const RnpDev& m,
   float q = num * rinv;           // S r
   float S = q * rinv;
   float rS2 = q * S;              // r S^2: exponent and break ratio
-  bool vex = fabsf(r - m.low_band_f) <= 3e-5f * m.low_band_f;
-  float fac = fminf((m.check_up_f - r) * m.rcp_h_f, 1.0f);
-  const bool brk_irr = brk && m.irreversible;
-  const float dbr = rS2 - m.break_rs2_f;
-  bool bex = brk_irr && !fs0 && fabsf(dbr) <= m.break_guard_f;
-  fs = fs0 || (brk_irr && dbr > 0.f);
-  exact_needed = vex || bex || (m.contact && brk && fs);
-  float ex = brk ? __builtin_amdgcn_exp2f(rS2 * m.nbeta_log2e_f) : 1.0f;
-  float sr = (S * ex) * ((cvj * fac) * rinv);
+  const unsigned long long v_vex =
+      __builtin_amdgcn_ballot_w64(fabsf(r - m.low_band_f) <= m.vex_tol_f);
+  float fac = fminf(__builtin_fmaf(r, -m.rcp_h_f, m.up_rcp_h_f), 1.0f);
+  const float dbr = rS2 - m.break_rs2_f;  // break_rs2_f: +inf if no breaking
+  const unsigned long long v_cross = __builtin_amdgcn_ballot_w64(dbr > 0.f);
+  const unsigned long long v_band =
+      __builtin_amdgcn_ballot_w64(fabsf(dbr) <= m.break_guard_f);
+  const unsigned long long cand = k.act & k.brk & ~k.w;
+  band = cand & v_band;
+  nb = cand & v_cross & ~band;
+  const unsigned long long fs = k.w | nb;
+  ex = (v_vex | band | (fs & k.brk & m.contact_mask)) & k.act;
+  const unsigned long long keep = k.act & ~(fs & k.brk) & ~ex;
+  float ex_ = __builtin_amdgcn_exp2f(rS2 * m.nbeta_log2e_f);
+  if (NOFAIL) ex_ = __builtin_amdgcn_inverse_ballot_w64(k.brk) ? ex_ : 1.0f;
+  float sr = (S * ex_) * ((cvj * fac) * rinv);
   if constexpr (INF != 0) sr *= influence_f32<INF>(m, r);
-  const bool keep = act & !(brk & fs);
-  sr = keep ? sr : 0.f;
-  F[0] = double(sr * dx0);
-  F[1] = double(sr * dx1);
-  F[2] = DIM == 3 ? double(sr * dx2) : 0.0;
+  sr = __builtin_amdgcn_inverse_ballot_w64(keep) ? sr : 0.f;
+  F[0] = sr * dx0;
+  F[1] = sr * dx1;
+  F[2] = DIM == 3 ? sr * dx2 : 0.f;
 }
 
 }  // namespace nlm

@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -74,7 +75,7 @@ __global__ __launch_bounds__(256) void k_fd(
   if (lb >= nblk) return;
   // TILED: the workgroup's 256 nodes are one block of the layout; its halo
   // (every node its rows reference) is staged in LDS before the bond loop
-  // and neighbors are read through halo-local indices.
+  // and neighbors are read through halo-local entries (LDS byte offsets).
   static_assert(!TILED || DIM == 2, "the LDS-staged kernel is 2D");
   extern __shared__ HaloRec smem_halo[];
   if constexpr (TILED) {
@@ -118,22 +119,38 @@ __global__ __launch_bounds__(256) void k_fd(
   const int32_t* __restrict__ colS = a.col + cb;
   const uint16_t* __restrict__ colL = a.colL + cb;
   // bond-bit words of this slice, read through the scalar path (each word is
-  // read once, before this wavefront may rewrite it)
+  // read once, before this wavefront may rewrite it).  A chunk may read up
+  // to U - 1 words past the slice (the next slice's, or the array's
+  // padding); those slots are inactive.
   const unsigned long long* __restrict__ bits_in = bits_ro + bb;
-  const int wlast = width > 0 ? width - 1 : 0;
   double acc[3] = {0., 0., 0.};
   // Neighbor indices are fetched one chunk of U slots ahead of the chunk
   // whose positions and displacements are gathered, so the index latency
   // overlaps the previous chunk's arithmetic.  Every load is unconditional:
-  // slots are clamped to the slice width (padding slots hold the node
-  // itself), and slots at or past the row length are masked out of the
-  // arithmetic.  A slot's 64 entries start at a wavefront-uniform address;
-  // the lane offset is the only per-lane part.
+  // index fetches past the slice return 0 (a valid halo entry or node), and
+  // slots at or past the row length are masked out of the arithmetic.  A
+  // slot's 64 entries start at a wavefront-uniform address; the lane offset
+  // is the only per-lane part.
+  // The indices are fetched through a buffer resource over the slice: the
+  // slot offset is a scalar and the lane offset a per-lane constant, so a
+  // fetch costs no 64-bit vector address arithmetic.
+  const __amdgpu_buffer_rsrc_t col_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+      TILED ? (void*)colL : (void*)colS, 0,
+      width * 64 * int(TILED ? sizeof(uint16_t) : sizeof(int32_t)),
+      kBufferRsrcWord3);
   auto slot_index = [&](int k) -> uint32_t {
-    const int kc = k < wlast ? k : wlast;
-    if constexpr (TILED) return uint32_t(ld32(colL + 64 * kc, uint32_t(lane)));
-    else return uint32_t(ld32(colS + 64 * kc, uint32_t(lane)));
+    if constexpr (TILED)
+      return uint32_t(__builtin_amdgcn_raw_buffer_load_b16(
+          col_rsrc, lane * int(sizeof(uint16_t)),
+          k * 64 * int(sizeof(uint16_t)), 0));
+    else
+      return uint32_t(__builtin_amdgcn_raw_buffer_load_b32(
+          col_rsrc, lane * int(sizeof(int32_t)),
+          k * 64 * int(sizeof(int32_t)), 0));
   };
+  // wavefront masks that do not change along the row
+  const unsigned long long int_m =
+      NOFAIL ? __builtin_amdgcn_ballot_w64(int_i) : ~0ull;
   uint32_t jn[U];
 #pragma unroll
   for (int q = 0; q < U; ++q) jn[q] = slot_index(q);
@@ -152,12 +169,11 @@ __global__ __launch_bounds__(256) void k_fd(
     for (int q = 0; q < U; ++q) {
       const uint32_t j = jc[q];
       if constexpr (TILED) {
-        // LDS addresses are 32-bit and halo indices 16-bit: the record
-        // offset is one 24-bit multiply-add on an address-space-3 pointer
-        typedef const __attribute__((address_space(3))) char LdsByte;
+        // The slot entry is the record's LDS address: the kernel has no
+        // static LDS, so the dynamic area starts at address 0 (the host
+        // checks the kernel's static LDS size before it selects this path).
         typedef const __attribute__((address_space(3))) HaloRec LdsRec;
-        LdsRec* r = (LdsRec*)((LdsByte*)smem_halo +
-                              __umul24(j, uint32_t(sizeof(HaloRec))));
+        LdsRec* r = (LdsRec*)j;
         xj[q][0] = r->x.x;
         xj[q][1] = r->x.y;
         xj[q][2] = 0.;
@@ -174,54 +190,122 @@ __global__ __launch_bounds__(256) void k_fd(
         cvj[q] = float(vj[q]) * a.m.c4f_infl;
         if (NOFAIL) flj[q] = ld32(a.flags, j);
       }
-      int kq = k0 + q;
-      w[q] = bits_in[kq < wlast ? kq : wlast];
+      w[q] = bits_in[k0 + q];
     }
-    // The arithmetic runs on every lane; slots past the row length (their
-    // neighbor is the node itself, giving NaN) are removed by selects, so no
-    // branch separates the gathers from their use.
+    // newly broken bonds of each slot (wavefront masks)
+    unsigned long long nb[U];
+    if constexpr (MODE == kModeExact) {
 #pragma unroll
-    for (int q = 0; q < U; ++q) {
-      const bool act = k0 + q < len;
-      bool brk = NOFAIL ? (int_i && !(flj[q] & kFlagNonInterior)) : true;
-      // w[q] is wavefront-uniform: its bits are the lanes' broken flags
-      bool fs0 = __builtin_amdgcn_inverse_ballot_w64(w[q]);
-      bool fs = fs0;
-      double F[3];
-      if constexpr (MODE == kModeExact) {
+      for (int q = 0; q < U; ++q) {
+        const bool act = k0 + q < len;
+        bool brk = NOFAIL ? (int_i && !(flj[q] & kFlagNonInterior)) : true;
+        // w[q] is wavefront-uniform: its bits are the lanes' broken flags
+        bool fs0 = __builtin_amdgcn_inverse_ballot_w64(w[q]);
+        bool fs = fs0;
+        double F[3];
         bond_mode<DIM, MODE, INF>(a.m, xi, ui, xj[q], uj[q], voli, vj[q], brk,
                                   fs, F);
-      } else {
-        bool exn;
-        if constexpr (MODE == kModeFp64)
-          bond_fp64_bf<DIM, INF>(a.m, xi, ui, xj[q], uj[q], vj[q], act, brk,
-                                 fs0, fs, exn, F);
-        else
-          bond_fp32_bf<DIM, INF>(a.m, xi, ui, xj[q], uj[q], cvj[q], act, brk,
-                                 fs0, fs, exn, F);
-        // rare: a decision inside its guard band, or a contact force
-        exn = exn && act;
-        if (exn) {  // skipped as a whole when no lane needs it
-          fs = fs0;
-          bond_mode<DIM, MODE, INF>(a.m, xi, ui, xj[q], uj[q], voli, vj[q],
-                                    brk, fs, F);
-        }
-      }
-      bool newly = act && fs && !fs0;
-      if constexpr (MODE == kModeExact) {
         acc[0] += act ? F[0] : 0.0;
         acc[1] += act ? F[1] : 0.0;
         if (DIM == 3) acc[2] += act ? F[2] : 0.0;
-      } else {  // F is already 0 on masked slots
-        acc[0] += F[0];
-        acc[1] += F[1];
-        if (DIM == 3) acc[2] += F[2];
+        nb[q] = __builtin_amdgcn_ballot_w64(act && fs && !fs0);
       }
-      if (k0 + q < width) {
-        const unsigned long long nb = __builtin_amdgcn_ballot_w64(newly);
-        if (nb != 0ull && lane == 0) a.bits[bb + k0 + q] = w[q] | nb;
+    } else {
+      // The arithmetic runs on every lane with no branch; its decisions are
+      // wavefront masks (votes on single comparisons).  Bonds with a
+      // decision inside a guard band, or a contact force, are left out of
+      // the common path and evaluated after the chunk.  Forces are summed in
+      // pairs of slots (2m, 2m+1) -- in FP32 in the FP32-force mode -- and
+      // the pairs are added to the FP64 accumulator in order: a two-term sum
+      // does not depend on the order of its terms, so neither the chunk
+      // width nor the deferred evaluation changes the result.
+      typedef typename std::conditional<MODE == kModeFp32, float, double>::type
+          Real;
+      constexpr int P = (U + 1) / 2;
+      unsigned long long band[U], ex[U];
+      unsigned long long ex_any = 0ull;
+      Real pf[P][3];
+#pragma unroll
+      for (int q = 0; q < U; ++q) {
+        SlotMasks km;
+        km.act = __builtin_amdgcn_ballot_w64(k0 + q < len);
+        km.brk = NOFAIL ? (int_m & __builtin_amdgcn_ballot_w64(
+                                       (flj[q] & kFlagNonInterior) == 0u))
+                        : ~0ull;
+        km.w = w[q];
+        Real F[3];
+        if constexpr (MODE == kModeFp64)
+          bond_fp64_fast<DIM, INF, NOFAIL>(a.m, xi, ui, xj[q], uj[q], vj[q],
+                                           km, nb[q], band[q], ex[q], F);
+        else
+          bond_fp32_fast<DIM, INF, NOFAIL>(a.m, xi, ui, xj[q], uj[q], cvj[q],
+                                           km, nb[q], band[q], ex[q], F);
+        if (q % 2 == 0) {
+          pf[q / 2][0] = F[0];
+          pf[q / 2][1] = F[1];
+          pf[q / 2][2] = F[2];
+        } else {
+          pf[q / 2][0] += F[0];
+          pf[q / 2][1] += F[1];
+          if (DIM == 3) pf[q / 2][2] += F[2];
+        }
+        ex_any |= ex[q];
+      }
+      if (ex_any != 0ull) {  // rare: whole wavefronts skip it
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+          if (ex[q] == 0ull) continue;
+          bool fsn = false;
+          if (__builtin_amdgcn_inverse_ballot_w64(ex[q])) {
+            bool brk = NOFAIL ? (int_i && !(flj[q] & kFlagNonInterior)) : true;
+            bool fs0 = __builtin_amdgcn_inverse_ballot_w64(w[q]);
+            bool fs = fs0;
+            double Fd[3];
+            double xs[3], us[3], vs;
+            if constexpr (TILED) {
+              // the neighbor's record is re-read from LDS (volatile, so the
+              // common path need not keep its FP64 values alive)
+              typedef volatile const __attribute__((address_space(3))) HaloRec
+                  LdsRecV;
+              LdsRecV* r = (LdsRecV*)jc[q];
+              xs[0] = r->x.x;
+              xs[1] = r->x.y;
+              xs[2] = 0.;
+              us[0] = r->u.x;
+              us[1] = r->u.y;
+              us[2] = 0.;
+              vs = r->vol;
+            } else {
+              for (int d = 0; d < 3; d++) {
+                xs[d] = xj[q][d];
+                us[d] = uj[q][d];
+              }
+              vs = vj[q];
+            }
+            bond_mode<DIM, MODE, INF>(a.m, xi, ui, xs, us, voli, vs, brk, fs,
+                                      Fd);
+            fsn = fs && !fs0;
+            // (the FP32-force bond's force is FP32: exact conversions)
+            pf[q / 2][0] += Real(Fd[0]);
+            pf[q / 2][1] += Real(Fd[1]);
+            if (DIM == 3) pf[q / 2][2] += Real(Fd[2]);
+          }
+          // the guard band's bonds were decided exactly here
+          nb[q] |= band[q] & __builtin_amdgcn_ballot_w64(fsn);
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < P; ++m) {
+        acc[0] += double(pf[m][0]);
+        acc[1] += double(pf[m][1]);
+        if (DIM == 3) acc[2] += double(pf[m][2]);
       }
     }
+    // a slot's new bits (rare): every lane stores the same word, so the
+    // test is a scalar branch
+#pragma unroll
+    for (int q = 0; q < U; ++q)
+      if (nb[q] != 0ull && k0 + q < width) a.bits[bb + k0 + q] = w[q] | nb[q];
   }
   if (!valid) return;
   double f[3] = {0., 0., 0.};
@@ -874,15 +958,38 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
     return rc;
   if ((rc = up(s.sell.col, L.colS.data(), sizeof(int32_t) * nent))) return rc;
   if ((rc = up(s.sell.rowlen, L.rowlen.data(), sizeof(uint16_t) * n))) return rc;
-  if ((rc = up(s.sell.bits, nullptr, sizeof(uint64_t) * (nent / 64 + 1)))) return rc;
+  // bond-bit words: one per (slice, slot), plus 8 words of padding that the
+  // step kernel's chunked reads may touch past the last slice
+  if ((rc = up(s.sell.bits, nullptr, sizeof(uint64_t) * (nent / 64 + 8))))
+    return rc;
   if ((rc = up(s.halo_ptr, L.halo_ptr.data(), sizeof(int64_t) * (L.nblocks + 1))))
     return rc;
   if ((rc = up(s.halo, L.halo.data(), sizeof(int32_t) * L.halo.size()))) return rc;
-  if ((rc = up(s.colL, L.colL.data(), sizeof(uint16_t) * nent))) return rc;
   s.halo_max = L.halo_max;
   // LDS budget of the staged kernel: one 48-byte record per halo node
   s.tiled = !es_mat && dim == 2 && L.halo_max > 0 &&
-            (L.halo_max + 1) * 48 <= 64 * 1024;
+            (L.halo_max + 1) * int64_t(sizeof(HaloRec)) <= 64 * 1024;
+  if (s.tiled) {
+    // the staged kernel addresses its dynamic LDS from 0, which holds while
+    // it has no static LDS (one instance checked: all share the source)
+    hipFuncAttributes fa;
+    if (hipFuncGetAttributes(
+            &fa, reinterpret_cast<const void*>(
+                     &k_fd<2, kModeFp32, KIND_CD, false, 0, 2, false, true>)) !=
+            hipSuccess ||
+        fa.sharedSizeBytes != 0)
+      s.tiled = false;
+  }
+  if (s.tiled) {
+    // the bond-based staged kernel reads halo-local entries as LDS byte
+    // offsets of the records (below 64 KiB by the budget above)
+    std::vector<uint16_t> off(L.colL.size());
+    for (size_t e = 0; e < off.size(); e++)
+      off[e] = uint16_t(uint32_t(L.colL[e]) * uint32_t(sizeof(HaloRec)));
+    if ((rc = up(s.colL, off.data(), sizeof(uint16_t) * nent))) return rc;
+  } else if ((rc = up(s.colL, L.colL.data(), sizeof(uint16_t) * nent))) {
+    return rc;
+  }
   // node arrays in internal order
   const int stride = dim == 2 ? 2 : 4;
   std::vector<double> hx(size_t(n) * stride, 0.);
@@ -987,8 +1094,14 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
   m.rcp_krbar_sq_f = float(m.rcp_krbar_sq);
   m.nbeta_log2e_f = float(-m.beta * 1.4426950408889634);
   m.c4f_infl = m.inf_type == 0 ? float(m.c4 * m.inf_a) : float(m.c4);
-  m.break_rs2_f = float(kBreakRho2 * m.krbar_sq);
+  m.break_rs2_f = m.irreversible ? float(kBreakRho2 * m.krbar_sq)
+                                 : std::numeric_limits<float>::infinity();
+  m.break_rho2 = m.irreversible ? kBreakRho2
+                                : std::numeric_limits<double>::infinity();
+  m.contact_mask = m.contact ? ~0ull : 0ull;
   m.break_guard_f = float(double(kGuard32) * kBreakRho2 * m.krbar_sq);
+  m.up_rcp_h_f = float(m.check_up * m.rcp_h);
+  m.vex_tol_f = float(3e-5 * m.low_band);
   s.unroll = 4;
   if (es_mat) {
     // ElasticState constants (ElasticState.cpp:47-158, volumeCorrection.cpp)

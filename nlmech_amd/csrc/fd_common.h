@@ -99,6 +99,10 @@ constexpr int KIND_FORCE = 0;
 constexpr int KIND_CD = 1;
 constexpr int KIND_VERLET = 2;
 
+// Fourth dword of a raw buffer resource on gfx9 targets (32-bit untyped
+// accesses, range-checked against the record count).
+constexpr int kBufferRsrcWord3 = 0x00020000;
+
 // Element idx of a device array addressed as base + 32-bit byte offset, so
 // the load uses the scalar-base + vector-offset form (no 64-bit address
 // arithmetic per lane).

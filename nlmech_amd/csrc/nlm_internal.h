@@ -84,6 +84,13 @@ struct RnpDev {
   float break_rs2_f;       // r S^2 at which |S| / (k rbar / sqrt(r)) reaches
                            // 1 / (1 - 1e-5): krbar_sq / (1 - 1e-5)^2
   float break_guard_f;     // half-width of the exact-decision band around it
+  float up_rcp_h_f;        // check_up / h: fac = min(up/h - r/h, 1)
+  float vex_tol_f;         // half-width of the volume-correction edge band
+  // fast paths: the break threshold of rho^2 (+inf when bonds never break,
+  // which removes the break test) and an all-ones mask when contact forces
+  // are on (else 0)
+  double break_rho2;
+  unsigned long long contact_mask;
 };
 
 // ElasticState constants as used by the kernels.
@@ -148,7 +155,8 @@ struct FdState {
   DevBuf ref_off;   // int64 per internal node: row start in reference CSR
   DevBuf halo_ptr;  // int64 per 256-node block
   DevBuf halo;      // int32 internal ids
-  DevBuf colL;      // uint16 halo-local slice entries
+  DevBuf colL;      // uint16 halo-local slice entries (bond-based staged
+                    // kernel: LDS byte offsets of the halo records)
   int64_t halo_max = 0;
   bool tiled = false;   // LDS-staged step kernel usable
   bool es_tiled = false;  // LDS-staged state-based passes usable
