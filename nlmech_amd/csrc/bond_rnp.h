@@ -373,18 +373,14 @@ __device__ __forceinline__ void bond_fp64_fast(
   F[2] = DIM == 3 ? sr * dx2 : 0.0;
 }
 
-// cvj = vol_j c4 (J) in FP32 (c4f_infl): the bond's constant factor.  The
-// force comes out in FP32; the caller sums a chunk of slots in FP32 and adds
-// the chunk to its FP64 accumulator.
+// FP32-force common path on the bond's position and displacement
+// differences (FP32).  cvj = vol_j c4 (J) in FP32 (c4f_infl): the bond's
+// constant factor.  The force comes out in FP32.
 template <int DIM, int INF, bool NOFAIL>
-__device__ __forceinline__ void bond_fp32_fast(
-    const RnpDev& m, const double* xi, const double* ui, const double* xj,
-    const double* uj, float cvj, const SlotMasks& k, unsigned long long& nb,
+__device__ __forceinline__ void bond_fp32_core(
+    const RnpDev& m, float dx0, float dx1, float dx2, float du0, float du1,
+    float du2, float cvj, const SlotMasks& k, unsigned long long& nb,
     unsigned long long& band, unsigned long long& ex, float* F) {
-  float dx0 = float(xj[0] - xi[0]), dx1 = float(xj[1] - xi[1]);
-  float dx2 = DIM == 3 ? float(xj[2] - xi[2]) : 0.f;
-  float du0 = float(uj[0] - ui[0]), du1 = float(uj[1] - ui[1]);
-  float du2 = DIM == 3 ? float(uj[2] - ui[2]) : 0.f;
   float dd = __builtin_fmaf(dx1, dx1, dx0 * dx0);
   float num = __builtin_fmaf(dx1, du1, dx0 * du0);
   if (DIM == 3) {
@@ -417,6 +413,20 @@ __device__ __forceinline__ void bond_fp32_fast(
   F[0] = sr * dx0;
   F[1] = sr * dx1;
   F[2] = DIM == 3 ? sr * dx2 : 0.f;
+}
+
+// The same from FP64 positions and displacements: the differences are
+// formed in FP64 (no cancellation) and rounded to FP32.
+template <int DIM, int INF, bool NOFAIL>
+__device__ __forceinline__ void bond_fp32_fast(
+    const RnpDev& m, const double* xi, const double* ui, const double* xj,
+    const double* uj, float cvj, const SlotMasks& k, unsigned long long& nb,
+    unsigned long long& band, unsigned long long& ex, float* F) {
+  bond_fp32_core<DIM, INF, NOFAIL>(
+      m, float(xj[0] - xi[0]), float(xj[1] - xi[1]),
+      DIM == 3 ? float(xj[2] - xi[2]) : 0.f, float(uj[0] - ui[0]),
+      float(uj[1] - ui[1]), DIM == 3 ? float(uj[2] - ui[2]) : 0.f, cvj, k, nb,
+      band, ex, F);
 }
 
 }  // namespace nlm

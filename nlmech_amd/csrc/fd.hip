@@ -61,6 +61,25 @@ struct alignas(16) HaloRec {
   float cv;  // FP32-force mode: vol * c4 (* J when constant), in FP32
 };
 
+// The FP32-force mode's record (32 bytes): position and displacement
+// relative to the block's origins in FP32, so a bond's differences are two
+// FP32 subtractions per component.  Positions within a block and its halo
+// lie within about (16 + 2 horizon/h) h of the origin, so the relative
+// coordinates carry FP32 rounding of that extent: a relative error of the
+// bond's differences of order 1e-6, far inside the guard bands of the break
+// and volume-correction decisions (which the exact path re-takes).
+struct alignas(16) HaloRec32 {
+  float2 xr;
+  float2 ur;
+  float cv;
+  uint32_t flags;
+  double vol;
+};
+
+template <int MODE>
+using HaloRecOf =
+    typename std::conditional<MODE == kModeFp32, HaloRec32, HaloRec>::type;
+
 template <int DIM, int MODE, int KIND, bool BASE_F, int INF, int U,
           bool NOFAIL, bool TILED>
 __global__ __launch_bounds__(256) void k_fd(
@@ -77,19 +96,34 @@ __global__ __launch_bounds__(256) void k_fd(
   // (every node its rows reference) is staged in LDS before the bond loop
   // and neighbors are read through halo-local entries (LDS byte offsets).
   static_assert(!TILED || DIM == 2, "the LDS-staged kernel is 2D");
+  constexpr bool REL = TILED && MODE == kModeFp32;  // block-relative FP32
+  typedef HaloRecOf<MODE> Rec;
   extern __shared__ HaloRec smem_halo[];
+  Rec* const halo_lds = reinterpret_cast<Rec*>(smem_halo);
+  double2 org = make_double2(0., 0.), uorg = make_double2(0., 0.);
+  int64_t hb = 0;
   if constexpr (TILED) {
-    const int64_t hb = a.halo_ptr[lb];
+    if constexpr (REL) {
+      org = a.blk_org[lb];
+      uorg = ld32(a.u, uint32_t(a.blk_cnode[lb]));
+    }
+    hb = a.halo_ptr[lb];
     const int hn = int(a.halo_ptr[lb + 1] - hb);
     for (int t = threadIdx.x; t < hn; t += 256) {
       const uint32_t g = uint32_t(a.halo[hb + t]);
-      HaloRec r;
-      r.x = ld32(a.x, g);
-      r.u = ld32(a.u, g);
+      Rec r;
+      const double2 xg = ld32(a.x, g), ug = ld32(a.u, g);
       r.vol = ld32(a.vol, g);
       r.flags = NOFAIL ? ld32(a.flags, g) : 0u;
       r.cv = float(r.vol) * a.m.c4f_infl;
-      smem_halo[t] = r;
+      if constexpr (REL) {
+        r.xr = make_float2(float(xg.x - org.x), float(xg.y - org.y));
+        r.ur = make_float2(float(ug.x - uorg.x), float(ug.y - uorg.y));
+      } else {
+        r.x = xg;
+        r.u = ug;
+      }
+      halo_lds[t] = r;
     }
     __syncthreads();
   }
@@ -115,6 +149,14 @@ __global__ __launch_bounds__(256) void k_fd(
     fl = a.flags[i];
   }
   bool int_i = !(fl & kFlagNonInterior);
+  // the node's block-relative FP32 position and displacement
+  float xir[2] = {0.f, 0.f}, uir[2] = {0.f, 0.f};
+  if constexpr (REL) {
+    xir[0] = float(xi[0] - org.x);
+    xir[1] = float(xi[1] - org.y);
+    uir[0] = float(ui[0] - uorg.x);
+    uir[1] = float(ui[1] - uorg.y);
+  }
   // this slice's neighbor slots: slot k of this lane at entry 64 k + lane
   const int32_t* __restrict__ colS = a.col + cb;
   const uint16_t* __restrict__ colL = a.colL + cb;
@@ -162,16 +204,26 @@ __global__ __launch_bounds__(256) void k_fd(
       jn[q] = slot_index(k0 + U + q);
     }
     double xj[U][3], uj[U][3], vj[U];
+    float xrj[U][2], urj[U][2];
     float cvj[U];
     unsigned long long w[U];
     uint32_t flj[U];
 #pragma unroll
     for (int q = 0; q < U; ++q) {
       const uint32_t j = jc[q];
-      if constexpr (TILED) {
+      if constexpr (REL) {
         // The slot entry is the record's LDS address: the kernel has no
         // static LDS, so the dynamic area starts at address 0 (the host
         // checks the kernel's static LDS size before it selects this path).
+        typedef const __attribute__((address_space(3))) HaloRec32 LdsRec;
+        LdsRec* r = (LdsRec*)j;
+        xrj[q][0] = r->xr.x;
+        xrj[q][1] = r->xr.y;
+        urj[q][0] = r->ur.x;
+        urj[q][1] = r->ur.y;
+        cvj[q] = r->cv;
+        if (NOFAIL) flj[q] = r->flags;
+      } else if constexpr (TILED) {
         typedef const __attribute__((address_space(3))) HaloRec LdsRec;
         LdsRec* r = (LdsRec*)j;
         xj[q][0] = r->x.x;
@@ -237,6 +289,11 @@ __global__ __launch_bounds__(256) void k_fd(
         if constexpr (MODE == kModeFp64)
           bond_fp64_fast<DIM, INF, NOFAIL>(a.m, xi, ui, xj[q], uj[q], vj[q],
                                            km, nb[q], band[q], ex[q], F);
+        else if constexpr (REL)
+          bond_fp32_core<DIM, INF, NOFAIL>(
+              a.m, xrj[q][0] - xir[0], xrj[q][1] - xir[1], 0.f,
+              urj[q][0] - uir[0], urj[q][1] - uir[1], 0.f, cvj[q], km, nb[q],
+              band[q], ex[q], F);
         else
           bond_fp32_fast<DIM, INF, NOFAIL>(a.m, xi, ui, xj[q], uj[q], cvj[q],
                                            km, nb[q], band[q], ex[q], F);
@@ -262,7 +319,14 @@ __global__ __launch_bounds__(256) void k_fd(
             bool fs = fs0;
             double Fd[3];
             double xs[3], us[3], vs;
-            if constexpr (TILED) {
+            if constexpr (REL) {
+              // FP64 values of the neighbor from its halo entry
+              const uint32_t g = uint32_t(
+                  a.halo[hb + jc[q] / uint32_t(sizeof(HaloRec32))]);
+              load3<DIM>(ld32(a.x, g), xs);
+              load3<DIM>(ld32(a.u, g), us);
+              vs = ld32(a.vol, g);
+            } else if constexpr (TILED) {
               // the neighbor's record is re-read from LDS (volatile, so the
               // common path need not keep its FP64 values alive)
               typedef volatile const __attribute__((address_space(3))) HaloRec
@@ -626,7 +690,7 @@ void launch_t(nlm_ctx* c, const FdArgs<DIM>& a, double tf, double tn) {
   const unsigned long long* bits_ro = a.bits;
   if constexpr (DIM == 2) {
     if (c->fd.tiled && c->fd.use_tiled) {
-      size_t lds = size_t(a.halo_cap) * sizeof(HaloRec);
+      size_t lds = size_t(a.halo_cap) * sizeof(HaloRecOf<MODE>);
       hipLaunchKernelGGL((k_fd<DIM, MODE, KIND, BASE_F, INF, U, NOFAIL, true>),
                          dim3(g), dim3(256), lds, c->stream, a, bits_ro, tf,
                          tn);
@@ -980,13 +1044,18 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
         fa.sharedSizeBytes != 0)
       s.tiled = false;
   }
+  // the bond-based staged kernel reads halo-local entries as LDS byte
+  // offsets of its records (below 64 KiB by the budget above; the FP32-force
+  // mode's records are the 32-byte block-relative ones)
+  std::vector<uint16_t> colL_off;
   if (s.tiled) {
-    // the bond-based staged kernel reads halo-local entries as LDS byte
-    // offsets of the records (below 64 KiB by the budget above)
-    std::vector<uint16_t> off(L.colL.size());
-    for (size_t e = 0; e < off.size(); e++)
-      off[e] = uint16_t(uint32_t(L.colL[e]) * uint32_t(sizeof(HaloRec)));
-    if ((rc = up(s.colL, off.data(), sizeof(uint16_t) * nent))) return rc;
+    const uint32_t rec = p->precision == NLM_PREC_FP32_FORCE
+                             ? uint32_t(sizeof(HaloRec32))
+                             : uint32_t(sizeof(HaloRec));
+    colL_off.resize(L.colL.size());
+    for (size_t e = 0; e < colL_off.size(); e++)
+      colL_off[e] = uint16_t(uint32_t(L.colL[e]) * rec);
+    if ((rc = up(s.colL, colL_off.data(), sizeof(uint16_t) * nent))) return rc;
   } else if ((rc = up(s.colL, L.colL.data(), sizeof(uint16_t) * nent))) {
     return rc;
   }
@@ -999,6 +1068,38 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
     for (int d = 0; d < dim; d++) hx[a * stride + d] = p->x[3 * i + d];
     hv[a] = p->vol[i];
   }
+  // Per 256-node block: the centre of its nodes' bounding box (origin of the
+  // FP32 block-relative positions) and the block node nearest to it (whose
+  // displacement is the displacement origin).
+  std::vector<double> borg(size_t(L.nblocks) * 2, 0.);
+  std::vector<int32_t> bcn(size_t(L.nblocks), 0);
+  if (dim == 2) {
+    for (int64_t b = 0; b < L.nblocks; b++) {
+      const int64_t a0 = 256 * b, a1 = std::min<int64_t>(n, a0 + 256);
+      double lo[2] = {hx[a0 * 2], hx[a0 * 2 + 1]}, hi[2] = {lo[0], lo[1]};
+      for (int64_t a = a0; a < a1; a++)
+        for (int d = 0; d < 2; d++) {
+          lo[d] = std::min(lo[d], hx[a * 2 + d]);
+          hi[d] = std::max(hi[d], hx[a * 2 + d]);
+        }
+      const double o[2] = {0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1])};
+      double best = -1.;
+      for (int64_t a = a0; a < a1; a++) {
+        const double e0 = hx[a * 2] - o[0], e1 = hx[a * 2 + 1] - o[1];
+        const double d2 = e0 * e0 + e1 * e1;
+        if (best < 0. || d2 < best) {
+          best = d2;
+          bcn[b] = int32_t(a);
+        }
+      }
+      borg[2 * b] = o[0];
+      borg[2 * b + 1] = o[1];
+    }
+  }
+  if ((rc = up(s.blk_org, borg.data(), sizeof(double) * borg.size())))
+    return rc;
+  if ((rc = up(s.blk_cnode, bcn.data(), sizeof(int32_t) * bcn.size())))
+    return rc;
   size_t vbytes = sizeof(double) * stride * size_t(n);
   if ((rc = alloc(c, s.x, vbytes))) return rc;
   if ((rc = alloc(c, s.vol, sizeof(double) * n))) return rc;

@@ -31,6 +31,10 @@ struct FdArgs {
   const int64_t* __restrict__ halo_ptr;
   const int32_t* __restrict__ halo;
   const uint16_t* __restrict__ colL;
+  // per 256-node block: origin of the FP32 block-relative coordinates and
+  // the node whose displacement is the displacement origin
+  const double2* __restrict__ blk_org;
+  const int32_t* __restrict__ blk_cnode;
   int64_t halo_cap;  // LDS entries per halo array
   RnpDev m;
   double dt;
@@ -171,6 +175,8 @@ inline FdArgs<DIM> make_args(nlm_ctx* c) {
   a.halo_ptr = s.halo_ptr.as<int64_t>();
   a.halo = s.halo.as<int32_t>();
   a.colL = s.colL.as<uint16_t>();
+  a.blk_org = s.blk_org.as<double2>();
+  a.blk_cnode = s.blk_cnode.as<int32_t>();
   a.halo_cap = (s.halo_max + 1) & ~int64_t(1);
   a.m = s.rnp;
   a.dt = s.dt;

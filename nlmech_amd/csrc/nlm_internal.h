@@ -155,6 +155,8 @@ struct FdState {
   DevBuf ref_off;   // int64 per internal node: row start in reference CSR
   DevBuf halo_ptr;  // int64 per 256-node block
   DevBuf halo;      // int32 internal ids
+  DevBuf blk_org;   // double2 per block: origin of block-relative positions
+  DevBuf blk_cnode; // int32 per block: node nearest that origin
   DevBuf colL;      // uint16 halo-local slice entries (bond-based staged
                     // kernel: LDS byte offsets of the halo records)
   int64_t halo_max = 0;
