@@ -316,6 +316,7 @@ struct SlotMasks {
   unsigned long long act;  // the slot lies within the lane's row
   unsigned long long brk;  // both ends breakable (no-fail regions)
   unsigned long long w;    // broken before this step
+  unsigned long long vex;  // length within the volume-correction edge band
 };
 
 // Branch-free common path of the fast variants.  Computes the force of a
@@ -346,20 +347,20 @@ __device__ __forceinline__ void bond_fp64_fast(
   double r = dd * rinv;
   double S = num * (rinv * rinv);
   // (check_up - r)/h >= 1 exactly when r <= check_low, so the clamp gives
-  // the reference's factor away from the relative-epsilon edge
-  const unsigned long long v_vex = __builtin_amdgcn_ballot_w64(
-      fabs(r - m.low_band) <= 2e-9 * m.low_band);
+  // the reference's factor away from the relative-epsilon edge (bonds near
+  // the edge are in k.vex)
   double fac = fmin((m.check_up - r) * m.rcp_h, 1.0);
-  // (break_rho2 is +inf when bonds never break)
-  double d = (S * S) * r * m.rcp_krbar_sq - m.break_rho2;
-  const unsigned long long v_cross = __builtin_amdgcn_ballot_w64(d > 0.);
-  const unsigned long long v_band =
-      __builtin_amdgcn_ballot_w64(fabs(d) <= kGuard64 * kBreakRho2);
+  // rho^2 against the guard band [break_lo, break_hi] (+inf when bonds
+  // never break)
+  const double rho2 = (S * S) * r * m.rcp_krbar_sq;
+  const unsigned long long v_hi = __builtin_amdgcn_ballot_w64(rho2 > m.break_hi);
+  const unsigned long long v_lo =
+      __builtin_amdgcn_ballot_w64(rho2 >= m.break_lo);
   const unsigned long long cand = k.act & k.brk & ~k.w;
-  band = cand & v_band;
-  nb = cand & v_cross & ~band;
+  nb = cand & v_hi;
+  band = cand & v_lo & ~v_hi;
   const unsigned long long fs = k.w | nb;
-  ex = (v_vex | band | (fs & k.brk & m.contact_mask)) & k.act;
+  ex = (k.vex | band | (fs & k.brk & m.contact_mask)) & k.act;
   const unsigned long long keep = k.act & ~(fs & k.brk) & ~ex;
   double infl = influence_f64<INF>(m, r);
   double ex_ = exp(-m.beta * r * S * S);
@@ -392,17 +393,24 @@ __device__ __forceinline__ void bond_fp32_core(
   float q = num * rinv;           // S r
   float S = q * rinv;
   float rS2 = q * S;              // r S^2: exponent and break ratio
+  // volume-correction factor (check_up - r)/h clamped to [0, 1] (the
+  // clamp is an output modifier of the multiply-add; r < check_up for every
+  // bond); bonds near the correction's edge are in k.vex
+  const float fac =
+      __builtin_amdgcn_fmed3f(__builtin_fmaf(r, -m.rcp_h_f, m.up_rcp_h_f),
+                              0.f, 1.f);
+  // r S^2 against the guard band [break_lo_f, break_hi_f] (+inf when bonds
+  // never break)
+  const unsigned long long v_hi =
+      __builtin_amdgcn_ballot_w64(rS2 > m.break_hi_f);
+  const unsigned long long v_lo =
+      __builtin_amdgcn_ballot_w64(rS2 >= m.break_lo_f);
+  const unsigned long long cand = k.act & k.brk & ~k.w;
+  nb = cand & v_hi;
+  band = cand & v_lo & ~v_hi;
+  const unsigned long long fs = k.w | nb;
   const unsigned long long v_vex =
       __builtin_amdgcn_ballot_w64(fabsf(r - m.low_band_f) <= m.vex_tol_f);
-  float fac = fminf(__builtin_fmaf(r, -m.rcp_h_f, m.up_rcp_h_f), 1.0f);
-  const float dbr = rS2 - m.break_rs2_f;  // break_rs2_f: +inf if no breaking
-  const unsigned long long v_cross = __builtin_amdgcn_ballot_w64(dbr > 0.f);
-  const unsigned long long v_band =
-      __builtin_amdgcn_ballot_w64(fabsf(dbr) <= m.break_guard_f);
-  const unsigned long long cand = k.act & k.brk & ~k.w;
-  band = cand & v_band;
-  nb = cand & v_cross & ~band;
-  const unsigned long long fs = k.w | nb;
   ex = (v_vex | band | (fs & k.brk & m.contact_mask)) & k.act;
   const unsigned long long keep = k.act & ~(fs & k.brk) & ~ex;
   float ex_ = __builtin_amdgcn_exp2f(rS2 * m.nbeta_log2e_f);

@@ -68,12 +68,11 @@ struct alignas(16) HaloRec {
 // coordinates carry FP32 rounding of that extent: a relative error of the
 // bond's differences of order 1e-6, far inside the guard bands of the break
 // and volume-correction decisions (which the exact path re-takes).
-struct alignas(16) HaloRec32 {
+struct alignas(8) HaloRec32 {
   float2 xr;
   float2 ur;
-  float cv;
+  float cv;  // vol c4 (J)
   uint32_t flags;
-  double vol;
 };
 
 template <int MODE>
@@ -84,7 +83,8 @@ template <int DIM, int MODE, int KIND, bool BASE_F, int INF, int U,
           bool NOFAIL, bool TILED>
 __global__ __launch_bounds__(256) void k_fd(
     FdArgs<DIM> a, const unsigned long long* __restrict__ bits_ro,
-    double t_force, double t_next) {
+    const unsigned long long* __restrict__ vex_ro, double t_force,
+    double t_next) {
   if (a.tdev) {  // graph-captured steps read the step's times
     t_force = a.tdev[0];
     t_next = a.tdev[1];
@@ -113,15 +113,16 @@ __global__ __launch_bounds__(256) void k_fd(
       const uint32_t g = uint32_t(a.halo[hb + t]);
       Rec r;
       const double2 xg = ld32(a.x, g), ug = ld32(a.u, g);
-      r.vol = ld32(a.vol, g);
+      const double vg = ld32(a.vol, g);
       r.flags = NOFAIL ? ld32(a.flags, g) : 0u;
-      r.cv = float(r.vol) * a.m.c4f_infl;
+      r.cv = float(vg) * a.m.c4f_infl;
       if constexpr (REL) {
         r.xr = make_float2(float(xg.x - org.x), float(xg.y - org.y));
         r.ur = make_float2(float(ug.x - uorg.x), float(ug.y - uorg.y));
       } else {
         r.x = xg;
         r.u = ug;
+        r.vol = vg;
       }
       halo_lds[t] = r;
     }
@@ -165,6 +166,9 @@ __global__ __launch_bounds__(256) void k_fd(
   // to U - 1 words past the slice (the next slice's, or the array's
   // padding); those slots are inactive.
   const unsigned long long* __restrict__ bits_in = bits_ro + bb;
+  // static per-slot masks of the bonds near the volume-correction edge
+  // (padded like the bond bits)
+  const unsigned long long* __restrict__ vex_in = vex_ro + bb;
   double acc[3] = {0., 0., 0.};
   // Neighbor indices are fetched one chunk of U slots ahead of the chunk
   // whose positions and displacements are gathered, so the index latency
@@ -206,7 +210,7 @@ __global__ __launch_bounds__(256) void k_fd(
     double xj[U][3], uj[U][3], vj[U];
     float xrj[U][2], urj[U][2];
     float cvj[U];
-    unsigned long long w[U];
+    unsigned long long w[U], vx[U];
     uint32_t flj[U];
 #pragma unroll
     for (int q = 0; q < U; ++q) {
@@ -243,6 +247,7 @@ __global__ __launch_bounds__(256) void k_fd(
         if (NOFAIL) flj[q] = ld32(a.flags, j);
       }
       w[q] = bits_in[k0 + q];
+      vx[q] = MODE == kModeFp32 ? 0ull : vex_in[k0 + q];
     }
     // newly broken bonds of each slot (wavefront masks)
     unsigned long long nb[U];
@@ -285,6 +290,7 @@ __global__ __launch_bounds__(256) void k_fd(
                                        (flj[q] & kFlagNonInterior) == 0u))
                         : ~0ull;
         km.w = w[q];
+        km.vex = vx[q];
         Real F[3];
         if constexpr (MODE == kModeFp64)
           bond_fp64_fast<DIM, INF, NOFAIL>(a.m, xi, ui, xj[q], uj[q], vj[q],
@@ -340,11 +346,16 @@ __global__ __launch_bounds__(256) void k_fd(
               us[2] = 0.;
               vs = r->vol;
             } else {
+              // re-read from global memory (volatile, so the common path
+              // need not keep the FP64 values alive)
+              const uint32_t g = jc[q];
+              const volatile double* px = (const volatile double*)(a.x + g);
+              const volatile double* pu = (const volatile double*)(a.u + g);
               for (int d = 0; d < 3; d++) {
-                xs[d] = xj[q][d];
-                us[d] = uj[q][d];
+                xs[d] = d < DIM ? px[d] : 0.;
+                us[d] = d < DIM ? pu[d] : 0.;
               }
-              vs = vj[q];
+              vs = *(const volatile double*)(a.vol + g);
             }
             bond_mode<DIM, MODE, INF>(a.m, xi, ui, xs, us, voli, vs, brk, fs,
                                       Fd);
@@ -692,14 +703,15 @@ void launch_t(nlm_ctx* c, const FdArgs<DIM>& a, double tf, double tn) {
     if (c->fd.tiled && c->fd.use_tiled) {
       size_t lds = size_t(a.halo_cap) * sizeof(HaloRecOf<MODE>);
       hipLaunchKernelGGL((k_fd<DIM, MODE, KIND, BASE_F, INF, U, NOFAIL, true>),
-                         dim3(g), dim3(256), lds, c->stream, a, bits_ro, tf,
-                         tn);
+                         dim3(g), dim3(256), lds, c->stream, a, bits_ro,
+                         a.vexw, tf, tn);
       return;
     }
   }
   {
     hipLaunchKernelGGL((k_fd<DIM, MODE, KIND, BASE_F, INF, U, NOFAIL, false>),
-                       dim3(g), dim3(256), 0, c->stream, a, bits_ro, tf, tn);
+                       dim3(g), dim3(256), 0, c->stream, a, bits_ro, a.vexw,
+                       tf, tn);
   }
 }
 
@@ -1201,8 +1213,49 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
                                 : std::numeric_limits<double>::infinity();
   m.contact_mask = m.contact ? ~0ull : 0ull;
   m.break_guard_f = float(double(kGuard32) * kBreakRho2 * m.krbar_sq);
+  m.break_lo = m.break_rho2 - kGuard64 * kBreakRho2;
+  m.break_hi = m.break_rho2 + kGuard64 * kBreakRho2;
+  m.break_lo_f = m.irreversible ? float(double(m.break_rs2_f) -
+                                        double(m.break_guard_f))
+                                : std::numeric_limits<float>::infinity();
+  m.break_hi_f = m.irreversible ? float(double(m.break_rs2_f) +
+                                        double(m.break_guard_f))
+                                : std::numeric_limits<float>::infinity();
   m.up_rcp_h_f = float(m.check_up * m.rcp_h);
   m.vex_tol_f = float(3e-5 * m.low_band);
+  {
+    // Bonds whose reference length lies within 3e-5 (relative) of the
+    // volume-correction edge: a static mask per (slice, slot), padded like
+    // the bond bits; the step kernels send them to the exact evaluation.
+    // The band is far wider than the fast paths' rounding of the length.
+    std::vector<unsigned long long> vw(size_t(nent / 64 + 8), 0ull);
+    if (!es_mat) {
+      const double tol = 3e-5 * m.low_band;
+      for (int64_t sl = 0; sl < L.nslices; sl++) {
+        const int64_t e0 = L.sl_ptr[sl];
+        const int64_t width = (L.sl_ptr[sl + 1] - e0) / 64;
+        for (int64_t k = 0; k < width; k++) {
+          unsigned long long word = 0ull;
+          for (int lane = 0; lane < 64; lane++) {
+            const int64_t a = sl * 64 + lane;
+            if (a >= n || k >= int64_t(L.rowlen[a])) continue;
+            const int64_t g = L.colS[e0 + 64 * k + lane];
+            double d2 = 0.;
+            for (int d = 0; d < dim; d++) {
+              const double e = hx[g * stride + d] - hx[a * stride + d];
+              d2 += e * e;
+            }
+            if (std::fabs(std::sqrt(d2) - m.low_band) <= tol)
+              word |= 1ull << lane;
+          }
+          vw[size_t(e0 / 64 + k)] = word;
+        }
+      }
+    }
+    if ((rc = up(s.vexw, vw.data(), sizeof(unsigned long long) * vw.size())))
+      return rc;
+    NLM_HIP(c, hipStreamSynchronize(st));
+  }
   s.unroll = 4;
   if (es_mat) {
     // ElasticState constants (ElasticState.cpp:47-158, volumeCorrection.cpp)

@@ -33,6 +33,7 @@ struct FdArgs {
   const uint16_t* __restrict__ colL;
   // per 256-node block: origin of the FP32 block-relative coordinates and
   // the node whose displacement is the displacement origin
+  const unsigned long long* __restrict__ vexw;
   const double2* __restrict__ blk_org;
   const int32_t* __restrict__ blk_cnode;
   int64_t halo_cap;  // LDS entries per halo array
@@ -175,6 +176,7 @@ inline FdArgs<DIM> make_args(nlm_ctx* c) {
   a.halo_ptr = s.halo_ptr.as<int64_t>();
   a.halo = s.halo.as<int32_t>();
   a.colL = s.colL.as<uint16_t>();
+  a.vexw = s.vexw.as<unsigned long long>();
   a.blk_org = s.blk_org.as<double2>();
   a.blk_cnode = s.blk_cnode.as<int32_t>();
   a.halo_cap = (s.halo_max + 1) & ~int64_t(1);

@@ -91,6 +91,10 @@ struct RnpDev {
   // are on (else 0)
   double break_rho2;
   unsigned long long contact_mask;
+  // bounds of the break test's guard band: on rho^2 (FP64 path) and on
+  // r S^2 (FP32 path); +inf when bonds never break
+  double break_lo, break_hi;
+  float break_lo_f, break_hi_f;
 };
 
 // ElasticState constants as used by the kernels.
@@ -157,6 +161,8 @@ struct FdState {
   DevBuf halo;      // int32 internal ids
   DevBuf blk_org;   // double2 per block: origin of block-relative positions
   DevBuf blk_cnode; // int32 per block: node nearest that origin
+  DevBuf vexw;      // uint64 per (slice, slot): bonds whose length lies in
+                    // the volume-correction edge band (exact evaluation)
   DevBuf colL;      // uint16 halo-local slice entries (bond-based staged
                     // kernel: LDS byte offsets of the halo records)
   int64_t halo_max = 0;
