@@ -286,6 +286,22 @@ int nlm_fd_compute_forces(nlm_ctx* ctx);
  * Z_i = max_j |S|/Sc(r), phi_i = 1 - sum unbroken V_j / sum V_j (float like
  * the reference), strain energy of node i.  Any pointer may be NULL. */
 int nlm_fd_postproc(nlm_ctx* ctx, float* Z, float* phi, double* energy);
+/* External force density of every node at time t, 3 components per node in
+ * reference order (z = 0 in 2D): FLoading::apply on a zero force vector
+ * (fLoading.cpp:117-279), which computePostProcFields takes for the work
+ * done u . f_ext (fDModel.cpp:655-662, 734-735). */
+int nlm_fd_external_force(nlm_ctx* ctx, double t, double* f_ext);
+/* Reaction force of computeForce (fDModel.cpp:548-580): for each node, the
+ * sum of V_i F_ij over its bonds for which is_reaction_force(i, j) holds
+ * (fDModel.cpp:585-605: node in max_y - 1.5 horizon < y < max_y - horizon,
+ * bond crossing the line y = max_y - horizon + 1e-6 between min_x and max_x,
+ * tested with util::geometry::doLinesIntersect as the reference writes it,
+ * utilGeom.cpp:281-326), at the current state.  box = {min_x, max_x, max_y}
+ * of the mesh bounding box (fe::Mesh::computeBBox), horizon = the model
+ * deck's.  R: 3 per node (reference order), R_len: |R_i| (the per-node
+ * Total_Reaction_Force).  Either output may be NULL.  Bond-based only. */
+int nlm_fd_reaction_force(nlm_ctx* ctx, const double* box, double horizon,
+                          double* R, double* R_len);
 
 /* State-based material: dilatation theta_i of the last force evaluation and
  * the weighted volume m_i (VolumeCorrection::weightedVolume,

@@ -690,6 +690,109 @@ __global__ __launch_bounds__(256) void k_postproc(FdArgs<DIM> a,
   if (energy) energy[i] = energy_i * voli;
 }
 
+// External force density of every node at time t: FLoading::apply on a zero
+// force vector (fLoading.cpp:117-279), as FDModel::computePostProcFields
+// takes it for the work done (fDModel.cpp:655-662).  Internal order, 3
+// components per node.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_fext(FdArgs<DIM> a, double t,
+                                              double* __restrict__ out) {
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= a.n) return;
+  double xi[3] = {0., 0., 0.}, f[3] = {0., 0., 0.};
+  load3<DIM>(a.x[i], xi);
+  if (a.has_fbc) fext_node<DIM>(a.bc, a.flags[i], xi, t, f);
+  out[3 * i] = f[0];
+  out[3 * i + 1] = f[1];
+  out[3 * i + 2] = f[2];
+}
+
+// util::geometry::direction (utilGeom.cpp:317-326) as the reference writes
+// it: the orientation determinant is stored in an int (truncation toward
+// zero), so only determinants of magnitude >= 1 count as turns.
+__device__ __forceinline__ int ref_direction(double ax, double ay, double bx,
+                                             double by, double cx, double cy) {
+  const int val = int((by - ay) * (cx - bx) - (bx - ax) * (cy - by));
+  if (val == 0) return 0;
+  return val < 0 ? 2 : 1;
+}
+
+// util::geometry::onLine (utilGeom.cpp:308-315), as written (C below or at
+// both the maximum and the minimum of each coordinate).
+__device__ __forceinline__ bool ref_on_line(double ax, double ay, double bx,
+                                            double by, double cx, double cy) {
+  return cx <= fmax(ax, bx) && cx <= fmin(ax, bx) && cy <= fmax(ay, by) &&
+         cy <= fmin(ay, by);
+}
+
+// util::geometry::doLinesIntersect (utilGeom.cpp:281-306).
+__device__ __forceinline__ bool ref_lines_intersect(double ax, double ay,
+                                                    double bx, double by,
+                                                    double cx, double cy,
+                                                    double dx, double dy) {
+  const int d1 = ref_direction(ax, ay, bx, by, cx, cy);
+  const int d2 = ref_direction(ax, ay, bx, by, dx, dy);
+  const int d3 = ref_direction(cx, cy, dx, dy, ax, ay);
+  const int d4 = ref_direction(cx, cy, dx, dy, bx, by);
+  if (d1 != d2 && d3 != d4) return true;
+  if (d1 == 0 && ref_on_line(ax, ay, bx, by, cx, cy)) return true;
+  if (d2 == 0 && ref_on_line(ax, ay, bx, by, dx, dy)) return true;
+  if (d3 == 0 && ref_on_line(cx, cy, dx, dy, ax, ay)) return true;
+  if (d4 == 0 && ref_on_line(cx, cy, dx, dy, bx, by)) return true;
+  return false;
+}
+
+// Reaction force of FDModel::computeForce (fDModel.cpp:548-580): for a node
+// in the layer max_y - 1.5 delta < y < max_y - delta, the volume-weighted
+// forces of its bonds that cross the line y = max_y - delta + 1e-6 between
+// min_x and max_x (FDModel::is_reaction_force, fDModel.cpp:585-605), at the
+// current state (the bond bits are read, not changed).  R: internal order,
+// 3 components per node; R_len: its length (Total_Reaction_Force per node).
+template <int DIM>
+__global__ __launch_bounds__(256) void k_reaction(
+    FdArgs<DIM> a, double min_x, double max_x, double max_y, double delta,
+    double* __restrict__ R, double* __restrict__ R_len) {
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int64_t s = i >> 6;
+  if (s >= a.nslices) return;
+  if (i >= a.n) return;
+  const int64_t cb = a.sl_ptr[s];
+  const int64_t bb = cb >> 6;
+  const int len = int(a.rowlen[i]);
+  double xi[3] = {0., 0., 0.}, ui[3] = {0., 0., 0.};
+  load3<DIM>(a.x[i], xi);
+  load3<DIM>(a.u[i], ui);
+  const double voli = a.vol[i];
+  const uint32_t fl = a.flags[i];
+  const bool int_i = !(fl & kFlagNonInterior);
+  double r[3] = {0., 0., 0.};
+  const double eps = 1e-6;
+  if (xi[1] > max_y - 1.5 * delta && xi[1] < max_y - delta) {
+    const double ly = max_y - delta + eps;
+    for (int k = 0; k < len; ++k) {
+      const int32_t j = a.col[cb + int64_t(k) * 64 + lane];
+      double xj[3] = {0., 0., 0.}, uj[3] = {0., 0., 0.};
+      load3<DIM>(a.x[j], xj);
+      if (!ref_lines_intersect(min_x, ly, max_x, ly, xi[0], xi[1], xj[0],
+                               xj[1]))
+        continue;
+      load3<DIM>(a.u[j], uj);
+      bool brk = int_i;
+      if (a.has_nofail) brk = brk && !(a.flags[j] & kFlagNonInterior);
+      bool fs = (a.bits[bb + k] >> lane) & 1ull;
+      double F[3];
+      bond_exact<DIM, false>(a.m, xi, ui, xj, uj, voli, a.vol[j], brk, fs, F,
+                             nullptr);
+      for (int d = 0; d < 3; d++) r[d] += voli * F[d];
+    }
+  }
+  R[3 * i] = r[0];
+  R[3 * i + 1] = r[1];
+  R[3 * i + 2] = r[2];
+  R_len[i] = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+}
+
 // ---- host side ------------------------------------------------------------
 
 // Launch helpers.  The neighbor-slot chunk U of the fused step kernel is a
@@ -1748,6 +1851,72 @@ extern "C" int nlm_fd_postproc(nlm_ctx* c, float* Z, float* phi,
     if (Z) Z[i] = hz[a];
     if (phi) phi[i] = hp[a];
     if (energy) energy[i] = he[a];
+  }
+  return NLM_OK;
+}
+
+extern "C" int nlm_fd_external_force(nlm_ctx* c, double t, double* f_ext) {
+  int rc = check_ready(c, "nlm_fd_external_force");
+  if (rc) return rc;
+  if (!f_ext)
+    return fail(c, NLM_ERR_INVALID, "nlm_fd_external_force: null output");
+  FdState& s = c->fd;
+  DevBuf d;
+  if ((rc = alloc(c, d, sizeof(double) * 3 * s.n))) return rc;
+  const unsigned g = unsigned((s.n + 255) / 256);
+  if (s.dim == 2)
+    hipLaunchKernelGGL(k_fext<2>, dim3(g), dim3(256), 0, c->stream,
+                       make_args<2>(c), t, d.as<double>());
+  else
+    hipLaunchKernelGGL(k_fext<3>, dim3(g), dim3(256), 0, c->stream,
+                       make_args<3>(c), t, d.as<double>());
+  NLM_HIP(c, hipGetLastError());
+  std::vector<double> h(size_t(3 * s.n));
+  NLM_HIP(c, hipMemcpyAsync(h.data(), d.p, sizeof(double) * 3 * s.n,
+                            hipMemcpyDeviceToHost, c->stream));
+  NLM_HIP(c, hipStreamSynchronize(c->stream));
+  for (int64_t a = 0; a < s.n; a++) {
+    const int64_t i = s.perm_host[a];
+    for (int k = 0; k < 3; k++) f_ext[3 * i + k] = h[3 * a + k];
+  }
+  return NLM_OK;
+}
+
+extern "C" int nlm_fd_reaction_force(nlm_ctx* c, const double* box,
+                                     double horizon, double* R,
+                                     double* R_len) {
+  int rc = check_ready(c, "nlm_fd_reaction_force");
+  if (rc) return rc;
+  FdState& s = c->fd;
+  if (s.material != NLM_MATERIAL_RNP_BOND)
+    return fail(c, NLM_ERR_INVALID,
+                "nlm_fd_reaction_force: bond-based material only");
+  if (!box)
+    return fail(c, NLM_ERR_INVALID, "nlm_fd_reaction_force: null box");
+  DevBuf dr, dl;
+  if ((rc = alloc(c, dr, sizeof(double) * 3 * s.n))) return rc;
+  if ((rc = alloc(c, dl, sizeof(double) * s.n))) return rc;
+  const unsigned g = unsigned((s.sell.nslices * 64 + 255) / 256);
+  if (s.dim == 2)
+    hipLaunchKernelGGL(k_reaction<2>, dim3(g), dim3(256), 0, c->stream,
+                       make_args<2>(c), box[0], box[1], box[2], horizon,
+                       dr.as<double>(), dl.as<double>());
+  else
+    hipLaunchKernelGGL(k_reaction<3>, dim3(g), dim3(256), 0, c->stream,
+                       make_args<3>(c), box[0], box[1], box[2], horizon,
+                       dr.as<double>(), dl.as<double>());
+  NLM_HIP(c, hipGetLastError());
+  std::vector<double> hr(size_t(3 * s.n)), hl(size_t(s.n));
+  NLM_HIP(c, hipMemcpyAsync(hr.data(), dr.p, sizeof(double) * 3 * s.n,
+                            hipMemcpyDeviceToHost, c->stream));
+  NLM_HIP(c, hipMemcpyAsync(hl.data(), dl.p, sizeof(double) * s.n,
+                            hipMemcpyDeviceToHost, c->stream));
+  NLM_HIP(c, hipStreamSynchronize(c->stream));
+  for (int64_t a = 0; a < s.n; a++) {
+    const int64_t i = s.perm_host[a];
+    if (R)
+      for (int k = 0; k < 3; k++) R[3 * i + k] = hr[3 * a + k];
+    if (R_len) R_len[i] = hl[a];
   }
   return NLM_OK;
 }

@@ -411,3 +411,19 @@ class GpuFD:
         self._call("nlm_fd_postproc", ptr(Z, C.c_float), ptr(phi, C.c_float),
                    ptr(e, C.c_double))
         return Z, phi, e
+
+    def external_force(self, t: float):
+        """External force density (n, 3) at time t (FLoading::apply)."""
+        f = np.zeros((self.p.n, 3))
+        self._call("nlm_fd_external_force", float(t), ptr(f, C.c_double))
+        return f
+
+    def reaction_force(self, box, horizon: float):
+        """Reaction force (n, 3) and its per-node length; box = (min_x,
+        max_x, max_y) of the mesh bounding box."""
+        b = np.ascontiguousarray(box, np.float64)
+        R = np.zeros((self.p.n, 3))
+        L = np.zeros(self.p.n)
+        self._call("nlm_fd_reaction_force", ptr(b, C.c_double), float(horizon),
+                   ptr(R, C.c_double), ptr(L, C.c_double))
+        return R, L

@@ -462,6 +462,23 @@ void FDModel<T>::computeForces() {
 }
 
 template <class T>
+bool FDModel<T>::populate(const std::string& tag) const {
+  // tags left out at memory-control levels 1..3 (policy.cpp:38-56)
+  const int lvl = d_input_p->getPolicyDeck()->d_memControlFlag;
+  static const char* const l1[] = {"Model_d_e", "Model_d_w", "Model_d_phi",
+                                   "Model_d_strain", "Model_d_stress"};
+  static const char* const l2[] = {"Model_d_eF", "Model_d_eFB"};
+  if (lvl >= 1)
+    for (const char* t : l1)
+      if (tag == t) return false;
+  if (lvl >= 2)
+    for (const char* t : l2)
+      if (tag == t) return false;
+  if (lvl >= 3 && tag == "Model_d_Z") return false;
+  return true;
+}
+
+template <class T>
 void FDModel<T>::computePostProcFields() {
   const size_t n = d_mesh->getNumNodes();
   d_Z.assign(n, 0.f);
@@ -470,6 +487,50 @@ void FDModel<T>::computePostProcFields() {
   if (T::kMaterial != NLM_MATERIAL_RNP_BOND) return;
   check(nlm_fd_postproc(d_ctx, d_Z.data(), d_phi.data(), d_energy.data()),
         "nlm_fd_postproc");
+  const inp::OutputDeck* od = d_outputDeck_p;
+  const auto& V = d_mesh->getNodalVolumes();
+  std::vector<double> u, v;
+  getState(&u, &v, nullptr);
+  // work done u . f_ext (fDModel.cpp:655-662, 734-735)
+  d_work.assign(n, 0.);
+  if (populate("Model_d_w")) {
+    std::vector<double> fe(3 * n, 0.);
+    check(nlm_fd_external_force(d_ctx, d_time, fe.data()),
+          "nlm_fd_external_force");
+    for (size_t i = 0; i < n; i++)
+      d_work[i] = u[3 * i] * fe[3 * i] + u[3 * i + 1] * fe[3 * i + 1] +
+                  u[3 * i + 2] * fe[3 * i + 2];
+  }
+  // kinetic energy (fDModel.cpp:755-761)
+  d_kinetic.assign(n, 0.);
+  if (populate("Model_d_e")) {
+    const double rho = d_input_p->getMaterialDeck()->d_density;
+    for (size_t i = 0; i < n; i++)
+      d_kinetic[i] = 0.5 * rho *
+                     (v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] +
+                      v[3 * i + 2] * v[3 * i + 2]) *
+                     V[i];
+  }
+  // fracture energies of fully damaged nodes; kept from earlier outputs
+  // elsewhere (fDModel.cpp:737-745)
+  if (d_fracE.size() != n) d_fracE.assign(n, 0.);
+  if (d_fracEB.size() != n) d_fracEB.assign(n, 0.);
+  for (size_t i = 0; i < n; i++)
+    if (greater_than(double(d_Z[i]), 1.0 - 1.0E-10)) {
+      if (populate("Model_d_eFB")) d_fracEB[i] = d_energy[i];
+      if (populate("Model_d_eF")) d_fracE[i] = d_energy[i];
+    }
+  // reaction force of computeForce (fDModel.cpp:548-580)
+  if (od->isTagInOutput("Reaction_Force") ||
+      od->isTagInOutput("Total_Reaction_Force")) {
+    const auto& bb = d_mesh->getBoundingBox();
+    const double box[3] = {bb.first[0], bb.second[0], bb.second[1]};
+    d_reaction.assign(3 * n, 0.);
+    d_reactionLen.assign(n, 0.);
+    check(nlm_fd_reaction_force(d_ctx, box, d_modelDeck_p->d_horizon,
+                                d_reaction.data(), d_reactionLen.data()),
+          "nlm_fd_reaction_force");
+  }
 }
 
 template <class T>
@@ -541,13 +602,56 @@ void FDModel<T>::output() {
     if (od->isTagInOutput("Node_Volume"))
       w.appendPointData("Node_Volume", d_mesh->getNodalVolumes().data(), 1);
     if (d_Z.size() == n) {
-      if (od->isTagInOutput("Damage_Phi"))
+      if (od->isTagInOutput("Damage_Phi") && populate("Model_d_phi"))
         w.appendPointData("Damage_Phi", d_phi.data(), 1);
-      if (od->isTagInOutput("Damage_Z"))
+      if (od->isTagInOutput("Damage_Z") && populate("Model_d_Z"))
         w.appendPointData("Damage_Z", d_Z.data(), 1);
       if (od->isTagInOutput("Strain_Energy"))
         w.appendPointData("Strain_Energy", d_energy.data(), 1);
     }
+    if (d_reaction.size() == 3 * n) {
+      if (od->isTagInOutput("Reaction_Force"))
+        w.appendPointData("Reaction_Force", d_reaction.data(), 3);
+      if (od->isTagInOutput("Total_Reaction_Force")) {
+        // model/util.cpp:79-96: std::accumulate with an int initial value
+        // accumulates in int (each partial sum truncated toward zero); times
+        // the area (|max_x - min_x| - horizon) (|max_y - min_y| - horizon)
+        int acc = 0;
+        for (size_t i = 0; i < n; i++) acc = int(acc + d_reactionLen[i]);
+        const auto& bb = d_mesh->getBoundingBox();
+        const double delta = d_modelDeck_p->d_horizon;
+        const double area = (std::abs(bb.second[0] - bb.first[0]) - delta) *
+                            (std::abs(bb.second[1] - bb.first[1]) - delta);
+        w.appendFieldData("Total_Reaction_Force", double(acc) * area);
+      }
+    }
+    if (d_work.size() == n && od->isTagInOutput("Work_Done") &&
+        populate("Model_d_w"))
+      w.appendPointData("Work_Done", d_work.data(), 1);
+    if (d_fracEB.size() == n && od->isTagInOutput("Fracture_Perienergy_Bond") &&
+        populate("Model_d_eFB"))
+      w.appendPointData("Fracture_Perienergy_Bond", d_fracEB.data(), 1);
+    if (d_fracE.size() == n && od->isTagInOutput("Fracture_Perienergy_Total") &&
+        populate("Model_d_eF"))
+      w.appendPointData("Fracture_Perienergy_Total", d_fracE.data(), 1);
+    auto total = [](const std::vector<double>& a) {
+      double s = 0.;
+      for (double x : a) s += x;
+      return s;
+    };
+    // model/util.cpp:140-164
+    if (d_energy.size() == n && od->isTagInOutput("Total_Energy") &&
+        populate("Model_d_e"))
+      w.appendFieldData("Total_Energy", total(d_energy) - total(d_work) +
+                                            total(d_kinetic));
+    if (d_fracEB.size() == n &&
+        od->isTagInOutput("Total_Fracture_Perienergy_Bond") &&
+        populate("Model_d_eFB"))
+      w.appendFieldData("Total_Fracture_Perienergy_Bond", total(d_fracEB));
+    if (d_fracE.size() == n &&
+        od->isTagInOutput("Total_Fracture_Perienergy_Total") &&
+        populate("Model_d_eF"))
+      w.appendFieldData("Total_Fracture_Perienergy_Total", total(d_fracE));
     if (od->isTagInOutput("Neighbors")) {
       std::vector<int64_t> rp(n + 1);
       check(nlm_neighbor_get(d_ctx, rp.data(), nullptr), "nlm_neighbor_get");
@@ -555,12 +659,7 @@ void FDModel<T>::output() {
       for (size_t i = 0; i < n; i++) cnt[i] = double(rp[i + 1] - rp[i]);
       w.appendPointData("Neighbors", cnt.data(), 1);
     }
-    for (const char* t :
-         {"Reaction_Force", "Total_Reaction_Force", "Work_Done",
-          "Fracture_Perienergy_Bond", "Fracture_Perienergy_Total",
-          "Total_Energy", "Total_Fracture_Perienergy_Bond",
-          "Total_Fracture_Perienergy_Total", "Strain_Tensor",
-          "Stress_Tensor"})
+    for (const char* t : {"Strain_Tensor", "Stress_Tensor"})
       if (od->isTagInOutput(t) && d_n == 0)
         std::cout << "Warning: output tag " << t
                   << " is not computed by the GPU engine.\n";

@@ -103,7 +103,9 @@ class FDModel : public Model {
   void integrate();
   // computeForces (fDModel.cpp:528-541) on the current state.
   void computeForces();
-  // computePostProcFields (fDModel.cpp:652-778): damage Z, phi, energy.
+  // computePostProcFields (fDModel.cpp:652-778): damage Z, phi, strain
+  // energy, work done, kinetic and fracture energies, and the reaction
+  // force of computeForce (fDModel.cpp:548-580) when output asks for it.
   void computePostProcFields();
   // model::Output (src/model/util.cpp:4-190) at the current step.
   void output();
@@ -135,8 +137,15 @@ class FDModel : public Model {
   bool d_changedToSmall = false;
   bool d_changedBackToLarge = false;
   bool d_pendingCracks = false;
+  // inp::Policy::populateData (policy.cpp:90-97) for the deck's
+  // Memory_Control_Flag.
+  bool populate(const std::string& tag) const;
+
   std::vector<float> d_Z, d_phi;
   std::vector<double> d_energy;
+  std::vector<double> d_work, d_kinetic;     // u . f_ext, (1/2) rho v.v V
+  std::vector<double> d_fracE, d_fracEB;     // kept across outputs
+  std::vector<double> d_reaction, d_reactionLen;
   std::vector<uint8_t> d_fix;
 };
 

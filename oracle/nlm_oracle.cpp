@@ -870,6 +870,80 @@ struct FD {
     }
   }
 
+  // External force density at time t: FLoading::apply on zeros
+  // (fLoading.cpp:117-279), fDModel.cpp:655-662.
+  void external_force(double t, double* out) {
+    std::vector<P3> ff(size_t(n), P3{0., 0., 0.});
+    apply_fbc(t, ff);
+    for (int64_t i = 0; i < n; i++) {
+      out[3 * i] = ff[i].x;
+      out[3 * i + 1] = ff[i].y;
+      out[3 * i + 2] = ff[i].z;
+    }
+  }
+
+  // util::geometry::direction, utilGeom.cpp:317-326 (int determinant).
+  static int ref_direction(const P3& A, const P3& B, const P3& C) {
+    int val = int((B.y - A.y) * (C.x - B.x) - (B.x - A.x) * (C.y - B.y));
+    if (val == 0) return 0;
+    if (val < 0) return 2;
+    return 1;
+  }
+  // util::geometry::onLine, utilGeom.cpp:308-315.
+  static bool ref_on_line(const P3& A, const P3& B, const P3& C) {
+    return C.x <= std::max(A.x, B.x) && C.x <= std::min(A.x, B.x) &&
+           (C.y <= std::max(A.y, B.y) && C.y <= std::min(A.y, B.y));
+  }
+  // util::geometry::doLinesIntersect, utilGeom.cpp:281-306.
+  static bool ref_lines_intersect(const P3& A, const P3& B, const P3& C,
+                                  const P3& D) {
+    int dir1 = ref_direction(A, B, C);
+    int dir2 = ref_direction(A, B, D);
+    int dir3 = ref_direction(C, D, A);
+    int dir4 = ref_direction(C, D, B);
+    if (dir1 != dir2 && dir3 != dir4) return true;
+    if (dir1 == 0 && ref_on_line(A, B, C)) return true;
+    if (dir2 == 0 && ref_on_line(A, B, D)) return true;
+    if (dir3 == 0 && ref_on_line(C, D, A)) return true;
+    if (dir4 == 0 && ref_on_line(C, D, B)) return true;
+    return false;
+  }
+
+  // Reaction force of FDModel::computeForce (fDModel.cpp:548-580) with
+  // FDModel::is_reaction_force (fDModel.cpp:585-605), at the current state;
+  // box = {min_x, max_x, max_y}.  The bond bits are read, not changed.
+  void reaction_force(const double* box, double delta, double* R,
+                      double* R_len) {
+    const double min_x = box[0], max_x = box[1], max_y = box[2];
+    const double eps = 1e-6;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+      P3 r{0., 0., 0.};
+      const P3& xi = x[i];
+      for (int64_t k = row_ptr[i]; k < row_ptr[i + 1]; k++) {
+        int64_t jl = k - row_ptr[i];
+        int64_t j = col[k];
+        bool crosses = false;
+        if (xi.y > max_y - 1.5 * delta && xi.y < max_y - delta) {
+          P3 A{min_x, max_y - delta + eps, 0.};
+          P3 B{max_x, max_y - delta + eps, 0.};
+          crosses = ref_lines_intersect(A, B, xi, x[j]);
+        }
+        if (!crosses) continue;
+        bool fs = get_bond(i, jl);
+        P3 F;
+        double E;
+        bond_ef(m, J, xi, u[i], x[j], u[j], vol[i], vol[j], interior[i] != 0,
+                interior[j] != 0, fs, F, E);
+        r = r + vol[i] * F;  // R_i += V_i F (fDModel.cpp:572-574)
+      }
+      R[3 * i] = r.x;
+      R[3 * i + 1] = r.y;
+      R[3 * i + 2] = r.z;
+      R_len[i] = length(r);
+    }
+  }
+
   // FDModel::computePostProcFields damage/energy part, fDModel.cpp:652-751
   void postproc(float* Z, float* phi, double* energy) {
     double h = m.h;
@@ -1453,6 +1527,13 @@ void ora_fd_get_fixity(void* h, uint8_t* fix) {
 }
 double ora_fd_time(void* h) { return static_cast<FD*>(h)->time; }
 int64_t ora_fd_step_count(void* h) { return static_cast<FD*>(h)->nstep; }
+void ora_fd_external_force(void* h, double t, double* out) {
+  static_cast<FD*>(h)->external_force(t, out);
+}
+void ora_fd_reaction_force(void* h, const double* box, double delta,
+                           double* R, double* R_len) {
+  static_cast<FD*>(h)->reaction_force(box, delta, R, R_len);
+}
 void ora_fd_postproc(void* h, float* Z, float* phi, double* energy) {
   static_cast<FD*>(h)->postproc(Z, phi, energy);
 }

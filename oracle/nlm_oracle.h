@@ -169,6 +169,9 @@ int64_t ora_fd_step_count(void* h);
 /* computePostProcFields damage Z / phi / strain energy density
  * (fDModel.cpp:652-778): Z and phi stored as float like the reference. */
 void ora_fd_postproc(void* h, float* Z, float* phi, double* energy);
+void ora_fd_external_force(void* h, double t, double* out);
+void ora_fd_reaction_force(void* h, const double* box, double delta,
+                           double* R, double* R_len);
 /* per node sum_j |F_ij| at the current state (bond bits untouched) */
 void ora_fd_force_abs_sum(void* h, double* out);
 void ora_fd_set_time(void* h, double t, int64_t step);

@@ -125,6 +125,8 @@ SIGNATURES = {
     "ora_fd_time": (C.c_double, [_P]),
     "ora_fd_step_count": (C.c_int64, [_P]),
     "ora_fd_postproc": (None, [_P, _F, _F, _D]),
+    "ora_fd_external_force": (None, [_P, C.c_double, _D]),
+    "ora_fd_reaction_force": (None, [_P, _D, C.c_double, _D, _D]),
     "ora_fd_force_abs_sum": (None, [_P, _D]),
     "ora_fd_set_time": (None, [_P, C.c_double, C.c_int64]),
     "ora_interior_flags": (None, [C.c_int64, _D, _D, _D, C.c_double, C.c_int,
@@ -365,6 +367,19 @@ class OracleFD:
         self.lib.ora_fd_postproc(self.h, _p(Z, C.c_float), _p(phi, C.c_float),
                                  _p(e, C.c_double))
         return Z, phi, e
+
+    def external_force(self, t):
+        f = np.zeros((self.p.n, 3))
+        self.lib.ora_fd_external_force(self.h, float(t), _p(f, C.c_double))
+        return f
+
+    def reaction_force(self, box, delta):
+        b = np.ascontiguousarray(box, np.float64)
+        R = np.zeros((self.p.n, 3))
+        L = np.zeros(self.p.n)
+        self.lib.ora_fd_reaction_force(self.h, _p(b, C.c_double), float(delta),
+                                       _p(R, C.c_double), _p(L, C.c_double))
+        return R, L
 
 
 def bond_ef(mat, xi, ui, xj, uj, voli, volj, int_i=1, int_j=1, fs=0):

@@ -248,6 +248,45 @@ def test_postproc_parity(engine, oracle):
     assert np.max(np.abs(eg - er) / np.maximum(np.abs(er), 1e-300)) <= 1e-12
 
 
+@pytest.mark.parametrize("scale", ["metre", "millimetre"])
+def test_reaction_and_external_force_parity(engine, oracle, scale):
+    """Reaction force of computeForce / is_reaction_force and the external
+    force of FLoading::apply against the oracle on the same state (GPU state
+    copied into the oracle): metre-sized plate, where the reference's int
+    orientation test selects the bonds crossing the line, and millimetre
+    plate, where its truncation leaves only the onLine endpoint cases."""
+    progress(f"[gpu] reaction force ({scale})")
+    if scale == "metre":
+        p = pr.plate2d(30, 30, 0.5, 3, 1e-5, 20, notch_len=0.0, pull=0.5)
+        box = (0.0, 14.5, 14.5)
+    else:
+        p = pr.plate2d(30, 30, 1e-3, 3, 1e-7, 20, pull=10.0)
+        box = (0.0, 0.029, 0.029)
+    h = p.h
+    top = (p.shape[1] - 1) * h
+    fb = [pr.BC("rectangle", [1, 2], "linear", [2.5], "constant", [4.0],
+                x1=-h, x2=0.3 * top, y1=-h, y2=top + h)]
+    p = pr.with_(p, fbc=fb)
+    o = oracle.OracleFD(p)
+    with engine.GpuFD(p) as g:
+        g.init()
+        g.step(20)
+        u, v, f = g.get_state()
+        o.set_state(u=u, v=v, f=f)
+        o.set_bonds(g.get_bonds())
+        Rg, Lg = g.reaction_force(box, p.horizon)
+        fg = g.external_force(7.0e-6)
+    Rr, Lr = o.reaction_force(box, p.horizon)
+    fr = o.external_force(7.0e-6)
+    scale_r = max(np.abs(Rr).max(), 1e-300)
+    assert np.array_equal(Rg != 0, Rr != 0)
+    assert np.abs(Rg - Rr).max() <= 1e-12 * scale_r
+    assert np.abs(Lg - Lr).max() <= 1e-12 * scale_r
+    if scale == "metre":
+        assert np.count_nonzero(Lr) >= 30
+    assert np.array_equal(fg, fr) and np.abs(fr).max() > 0
+
+
 def test_verlet_parity(engine, oracle):
     progress("[gpu] velocity verlet")
     p = pr.plate2d(30, 30, 1e-3, 3, 1e-7, 40, pull=10.0)

@@ -16,6 +16,9 @@ Bars:
   - a 3D state-based deck (ElasticState with Critical_Stretch, .csv mesh,
     cuboid displacement sets, explicit dynamics) matches the oracle: relative
     L2 displacement error <= 1e-9 and identical bond bits;
+  - the post-processing tags (work done, strain / kinetic / fracture
+    energies and totals, reaction force and its integer-accumulated total)
+    equal the oracle's on the GPU's own state at each output;
   - the reference's quasi-static example deck (examples/qsModel/2D, state-
     based material, its mesh made by nlm_mesh) reproduces the reference-held
     result output_res.vtu within the reference test's own dc tolerance
@@ -343,3 +346,107 @@ def test_state_based_3d_deck_matches_oracle(host_bin, oracle, tmp_path):
     assert np.abs(u_ref[:, 2]).max() > 0
     assert np.linalg.norm(u - u_ref) <= 1e-9 * np.linalg.norm(u_ref)
     assert np.array_equal(out["Bond_State"], pack_bits(rp, bits))
+
+
+ENERGY_TAGS = """    - Neighbors
+    - Strain_Energy
+    - Work_Done
+    - Reaction_Force
+    - Total_Reaction_Force
+    - Fracture_Perienergy_Bond
+    - Fracture_Perienergy_Total
+    - Total_Energy
+    - Total_Fracture_Perienergy_Bond
+    - Total_Fracture_Perienergy_Total
+"""
+
+FORCE_BC = """Force_BC:
+  Sets: 1
+  Set_1:
+    Location:
+      Rectangle: [-0.001, -0.001, 0.0105, 0.032]
+    Direction: [1, 2]
+    Time_Function:
+      Type: linear
+      Parameters: [2.5e10]
+    Spatial_Function:
+      Type: constant
+      Parameters: [4.0]
+"""
+
+
+def unpack_bits(row_ptr, by):
+    bits = np.zeros(int(row_ptr[-1]), np.uint8)
+    off = 0
+    for i in range(len(row_ptr) - 1):
+        m = int(row_ptr[i + 1] - row_ptr[i])
+        for jl in range(m):
+            bits[row_ptr[i] + jl] = (by[off + jl // 8] >> (jl % 8)) & 1
+        off += (m + 7) // 8
+    return bits
+
+
+def test_postprocessing_tags_match_oracle(host_bin, oracle, tmp_path):
+    """Work done, kinetic, strain and fracture energies, reaction force and
+    their totals as model::Output writes them (model/util.cpp:74-164), from
+    computePostProcFields (fDModel.cpp:652-778) and computeForce's reaction
+    force (fDModel.cpp:548-605), against the oracle on the GPU's own state
+    at both outputs (fracture energies keep their last value where Z has
+    dropped below the threshold).  Total_Reaction_Force accumulates in int
+    as the reference's std::accumulate(..., 0) does."""
+    d = tmp_path / "run"
+    d.mkdir()
+    (tmp_path / "m.yaml").write_text(PLATE_MESH)
+    run_tool(host_bin / "nlm_mesh", "-i", "m.yaml", "-d", "2", cwd=tmp_path)
+    deck = PLATE_DECK.format(time="central_difference",
+                             mesh=tmp_path / "mesh.vtu", extra=FORCE_BC)
+    deck = deck.replace("    - Neighbors\n", ENERGY_TAGS)
+    (d / "input.yaml").write_text(deck)
+    run_deck(host_bin, d)
+    p = plate_problem(tmp_path)
+    p = pr.with_(p, fbc=[pr.BC("rectangle", [1, 2], "linear", [2.5e10],
+                               "constant", [4.0], x1=-0.001, x2=0.0105,
+                               y1=-0.001, y2=0.032)])
+    o = oracle.OracleFD(p)
+    rp = o.row_ptr
+    x = p.x
+    box = (min(0.0, x[:, 0].min()), x[:, 0].max(), x[:, 1].max())
+    fE = np.zeros(p.n)
+    for k, steps in ((1, 20), (2, 40)):
+        out = read_vtu(str(d / f"output_{k}.vtu"))
+        u, v = out["Displacement"], out["Velocity"]
+        o.set_state(u=u, v=v, f=np.zeros_like(u))
+        o.set_bonds(unpack_bits(rp, out["Bond_State"]))
+        Z, _, e = o.postproc()
+        t = steps * p.dt
+        W = np.einsum("ij,ij->i", u, o.external_force(t))
+        KE = 0.5 * 1200.0 * np.einsum("ij,ij->i", v, v) * p.vol
+        # definitelyGreaterThan(Z, 1 - 1e-10) on the stored (float) Z
+        hit = np.array([oracle.definitely_greater(float(z), 1.0 - 1.0e-10)
+                        for z in Z])
+        fE = np.where(hit, e, fE)
+        R, L = o.reaction_force(box, p.horizon)
+        acc = 0
+        for val in L:
+            acc = int(acc + val)
+        area = (abs(box[1] - box[0]) - p.horizon) * \
+            (abs(box[2] - 0.0) - p.horizon)
+        assert np.abs(W).max() > 0
+        np.testing.assert_allclose(out["Work_Done"], W, rtol=1e-12,
+                                   atol=1e-12 * np.abs(W).max())
+        np.testing.assert_allclose(out["Strain_Energy"], e, rtol=1e-12,
+                                   atol=1e-300)
+        np.testing.assert_allclose(out["Fracture_Perienergy_Total"], fE,
+                                   rtol=1e-12, atol=1e-300)
+        np.testing.assert_allclose(out["Fracture_Perienergy_Bond"], fE,
+                                   rtol=1e-12, atol=1e-300)
+        assert out["Total_Energy"][0] == pytest.approx(
+            e.sum() - W.sum() + KE.sum(), rel=1e-10)
+        assert out["Total_Fracture_Perienergy_Total"][0] == pytest.approx(
+            fE.sum(), rel=1e-10, abs=1e-300)
+        sr = max(np.abs(R).max(), 1e-300)
+        assert np.abs(out["Reaction_Force"] - R).max() <= 1e-12 * sr
+        assert out["Total_Reaction_Force"][0] == pytest.approx(acc * area,
+                                                               rel=1e-12)
+    assert (fE > 0).sum() > 0
+    o.close()

@@ -421,3 +421,78 @@ def test_dc_metric_known_answer(golden):
     exp_l2, exp_sup = d["expected_l2_sup"]
     assert float(f"{l2:.6g}") == pytest.approx(exp_l2, abs=5e-5)
     assert float(f"{sup:.6g}") == pytest.approx(exp_sup, abs=5e-7)
+
+
+# ------------------------------------------- reaction force, external force
+def test_reaction_force_selection(oracle):
+    """FDModel::is_reaction_force (fDModel.cpp:585-605) through
+    util::geometry::doLinesIntersect as written (utilGeom.cpp:281-326).  On
+    a 14.5 m plate (h = 0.5 m, horizon 1.5 m) the layer max_y - 1.5 horizon
+    < y < max_y - horizon is the row y = 12.5 and the line is y0 = 13.000001.
+    Worked by hand: a bond from that row to a row at y >= 13 counts -- for
+    the row y = 13 only because direction() truncates -14.5 (13 - y0) =
+    1.45e-5 to the int 0 (geometrically the bond stops short of the line);
+    bonds to rows y <= 12.5 have direction() 7 or more on both ends and do
+    not count.  So R_i = sum over neighbors with y_j >= 13 of V_i F_ij for
+    the row y = 12.5 and 0 elsewhere, each F_ij from the bond force."""
+    p = pr.plate2d(30, 30, 0.5, 3, 1e-5, 20, notch_len=0.0, pull=0.5,
+                   jitter=0.0)
+    o = oracle.OracleFD(p)
+    o.init()
+    o.step(20)
+    u, _, _ = o.get_state()
+    bits = o.get_bonds()
+    box = (0.0, 14.5, 14.5)
+    R, L = o.reaction_force(box, p.horizon)
+    m = oracle.rnp_params(p)
+    y = p.x[:, 1]
+    layer = (y > 14.5 - 2.25) & (y < 13.0)
+    assert layer.sum() == 30 and np.all(y[layer] == 12.5)
+    exp = np.zeros((p.n, 3))
+    counted = 0
+    for i in np.nonzero(layer)[0]:
+        for k in range(o.row_ptr[i], o.row_ptr[i + 1]):
+            j = o.col[k]
+            if y[j] < 12.9:
+                continue
+            F, _, _ = oracle.bond_ef(m, p.x[i], u[i], p.x[j], u[j],
+                                     p.vol[i], p.vol[j], fs=int(bits[k]))
+            exp[i] += p.vol[i] * F
+            counted += 1
+    assert counted > 200
+    assert np.abs(exp).max() > 0
+    np.testing.assert_allclose(R, exp, rtol=1e-13, atol=1e-13 * np.abs(exp).max())
+    np.testing.assert_allclose(L, np.linalg.norm(R, axis=1), rtol=1e-15)
+    # millimetre coordinates: every determinant truncates to 0, and onLine
+    # (C <= max and C <= min of each coordinate) admits only endpoints at or
+    # left of min_x below the line: nodes off the left edge give 0
+    q = pr.plate2d(30, 30, 1e-3, 3, 1e-7, 5, notch_len=0.0)
+    oq = oracle.OracleFD(q)
+    oq.init()
+    oq.step(5)
+    Rq, _ = oq.reaction_force((0.0, 0.029, 0.029), q.horizon)
+    assert np.all(Rq[q.x[:, 0] > 1e-5] == 0.0)
+
+
+def test_external_force_closed_form(oracle):
+    """FLoading::apply (fLoading.cpp:117-279): constant spatial function,
+    linear time function: f = params_space[0] t params_time[0] on the set's
+    nodes and directions, summed over sets."""
+    p = pr.plate2d(12, 12, 1e-3, 3, 1e-7, 1, notch_len=0.0, pull=0.0)
+    fb = [pr.BC("rectangle", [1, 2], "linear", [2.5], "constant", [4.0],
+                x1=-1e-4, x2=0.0035, y1=-1e-4, y2=0.0115),
+          pr.BC("rectangle", [2], "linear", [-1.5], "constant", [2.0],
+                x1=0.0025, x2=0.0115, y1=0.0055, y2=0.0115)]
+    p = pr.with_(p, fbc=fb)
+    o = oracle.OracleFD(p)
+    t = 3.0e-6
+    f = o.external_force(t)
+    x, y = p.x[:, 0], p.x[:, 1]
+    s1 = (x <= 0.0035) & (y <= 0.0115)
+    s2 = (x >= 0.0025) & (y >= 0.0055)
+    exp = np.zeros((p.n, 3))
+    exp[s1, 0] += 4.0 * t * 2.5
+    exp[s1, 1] += 4.0 * t * 2.5
+    exp[s2, 1] += 2.0 * t * -1.5
+    assert s1.sum() > 0 and s2.sum() > 0 and (s1 & s2).sum() > 0
+    np.testing.assert_allclose(f, exp, rtol=1e-15, atol=0)
