@@ -209,6 +209,11 @@ typedef struct {
 #define NLM_SFN_HAT_X 7
 #define NLM_SFN_HAT_Y 8
 #define NLM_SFN_HAT_Z 9
+/* Force BCs only: FLoading's line_load (fLoading.cpp:172-228), for each
+ * loaded direction d the factor floor((x_d - min_d)/h) / floor(|max_d -
+ * min_d|/h) over the set's extent in d (floor((max_d - x_d)/h) when
+ * space_params[0] == -1), h the mesh size given at setup. */
+#define NLM_SFN_LINE_LOAD 10
 
 typedef struct {
   int region;

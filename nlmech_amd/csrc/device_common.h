@@ -189,6 +189,23 @@ __device__ __forceinline__ double fbc_eval(const BcSetDev& s, double px,
       double a = sp[0];
       fmax = sp[0] * a * pz;
     } break;
+    case NLM_SFN_LINE_LOAD:
+      // fLoading.cpp:172-228: per loaded direction, the node's position in
+      // whole mesh steps over the extent's whole number of steps (size_t
+      // truncations as written; the reference's swap for params[0] == -1
+      // only shadows min and max, so the extent keeps its order)
+      for (int d = 0; d < 3; d++) {
+        if (!((s.dirmask >> d) & 1)) continue;
+        const double mn = d == 0 ? s.x1 : (d == 1 ? s.y1 : s.z1);
+        const double mx = d == 0 ? s.x2 : (d == 1 ? s.y2 : s.z2);
+        const double xc = d == 0 ? px : (d == 1 ? py : pz);
+        const uint64_t nodes = uint64_t(fabs(mx - mn) / s.h);
+        const double scale = 1. / double(nodes);
+        uint64_t pos = uint64_t((xc - mn) / s.h);
+        if (s.n_space_params > 0 && sp[0] == -1) pos = uint64_t((mx - xc) / s.h);
+        fmax *= double(pos) * scale;
+      }
+      break;
     case NLM_SFN_CONSTANT:
       fmax = s.n_space_params > 0 ? sp[0] : 1.0;
       break;

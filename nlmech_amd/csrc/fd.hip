@@ -1516,10 +1516,11 @@ extern "C" int nlm_fd_set_fbc(nlm_ctx* c, int nsets, const nlm_bc* sets) {
     if (b.n_time_params != ntp)
       return fail(c, NLM_ERR_INVALID,
                   "nlm_fd_set_fbc: wrong number of time parameters");
-    if (b.space_fn < 0 || b.space_fn > NLM_SFN_HAT_Z)
+    if (b.space_fn < 0 || b.space_fn > NLM_SFN_LINE_LOAD)
       return fail(c, NLM_ERR_INVALID,
                   "nlm_fd_set_fbc: unknown spatial function");
-    if (b.space_fn != NLM_SFN_CONSTANT && b.n_space_params < 1)
+    if (b.space_fn != NLM_SFN_CONSTANT && b.space_fn != NLM_SFN_LINE_LOAD &&
+        b.n_space_params < 1)
       return fail(c, NLM_ERR_INVALID,
                   "nlm_fd_set_fbc: spatial function needs a parameter");
     if (b.ndir < 1 || b.ndir > 3)
@@ -1528,6 +1529,9 @@ extern "C" int nlm_fd_set_fbc(nlm_ctx* c, int nsets, const nlm_bc* sets) {
       if (b.dir[d] < 1 || b.dir[d] > s.dim)
         return fail(c, NLM_ERR_INVALID, "nlm_fd_set_fbc: bad direction");
     s.bc_host.f[k] = to_dev(b);
+    // mesh size of FLoading's line_load
+    s.bc_host.f[k].h =
+        s.material == NLM_MATERIAL_ELASTIC_STATE ? s.es.h : s.rnp.h;
     for (int64_t i = 0; i < s.n; i++) {
       const double* p = &hx[3 * i];
       bool sel = false;

@@ -48,6 +48,7 @@ struct BcSetDev {
   double tp[4];
   double sp[4];
   double x1, x2, y1, y2, z1, z2;
+  double h;  // mesh size (line_load)
 };
 
 struct BcTable {

@@ -56,8 +56,8 @@ int time_code(const std::string& t) {
 int space_code(const std::string& s) {
   static const char* names[] = {"constant", "sin_x",    "sin_y", "sin_z",
                                 "linear_x", "linear_y", "linear_z",
-                                "hat_x",    "hat_y",    "hat_z"};
-  for (int k = 0; k < 10; k++)
+                                "hat_x",    "hat_y",    "hat_z", "line_load"};
+  for (int k = 0; k < 11; k++)
     if (s == names[k]) return k;
   deck_error("Error: Spatial function type " + s + " not recognized.");
 }

@@ -649,8 +649,8 @@ struct FD {
     }
   }
 
-  // FLoading::apply, src/loading/fLoading.cpp:117-279 (line_load omitted; a
-  // "constant" spatial function without parameters uses 1)
+  // FLoading::apply, src/loading/fLoading.cpp:117-279 (a "constant" spatial
+  // function without parameters uses 1)
   void apply_fbc(double t, std::vector<P3>& ff) {
     for (size_t s = 0; s < fbc.size(); s++) {
       const ora_bc& bc = fbc[s];
@@ -695,6 +695,22 @@ struct FD {
           } break;
           case ORA_SFN_CONSTANT:
             fmax = bc.n_space_params > 0 ? sp[0] : 1.0;
+            break;
+          case ORA_SFN_LINE_LOAD:
+            // fLoading.cpp:172-228; the block that would swap min and max
+            // for params[0] == -1 only declares shadowing locals
+            for (int k = 0; k < bc.ndir; k++) {
+              const int d = bc.dir[k];
+              double mn = d == 1 ? bc.x1 : (d == 2 ? bc.y1 : bc.z1);
+              double mx = d == 1 ? bc.x2 : (d == 2 ? bc.y2 : bc.z2);
+              double xd = d == 1 ? xx.x : (d == 2 ? xx.y : xx.z);
+              double length = std::abs(mx - mn);
+              size_t nodes = length / m.h;
+              double scale = 1. / nodes;
+              size_t pos = (xd - mn) / m.h;
+              if (bc.n_space_params > 0 && sp[0] == -1) pos = (mx - xd) / m.h;
+              fmax *= pos * scale;
+            }
             break;
           default:
             break;

@@ -92,7 +92,8 @@ enum {
   ORA_SFN_LINEAR_Z = 6,
   ORA_SFN_HAT_X = 7,
   ORA_SFN_HAT_Y = 8,
-  ORA_SFN_HAT_Z = 9
+  ORA_SFN_HAT_Z = 9,
+  ORA_SFN_LINE_LOAD = 10
 };
 typedef struct {
   int region;

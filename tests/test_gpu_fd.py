@@ -251,7 +251,8 @@ def test_postproc_parity(engine, oracle):
 @pytest.mark.parametrize("scale", ["metre", "millimetre"])
 def test_reaction_and_external_force_parity(engine, oracle, scale):
     """Reaction force of computeForce / is_reaction_force and the external
-    force of FLoading::apply against the oracle on the same state (GPU state
+    force of FLoading::apply (constant and line_load sets) against the
+    oracle on the same state (GPU state
     copied into the oracle): metre-sized plate, where the reference's int
     orientation test selects the bonds crossing the line, and millimetre
     plate, where its truncation leaves only the onLine endpoint cases."""
@@ -265,7 +266,9 @@ def test_reaction_and_external_force_parity(engine, oracle, scale):
     h = p.h
     top = (p.shape[1] - 1) * h
     fb = [pr.BC("rectangle", [1, 2], "linear", [2.5], "constant", [4.0],
-                x1=-h, x2=0.3 * top, y1=-h, y2=top + h)]
+                x1=-h, x2=0.3 * top, y1=-h, y2=top + h),
+          pr.BC("rectangle", [2], "linear", [1.5], "line_load", [-1.0],
+                x1=0.4 * top, x2=top + 0.5 * h, y1=-h, y2=0.5 * top)]
     p = pr.with_(p, fbc=fb)
     o = oracle.OracleFD(p)
     with engine.GpuFD(p) as g:

@@ -496,3 +496,31 @@ def test_external_force_closed_form(oracle):
     exp[s2, 1] += 2.0 * t * -1.5
     assert s1.sum() > 0 and s2.sum() > 0 and (s1 & s2).sum() > 0
     np.testing.assert_allclose(f, exp, rtol=1e-15, atol=0)
+
+
+def test_external_force_line_load(oracle):
+    """FLoading line_load (fLoading.cpp:172-228): the force of a node is
+    t params[0] floor((x - x1)/h) / floor((x2 - x1)/h) in the loaded direction
+    (worked here with integer grid positions, so the floors are exact), and
+    with space params [-1] floor((x2 - x)/h) instead."""
+    h = 1e-3
+    p = pr.plate2d(12, 12, h, 3, 1e-7, 1, notch_len=0.0, pull=0.0,
+                   jitter=0.0)
+    x1, x2 = -0.25e-3, 0.01075
+    sets = {"up": [1.0], "down": [-1.0]}
+    for name, sp in sets.items():
+        fb = [pr.BC("rectangle", [1], "linear", [3.0], "line_load", sp,
+                    x1=x1, x2=x2, y1=0.0045, y2=0.0065)]
+        q = pr.with_(p, fbc=fb)
+        o = oracle.OracleFD(q)
+        t = 2.0e-6
+        f = o.external_force(t)
+        sel = ((q.x[:, 1] >= 0.0045) & (q.x[:, 1] <= 0.0065) &
+               (q.x[:, 0] <= x2))
+        col = np.rint(q.x[:, 0] / h).astype(int)
+        nodes = 11   # floor(11.0 / 1) steps over [x1, x2]
+        pos = col if name == "up" else 10 - col   # floor((x - x1)/h) etc.
+        exp = np.zeros((q.n, 3))
+        exp[sel, 0] = t * 3.0 * pos[sel] / nodes
+        assert sel.sum() == 22
+        np.testing.assert_allclose(f, exp, rtol=1e-14, atol=1e-300)
