@@ -305,10 +305,16 @@ def test_verlet_parity(engine, oracle):
     assert rel_l2(ug, ur) <= 1e-9 and rel_l2(vg, vr) <= 1e-8
 
 
-def test_force_bc_nofail_contact_timed_crack(engine, oracle):
+@pytest.mark.parametrize("mode", [NLM_PREC_FP64_EXACT, NLM_PREC_FP64,
+                                  NLM_PREC_FP32_FORCE])
+def test_force_bc_nofail_contact_timed_crack(engine, oracle, mode):
     """Force BCs (hat / linear / sin time), a no-fail region, the contact
-    force of broken bonds and a crack activated during stepping."""
-    progress("[gpu] fbc + no-fail + contact + timed crack")
+    force of broken bonds and a crack activated during stepping, in every
+    precision mode (the fast modes take the no-fail and contact branches
+    through their exact evaluation): exact and FP64 modes to 1e-9 with
+    identical bonds, FP32-force within its 1e-5 bound and at most 0.1 % of
+    the broken bonds differing."""
+    progress(f"[gpu] fbc + no-fail + contact + timed crack, mode {mode}")
     p = pr.plate2d(36, 36, 1e-3, 3, 1e-7, 50, pull=20.0, no_penetration=True)
     p.fbc = [pr.BC("rectangle", [1], "linear", [4e9], "hat_y", [1.0],
                    x1=-1e-3, x2=2.5e-3, y1=-1e-3, y2=0.036),
@@ -324,13 +330,19 @@ def test_force_bc_nofail_contact_timed_crack(engine, oracle):
     o.init()
     o.step(50)
     ur, vr, fr = o.get_state()
-    with engine.GpuFD(p, precision=NLM_PREC_FP64_EXACT) as g:
+    with engine.GpuFD(p, precision=mode) as g:
         g.init()
         g.step(50)
         ug, vg, fg = g.get_state()
-        assert np.array_equal(g.get_bonds(), o.get_bonds())
-    assert rel_l2(ug, ur) <= 1e-9
-    assert o.get_bonds().sum() > 0
+        bg = g.get_bonds()
+    br = o.get_bonds()
+    assert br.sum() > 0
+    if mode == NLM_PREC_FP32_FORCE:
+        assert rel_l2(ug, ur) <= 1e-5
+        assert np.count_nonzero(bg != br) <= 1e-3 * br.sum()
+    else:
+        assert np.array_equal(bg, br)
+        assert rel_l2(ug, ur) <= 1e-9
 
 
 @pytest.mark.parametrize("mode", [NLM_PREC_FP64, NLM_PREC_FP32_FORCE])
