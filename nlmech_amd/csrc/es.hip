@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -82,6 +83,50 @@ __device__ __forceinline__ const EsRec& es_rec(const EsRec* smem, uint32_t j) {
                                          j * uint32_t(sizeof(EsRec)));
 }
 
+// The fast passes stage the halo as two arrays: FP64 position and
+// displacement (48 bytes a node) and the volume and force coefficient (in
+// FP32 in the FP32-force mode, which rounds both anyway).  The 12-dword
+// stride of the first spreads a wavefront's reads over four times as many
+// LDS bank groups as a 16-dword record, at no more LDS.
+struct alignas(16) EsXU {
+  double x[3];
+  double u[3];
+};
+template <class R>
+struct EsVP {
+  R vol;
+  R P;
+};
+template <int MODE>
+using EsVPOf =
+    EsVP<typename std::conditional<MODE == kModeFp32, float, double>::type>;
+
+template <bool WITH_P, class VP>
+__device__ __forceinline__ void es_stage_halo_split(
+    const FdArgs<3>& a, const double* __restrict__ P, int64_t lb, EsXU* xu,
+    VP* vp) {
+  const int64_t hb = a.halo_ptr[lb];
+  const int hn = int(a.halo_ptr[lb + 1] - hb);
+  for (int t = threadIdx.x; t < hn; t += 256) {
+    const uint32_t g = uint32_t(a.halo[hb + t]);
+    const double4 xx = ld32(a.x, g);
+    const double4 uu = ld32(a.u, g);
+    EsXU r;
+    r.x[0] = xx.x;
+    r.x[1] = xx.y;
+    r.x[2] = xx.z;
+    r.u[0] = uu.x;
+    r.u[1] = uu.y;
+    r.u[2] = uu.z;
+    xu[t] = r;
+    VP q;
+    q.vol = decltype(q.vol)(ld32(a.vol, g));
+    q.P = WITH_P ? decltype(q.P)(ld32(P, g)) : decltype(q.P)(0);
+    vp[t] = q;
+  }
+  __syncthreads();
+}
+
 // m_i = sum_j |X| |X| c_ij V_j in list order (weightedVolume, :51-67), over
 // the internal CSR whose rows keep the reference order.
 template <int DIM>
@@ -119,7 +164,15 @@ __global__ __launch_bounds__(256) void k_es_dil(
   const int64_t lb = xcd_block(nblk);
   if (lb >= nblk) return;
   extern __shared__ EsRec es_smem[];
-  if constexpr (TILED) es_stage_halo<false>(a, e.P, lb, es_smem);
+  // fast staged passes: split arrays (FP64 x, u; volume)
+  constexpr bool SPLIT = TILED && MODE != kModeExact;
+  typedef EsVPOf<MODE> VP;
+  EsXU* const es_xu = reinterpret_cast<EsXU*>(es_smem);
+  VP* const es_vp = reinterpret_cast<VP*>(es_xu + a.halo_cap);
+  if constexpr (SPLIT)
+    es_stage_halo_split<false>(a, e.P, lb, es_xu, es_vp);
+  else if constexpr (TILED)
+    es_stage_halo<false>(a, e.P, lb, es_smem);
   const int lane = threadIdx.x & 63;
   const int64_t s = lb * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (s >= a.nslices) return;  // whole wavefront
@@ -164,7 +217,14 @@ __global__ __launch_bounds__(256) void k_es_dil(
     unsigned long long w[U];
 #pragma unroll
     for (int q = 0; q < U; ++q) {
-      if constexpr (TILED) {
+      if constexpr (SPLIT) {
+        const EsXU& r = es_xu[jc[q]];
+        for (int d = 0; d < 3; d++) {
+          xj[q][d] = r.x[d];
+          uj[q][d] = r.u[d];
+        }
+        vj[q] = double(es_vp[jc[q]].vol);
+      } else if constexpr (TILED) {
         const EsRec& r = es_rec(es_smem, jc[q]);
         xj[q][0] = r.x0;
         xj[q][1] = r.x1;
@@ -246,7 +306,16 @@ __global__ __launch_bounds__(256) void k_es_force(
   const int64_t lb = xcd_block(nblk);
   if (lb >= nblk) return;
   extern __shared__ EsRec es_smem[];
-  if constexpr (TILED) es_stage_halo<true>(a, e.P, lb, es_smem);
+  // fast staged passes: split arrays (FP64 x, u; volume and force
+  // coefficient)
+  constexpr bool SPLIT = TILED && MODE != kModeExact;
+  typedef EsVPOf<MODE> VP;
+  EsXU* const es_xu = reinterpret_cast<EsXU*>(es_smem);
+  VP* const es_vp = reinterpret_cast<VP*>(es_xu + a.halo_cap);
+  if constexpr (SPLIT)
+    es_stage_halo_split<true>(a, e.P, lb, es_xu, es_vp);
+  else if constexpr (TILED)
+    es_stage_halo<true>(a, e.P, lb, es_smem);
   const int lane = threadIdx.x & 63;
   const int64_t s = lb * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (s >= a.nslices) return;
@@ -295,7 +364,18 @@ __global__ __launch_bounds__(256) void k_es_force(
 #pragma unroll
     for (int q = 0; q < U; ++q) {
       const uint32_t j = jc[q];
-      if constexpr (TILED) {
+      if constexpr (SPLIT) {
+        const EsXU& r = es_xu[j];
+        for (int d = 0; d < 3; d++) {
+          xj[q][d] = r.x[d];
+          uj[q][d] = r.u[d];
+        }
+        const VP vpj = es_vp[j];
+        vj[q] = double(vpj.vol);
+        Pj[q] = double(vpj.P);
+        Qj[q] = 0.;
+        thj[q] = 0.;
+      } else if constexpr (TILED) {
         const EsRec& r = es_rec(es_smem, j);
         xj[q][0] = r.x0;
         xj[q][1] = r.x1;
@@ -431,9 +511,14 @@ EsArgs<DIM> make_es_args(nlm_ctx* c) {
   return e;
 }
 
-// LDS bytes of a tiled launch: one record per halo entry.
+// LDS bytes of a tiled launch: one record per halo entry (the fast passes:
+// the two split arrays; the exact pass 1 keeps one record).
 size_t es_lds_bytes(const FdState& s) {
-  return size_t(s.halo_max + 1) * sizeof(EsRec);
+  const size_t fast = s.precision == kModeFp32
+                          ? sizeof(EsXU) + sizeof(EsVPOf<kModeFp32>)
+                          : sizeof(EsXU) + sizeof(EsVPOf<kModeFp64>);
+  return size_t(s.halo_max + 1) *
+         (s.precision == kModeExact ? sizeof(EsRec) : fast);
 }
 
 // Raises a kernel's dynamic-LDS limit to what the launch needs (the first
