@@ -81,7 +81,7 @@ using HaloRecOf =
 
 template <int DIM, int MODE, int KIND, bool BASE_F, int INF, int U,
           bool NOFAIL, bool TILED>
-__global__ __launch_bounds__(256) void k_fd(
+__device__ __forceinline__ void fd_step_body(
     FdArgs<DIM> a, const unsigned long long* __restrict__ bits_ro,
     const unsigned long long* __restrict__ vex_ro, double t_force,
     double t_next) {
@@ -376,11 +376,17 @@ __global__ __launch_bounds__(256) void k_fd(
         if (DIM == 3) acc[2] += double(pf[m][2]);
       }
     }
-    // a slot's new bits (rare): every lane stores the same word, so the
-    // test is a scalar branch
+    // new broken bits (rare): one scalar test per chunk; every lane stores
+    // the same word
+    unsigned long long nb_any = 0ull;
 #pragma unroll
-    for (int q = 0; q < U; ++q)
-      if (nb[q] != 0ull && k0 + q < width) a.bits[bb + k0 + q] = w[q] | nb[q];
+    for (int q = 0; q < U; ++q) nb_any |= nb[q];
+    if (nb_any != 0ull) {
+#pragma unroll
+      for (int q = 0; q < U; ++q)
+        if (nb[q] != 0ull && k0 + q < width)
+          a.bits[bb + k0 + q] = w[q] | nb[q];
+    }
   }
   if (!valid) return;
   double f[3] = {0., 0., 0.};
@@ -415,6 +421,31 @@ __global__ __launch_bounds__(256) void k_fd(
   }
   a.u_out[i] = make_v<DIM>(un);
   a.v_out[i] = make_v<DIM>(vn);
+}
+
+// The fused step kernel (one integrateCD iteration or a force evaluation).
+template <int DIM, int MODE, int KIND, bool BASE_F, int INF, int U,
+          bool NOFAIL, bool TILED>
+__global__ __launch_bounds__(256) void k_fd(
+    FdArgs<DIM> a, const unsigned long long* __restrict__ bits_ro,
+    const unsigned long long* __restrict__ vex_ro, double t_force,
+    double t_next) {
+  fd_step_body<DIM, MODE, KIND, BASE_F, INF, U, NOFAIL, TILED>(
+      a, bits_ro, vex_ro, t_force, t_next);
+}
+
+// The same kernel held to at least 6 wavefronts per SIMD (at most 80
+// vector registers): the FP32-force staged kernel with two slots per chunk
+// fits without spilling, and the compiler's own register choice can miss
+// that occupancy step by a register.
+template <int DIM, int MODE, int KIND, bool BASE_F, int INF, int U,
+          bool NOFAIL, bool TILED>
+__global__ __launch_bounds__(256, 6) void k_fd_w6(
+    FdArgs<DIM> a, const unsigned long long* __restrict__ bits_ro,
+    const unsigned long long* __restrict__ vex_ro, double t_force,
+    double t_next) {
+  fd_step_body<DIM, MODE, KIND, BASE_F, INF, U, NOFAIL, TILED>(
+      a, bits_ro, vex_ro, t_force, t_next);
 }
 
 // Update from a stored force (the first iteration of a stepping call):
@@ -805,6 +836,12 @@ void launch_t(nlm_ctx* c, const FdArgs<DIM>& a, double tf, double tn) {
   if constexpr (DIM == 2) {
     if (c->fd.tiled && c->fd.use_tiled) {
       size_t lds = size_t(a.halo_cap) * sizeof(HaloRecOf<MODE>);
+      if constexpr (MODE == kModeFp32 && U == 2) {
+        hipLaunchKernelGGL(
+            (k_fd_w6<DIM, MODE, KIND, BASE_F, INF, U, NOFAIL, true>), dim3(g),
+            dim3(256), lds, c->stream, a, bits_ro, a.vexw, tf, tn);
+        return;
+      }
       hipLaunchKernelGGL((k_fd<DIM, MODE, KIND, BASE_F, INF, U, NOFAIL, true>),
                          dim3(g), dim3(256), lds, c->stream, a, bits_ro,
                          a.vexw, tf, tn);
@@ -1154,7 +1191,7 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
     hipFuncAttributes fa;
     if (hipFuncGetAttributes(
             &fa, reinterpret_cast<const void*>(
-                     &k_fd<2, kModeFp32, KIND_CD, false, 0, 2, false, true>)) !=
+                     &k_fd_w6<2, kModeFp32, KIND_CD, false, 0, 2, false, true>)) !=
             hipSuccess ||
         fa.sharedSizeBytes != 0)
       s.tiled = false;
