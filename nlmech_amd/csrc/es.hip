@@ -241,6 +241,85 @@ __global__ __launch_bounds__(256) void k_es_dil(
       const int kq = k0 + q;
       w[q] = bits_in[kq < wlast ? kq : wlast];
     }
+    if constexpr (MODE == kModeFp32) {
+      // FP32-force: branch-free common path; decisions are votes on single
+      // comparisons (their lane masks) combined by scalar mask arithmetic;
+      // bonds inside a guard band (break test, volume-correction edge) are
+      // evaluated exactly after the chunk; dilatation terms summed in FP32
+      // over slot pairs (2m, 2m+1), each pair added to the FP64 sum.
+      float pf[(U + 1) / 2];
+      unsigned long long nb[U], band[U], ex[U];
+      unsigned long long ex_any = 0ull;
+      const float vtol = e.m.vex_tol_f;
+#pragma unroll
+      for (int q = 0; q < U; ++q) {
+        const unsigned long long act = __builtin_amdgcn_ballot_w64(k0 + q < len);
+        float dx[3], du[3];
+        for (int d = 0; d < 3; d++) {
+          dx[d] = d < DIM ? float(xj[q][d] - xi[d]) : 0.f;
+          du[d] = d < DIM ? float(uj[q][d] - ui[d]) : 0.f;
+        }
+        float dd = __builtin_fmaf(dx[1], dx[1], dx[0] * dx[0]);
+        float g = __builtin_fmaf(du[1], __builtin_fmaf(2.f, dx[1], du[1]),
+                                 du[0] * __builtin_fmaf(2.f, dx[0], du[0]));
+        if (DIM == 3) {
+          dd = __builtin_fmaf(dx[2], dx[2], dd);
+          g = __builtin_fmaf(du[2], __builtin_fmaf(2.f, dx[2], du[2]), g);
+        }
+        const float rx = __builtin_amdgcn_rsqf(dd);
+        const float lx = dd * rx;
+        const float yy = dd + g;
+        const float ly = yy * __builtin_amdgcn_rsqf(yy);
+        const float ext = g * __builtin_amdgcn_rcpf(ly + lx);
+        const float st = ext * rx;
+        const unsigned long long v_vex =
+            __builtin_amdgcn_ballot_w64(fabsf(lx - e.m.low_band_f) <= vtol);
+        const unsigned long long v_hi =
+            __builtin_amdgcn_ballot_w64(st > e.m.brk_hi_f);
+        const unsigned long long v_lo =
+            __builtin_amdgcn_ballot_w64(st >= e.m.brk_lo_f);
+        const unsigned long long cand = act & ~w[q];
+        nb[q] = cand & v_hi;
+        band[q] = cand & v_lo & ~v_hi;
+        ex[q] = (v_vex & act) | band[q];
+        const unsigned long long keep = act & ~(w[q] | nb[q]) & ~ex[q];
+        const float c = fminf((e.m.check_up_f - lx) * e.m.rcp_h_f, 1.0f);
+        float dterm = c3f * lx * ext * c * float(vj[q]);
+        dterm = __builtin_amdgcn_inverse_ballot_w64(keep) ? dterm : 0.f;
+        if (q % 2 == 0) pf[q / 2] = dterm;
+        else pf[q / 2] += dterm;
+        ex_any |= ex[q];
+      }
+      if (ex_any != 0ull) {  // rare: whole wavefronts skip it
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+          if (ex[q] == 0ull) continue;
+          bool fsn = false;
+          if (__builtin_amdgcn_inverse_ballot_w64(ex[q])) {
+            const bool fs0 = __builtin_amdgcn_inverse_ballot_w64(w[q]);
+            bool fs = fs0;
+            const double d = es_dil_exact(e.m, xi, ui, xj[q], uj[q], vj[q],
+                                          c3, fs);
+            fsn = fs && !fs0;
+            if (!fs) pf[q / 2] += float(d);
+          }
+          // the guard band's bonds were decided exactly here
+          nb[q] |= band[q] & __builtin_amdgcn_ballot_w64(fsn);
+        }
+      }
+#pragma unroll
+      for (int mm = 0; mm < (U + 1) / 2; ++mm) th += double(pf[mm]);
+      unsigned long long nb_any = 0ull;
+#pragma unroll
+      for (int q = 0; q < U; ++q) nb_any |= nb[q];
+      if (nb_any != 0ull) {
+#pragma unroll
+        for (int q = 0; q < U; ++q)
+          if (nb[q] != 0ull && k0 + q < width)
+            a.bits[bb + k0 + q] = w[q] | nb[q];
+      }
+      continue;
+    }
 #pragma unroll
     for (int q = 0; q < U; ++q) {
       const bool act = k0 + q < len;
@@ -398,6 +477,7 @@ __global__ __launch_bounds__(256) void k_es_force(
       const int kq = k0 + q;
       w[q] = bits_in[kq < wlast ? kq : wlast];
     }
+    float pf[(U + 1) / 2][3];  // FP32-force: slot-pair sums
 #pragma unroll
     for (int q = 0; q < U; ++q) {
       const bool act = k0 + q < len;
@@ -447,6 +527,11 @@ __global__ __launch_bounds__(256) void k_es_force(
         sr = keep ? sr : 0.;
         for (int d = 0; d < DIM; d++) acc[d] = __builtin_fma(sr, Y[d], acc[d]);
       } else {
+        // FP32-force: the volume-correction edge test is a vote on a single
+        // comparison (its lane mask), the forces are summed in FP32 over
+        // slot pairs (2m, 2m+1) and each pair is added to the FP64 sum
+        const unsigned long long live =
+            __builtin_amdgcn_ballot_w64(k0 + q < len) & ~w[q];
         float dx[3], du[3], Y[3];
         for (int d = 0; d < 3; d++) {
           dx[d] = d < DIM ? float(xj[q][d] - xi[d]) : 0.f;
@@ -462,11 +547,13 @@ __global__ __launch_bounds__(256) void k_es_force(
         const float ry = __builtin_amdgcn_rsqf(yy);
         const float rx = __builtin_amdgcn_rsqf(dd);
         const float lx = dd * rx;
-        const bool vex = keep && fabsf(lx - e.m.low_band_f) <=
-                                     3e-5f * e.m.low_band_f;
+        const unsigned long long vexm =
+            __builtin_amdgcn_ballot_w64(fabsf(lx - e.m.low_band_f) <=
+                                        3e-5f * e.m.low_band_f) &
+            live;
         float c = fminf((e.m.check_up_f - lx) * e.m.rcp_h_f, 1.0f);
-        if (__ballot(vex) != 0ull) {
-          if (vex) {
+        if (vexm != 0ull) {  // rare: lengths at the correction's edge
+          if (__builtin_amdgcn_inverse_ballot_w64(vexm)) {
             double X0 = xj[q][0] - xi[0], X1 = xj[q][1] - xi[1];
             double X2 = DIM == 3 ? xj[q][2] - xi[2] : 0.;
             c = float(es_corr_exact(e.m, sqrt(X0 * X0 + X1 * X1 + X2 * X2)));
@@ -480,9 +567,17 @@ __global__ __launch_bounds__(256) void k_es_force(
           t = __builtin_fmaf(Qif + float(Qj[q]), ext, t);
         }
         float sr = t * c * float(vj[q]) * ry;
-        sr = keep ? sr : 0.f;
-        for (int d = 0; d < DIM; d++) acc[d] += double(sr * Y[d]);
+        sr = __builtin_amdgcn_inverse_ballot_w64(live) ? sr : 0.f;
+        for (int d = 0; d < DIM; d++) {
+          if (q % 2 == 0) pf[q / 2][d] = sr * Y[d];
+          else pf[q / 2][d] += sr * Y[d];
+        }
       }
+    }
+    if constexpr (MODE == kModeFp32) {
+#pragma unroll
+      for (int mm = 0; mm < (U + 1) / 2; ++mm)
+        for (int d = 0; d < DIM; d++) acc[d] += double(pf[mm][d]);
     }
   }
   if (!valid) return;

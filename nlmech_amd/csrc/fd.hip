@@ -27,6 +27,7 @@
 #include <string>
 #include <vector>
 
+#include "bond_es.h"
 #include "bond_rnp.h"
 #include "device_common.h"
 #include "fd_common.h"
@@ -1425,6 +1426,11 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
     e.low_band_f = float(e.low_band);
     e.rcp_h_f = float(e.rcp_h);
     e.sc_break_f = float(e.sc_break);
+    e.vex_tol_f = 3e-5f * e.low_band_f;
+    e.brk_lo_f = e.breaking ? e.sc_break_f - kEsGuard32 * e.sc_break_f
+                            : std::numeric_limits<float>::infinity();
+    e.brk_hi_f = e.breaking ? e.sc_break_f + kEsGuard32 * e.sc_break_f
+                            : std::numeric_limits<float>::infinity();
     if ((rc = es_prepare(c))) return rc;
   }
   if (!s.ev0) NLM_HIP(c, hipEventCreate(&s.ev0));

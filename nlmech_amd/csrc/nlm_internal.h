@@ -112,6 +112,9 @@ struct EsDev {
   int breaking;           // sc > 0
   int pad;
   float check_up_f, low_band_f, rcp_h_f, sc_break_f;
+  // FP32 pass 1: half-width of the volume-correction edge band; bounds of
+  // the break test's guard band on e/|X| (+inf when bonds never break)
+  float vex_tol_f, brk_lo_f, brk_hi_f;
 };
 
 // Slice layout of the neighbor lists: 64 consecutive nodes (one wavefront)
