@@ -318,8 +318,8 @@ def main():
         copy_probe()
         return
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not "
-              "supported (the sharding design is described in DESIGN.md)",
+        print(f"bench.py: the engine runs on one GPU; --gpus {a.gpus} is not "
+              "supported (multi-GPU sharding is designed but not built)",
               file=sys.stderr)
         sys.exit(2)
     from nlmech_amd import problems as pr
