@@ -171,7 +171,6 @@ __device__ __forceinline__ void fd_step_body(
   // (padded like the bond bits)
   const unsigned long long* __restrict__ vex_in = vex_ro + bb;
   double acc[3] = {0., 0., 0.};
-  float accf[3] = {0.f, 0.f, 0.f};
   // Neighbor indices are fetched one chunk of U slots ahead of the chunk
   // whose positions and displacements are gathered, so the index latency
   // overlaps the previous chunk's arithmetic.  Every load is unconditional:
@@ -275,10 +274,9 @@ __device__ __forceinline__ void fd_step_body(
       // decision inside a guard band, or a contact force, are left out of
       // the common path and evaluated after the chunk.  Forces are summed in
       // pairs of slots (2m, 2m+1) -- in FP32 in the FP32-force mode -- and
-      // the pairs are added to the row's sum in order (FP64; FP32 in the
-      // FP32-force mode): a two-term sum does not depend on the order of its
-      // terms, and the chunk widths are even, so neither the chunk width nor
-      // the deferred evaluation changes the result.
+      // the pairs are added to the FP64 accumulator in order: a two-term sum
+      // does not depend on the order of its terms, so neither the chunk
+      // width nor the deferred evaluation changes the result.
       typedef typename std::conditional<MODE == kModeFp32, float, double>::type
           Real;
       constexpr int P = (U + 1) / 2;
@@ -374,17 +372,9 @@ __device__ __forceinline__ void fd_step_body(
       }
 #pragma unroll
       for (int m = 0; m < P; ++m) {
-        if constexpr (MODE == kModeFp32) {
-          // FP32-force: the pairs are summed in FP32 along the row, in
-          // slot order whatever the chunk width
-          accf[0] += pf[m][0];
-          accf[1] += pf[m][1];
-          if (DIM == 3) accf[2] += pf[m][2];
-        } else {
-          acc[0] += double(pf[m][0]);
-          acc[1] += double(pf[m][1]);
-          if (DIM == 3) acc[2] += double(pf[m][2]);
-        }
+        acc[0] += double(pf[m][0]);
+        acc[1] += double(pf[m][1]);
+        if (DIM == 3) acc[2] += double(pf[m][2]);
       }
     }
     // new broken bits (rare): one scalar test per chunk; every lane stores
@@ -400,8 +390,6 @@ __device__ __forceinline__ void fd_step_body(
     }
   }
   if (!valid) return;
-  if constexpr (MODE == kModeFp32)
-    for (int d = 0; d < DIM; d++) acc[d] = double(accf[d]);
   double f[3] = {0., 0., 0.};
   if (BASE_F) {
     load3<DIM>(a.f_in[i], f);
