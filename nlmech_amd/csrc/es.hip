@@ -241,51 +241,70 @@ __global__ __launch_bounds__(256) void k_es_dil(
       const int kq = k0 + q;
       w[q] = bits_in[kq < wlast ? kq : wlast];
     }
-    if constexpr (MODE == kModeFp32) {
-      // FP32-force: branch-free common path; decisions are votes on single
+    if constexpr (MODE != kModeExact) {
+      // Fast modes: branch-free common path; decisions are votes on single
       // comparisons (their lane masks) combined by scalar mask arithmetic;
       // bonds inside a guard band (break test, volume-correction edge) are
-      // evaluated exactly after the chunk; dilatation terms summed in FP32
-      // over slot pairs (2m, 2m+1), each pair added to the FP64 sum.
-      float pf[(U + 1) / 2];
+      // evaluated exactly after the chunk; dilatation terms summed over slot
+      // pairs (2m, 2m+1) -- in FP32 in the FP32-force mode -- each pair
+      // added to the FP64 sum.
+      typedef typename std::conditional<MODE == kModeFp32, float, double>::type
+          Real;
+      constexpr bool F32 = MODE == kModeFp32;
+      auto fmaR = [](Real x, Real y, Real z) -> Real {
+        if constexpr (F32) return __builtin_fmaf(x, y, z);
+        else return __builtin_fma(x, y, z);
+      };
+      Real pf[(U + 1) / 2];
       unsigned long long nb[U], band[U], ex[U];
       unsigned long long ex_any = 0ull;
-      const float vtol = e.m.vex_tol_f;
+      const Real vtol = F32 ? Real(e.m.vex_tol_f) : Real(e.m.vex_tol);
+      const Real lowb = F32 ? Real(e.m.low_band_f) : Real(e.m.low_band);
+      const Real bhi = F32 ? Real(e.m.brk_hi_f) : Real(e.m.brk_hi);
+      const Real blo = F32 ? Real(e.m.brk_lo_f) : Real(e.m.brk_lo);
+      const Real cup = F32 ? Real(e.m.check_up_f) : Real(e.m.check_up);
+      const Real rch = F32 ? Real(e.m.rcp_h_f) : Real(e.m.rcp_h);
+      const Real c3r = F32 ? Real(c3f) : Real(c3);
 #pragma unroll
       for (int q = 0; q < U; ++q) {
         const unsigned long long act = __builtin_amdgcn_ballot_w64(k0 + q < len);
-        float dx[3], du[3];
+        Real dx[3], du[3];
         for (int d = 0; d < 3; d++) {
-          dx[d] = d < DIM ? float(xj[q][d] - xi[d]) : 0.f;
-          du[d] = d < DIM ? float(uj[q][d] - ui[d]) : 0.f;
+          dx[d] = d < DIM ? Real(xj[q][d] - xi[d]) : Real(0);
+          du[d] = d < DIM ? Real(uj[q][d] - ui[d]) : Real(0);
         }
-        float dd = __builtin_fmaf(dx[1], dx[1], dx[0] * dx[0]);
-        float g = __builtin_fmaf(du[1], __builtin_fmaf(2.f, dx[1], du[1]),
-                                 du[0] * __builtin_fmaf(2.f, dx[0], du[0]));
+        Real dd = fmaR(dx[1], dx[1], dx[0] * dx[0]);
+        Real g = fmaR(du[1], fmaR(Real(2), dx[1], du[1]),
+                      du[0] * fmaR(Real(2), dx[0], du[0]));
         if (DIM == 3) {
-          dd = __builtin_fmaf(dx[2], dx[2], dd);
-          g = __builtin_fmaf(du[2], __builtin_fmaf(2.f, dx[2], du[2]), g);
+          dd = fmaR(dx[2], dx[2], dd);
+          g = fmaR(du[2], fmaR(Real(2), dx[2], du[2]), g);
         }
-        const float rx = __builtin_amdgcn_rsqf(dd);
-        const float lx = dd * rx;
-        const float yy = dd + g;
-        const float ly = yy * __builtin_amdgcn_rsqf(yy);
-        const float ext = g * __builtin_amdgcn_rcpf(ly + lx);
-        const float st = ext * rx;
+        Real rx, ly, ext;
+        const Real yy = dd + g;
+        if constexpr (F32) {
+          rx = __builtin_amdgcn_rsqf(dd);
+          ly = yy * __builtin_amdgcn_rsqf(yy);
+        } else {
+          rx = rsqrt_f64(dd);
+          ly = yy * rsqrt_f64(yy);
+        }
+        const Real lx = dd * rx;
+        if constexpr (F32) ext = g * __builtin_amdgcn_rcpf(ly + lx);
+        else ext = g / (ly + lx);
+        const Real st = ext * rx;
         const unsigned long long v_vex =
-            __builtin_amdgcn_ballot_w64(fabsf(lx - e.m.low_band_f) <= vtol);
-        const unsigned long long v_hi =
-            __builtin_amdgcn_ballot_w64(st > e.m.brk_hi_f);
-        const unsigned long long v_lo =
-            __builtin_amdgcn_ballot_w64(st >= e.m.brk_lo_f);
+            __builtin_amdgcn_ballot_w64(fabs(lx - lowb) <= vtol);
+        const unsigned long long v_hi = __builtin_amdgcn_ballot_w64(st > bhi);
+        const unsigned long long v_lo = __builtin_amdgcn_ballot_w64(st >= blo);
         const unsigned long long cand = act & ~w[q];
         nb[q] = cand & v_hi;
         band[q] = cand & v_lo & ~v_hi;
         ex[q] = (v_vex & act) | band[q];
         const unsigned long long keep = act & ~(w[q] | nb[q]) & ~ex[q];
-        const float c = fminf((e.m.check_up_f - lx) * e.m.rcp_h_f, 1.0f);
-        float dterm = c3f * lx * ext * c * float(vj[q]);
-        dterm = __builtin_amdgcn_inverse_ballot_w64(keep) ? dterm : 0.f;
+        const Real c = fmin((cup - lx) * rch, Real(1));
+        Real dterm = c3r * lx * ext * c * Real(vj[q]);
+        dterm = __builtin_amdgcn_inverse_ballot_w64(keep) ? dterm : Real(0);
         if (q % 2 == 0) pf[q / 2] = dterm;
         else pf[q / 2] += dterm;
         ex_any |= ex[q];
@@ -301,7 +320,7 @@ __global__ __launch_bounds__(256) void k_es_dil(
             const double d = es_dil_exact(e.m, xi, ui, xj[q], uj[q], vj[q],
                                           c3, fs);
             fsn = fs && !fs0;
-            if (!fs) pf[q / 2] += float(d);
+            if (!fs) pf[q / 2] += Real(d);
           }
           // the guard band's bonds were decided exactly here
           nb[q] |= band[q] & __builtin_amdgcn_ballot_w64(fsn);

@@ -1427,6 +1427,11 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
     e.rcp_h_f = float(e.rcp_h);
     e.sc_break_f = float(e.sc_break);
     e.vex_tol_f = 3e-5f * e.low_band_f;
+    e.vex_tol = kEsGuard64 * e.low_band;
+    e.brk_lo = e.breaking ? e.sc_break - kEsGuard64 * e.sc_break
+                          : std::numeric_limits<double>::infinity();
+    e.brk_hi = e.breaking ? e.sc_break + kEsGuard64 * e.sc_break
+                          : std::numeric_limits<double>::infinity();
     e.brk_lo_f = e.breaking ? e.sc_break_f - kEsGuard32 * e.sc_break_f
                             : std::numeric_limits<float>::infinity();
     e.brk_hi_f = e.breaking ? e.sc_break_f + kEsGuard32 * e.sc_break_f

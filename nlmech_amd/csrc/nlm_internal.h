@@ -115,6 +115,8 @@ struct EsDev {
   // FP32 pass 1: half-width of the volume-correction edge band; bounds of
   // the break test's guard band on e/|X| (+inf when bonds never break)
   float vex_tol_f, brk_lo_f, brk_hi_f;
+  // the same bounds in FP64 for the FP64 mode
+  double vex_tol, brk_lo, brk_hi;
 };
 
 // Slice layout of the neighbor lists: 64 consecutive nodes (one wavefront)
