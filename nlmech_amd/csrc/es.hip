@@ -21,6 +21,8 @@
 
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <utility>
@@ -635,16 +637,21 @@ size_t es_lds_bytes(const FdState& s) {
          (s.precision == kModeExact ? sizeof(EsRec) : fast);
 }
 
-// Raises a kernel's dynamic-LDS limit to what the launch needs (the first
-// time the kernel is launched with that size or more).
-template <class K>
-void allow_lds(K kernel, size_t bytes) {
-  static size_t granted = 0;
-  if (bytes > granted) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+// Raises a kernel's dynamic-LDS limit to what the launch needs.  The grant
+// is kept per kernel (keyed by its address: every template instance has its
+// own attribute) and per device.
+void allow_lds(const void* kernel, size_t bytes) {
+  static std::mutex mu;
+  static std::map<std::pair<int, const void*>, size_t> granted;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lock(mu);
+  size_t& g = granted[{dev, kernel}];
+  if (bytes > g) {
+    (void)hipFuncSetAttribute(kernel,
                               hipFuncAttributeMaxDynamicSharedMemorySize,
                               int(bytes));
-    granted = bytes;
+    g = bytes;
   }
 }
 
@@ -654,7 +661,8 @@ void launch_dil(nlm_ctx* c, const EsArgs<DIM>& e) {
   if constexpr (DIM == 3) {
     if (c->fd.es_tiled && c->fd.use_tiled) {
       const size_t lds = es_lds_bytes(c->fd);
-      allow_lds(k_es_dil<DIM, MODE, 4, true>, lds);
+      allow_lds(reinterpret_cast<const void*>(k_es_dil<DIM, MODE, 4, true>),
+                lds);
       hipLaunchKernelGGL((k_es_dil<DIM, MODE, 4, true>), dim3(g), dim3(256),
                          lds, c->stream, e, e.f.bits);
       return;
@@ -670,7 +678,9 @@ void launch_force_k(nlm_ctx* c, const EsArgs<DIM>& e, double tf, double tn) {
   if constexpr (DIM == 3 && MODE != kModeExact) {
     if (c->fd.es_tiled && c->fd.use_tiled) {
       const size_t lds = es_lds_bytes(c->fd);
-      allow_lds(k_es_force<DIM, MODE, KIND, BASE_F, U, true>, lds);
+      allow_lds(reinterpret_cast<const void*>(
+                    k_es_force<DIM, MODE, KIND, BASE_F, U, true>),
+                lds);
       hipLaunchKernelGGL((k_es_force<DIM, MODE, KIND, BASE_F, U, true>),
                          dim3(g), dim3(256), lds, c->stream, e, e.f.bits, tf,
                          tn);

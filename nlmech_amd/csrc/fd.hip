@@ -62,7 +62,7 @@ struct alignas(16) HaloRec {
   float cv;  // FP32-force mode: vol * c4 (* J when constant), in FP32
 };
 
-// The FP32-force mode's record (32 bytes): position and displacement
+// The FP32-force mode's record (24 bytes): position and displacement
 // relative to the block's origins in FP32, so a bond's differences are two
 // FP32 subtractions per component.  Positions within a block and its halo
 // lie within about (16 + 2 horizon/h) h of the origin, so the relative
@@ -361,7 +361,10 @@ __device__ __forceinline__ void fd_step_body(
             bond_mode<DIM, MODE, INF>(a.m, xi, ui, xs, us, voli, vs, brk, fs,
                                       Fd);
             fsn = fs && !fs0;
-            // (the FP32-force bond's force is FP32: exact conversions)
+            // (in the FP32-force mode a bond force comes out in FP32, so its
+            // conversion is exact; a contact force of a broken bond is
+            // computed in FP64 and rounded to FP32 here, like every bond
+            // force of that mode)
             pf[q / 2][0] += Real(Fd[0]);
             pf[q / 2][1] += Real(Fd[1]);
             if (DIM == 3) pf[q / 2][2] += Real(Fd[2]);
@@ -1426,8 +1429,12 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
     e.low_band_f = float(e.low_band);
     e.rcp_h_f = float(e.rcp_h);
     e.sc_break_f = float(e.sc_break);
+    // The fast first pass takes the factor as min((check_up - |X|)/h, 1),
+    // which is below the reference's 1 for |X| in (check_low, low_band]
+    // (definitelyGreaterThan's relative window, 1e-5 check_low wide): the
+    // exact-evaluation band of both fast modes covers that whole window.
     e.vex_tol_f = 3e-5f * e.low_band_f;
-    e.vex_tol = kEsGuard64 * e.low_band;
+    e.vex_tol = 3e-5 * e.low_band;
     e.brk_lo = e.breaking ? e.sc_break - kEsGuard64 * e.sc_break
                           : std::numeric_limits<double>::infinity();
     e.brk_hi = e.breaking ? e.sc_break + kEsGuard64 * e.sc_break

@@ -183,6 +183,16 @@ class GpuFD:
         if getattr(p, "v0", None) is not None:
             self.set_state(v=p.v0)
 
+    def _node_vec(self, a, name):
+        """A per-node vector field as the C-ABI reads it: double[n][3]."""
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, np.float64)
+        if a.shape != (self.p.n, 3):
+            raise ValueError(f"{name}: expected shape ({self.p.n}, 3), got "
+                             f"{a.shape}")
+        return a
+
     def _call(self, name, *args):
         rc = getattr(self.lib, name)(self.ctx, *args)
         _check(self.lib, self.ctx, rc, name)
@@ -232,9 +242,17 @@ class GpuFD:
 
     def set_node_ubc(self, nodes, dofs, rates):
         """Node-list displacement BCs: u = rate t on the masked dofs."""
-        nodes = np.ascontiguousarray(nodes, np.int64)
-        dofs = np.ascontiguousarray(dofs, np.uint8)
-        rates = np.ascontiguousarray(rates, np.float64).reshape(-1, 3)
+        nodes = np.ascontiguousarray(nodes, np.int64).ravel()
+        dofs = np.ascontiguousarray(dofs, np.uint8).ravel()
+        rates = np.ascontiguousarray(rates, np.float64)
+        k = nodes.size
+        if dofs.size != k or rates.size != 3 * k:
+            raise ValueError(f"set_node_ubc: {k} nodes need {k} dof masks and "
+                             f"{k} x 3 rates, got {dofs.size} and "
+                             f"{rates.shape}")
+        if k and (nodes.min() < 0 or nodes.max() >= self.p.n):
+            raise ValueError("set_node_ubc: node index out of range")
+        rates = rates.reshape(-1, 3)
         self._call("nlm_fd_set_node_ubc", int(nodes.size), ptr(nodes, C.c_int64),
                    ptr(dofs, C.c_uint8), ptr(rates, C.c_double))
 
@@ -264,9 +282,8 @@ class GpuFD:
         return u, v, f
 
     def set_state(self, u=None, v=None, f=None):
-        def c(a):
-            return None if a is None else np.ascontiguousarray(a, np.float64)
-        u, v, f = c(u), c(v), c(f)
+        u, v, f = (self._node_vec(a, nm) for a, nm in ((u, "u"), (v, "v"),
+                                                        (f, "f")))
         self._call("nlm_fd_set_state", ptr(u, C.c_double), ptr(v, C.c_double),
                    ptr(f, C.c_double))
 
@@ -277,6 +294,9 @@ class GpuFD:
 
     def set_bonds(self, b: np.ndarray):
         b = np.ascontiguousarray(b, np.uint8)
+        if b.size != self.nnz:
+            raise ValueError(f"set_bonds: {b.size} bond states given, the "
+                             f"model has {self.nnz} directed bonds")
         self._call("nlm_fd_set_bonds", ptr(b, C.c_uint8))
 
     def get_bond_bytes(self) -> np.ndarray:
@@ -332,7 +352,7 @@ class GpuFD:
         return b[: self.nnz].reshape(-1, 2, 2), d.reshape(-1, 2, 2)
 
     def qs_apply(self, x):
-        x = np.ascontiguousarray(x, np.float64)
+        x = self._node_vec(x, "x")
         y = np.zeros((self.p.n, 3))
         self._call("nlm_qs_apply", ptr(x, C.c_double), ptr(y, C.c_double))
         return y
@@ -340,10 +360,14 @@ class GpuFD:
     def qs_set_precond(self, blocks, diag):
         b = np.ascontiguousarray(np.asarray(blocks).reshape(-1, 4), np.float64)
         d = np.ascontiguousarray(np.asarray(diag).reshape(-1, 4), np.float64)
+        if b.shape[0] != self.nnz or d.shape[0] != self.p.n:
+            raise ValueError(f"qs_set_precond: need {self.nnz} bond blocks and "
+                             f"{self.p.n} node blocks, got {b.shape[0]} and "
+                             f"{d.shape[0]}")
         self._call("nlm_qs_set_precond", ptr(b, C.c_double), ptr(d, C.c_double))
 
     def qs_coarse_apply(self, r):
-        r = np.ascontiguousarray(r, np.float64)
+        r = self._node_vec(r, "r")
         z = np.zeros((self.p.n, 3))
         self._call("nlm_qs_coarse_apply", ptr(r, C.c_double), ptr(z, C.c_double))
         return z
@@ -366,6 +390,9 @@ class GpuFD:
 
     def gnn_set_weights(self, w):
         w = np.ascontiguousarray(w, np.float32)
+        if w.size != self.lib.nlm_gnn_num_params():
+            raise ValueError(f"gnn_set_weights: {w.size} weights given, the "
+                             f"network has {self.lib.nlm_gnn_num_params()}")
         self._call("nlm_gnn_set_weights", ptr(w, C.c_float))
 
     def gnn_predict(self):
@@ -376,6 +403,9 @@ class GpuFD:
         r = np.ascontiguousarray(r, np.float64)
         e = np.ascontiguousarray(e, np.float64)
         K = r.shape[0]
+        if r.shape != (K, self.p.n, 3) or e.shape != r.shape:
+            raise ValueError(f"gnn_loss_grad: r and e must be ({K}, "
+                             f"{self.p.n}, 3), got {r.shape} and {e.shape}")
         loss = C.c_double()
         g = np.zeros(self.lib.nlm_gnn_num_params(), np.float32) if want_grad else None
         self._call("nlm_gnn_loss_grad", int(K), ptr(r, C.c_double),
@@ -386,7 +416,7 @@ class GpuFD:
         self._call("nlm_gnn_adam_step", float(lr))
 
     def qs_linear_solve(self, b, precond=1, tol=1e-8, max_iter=5000):
-        b = np.ascontiguousarray(b, np.float64)
+        b = self._node_vec(b, "b")
         x = np.zeros((self.p.n, 3))
         it = C.c_int()
         rr = C.c_double()

@@ -97,3 +97,26 @@ def test_es_params_reject_bad_deck():
     for bad in (dict(K=-1.0), dict(G=-1.0), dict(E=1e3)):
         with pytest.raises(capi.NlmError):
             es_params(pr.with_(p, material=dict(p.material, **bad)))
+
+
+def test_engine_rejects_misshapen_arrays():
+    """The driver checks array shapes before it hands raw pointers to the
+    C-ABI (which reads n x 3 doubles, nnz bond bytes, ...): a wrong shape is
+    a ValueError, never an out-of-bounds host read."""
+    from nlmech_amd.engine import GpuFD
+    p = pr.plate2d(6, 6, 1e-3, 3, 1e-7, 1)
+    g = GpuFD.__new__(GpuFD)       # no context: nothing reaches the library
+    g.p, g.nnz, g.ctx = p, 100, None
+    g.lib = None
+    with pytest.raises(ValueError):
+        g.set_state(u=np.zeros((p.n, 2)))
+    with pytest.raises(ValueError):
+        g.set_state(v=np.zeros((p.n - 1, 3)))
+    with pytest.raises(ValueError):
+        g.set_bonds(np.zeros(99, np.uint8))
+    with pytest.raises(ValueError):
+        g.set_node_ubc([0, 1], [1], np.zeros((2, 3)))
+    with pytest.raises(ValueError):
+        g.set_node_ubc([0, p.n], [1, 1], np.zeros((2, 3)))
+    with pytest.raises(ValueError):
+        g.qs_apply(np.zeros(3 * p.n))

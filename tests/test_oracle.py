@@ -333,6 +333,70 @@ def test_cd_free_node_kinematics(oracle):
     assert np.all(u[1] == 0) and np.all(v[1] == 0)
 
 
+def test_verlet_free_node_kinematics(oracle):
+    """An isolated node under a constant force density f, velocity Verlet
+    (integrateVerlet, fDModel.cpp:446-525): two half kicks of dt f/(2 rho)
+    per step give v_k = k dt f/rho and u_k = k^2 dt^2 f/(2 rho) -- not the
+    central difference's k(k+1)/2."""
+    x = np.array([[0.0, 0.0, 0.0], [10.0, 0.0, 0.0]])
+    p = pr.Problem("free", 2, x, np.ones(2), 1e-3, 1e-3 / 3, 1e-4, 1,
+                   pr.rnp_material(),
+                   fbc=[pr.BC("rectangle", [1, 2], "constant", [2.0],
+                              space_params=[3.0], x1=-1, x2=1, y1=-1, y2=1)])
+    o = oracle.OracleFD(p)
+    o.init()
+    a = 6.0 / 1200.0
+    done = 0
+    for K in (1, 7, 200):
+        o.step(K - done, scheme="verlet")
+        done = K
+        u, v, f = o.get_state()
+        assert f[0, 0] == 6.0 and f[0, 1] == 6.0 and f[1, 0] == 0.0
+        for d in (0, 1):
+            assert v[0, d] == pytest.approx(K * 1e-4 * a, rel=1e-12)
+            assert u[0, d] == pytest.approx(1e-8 * a * K * K / 2, rel=1e-12)
+        assert np.all(u[1] == 0) and np.all(v[1] == 0)
+
+
+def test_verlet_harmonic_pair_closed_form(oracle):
+    """Two free nodes joined by one linearised bond (non-interior nodes,
+    rnpBond.cpp:296-300) oscillate as a harmonic oscillator f = -k u.
+    Velocity Verlet maps it exactly onto u_k = u_0 cos(k theta) with
+    cos(theta) = 1 - w^2 dt^2/2 for v_0 = 0, and conserves
+    Q = w^2 (1 - w^2 dt^2/4) u^2 + v^2 to rounding; a second half kick with
+    the old force, or the central difference's start u_1 = (1 - w^2 dt^2)
+    u_0, breaks both."""
+    r = 1e-3
+    x = np.array([[0.0, 0.0, 0.0], [r, 0.0, 0.0]])
+    p = pr.Problem("pair", 2, x, np.full(2, 1e-6), 1.5 * r, r, 1.0, 1,
+                   pr.rnp_material(), interior=np.zeros(2, np.uint8))
+    o = oracle.OracleFD(p)
+    a0 = 1e-7
+    u0 = np.array([[a0, 0, 0], [-a0, 0, 0]])
+    z = np.zeros((2, 3))
+    o.set_state(u=u0, v=z, f=z)
+    o.compute_forces()
+    _, _, f0 = o.get_state()
+    k = -f0[0, 0] / a0          # force density per unit displacement
+    assert k > 0 and f0[1, 0] == pytest.approx(-f0[0, 0], rel=1e-15)
+    w2 = k / 1200.0
+    dt = 0.3 / math.sqrt(w2)    # w dt = 0.3: the dt^2 terms matter
+    p2 = pr.with_(p, dt=dt)
+    o = oracle.OracleFD(p2)
+    o.set_state(u=u0, v=z, f=z)
+    o.compute_forces()          # f of u_0, as integrate() leaves it
+    theta = math.acos(1.0 - 0.5 * w2 * dt * dt)
+    q0 = w2 * (1 - 0.25 * w2 * dt * dt) * a0 * a0
+    for k_step in range(1, 41):
+        o.step(1, scheme="verlet")
+        u, v, _ = o.get_state()
+        assert u[0, 0] == pytest.approx(a0 * math.cos(k_step * theta),
+                                        abs=1e-12 * a0)
+        assert u[1, 0] == pytest.approx(-u[0, 0], rel=1e-12, abs=1e-22)
+        q = w2 * (1 - 0.25 * w2 * dt * dt) * u[0, 0] ** 2 + v[0, 0] ** 2
+        assert q == pytest.approx(q0, rel=1e-12)
+
+
 def test_displacement_bc_time_functions(oracle):
     p = pr.plate2d(12, 12, 1e-3, 3, 1e-7, 1, notch_len=0.0, pull=0.7)
     o = oracle.OracleFD(p)
@@ -386,6 +450,50 @@ def test_postproc_damage(oracle):
     assert np.array_equal(phi > 0, has_broken)
     # only broken bonds carry energy at u = 0: J C / f V_j per broken bond
     assert np.array_equal(e > 0, has_broken)
+
+
+def test_postproc_hand_worked(oracle):
+    """Z and phi of computePostProcFields (fDModel.cpp:700-745) on four
+    nodes worked by hand.  h = 1, horizon 2 (volume correction for
+    r > 1.5: V_j (2.5 - r)/h), Check_Sc_Factor 10 so that no bond breaks
+    while |S|/Sc reaches 5 (Z divides by Sc = rbar/sqrt(r) without the
+    factor, rnpBond.cpp:319-321).
+
+        node  x          V     bonds (r)
+        0     (0, 0)     1.0   1 (1.0), 2 (1.8: corrected 0.7, broken)
+        1     (1, 0)     2.0   0 (1.0), 3 (1.9: corrected 0.6)
+        2     (0, 1.8)   0.5   0 (1.8: corrected 0.7)
+        3     (2.9, 0)   1.5   1 (1.9: corrected 0.6, broken)
+
+    Displacements give |S| sqrt(r)/rbar = 3 on bond 0-1, 5 on 0-2, 2 on
+    1-3, so Z = (5, 3, 5, 2) (a broken bond still counts, the max is over
+    every bond); phi_0 = 1 - 2/(2 + 0.5*0.7), phi_3 = 1, others 0 (a
+    missing volume correction gives 1 - 2/2.5 at node 0)."""
+    x = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.8, 0.0],
+                  [2.9, 0.0, 0.0]])
+    vol = np.array([1.0, 2.0, 0.5, 1.5])
+    mat = pr.rnp_material(check_sc=10.0)
+    p = pr.Problem("pp4", 2, x, vol, 2.0, 1.0, 1e-3, 1, mat)
+    o = oracle.OracleFD(p)
+    assert o.row_ptr.tolist() == [0, 2, 4, 5, 6]
+    assert o.col.tolist() == [1, 2, 0, 3, 0, 1]
+    rbar = o.mat.rbar
+    u = np.zeros((4, 3))
+    u[1, 0] = 3.0 * rbar                                   # S_01 = u1x / 1
+    u[2, 1] = 1.8 * 5.0 * rbar / math.sqrt(1.8)            # S_02 = u2y / 1.8
+    u[3, 0] = u[1, 0] + 1.9 * 2.0 * rbar / math.sqrt(1.9)  # S_13
+    z = np.zeros((4, 3))
+    o.set_state(u=u, v=z, f=z)
+    o.set_bonds(np.array([0, 1, 0, 0, 0, 1], np.uint8))
+    Z, phi, _ = o.postproc()
+    assert o.get_bonds().tolist() == [0, 1, 0, 0, 0, 1]  # nothing broke
+    assert Z.tolist() == pytest.approx([5.0, 3.0, 5.0, 2.0], rel=1e-6)
+    assert phi.tolist() == pytest.approx(
+        [1.0 - 2.0 / (2.0 + 0.5 * 0.7), 0.0, 0.0, 1.0], abs=1e-7)
+    # the same bonds intact: phi = 0 everywhere, Z unchanged
+    o.set_bonds(np.zeros(6, np.uint8))
+    Z2, phi2, _ = o.postproc()
+    assert np.array_equal(Z2, Z) and np.all(phi2 == 0)
 
 
 @pytest.mark.parametrize("name", ["oracle_plate24_pull05",
