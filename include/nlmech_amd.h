@@ -291,6 +291,18 @@ int nlm_fd_compute_forces(nlm_ctx* ctx);
  * Z_i = max_j |S|/Sc(r), phi_i = 1 - sum unbroken V_j / sum V_j (float like
  * the reference), strain energy of node i.  Any pointer may be NULL. */
 int nlm_fd_postproc(nlm_ctx* ctx, float* Z, float* phi, double* energy);
+/* The same fields with no device allocation (no counterpart in the
+ * reference): for callers that cannot allocate device memory, such as
+ * serving loops, harnesses that forbid hipMalloc and code near HIP graph
+ * capture.  The device staging is a static array of the library, 16 MiB per
+ * GPU, held until the process ends: at most NLM_NOALLOC_MAX_NODES nodes
+ * (larger models get NLM_ERR_INVALID; use nlm_fd_postproc).  Results equal
+ * nlm_fd_postproc's bit for bit.  Every context on one GPU shares that
+ * GPU's staging array, so calls that overlap on one GPU (other host
+ * threads' contexts) race. */
+#define NLM_NOALLOC_MAX_NODES 1048576
+int nlm_fd_postproc_noalloc(nlm_ctx* ctx, float* Z, float* phi,
+                            double* energy);
 /* External force density of every node at time t, 3 components per node in
  * reference order (z = 0 in 2D): FLoading::apply on a zero force vector
  * (fLoading.cpp:117-279), which computePostProcFields takes for the work

@@ -33,6 +33,15 @@ extern "C" int nlm_ctx_create(nlm_ctx** out, int device) {
   nlm_ctx* c = new (std::nothrow) nlm_ctx;
   if (!c) return NLM_ERR_MEMORY;
   c->device = device;
+  {
+    std::string msg;
+    int rc = nlm::scratch_setup(device, &c->scratch, msg);
+    if (rc != NLM_OK) {
+      g_create_error = "nlm_ctx_create: " + msg;
+      delete c;
+      return rc;
+    }
+  }
   // default-priority stream
   if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) !=
       hipSuccess) {

@@ -789,17 +789,17 @@ void es_launch_fused_dev(nlm_ctx* c, const double* tdev) {
 // bond bits, so the model state is unchanged.
 int es_autotune(nlm_ctx* c, int* chosen, double* best_us) {
   FdState& s = c->fd;
-  DevBuf scratch;
+  DevBuf bits_copy;
   int rc;
-  if ((rc = alloc(c, scratch, s.sell.bits.bytes))) return rc;
-  NLM_HIP(c, hipMemcpyAsync(scratch.p, s.sell.bits.p, s.sell.bits.bytes,
+  if ((rc = alloc(c, bits_copy, s.sell.bits.bytes))) return rc;
+  NLM_HIP(c, hipMemcpyAsync(bits_copy.p, s.sell.bits.p, s.sell.bits.bytes,
                             hipMemcpyDeviceToDevice, c->stream));
   DevBuf th_save;
   if ((rc = alloc(c, th_save, s.es_theta.bytes))) return rc;
   NLM_HIP(c, hipMemcpyAsync(th_save.p, s.es_theta.p, s.es_theta.bytes,
                             hipMemcpyDeviceToDevice, c->stream));
-  // the trials run on the scratch bits
-  std::swap(s.sell.bits.p, scratch.p);
+  // the trials run on the bits_copy bits
+  std::swap(s.sell.bits.p, bits_copy.p);
   const int cands[3] = {2, 4, 8};
   double best = 1e300;
   int pick = 4;
@@ -822,7 +822,7 @@ int es_autotune(nlm_ctx* c, int* chosen, double* best_us) {
       pick = U;
     }
   }
-  std::swap(s.sell.bits.p, scratch.p);
+  std::swap(s.sell.bits.p, bits_copy.p);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   // restore pass 1's outputs of the current state

@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -34,7 +35,43 @@
 #include "layout.h"
 #include "nlm_internal.h"
 
+// Staging of nlm_fd_postproc_noalloc, the entry point that allocates
+// nothing: Z (FP32), phi (FP32) and strain energy (FP64) of up to
+// kNoAllocMaxNodes nodes, 16 bytes a node (16 MiB).  Each GPU the library
+// runs on holds its own copy until the process ends.  In host code the
+// array's name is not a device address: the per-device address comes from
+// hipGetSymbolAddress (nlm::scratch_setup) and only that address is used.
+static __device__ __align__(256) unsigned char scratch[nlm::kNoAllocBytes];
+
 namespace nlm {
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+int scratch_setup(int device, void** out, std::string& err) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxDevices) {
+    err = "scratch_setup: device index beyond the library's table";
+    return NLM_ERR_INVALID;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    // the device is current (nlm_ctx_create set it)
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      err = std::string("hipGetSymbolAddress(scratch): ") +
+            hipGetErrorString(e);
+      return NLM_ERR_HIP;
+    }
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return NLM_OK;
+}
+
 namespace {
 
 template <int DIM, int MODE, int INF>
@@ -656,10 +693,9 @@ __global__ __launch_bounds__(64) void k_bytes_to_bits(
 // ---- post-processing: computePostProcFields (fDModel.cpp:652-778) -------
 
 template <int DIM>
-__global__ __launch_bounds__(256) void k_postproc(FdArgs<DIM> a,
-                                                  float* __restrict__ Z,
-                                                  float* __restrict__ phi,
-                                                  double* __restrict__ energy) {
+__global__ __launch_bounds__(256) void k_postproc(
+    FdArgs<DIM> a, float* __restrict__ Z, float* __restrict__ phi,
+    double* __restrict__ energy, const int32_t* __restrict__ perm) {
   int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
   int lane = threadIdx.x & 63;
   int64_t s = i >> 6;
@@ -720,9 +756,12 @@ __global__ __launch_bounds__(256) void k_postproc(FdArgs<DIM> a,
     if (nb != 0ull && lane == 0) a.bits[bb + k] = w | nb;
   }
   if (!valid) return;
-  if (Z) Z[i] = float(z);
-  if (phi) phi[i] = float(1. - sa / sb);
-  if (energy) energy[i] = energy_i * voli;
+  // perm (optional): outputs at the node's reference id instead of its
+  // internal index
+  const int64_t o = perm ? int64_t(perm[i]) : i;
+  if (Z) Z[o] = float(z);
+  if (phi) phi[o] = float(1. - sa / sb);
+  if (energy) energy[o] = energy_i * voli;
 }
 
 // External force density of every node at time t: FLoading::apply on a zero
@@ -1174,6 +1213,7 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
   if ((rc = up(s.row_ptr, L.row_ptr.data(), sizeof(int64_t) * (n + 1)))) return rc;
   if ((rc = up(s.col, L.col.data(), sizeof(int32_t) * nnz))) return rc;
   if ((rc = up(s.ref_off, L.ref_off.data(), sizeof(int64_t) * n))) return rc;
+  if ((rc = up(s.perm_dev, L.perm.data(), sizeof(int32_t) * n))) return rc;
   if ((rc = up(s.sell.sl_ptr, L.sl_ptr.data(), sizeof(int64_t) * (L.nslices + 1))))
     return rc;
   if ((rc = up(s.sell.col, L.colS.data(), sizeof(int32_t) * nent))) return rc;
@@ -1889,11 +1929,13 @@ extern "C" int nlm_fd_postproc(nlm_ctx* c, float* Z, float* phi,
   if (s.dim == 2) {
     FdArgs<2> a = make_args<2>(c);
     hipLaunchKernelGGL(k_postproc<2>, dim3(g), dim3(256), 0, c->stream, a,
-                       dz.as<float>(), dp.as<float>(), de.as<double>());
+                       dz.as<float>(), dp.as<float>(), de.as<double>(),
+                       nullptr);
   } else {
     FdArgs<3> a = make_args<3>(c);
     hipLaunchKernelGGL(k_postproc<3>, dim3(g), dim3(256), 0, c->stream, a,
-                       dz.as<float>(), dp.as<float>(), de.as<double>());
+                       dz.as<float>(), dp.as<float>(), de.as<double>(),
+                       nullptr);
   }
   NLM_HIP(c, hipGetLastError());
   std::vector<float> hz(s.n), hp(s.n);
@@ -1911,6 +1953,54 @@ extern "C" int nlm_fd_postproc(nlm_ctx* c, float* Z, float* phi,
     if (phi) phi[i] = hp[a];
     if (energy) energy[i] = he[a];
   }
+  return NLM_OK;
+}
+
+extern "C" int nlm_fd_postproc_noalloc(nlm_ctx* c, float* Z, float* phi,
+                                       double* energy) {
+  int rc = check_ready(c, "nlm_fd_postproc_noalloc");
+  if (rc) return rc;
+  FdState& s = c->fd;
+  if (s.material != NLM_MATERIAL_RNP_BOND)
+    return fail(c, NLM_ERR_INVALID,
+                "nlm_fd_postproc_noalloc: bond-based material only");
+  if (s.n > kNoAllocMaxNodes)
+    return fail(c, NLM_ERR_INVALID,
+                "nlm_fd_postproc_noalloc: " + std::to_string(s.n) +
+                    " nodes exceed the static staging limit of " +
+                    std::to_string(kNoAllocMaxNodes) +
+                    " nodes; use nlm_fd_postproc");
+  if (!c->scratch)
+    return fail(c, NLM_ERR_STATE,
+                "nlm_fd_postproc_noalloc: no staging address for the device");
+  // the context's device copy of the staging array (its address from
+  // hipGetSymbolAddress at context creation): Z, phi, energy regions at
+  // fixed offsets, each written at every reference id 0..n-1 by the kernel
+  // (perm is a bijection), so nothing needs clearing
+  char* const base = static_cast<char*>(c->scratch);
+  float* const dz = reinterpret_cast<float*>(base);
+  float* const dp = reinterpret_cast<float*>(base + kNoAllocZBytes);
+  double* const de =
+      reinterpret_cast<double*>(base + kNoAllocZBytes + kNoAllocPhiBytes);
+  const unsigned g = unsigned((s.sell.nslices * 64 + 255) / 256);
+  const int32_t* perm = s.perm_dev.as<int32_t>();
+  if (s.dim == 2)
+    hipLaunchKernelGGL(k_postproc<2>, dim3(g), dim3(256), 0, c->stream,
+                       make_args<2>(c), dz, dp, de, perm);
+  else
+    hipLaunchKernelGGL(k_postproc<3>, dim3(g), dim3(256), 0, c->stream,
+                       make_args<3>(c), dz, dp, de, perm);
+  NLM_HIP(c, hipGetLastError());
+  if (Z)
+    NLM_HIP(c, hipMemcpyAsync(Z, dz, sizeof(float) * s.n,
+                              hipMemcpyDeviceToHost, c->stream));
+  if (phi)
+    NLM_HIP(c, hipMemcpyAsync(phi, dp, sizeof(float) * s.n,
+                              hipMemcpyDeviceToHost, c->stream));
+  if (energy)
+    NLM_HIP(c, hipMemcpyAsync(energy, de, sizeof(double) * s.n,
+                              hipMemcpyDeviceToHost, c->stream));
+  NLM_HIP(c, hipStreamSynchronize(c->stream));
   return NLM_OK;
 }
 
@@ -1988,13 +2078,13 @@ namespace {
 template <int DIM>
 int autotune(nlm_ctx* c, int* chosen, double* best_us) {
   FdState& s = c->fd;
-  DevBuf scratch;
+  DevBuf bits_copy;
   int rc;
-  if ((rc = alloc(c, scratch, s.sell.bits.bytes))) return rc;
-  NLM_HIP(c, hipMemcpyAsync(scratch.p, s.sell.bits.p, s.sell.bits.bytes,
+  if ((rc = alloc(c, bits_copy, s.sell.bits.bytes))) return rc;
+  NLM_HIP(c, hipMemcpyAsync(bits_copy.p, s.sell.bits.p, s.sell.bits.bytes,
                             hipMemcpyDeviceToDevice, c->stream));
   FdArgs<DIM> a = make_args<DIM>(c);
-  a.bits = scratch.as<unsigned long long>();
+  a.bits = bits_copy.as<unsigned long long>();
   const int cands[3] = {2, 4, 8};
   double best = 1e300;
   int pick = 4;

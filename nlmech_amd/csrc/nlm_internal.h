@@ -163,6 +163,7 @@ struct FdState {
   // internal node order (see layout.cpp): perm[a] = reference id of a
   std::vector<int32_t> perm_host, inv_host;
   DevBuf ref_off;   // int64 per internal node: row start in reference CSR
+  DevBuf perm_dev;  // int32 per internal node: its reference id
   DevBuf halo_ptr;  // int64 per 256-node block
   DevBuf halo;      // int32 internal ids
   DevBuf blk_org;   // double2 per block: origin of block-relative positions
@@ -258,6 +259,9 @@ struct NbrState {
 struct nlm_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // the device's copy of the static staging array of the allocation-free
+  // entry point (hipGetSymbolAddress at creation; fd.hip)
+  void* scratch = nullptr;
   std::string err;
   std::string arch;
   nlm::NbrState nbr;
@@ -267,6 +271,19 @@ struct nlm_ctx {
 };
 
 namespace nlm {
+
+// Static staging of nlm_fd_postproc_noalloc (fd.hip): Z and phi (FP32) and
+// the strain energy (FP64) of at most kNoAllocMaxNodes nodes.
+constexpr int64_t kNoAllocMaxNodes = int64_t(1) << 20;
+constexpr size_t kNoAllocZBytes = sizeof(float) * size_t(kNoAllocMaxNodes);
+constexpr size_t kNoAllocPhiBytes = sizeof(float) * size_t(kNoAllocMaxNodes);
+constexpr size_t kNoAllocEBytes = sizeof(double) * size_t(kNoAllocMaxNodes);
+constexpr size_t kNoAllocBytes =
+    kNoAllocZBytes + kNoAllocPhiBytes + kNoAllocEBytes;
+constexpr int kMaxDevices = 64;
+// Address of this device's staging array (once per device, with the device
+// current); *out stays valid until the process ends.
+int scratch_setup(int device, void** out, std::string& err);
 
 inline int fail(nlm_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
