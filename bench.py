@@ -12,8 +12,12 @@ updates/s = B * K / t.
 Reported: FP32-force mode (value) and FP64 plain mode, the dominant
 kernel's roofline from HIP events on the stepping stream, the displacement
 error and broken-bond mismatch of both modes against the FP64 exact GPU mode
-after the same steps, and the CPU oracle (OpenMP restatement) timed on the
-host on a bounded sample of steps of the same plate.
+after the same steps, and the CPU oracle (OpenMP restatement, lean and
+reference-faithful variants) timed on the host on a bounded sample of steps
+of the same plate.  The oracle's trajectory over that sample is the
+reference of each mode's error "accuracy_vs_oracle"; the same plate pulled
+at +/-30 m/s, where about 1.3e5 bonds break within 60 steps, gives
+"accuracy_vs_oracle_breaking".
 
 Beside it ("state_based_3d"): the 160^3-node ElasticState block (horizon 3h,
 369,653,064 bonds, z faces pulled at +/-1 m/s, seeded Gaussian initial
@@ -278,24 +282,99 @@ def quasi_static(a):
 
 
 def cpu_baseline(p, csr, seconds):
+    """The CPU oracle timed on the host's cores on a bounded sample of steps
+    of the same plate, in two variants: the lean restatement and the
+    reference-faithful one (FDModel::computeForce's per-node output-tag
+    lookups and per-bond is_reaction_force, virtual interior reads; same
+    forces).  Returns the baseline record and the lean run's state after its
+    1 + k steps, the reference trajectory of the accuracy check."""
     from oracle import pyoracle as po
-    o = po.OracleFD(p, csr)
-    o.init()
-    t0 = time.perf_counter()
-    o.step(1)
-    one = time.perf_counter() - t0
-    k = max(1, min(200, int(seconds / max(one, 1e-6))))
-    t0 = time.perf_counter()
-    o.step(k)
-    dt = time.perf_counter() - t0
     nnz = int(csr[0][-1])
     cores = po.load().ora_num_threads()
+    rates, state, k_lean = {}, None, 0
+    for variant in ("lean", "reference_faithful"):
+        o = po.OracleFD(p, csr)
+        o.set_faithful(variant == "reference_faithful")
+        o.init()
+        t0 = time.perf_counter()
+        o.step(1)
+        one = time.perf_counter() - t0
+        k = max(1, min(200, int(0.5 * seconds / max(one, 1e-6))))
+        t0 = time.perf_counter()
+        o.step(k)
+        dt = time.perf_counter() - t0
+        rates[variant] = {"value": nnz * k / dt, "steps": k, "seconds": dt}
+        if variant == "lean":
+            u, _, _ = o.get_state()
+            state = (1 + k, u, o.get_bonds())
+            k_lean = k
+        o.close()
+        print(f"[bench] cpu {variant}: {nnz * k / dt:.3e} bond updates/s "
+              f"({k} steps, {dt:.1f} s, {cores} threads)", file=sys.stderr,
+              flush=True)
+    rf = rates["reference_faithful"]
+    rec = {"value": rf["value"], "unit": "bond updates/s", "cores": cores,
+           "kind": "port",
+           "sample": f"{rf['steps']} integrateCD steps of the same {p.n}-node "
+                     f"plate ({nnz} bonds) after 1 untimed step, OpenMP "
+                     f"restatement with the reference's per-node and per-bond "
+                     f"bookkeeping, {rf['seconds']:.1f} s; the lean "
+                     f"restatement: {k_lean} steps",
+           "variants": rates}
+    return rec, state
+
+
+def accuracy_vs_oracle(p, csr, state, modes):
+    """Displacement error and broken-bond-set difference of each GPU mode
+    against the CPU oracle (FP64, the reference's operation order) after the
+    same steps from integrate()'s prologue."""
+    from nlmech_amd.engine import GpuFD
+    steps, uref, bref = state
+    out = {"steps": steps}
+    for name, mode in modes.items():
+        g = GpuFD(p, precision=mode, csr=csr)
+        g.init()
+        g.step(steps)
+        u, _, _ = g.get_state()
+        b = g.get_bonds()
+        g.close()
+        d = u - uref
+        out[name] = {
+            "l2_rel": float(np.linalg.norm(d) / np.linalg.norm(uref)),
+            "linf_rel": float(np.abs(d).max() / np.abs(uref).max()),
+            "broken_bond_mismatch": int(np.count_nonzero(b != bref)),
+            "broken_bonds_ref": int(bref.sum())}
+    return out
+
+
+def breaking_variant(p, steps):
+    """The same plate pulled at +/-30 m/s: the loaded layers tear off within
+    the first ~40 steps (about 1.3e5 bonds break), so the break decisions of
+    every mode are exercised at full size.  CPU oracle trajectory against
+    each GPU mode after the same steps."""
+    from nlmech_amd import problems as pr
+    from nlmech_amd.capi import (NLM_PREC_FP32_FORCE, NLM_PREC_FP64,
+                                 NLM_PREC_FP64_EXACT)
+    from oracle import pyoracle as po
+    q = pr.with_(pr.plate2d(p.shape[0], p.shape[1], p.h,
+                            int(round(p.horizon / p.h)), p.dt, steps,
+                            pull=30.0), name=p.name + "_pull30")
+    csr = po.neighbors(q.x, q.horizon, "cells")
+    o = po.OracleFD(q, csr)
+    o.init()
+    b0 = int(o.get_bonds().sum())
+    o.step(steps)
+    u, _, _ = o.get_state()
+    b = o.get_bonds()
     o.close()
-    return {"value": nnz * k / dt, "unit": "bond updates/s", "cores": cores,
-            "kind": "port",
-            "sample": f"{k} integrateCD steps of the same {p.n}-node plate "
-                      f"({nnz} bonds) after 1 untimed step, OpenMP "
-                      f"restatement, {dt:.1f} s"}
+    out = accuracy_vs_oracle(q, csr, (steps, u, b),
+                             {"fp32": NLM_PREC_FP32_FORCE,
+                              "fp64": NLM_PREC_FP64,
+                              "exact": NLM_PREC_FP64_EXACT})
+    out["workload"] = q.name
+    out["pull"] = 30.0
+    out["bonds_broken_during_run"] = int(b.sum()) - b0
+    return out
 
 
 def copy_probe(nbytes: int = 1 << 28) -> None:
@@ -406,7 +485,15 @@ def main():
     if not a.no_qs:
         out["quasi_static"] = quasi_static(a)
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(p, csr, a.cpu_seconds)
+        out["cpu_baseline"], ref_state = cpu_baseline(p, csr, a.cpu_seconds)
+        if not a.no_accuracy:
+            out["accuracy_vs_oracle"] = accuracy_vs_oracle(
+                p, csr, ref_state, {"fp32": NLM_PREC_FP32_FORCE,
+                                    "fp64": NLM_PREC_FP64,
+                                    "exact": NLM_PREC_FP64_EXACT})
+            out["accuracy_vs_oracle_breaking"] = breaking_variant(p, 60)
+            print("[bench] accuracy against the CPU oracle done",
+                  file=sys.stderr, flush=True)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #ifdef _OPENMP
@@ -416,6 +417,18 @@ struct Crack {
   bool activated = false;
 };
 
+// Interior flags behind a virtual call, as RNPBond::getBondEF reads them
+// (geometry::InteriorFlags, interiorFlags.cpp:27-30, 228-231): used by the
+// reference-faithful timing variant only.
+struct InteriorIface {
+  virtual ~InteriorIface() = default;
+  virtual bool get(int64_t i) const = 0;
+};
+struct InteriorFromData : InteriorIface {
+  const uint8_t* flags = nullptr;
+  bool get(int64_t i) const override { return flags[i] != 0; }
+};
+
 struct FD {
   int dim = 2;
   int64_t n = 0;
@@ -765,6 +778,86 @@ struct FD {
     return force_i;
   }
 
+  // Reference-faithful timing variant (the CPU baseline that keeps the
+  // reference's per-node and per-bond bookkeeping; forces identical to
+  // compute_force's): FDModel::computeForce (fDModel.cpp:544-582) looks up
+  // two output tags per node (OutputDeck::isTagInOutput, a string compare
+  // over the tag list, outputDeck.h:95-103) and tests is_reaction_force per
+  // bond (fDModel.cpp:585-605); RNPBond::getBondEF reads both interior flags
+  // through a virtual call per bond (rnpBond.cpp:236-240).
+  bool faithful = false;
+  std::vector<std::string> out_tags;
+  std::vector<P3> react;
+  std::vector<double> react_tot;
+  double bb_min_x = 0., bb_max_x = 0., bb_max_y = 0.;
+  InteriorFromData interior_if;
+
+  bool tag_in_output(const std::string& tag) const {
+    for (const auto& t : out_tags)
+      if (tag == t) return true;
+    return false;
+  }
+
+  bool is_reaction_force(int64_t i, int64_t j) const {
+    const double delta = m.horizon, eps = 1e-6;
+    const P3 xi = x[i], xj = x[j];
+    if (xi.y > bb_max_y - 1.5 * delta && xi.y < bb_max_y - delta) {
+      P3 A{bb_min_x, bb_max_y - delta + eps, 0.};
+      P3 B{bb_max_x, bb_max_y - delta + eps, 0.};
+      if (ref_lines_intersect(A, B, xi, xj)) return true;
+    }
+    return false;
+  }
+
+  P3 compute_force_faithful(int64_t i) {
+    P3 force_i{0., 0., 0.};
+    double energy_i = 0.;
+    const InteriorIface& inter = interior_if;
+    if (tag_in_output("Reaction_Force") or
+        tag_in_output("Total_Reaction_Force")) {
+      react[i] = P3{};
+      react_tot[i] = 0.;
+    }
+    for (int64_t k = row_ptr[i]; k < row_ptr[i + 1]; k++) {
+      int64_t jl = k - row_ptr[i];
+      int64_t j = col[k];
+      bool fs = get_bond(i, jl);
+      const bool int_i = inter.get(i), int_j = inter.get(j);
+      P3 F;
+      double E;
+      bond_ef(m, J, x[i], u[i], x[j], u[j], vol[i], vol[j], int_i, int_j, fs,
+              F, E);
+      if (int_i && int_j) set_bond(i, jl, fs);
+      force_i = force_i + F;
+      energy_i += E;
+      if (is_reaction_force(i, j) and
+          (tag_in_output("Reaction_Force") or
+           tag_in_output("Total_Reaction_Force")))
+        react[i] = react[i] + vol[i] * F;
+    }
+    if (tag_in_output("Total_Reaction_Force"))
+      react_tot[i] = length(react[i]);
+    return force_i;
+  }
+
+  void set_faithful(bool on) {
+    faithful = on;
+    if (!on) return;
+    // the output tags of the reference's example explicit deck
+    // (examples/fdModel/crack_propagation/input.yaml:66-70)
+    out_tags = {"Displacement", "Velocity", "Force", "Damage_Z"};
+    react.assign(size_t(n), P3{});
+    react_tot.assign(size_t(n), 0.);
+    bb_min_x = bb_max_x = x[0].x;
+    bb_max_y = x[0].y;
+    for (int64_t i = 0; i < n; i++) {
+      bb_min_x = std::min(bb_min_x, x[i].x);
+      bb_max_x = std::max(bb_max_x, x[i].x);
+      bb_max_y = std::max(bb_max_y, x[i].y);
+    }
+    interior_if.flags = interior.data();
+  }
+
   // per-bond tangent scalars k (CSR order) at the current u and bond state
   void tangent(double* kout) {
 #pragma omp parallel for schedule(static)
@@ -780,6 +873,11 @@ struct FD {
 
   // FDModel::computeForces, fDModel.cpp:528-541
   void compute_forces() {
+    if (faithful) {
+#pragma omp parallel for schedule(static)
+      for (int64_t i = 0; i < n; i++) f[i] = f[i] + compute_force_faithful(i);
+      return;
+    }
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) f[i] = f[i] + compute_force(i, nullptr);
   }
@@ -1153,6 +1251,7 @@ struct ES {
     dilatation();
     m.sc = sc;
     broken = keep;
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) {
       double acc = 0.;
       for (int64_t k = row_ptr[i]; k < row_ptr[i + 1]; k++) {
@@ -1323,6 +1422,7 @@ void ora_es_get_node_fields(void* h, double* theta, double* mvol) {
   }
 }
 double ora_es_time(void* h) { return static_cast<ES*>(h)->time; }
+void ora_es_set_time(void* h, double t) { static_cast<ES*>(h)->time = t; }
 void ora_fd_tangent(void* h, double* k) { static_cast<FD*>(h)->tangent(k); }
 void ora_es_force_abs_sum(void* h, double* out) {
   static_cast<ES*>(h)->force_abs_sum(out);
@@ -1555,6 +1655,9 @@ void ora_fd_postproc(void* h, float* Z, float* phi, double* energy) {
 }
 void ora_fd_force_abs_sum(void* h, double* out) {
   static_cast<FD*>(h)->force_abs_sum(out);
+}
+void ora_fd_set_faithful(void* h, int on) {
+  static_cast<FD*>(h)->set_faithful(on != 0);
 }
 void ora_fd_set_time(void* h, double t, int64_t step) {
   static_cast<FD*>(h)->time = t;

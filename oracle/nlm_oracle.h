@@ -231,6 +231,9 @@ void ora_es_get_bonds(void* h, uint8_t* bond_state);
 void ora_es_set_bonds(void* h, const uint8_t* bond_state);
 void ora_es_get_node_fields(void* h, double* theta, double* mvol);
 double ora_es_time(void* h);
+/* Reference-faithful timing variant of the FD force loop (same results). */
+void ora_fd_set_faithful(void* h, int on);
+void ora_es_set_time(void* h, double t);
 /* per node sum_j |V_j T_ij| + |V_j T_ji| (the scale of the force sum) */
 void ora_es_force_abs_sum(void* h, double* out);
 

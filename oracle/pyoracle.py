@@ -148,6 +148,8 @@ SIGNATURES = {
     "ora_es_set_bonds": (None, [_P, _U8]),
     "ora_es_get_node_fields": (None, [_P, _D, _D]),
     "ora_es_time": (C.c_double, [_P]),
+    "ora_es_set_time": (None, [_P, C.c_double]),
+    "ora_fd_set_faithful": (None, [_P, C.c_int]),
     "ora_es_force_abs_sum": (None, [_P, _D]),
     "ora_fd_tangent": (None, [_P, _D]),
 }
@@ -300,6 +302,12 @@ class OracleFD:
 
     def compute_forces(self):
         self.lib.ora_fd_compute_forces(self.h)
+
+    def set_faithful(self, on=True):
+        """Reference-faithful timing variant: the per-node output-tag
+        lookups and per-bond is_reaction_force of FDModel::computeForce and
+        virtual interior-flag reads (same forces, more work per bond)."""
+        self.lib.ora_fd_set_faithful(self.h, int(bool(on)))
 
     def get_state(self):
         n = self.p.n
@@ -500,6 +508,9 @@ class OracleES:
 
     def time(self):
         return self.lib.ora_es_time(self.h)
+
+    def set_time(self, t):
+        self.lib.ora_es_set_time(self.h, float(t))
 
     def force_abs_sum(self):
         out = np.zeros(self.p.n)

@@ -370,13 +370,15 @@ def test_autotune_and_unroll_do_not_change_results(engine, mode):
 
 
 # ------------------------------------------------------------ full size
-def test_plate2d_1e6_properties(engine):
-    """1000x1000 plate at horizon 4h (no oracle run at this size):
-    B = 43,856,108 directed bonds, symmetric sorted lists, and sum_i V_i f_i
-    = 0 to rounding after 20 steps in every mode."""
-    progress("[gpu] plate2d_1e6 properties")
+@pytest.mark.parametrize("mode", [NLM_PREC_FP64, NLM_PREC_FP32_FORCE],
+                         ids=["fp64", "fp32"])
+def test_plate2d_1e6_properties(engine, mode):
+    """1000x1000 plate at horizon 4h: B = 43,856,108 directed bonds,
+    symmetric sorted lists, and sum_i V_i f_i = 0 to rounding after 20
+    steps, in the FP64 and the FP32-force modes."""
+    progress(f"[gpu] plate2d_1e6 properties mode {mode}")
     p = pr.workload("plate2d_1e6")
-    with engine.GpuFD(p, precision=NLM_PREC_FP64) as g:
+    with engine.GpuFD(p, precision=mode) as g:
         assert g.nnz == 43856108
         rp, col = g.neighbors()
         rows = np.repeat(np.arange(p.n, dtype=np.int64), np.diff(rp))
@@ -393,3 +395,47 @@ def test_plate2d_1e6_properties(engine):
     assert nb > 0  # the notch
     tot = (p.vol[:, None] * f).sum(0)
     assert np.all(np.abs(tot) <= 1e-10 * (p.vol[:, None] * np.abs(f)).sum())
+
+
+@pytest.mark.parametrize("mode", [NLM_PREC_FP64, NLM_PREC_FP32_FORCE],
+                         ids=["fp64", "fp32"])
+def test_plate2d_1e6_stepwise_breaking(engine, oracle, mode):
+    """Full-size per-step parity on the 1000x1000 plate (horizon 4h) pulled
+    at +/-30 m/s, where the loaded layers tear off (about 1.3e5 bonds break
+    in the first 40 steps): the GPU's neighbor lists equal the oracle's,
+    and at three checkpoints the GPU state is loaded into the oracle and one
+    step is taken on both -- bond bits identical (including the bonds that
+    break in that step) and per-node force within 1e-12 (FP64) / 2e-6
+    (FP32-force) of sum_j |F_ij|."""
+    progress(f"[gpu] plate2d_1e6 stepwise breaking mode {mode}")
+    p = pr.plate2d(1000, 1000, 1e-4, 4, 1e-8, 100, pull=30.0)
+    csr = oracle.neighbors(p.x, p.horizon, "cells")
+    o = oracle.OracleFD(p, csr)
+    new_total = 0
+    with engine.GpuFD(p, precision=mode) as g:
+        rp, col = g.neighbors()
+        assert np.array_equal(rp, csr[0]) and np.array_equal(col, csr[1])
+        g.init()
+        done = 0
+        for chk in (3, 12, 25):
+            g.step(chk - done)
+            done = chk + 1
+            u, v, f = g.get_state()
+            b = g.get_bonds()
+            t, n = g.time()
+            o.set_state(u=u, v=v, f=f)
+            o.set_bonds(b)
+            o.set_time(t, n)
+            g.step(1)
+            o.step(1)
+            ur, vr, fr = o.get_state()
+            br = o.get_bonds()
+            absF = o.force_abs_sum()
+            ug, vg, fg = g.get_state()
+            bg = g.get_bonds()
+            progress(f"  step {done}: {int(br.sum() - b.sum())} bonds broke")
+            new_total += int(br.sum() - b.sum())
+            assert np.array_equal(bg, br)
+            assert force_err(fg, fr, absF) <= FORCE_TOL[mode]
+    o.close()
+    assert new_total > 1000

@@ -632,3 +632,20 @@ def test_external_force_line_load(oracle):
         exp[sel, 0] = t * 3.0 * pos[sel] / nodes
         assert sel.sum() == 22
         np.testing.assert_allclose(f, exp, rtol=1e-14, atol=1e-300)
+
+
+def test_reference_faithful_variant_same_trajectory(oracle):
+    """The reference-faithful timing variant (FDModel::computeForce's tag
+    lookups and is_reaction_force per bond, virtual interior reads) does the
+    same arithmetic as the lean oracle: identical trajectories and bonds."""
+    p = pr.plate2d(30, 30, 1e-3, 3, 1e-7, 1, notch_len=0.008, pull=30.0)
+    runs = []
+    for faithful in (False, True):
+        o = oracle.OracleFD(p)
+        o.set_faithful(faithful)
+        o.init()
+        o.step(25)
+        runs.append((*o.get_state(), o.get_bonds()))
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)
+    assert runs[0][3].sum() > 0
