@@ -99,16 +99,18 @@ struct alignas(16) HaloRec {
   float cv;  // FP32-force mode: vol * c4 (* J when constant), in FP32
 };
 
-// The FP32-force mode's record (24 bytes): position and displacement
-// relative to the block's origins in FP32, so a bond's differences are two
-// FP32 subtractions per component.  Positions within a block and its halo
-// lie within about (16 + 2 horizon/h) h of the origin, so the relative
-// coordinates carry FP32 rounding of that extent: a relative error of the
-// bond's differences of order 1e-6, far inside the guard bands of the break
-// and volume-correction decisions (which the exact path re-takes).
-struct alignas(8) HaloRec32 {
+// The FP32-force mode's record (32 bytes): the displacement in FP64 and the
+// position relative to the block's origin in FP32.  A bond's displacement
+// difference is formed in FP64 and rounded once to FP32, so its error is
+// relative to the difference itself however large the displacements are
+// (cancellation-free).  Positions within a block and its halo lie within
+// about (16 + 2 horizon/h) h of the origin, so the relative coordinates
+// carry FP32 rounding of that extent: a relative error of the bond's
+// position difference of order 1e-6, far inside the guard bands of the
+// break and volume-correction decisions (which the exact path re-takes).
+struct alignas(16) HaloRec32 {
+  double2 u;
   float2 xr;
-  float2 ur;
   float cv;  // vol c4 (J)
   uint32_t flags;
 };
@@ -138,13 +140,10 @@ __device__ __forceinline__ void fd_step_body(
   typedef HaloRecOf<MODE> Rec;
   extern __shared__ HaloRec smem_halo[];
   Rec* const halo_lds = reinterpret_cast<Rec*>(smem_halo);
-  double2 org = make_double2(0., 0.), uorg = make_double2(0., 0.);
+  double2 org = make_double2(0., 0.);
   int64_t hb = 0;
   if constexpr (TILED) {
-    if constexpr (REL) {
-      org = a.blk_org[lb];
-      uorg = ld32(a.u, uint32_t(a.blk_cnode[lb]));
-    }
+    if constexpr (REL) org = a.blk_org[lb];
     hb = a.halo_ptr[lb];
     const int hn = int(a.halo_ptr[lb + 1] - hb);
     for (int t = threadIdx.x; t < hn; t += 256) {
@@ -156,7 +155,7 @@ __device__ __forceinline__ void fd_step_body(
       r.cv = float(vg) * a.m.c4f_infl;
       if constexpr (REL) {
         r.xr = make_float2(float(xg.x - org.x), float(xg.y - org.y));
-        r.ur = make_float2(float(ug.x - uorg.x), float(ug.y - uorg.y));
+        r.u = ug;
       } else {
         r.x = xg;
         r.u = ug;
@@ -188,13 +187,11 @@ __device__ __forceinline__ void fd_step_body(
     fl = a.flags[i];
   }
   bool int_i = !(fl & kFlagNonInterior);
-  // the node's block-relative FP32 position and displacement
-  float xir[2] = {0.f, 0.f}, uir[2] = {0.f, 0.f};
+  // the node's block-relative FP32 position
+  float xir[2] = {0.f, 0.f};
   if constexpr (REL) {
     xir[0] = float(xi[0] - org.x);
     xir[1] = float(xi[1] - org.y);
-    uir[0] = float(ui[0] - uorg.x);
-    uir[1] = float(ui[1] - uorg.y);
   }
   // this slice's neighbor slots: slot k of this lane at entry 64 k + lane
   const int32_t* __restrict__ colS = a.col + cb;
@@ -246,7 +243,7 @@ __device__ __forceinline__ void fd_step_body(
       jn[q] = slot_index(k0 + U + q);
     }
     double xj[U][3], uj[U][3], vj[U];
-    float xrj[U][2], urj[U][2];
+    float xrj[U][2], duj[U][2];
     float cvj[U];
     unsigned long long w[U], vx[U];
     uint32_t flj[U];
@@ -261,8 +258,9 @@ __device__ __forceinline__ void fd_step_body(
         LdsRec* r = (LdsRec*)j;
         xrj[q][0] = r->xr.x;
         xrj[q][1] = r->xr.y;
-        urj[q][0] = r->ur.x;
-        urj[q][1] = r->ur.y;
+        // displacement difference in FP64, rounded once
+        duj[q][0] = float(r->u.x - ui[0]);
+        duj[q][1] = float(r->u.y - ui[1]);
         cvj[q] = r->cv;
         if (NOFAIL) flj[q] = r->flags;
       } else if constexpr (TILED) {
@@ -335,9 +333,8 @@ __device__ __forceinline__ void fd_step_body(
                                            km, nb[q], band[q], ex[q], F);
         else if constexpr (REL)
           bond_fp32_core<DIM, INF, NOFAIL>(
-              a.m, xrj[q][0] - xir[0], xrj[q][1] - xir[1], 0.f,
-              urj[q][0] - uir[0], urj[q][1] - uir[1], 0.f, cvj[q], km, nb[q],
-              band[q], ex[q], F);
+              a.m, xrj[q][0] - xir[0], xrj[q][1] - xir[1], 0.f, duj[q][0],
+              duj[q][1], 0.f, cvj[q], km, nb[q], band[q], ex[q], F);
         else
           bond_fp32_fast<DIM, INF, NOFAIL>(a.m, xi, ui, xj[q], uj[q], cvj[q],
                                            km, nb[q], band[q], ex[q], F);
@@ -1264,11 +1261,9 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
     for (int d = 0; d < dim; d++) hx[a * stride + d] = p->x[3 * i + d];
     hv[a] = p->vol[i];
   }
-  // Per 256-node block: the centre of its nodes' bounding box (origin of the
-  // FP32 block-relative positions) and the block node nearest to it (whose
-  // displacement is the displacement origin).
+  // Per 256-node block: the centre of its nodes' bounding box, the origin of
+  // the FP32 block-relative positions.
   std::vector<double> borg(size_t(L.nblocks) * 2, 0.);
-  std::vector<int32_t> bcn(size_t(L.nblocks), 0);
   if (dim == 2) {
     for (int64_t b = 0; b < L.nblocks; b++) {
       const int64_t a0 = 256 * b, a1 = std::min<int64_t>(n, a0 + 256);
@@ -1279,22 +1274,11 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
           hi[d] = std::max(hi[d], hx[a * 2 + d]);
         }
       const double o[2] = {0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1])};
-      double best = -1.;
-      for (int64_t a = a0; a < a1; a++) {
-        const double e0 = hx[a * 2] - o[0], e1 = hx[a * 2 + 1] - o[1];
-        const double d2 = e0 * e0 + e1 * e1;
-        if (best < 0. || d2 < best) {
-          best = d2;
-          bcn[b] = int32_t(a);
-        }
-      }
       borg[2 * b] = o[0];
       borg[2 * b + 1] = o[1];
     }
   }
   if ((rc = up(s.blk_org, borg.data(), sizeof(double) * borg.size())))
-    return rc;
-  if ((rc = up(s.blk_cnode, bcn.data(), sizeof(int32_t) * bcn.size())))
     return rc;
   size_t vbytes = sizeof(double) * stride * size_t(n);
   if ((rc = alloc(c, s.x, vbytes))) return rc;

@@ -35,7 +35,6 @@ struct FdArgs {
   // the node whose displacement is the displacement origin
   const unsigned long long* __restrict__ vexw;
   const double2* __restrict__ blk_org;
-  const int32_t* __restrict__ blk_cnode;
   int64_t halo_cap;  // LDS entries per halo array
   RnpDev m;
   double dt;
@@ -178,7 +177,6 @@ inline FdArgs<DIM> make_args(nlm_ctx* c) {
   a.colL = s.colL.as<uint16_t>();
   a.vexw = s.vexw.as<unsigned long long>();
   a.blk_org = s.blk_org.as<double2>();
-  a.blk_cnode = s.blk_cnode.as<int32_t>();
   a.halo_cap = (s.halo_max + 1) & ~int64_t(1);
   a.m = s.rnp;
   a.dt = s.dt;

@@ -167,7 +167,6 @@ struct FdState {
   DevBuf halo_ptr;  // int64 per 256-node block
   DevBuf halo;      // int32 internal ids
   DevBuf blk_org;   // double2 per block: origin of block-relative positions
-  DevBuf blk_cnode; // int32 per block: node nearest that origin
   DevBuf vexw;      // uint64 per (slice, slot): bonds whose length lies in
                     // the volume-correction edge band (exact evaluation)
   DevBuf colL;      // uint16 halo-local slice entries (bond-based staged

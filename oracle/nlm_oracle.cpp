@@ -1270,6 +1270,49 @@ struct ES {
     }
   }
 
+  // Test tolerance (3D): a bound on the per-node force difference that the
+  // reference's own rounding of the extension e = |(x_j + u_j) - (x_i +
+  // u_i)| - |x_j - x_i| (stateBasedHelperFunctions.cpp:39-47) can cause
+  // against a cancellation-free evaluation of the same state.  The sums x +
+  // u round at the coordinates' magnitude, so per bond
+  //   de <= 8 eps (max(|x_i + u_i|_inf, |x_j + u_j|_inf) + |X|);
+  // the dilatation then carries dth_i <= (3/m_i) sum_j |X| c V_j de, and
+  // the force f_i = sum_j V_j c (P_i + P_j) |X| Y/|Y| with P = 3 K th/m
+  // (e_d = 0 in 3D, ElasticState.cpp:227-245) carries
+  //   df_i <= sum_j V_j c |X| (3K/m_i dth_i + 3K/m_j dth_j).
+  void extension_rounding_bound(double* out) {
+    const double eps = 0x1p-53;
+    std::vector<double> dth(size_t(n), 0.);
+    auto inf_norm = [&](int64_t a) {
+      const P3 y = x[a] + u[a];
+      return std::max(std::fabs(y.x), std::max(std::fabs(y.y), std::fabs(y.z)));
+    };
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+      double acc = 0.;
+      const double yi = inf_norm(i);
+      for (int64_t k = row_ptr[i]; k < row_ptr[i + 1]; k++) {
+        const int64_t j = col[k];
+        const double lx = length(x[j] - x[i]);
+        const double de = 8. * eps * (std::max(yi, inf_norm(j)) + lx);
+        acc += lx * corr[k] * vol[j] * de;
+      }
+      dth[size_t(i)] = 3. / mw[i] * acc;
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+      double acc = 0.;
+      for (int64_t k = row_ptr[i]; k < row_ptr[i + 1]; k++) {
+        const int64_t j = col[k];
+        const double lx = length(x[j] - x[i]);
+        acc += vol[j] * corr[k] * lx *
+               (3. * m.K / mw[i] * dth[size_t(i)] +
+                3. * m.K / mw[j] * dth[size_t(j)]);
+      }
+      out[i] = acc;
+    }
+  }
+
   void apply_ubc(double t) {
     loader->u = u;
     loader->v = v;
@@ -1423,6 +1466,9 @@ void ora_es_get_node_fields(void* h, double* theta, double* mvol) {
 }
 double ora_es_time(void* h) { return static_cast<ES*>(h)->time; }
 void ora_es_set_time(void* h, double t) { static_cast<ES*>(h)->time = t; }
+void ora_es_extension_rounding_bound(void* h, double* out) {
+  static_cast<ES*>(h)->extension_rounding_bound(out);
+}
 void ora_fd_tangent(void* h, double* k) { static_cast<FD*>(h)->tangent(k); }
 void ora_es_force_abs_sum(void* h, double* out) {
   static_cast<ES*>(h)->force_abs_sum(out);

@@ -234,6 +234,9 @@ double ora_es_time(void* h);
 /* Reference-faithful timing variant of the FD force loop (same results). */
 void ora_fd_set_faithful(void* h, int on);
 void ora_es_set_time(void* h, double t);
+/* Test tolerance: per-node bound on the force difference caused by the
+ * reference's rounding of |Y| - |X| (3D). */
+void ora_es_extension_rounding_bound(void* h, double* out);
 /* per node sum_j |V_j T_ij| + |V_j T_ji| (the scale of the force sum) */
 void ora_es_force_abs_sum(void* h, double* out);
 

@@ -149,6 +149,7 @@ SIGNATURES = {
     "ora_es_get_node_fields": (None, [_P, _D, _D]),
     "ora_es_time": (C.c_double, [_P]),
     "ora_es_set_time": (None, [_P, C.c_double]),
+    "ora_es_extension_rounding_bound": (None, [_P, _D]),
     "ora_fd_set_faithful": (None, [_P, C.c_int]),
     "ora_es_force_abs_sum": (None, [_P, _D]),
     "ora_fd_tangent": (None, [_P, _D]),
@@ -511,6 +512,13 @@ class OracleES:
 
     def set_time(self, t):
         self.lib.ora_es_set_time(self.h, float(t))
+
+    def extension_rounding_bound(self):
+        """Per-node bound (3D) on the force difference that the reference's
+        rounding of e = |Y| - |X| alone can cause at the current state."""
+        out = np.zeros(self.p.n)
+        self.lib.ora_es_extension_rounding_bound(self.h, _p(out, C.c_double))
+        return out
 
     def force_abs_sum(self):
         out = np.zeros(self.p.n)

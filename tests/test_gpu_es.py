@@ -261,17 +261,25 @@ def test_es_cube_full_size_properties():
     g.close()
 
 
-@pytest.mark.parametrize("mode", [NLM_PREC_FP64, NLM_PREC_FP32_FORCE],
-                         ids=["fp64", "fp32"])
+@pytest.mark.parametrize("mode", MODES, ids=lambda m: NAMES[m])
 def test_es_cube3d_4e6_stepwise(oracle, mode):
     """Full-size per-step parity on the 160^3 state-based block (horizon 3h,
-    369,653,064 bonds, z faces pulled at +/-1 m/s): at two checkpoints
-    along a run in which bonds break, the GPU state is loaded into the
-    oracle and one step is taken on both -- bond bits identical and
-    per-node force within 1e-12 (FP64) / 2e-6 (FP32-force) of
-    sum_j (|V_j T_ij| + |V_j T_ji|)."""
+    369,653,064 bonds, z faces pulled at +/-1 m/s, Gaussian initial
+    velocity): at two checkpoints of a run in which bonds break, the GPU
+    state is loaded into the oracle and one step is taken on both.  Bond
+    bits must be identical.  Per-node force, scale S_i = sum_j (|V_j T_ij| +
+    |V_j T_ji|):
+      exact mode (the reference's operation order)  <= 1e-12 S_i;
+      FP64 mode                                    <= 1e-12 S_i + B_i;
+      FP32-force mode                              <= 2e-6 S_i + B_i,
+    where B_i bounds what the reference's own rounding of the extension
+    |(x_j + u_j) - (x_i + u_i)| - |X| contributes (the sums round at the
+    coordinates' magnitude, 0.1 m here, while the interior strains of the
+    random velocity field are ~1e-7): the fast modes evaluate the extension
+    without that cancellation, so they differ from the oracle by up to that
+    rounding, not by their own."""
     from nlmech_amd.engine import GpuFD
-    progress(f"[gpu] es cube3d_4e6 stepwise mode {mode}")
+    progress(f"[gpu] es cube3d_4e6 stepwise {NAMES[mode]}")
     p = pr.workload("cube3d_4e6")
     csr = oracle.neighbors(p.x, p.horizon, "cells")
     o = oracle.OracleES(p, csr)
@@ -299,7 +307,13 @@ def test_es_cube3d_4e6_stepwise(oracle, mode):
             new_total += nb
             assert np.array_equal(bg, br)
             scale = o.force_abs_sum()
-            err = force_err(fg, fr, scale)
-            assert err <= FORCE_TOL[mode], err
+            bound = (np.zeros(p.n) if mode == NLM_PREC_FP64_EXACT
+                     else o.extension_rounding_bound())
+            d = np.abs(fg - fr).max(axis=1)
+            excess = d - (FORCE_TOL[mode] * scale + bound)
+            k = int(excess.argmax())
+            assert excess[k] <= 0.0, (f"node {k}: |df| {d[k]:.3e} scale "
+                                      f"{scale[k]:.3e} bound {bound[k]:.3e}")
+            progress(f"  max |df|/S {force_err(fg, fr, scale):.2e}")
     o.close()
     assert new_total > 0

@@ -369,6 +369,24 @@ def test_autotune_and_unroll_do_not_change_results(engine, mode):
             assert np.array_equal(a, b)
 
 
+def fp32_underflow_floor(o, p, csr):
+    """Per-node force that the FP32-force mode may lose to underflow: its
+    bond arithmetic (du, dx.du, S) runs in FP32, whose smallest normal
+    number is 1.18e-38, so a bond whose displacement difference is of that
+    order relative to its length (the undisturbed region ahead of a wave,
+    where FP64 displacements are ~1e-40 m) carries a force of at most
+    c4 V_j FLT_MIN / r^2 (c4 = 4 C beta J / invFactor, the linear force per
+    unit stretch, rnpBond.cpp:277-279) that FP32 cannot resolve; a factor 8
+    covers the product chain."""
+    m = o.mat
+    c4 = 4.0 * m.C * m.beta * m.inf_a / m.inv_factor
+    rp, col = csr
+    rows = np.repeat(np.arange(p.n), np.diff(rp))
+    r2 = ((p.x[col] - p.x[rows]) ** 2).sum(1)
+    per = np.bincount(rows, p.vol[col] / r2, minlength=p.n)
+    return 8.0 * np.finfo(np.float32).tiny * c4 * per
+
+
 # ------------------------------------------------------------ full size
 @pytest.mark.parametrize("mode", [NLM_PREC_FP64, NLM_PREC_FP32_FORCE],
                          ids=["fp64", "fp32"])
@@ -393,8 +411,12 @@ def test_plate2d_1e6_properties(engine, mode):
         nb = g.get_bonds().sum()
     assert np.isfinite(u).all()
     assert nb > 0  # the notch
+    # bond (i, j) and (j, i) carry opposite forces up to rounding: FP64 to
+    # ~1e-16 relative each, FP32-force to ~1e-7 (each bond force rounded
+    # once from its own FP32 differences), so the bound is per mode
     tot = (p.vol[:, None] * f).sum(0)
-    assert np.all(np.abs(tot) <= 1e-10 * (p.vol[:, None] * np.abs(f)).sum())
+    tol = 1e-10 if mode == NLM_PREC_FP64 else 1e-6
+    assert np.all(np.abs(tot) <= tol * (p.vol[:, None] * np.abs(f)).sum())
 
 
 @pytest.mark.parametrize("mode", [NLM_PREC_FP64, NLM_PREC_FP32_FORCE],
@@ -406,11 +428,14 @@ def test_plate2d_1e6_stepwise_breaking(engine, oracle, mode):
     and at three checkpoints the GPU state is loaded into the oracle and one
     step is taken on both -- bond bits identical (including the bonds that
     break in that step) and per-node force within 1e-12 (FP64) / 2e-6
-    (FP32-force) of sum_j |F_ij|."""
+    (FP32-force) of sum_j |F_ij|, plus in FP32 the underflow floor of nodes
+    the waves have not reached (fp32_underflow_floor)."""
     progress(f"[gpu] plate2d_1e6 stepwise breaking mode {mode}")
     p = pr.plate2d(1000, 1000, 1e-4, 4, 1e-8, 100, pull=30.0)
     csr = oracle.neighbors(p.x, p.horizon, "cells")
     o = oracle.OracleFD(p, csr)
+    floor = fp32_underflow_floor(o, p, csr) if mode == NLM_PREC_FP32_FORCE \
+        else 0.0
     new_total = 0
     with engine.GpuFD(p, precision=mode) as g:
         rp, col = g.neighbors()
@@ -436,6 +461,10 @@ def test_plate2d_1e6_stepwise_breaking(engine, oracle, mode):
             progress(f"  step {done}: {int(br.sum() - b.sum())} bonds broke")
             new_total += int(br.sum() - b.sum())
             assert np.array_equal(bg, br)
-            assert force_err(fg, fr, absF) <= FORCE_TOL[mode]
+            d = np.abs(fg - fr).max(axis=1)
+            excess = d - (FORCE_TOL[mode] * absF + floor)
+            k = int(excess.argmax())
+            assert excess[k] <= 0.0, (f"node {k}: |df| {d[k]:.3e} "
+                                      f"sum|F| {absF[k]:.3e}")
     o.close()
     assert new_total > 1000
