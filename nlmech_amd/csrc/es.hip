@@ -129,6 +129,56 @@ __device__ __forceinline__ void es_stage_halo_split(
   __syncthreads();
 }
 
+// The FP32-force passes stage each halo node as its displacement in FP64
+// (24 bytes) and its position as a fixed-point offset from the block's
+// origin (three int32 in units of the quantum q, a power of two) with its
+// FP32 volume (16 bytes); the force pass adds the force coefficient (4
+// bytes).  The bond arithmetic runs in units of q: the position difference
+// is an exact integer difference rounded once to FP32, the displacement
+// difference (the displacements scaled by 1/q, exact) is formed in FP64 and
+// rounded once -- both cancellation-free, a relative error of one FP32
+// rounding -- and a block's halo takes 40 / 44 bytes a node instead of 56,
+// so three blocks fit a CU's LDS instead of two.
+struct EsU64 {
+  double u[3];
+};
+struct alignas(16) EsXV32 {
+  int32_t x[3];
+  float vol;
+};
+
+// fixed-point offset of a coordinate from the origin, in units of q
+__device__ __forceinline__ int32_t es_fix(double x, double o, double inv_q) {
+  return int32_t(__double2int_rn((x - o) * inv_q));
+}
+
+template <bool WITH_P>
+__device__ __forceinline__ void es_stage_halo_r32(
+    const FdArgs<3>& a, const double* __restrict__ P, int64_t lb,
+    const double4& org, const EsDev& m, EsU64* su, EsXV32* sx, float* sp) {
+  const int64_t hb = a.halo_ptr[lb];
+  const int hn = int(a.halo_ptr[lb + 1] - hb);
+  for (int t = threadIdx.x; t < hn; t += 256) {
+    const uint32_t g = uint32_t(a.halo[hb + t]);
+    const double4 xx = ld32(a.x, g);
+    const double4 uu = ld32(a.u, g);
+    EsU64 ru;
+    ru.u[0] = uu.x * m.inv_q;
+    ru.u[1] = uu.y * m.inv_q;
+    ru.u[2] = uu.z * m.inv_q;
+    su[t] = ru;
+    EsXV32 rx;
+    rx.x[0] = es_fix(xx.x, org.x, m.inv_q);
+    rx.x[1] = es_fix(xx.y, org.y, m.inv_q);
+    rx.x[2] = es_fix(xx.z, org.z, m.inv_q);
+    rx.vol = float(ld32(a.vol, g));
+    sx[t] = rx;
+    // force coefficient in units of the bond arithmetic (times q)
+    if (WITH_P) sp[t] = float(ld32(P, g) * m.q);
+  }
+  __syncthreads();
+}
+
 // m_i = sum_j |X| |X| c_ij V_j in list order (weightedVolume, :51-67), over
 // the internal CSR whose rows keep the reference order.
 template <int DIM>
@@ -166,15 +216,24 @@ __global__ __launch_bounds__(256) void k_es_dil(
   const int64_t lb = xcd_block(nblk);
   if (lb >= nblk) return;
   extern __shared__ EsRec es_smem[];
-  // fast staged passes: split arrays (FP64 x, u; volume)
-  constexpr bool SPLIT = TILED && MODE != kModeExact;
+  // fast staged passes: split arrays (FP64 mode: FP64 x, u; volume.
+  // FP32-force mode: FP64 u; FP32 block-relative x and volume)
+  constexpr bool R32 = TILED && MODE == kModeFp32;
+  constexpr bool SPLIT = TILED && MODE == kModeFp64;
   typedef EsVPOf<MODE> VP;
   EsXU* const es_xu = reinterpret_cast<EsXU*>(es_smem);
   VP* const es_vp = reinterpret_cast<VP*>(es_xu + a.halo_cap);
-  if constexpr (SPLIT)
+  EsU64* const es_u64 = reinterpret_cast<EsU64*>(es_smem);
+  EsXV32* const es_xv = reinterpret_cast<EsXV32*>(es_u64 + a.halo_cap);
+  double4 org = make_double4(0., 0., 0., 0.);
+  if constexpr (R32) {
+    org = reinterpret_cast<const double4*>(a.blk_org)[lb];
+    es_stage_halo_r32<false>(a, e.P, lb, org, e.m, es_u64, es_xv, nullptr);
+  } else if constexpr (SPLIT) {
     es_stage_halo_split<false>(a, e.P, lb, es_xu, es_vp);
-  else if constexpr (TILED)
+  } else if constexpr (TILED) {
     es_stage_halo<false>(a, e.P, lb, es_smem);
+  }
   const int lane = threadIdx.x & 63;
   const int64_t s = lb * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (s >= a.nslices) return;  // whole wavefront
@@ -191,6 +250,17 @@ __global__ __launch_bounds__(256) void k_es_dil(
     load3<DIM>(a.u[i], ui);
     mwi = e.mw[i];
   }
+  // the node's fixed-point position and scaled displacement (FP32-force
+  // staged pass)
+  int32_t xqi[3] = {0, 0, 0};
+  double uqi[3] = {0., 0., 0.};
+  if constexpr (R32) {
+    xqi[0] = es_fix(xi[0], org.x, e.m.inv_q);
+    xqi[1] = es_fix(xi[1], org.y, e.m.inv_q);
+    xqi[2] = es_fix(xi[2], org.z, e.m.inv_q);
+    for (int d = 0; d < 3; d++) uqi[d] = ui[d] * e.m.inv_q;
+  }
+  const int64_t hb = TILED ? a.halo_ptr[lb] : 0;
   // (3/m_i) in 3D, (2/m_i) factor2D in 2D, leading factors of the reference
   // product (stateBasedHelperFunctions.cpp:58-71)
   const double c3 = DIM == 3 ? 3. / mwi : (2. / mwi) * e.m.f2d;
@@ -216,10 +286,19 @@ __global__ __launch_bounds__(256) void k_es_dil(
       jn[q] = slot_index(k0 + U + q);
     }
     double xj[U][3], uj[U][3], vj[U];
+    float dxf[U][3], duf[U][3];
     unsigned long long w[U];
 #pragma unroll
     for (int q = 0; q < U; ++q) {
-      if constexpr (SPLIT) {
+      if constexpr (R32) {
+        const EsU64& ru = es_u64[jc[q]];
+        const EsXV32 rx = es_xv[jc[q]];
+        for (int d = 0; d < 3; d++) {
+          dxf[q][d] = float(rx.x[d] - xqi[d]);
+          duf[q][d] = float(ru.u[d] - uqi[d]);
+        }
+        vj[q] = double(rx.vol);
+      } else if constexpr (SPLIT) {
         const EsXU& r = es_xu[jc[q]];
         for (int d = 0; d < 3; d++) {
           xj[q][d] = r.x[d];
@@ -260,20 +339,37 @@ __global__ __launch_bounds__(256) void k_es_dil(
       Real pf[(U + 1) / 2];
       unsigned long long nb[U], band[U], ex[U];
       unsigned long long ex_any = 0ull;
-      const Real vtol = F32 ? Real(e.m.vex_tol_f) : Real(e.m.vex_tol);
-      const Real lowb = F32 ? Real(e.m.low_band_f) : Real(e.m.low_band);
+      // (the staged FP32-force pass works in units of q: lengths of the
+      // edge test and the dilatation factor scaled accordingly)
+      const Real vtol = R32   ? Real(e.m.vex_tol_qf)
+                        : F32 ? Real(e.m.vex_tol_f)
+                              : Real(e.m.vex_tol);
+      const Real lowb = R32   ? Real(e.m.low_band_qf)
+                        : F32 ? Real(e.m.low_band_f)
+                              : Real(e.m.low_band);
       const Real bhi = F32 ? Real(e.m.brk_hi_f) : Real(e.m.brk_hi);
       const Real blo = F32 ? Real(e.m.brk_lo_f) : Real(e.m.brk_lo);
-      const Real cup = F32 ? Real(e.m.check_up_f) : Real(e.m.check_up);
-      const Real rch = F32 ? Real(e.m.rcp_h_f) : Real(e.m.rcp_h);
-      const Real c3r = F32 ? Real(c3f) : Real(c3);
+      const Real cup = R32   ? Real(e.m.check_up_qf)
+                       : F32 ? Real(e.m.check_up_f)
+                             : Real(e.m.check_up);
+      const Real rch = R32   ? Real(e.m.rcp_h_qf)
+                       : F32 ? Real(e.m.rcp_h_f)
+                             : Real(e.m.rcp_h);
+      const Real c3r = R32   ? Real(float(c3 * e.m.q * e.m.q))
+                       : F32 ? Real(c3f)
+                             : Real(c3);
 #pragma unroll
       for (int q = 0; q < U; ++q) {
         const unsigned long long act = __builtin_amdgcn_ballot_w64(k0 + q < len);
         Real dx[3], du[3];
         for (int d = 0; d < 3; d++) {
-          dx[d] = d < DIM ? Real(xj[q][d] - xi[d]) : Real(0);
-          du[d] = d < DIM ? Real(uj[q][d] - ui[d]) : Real(0);
+          if constexpr (R32) {
+            dx[d] = dxf[q][d];
+            du[d] = duf[q][d];
+          } else {
+            dx[d] = d < DIM ? Real(xj[q][d] - xi[d]) : Real(0);
+            du[d] = d < DIM ? Real(uj[q][d] - ui[d]) : Real(0);
+          }
         }
         Real dd = fmaR(dx[1], dx[1], dx[0] * dx[0]);
         Real g = fmaR(du[1], fmaR(Real(2), dx[1], du[1]),
@@ -319,8 +415,21 @@ __global__ __launch_bounds__(256) void k_es_dil(
           if (__builtin_amdgcn_inverse_ballot_w64(ex[q])) {
             const bool fs0 = __builtin_amdgcn_inverse_ballot_w64(w[q]);
             bool fs = fs0;
-            const double d = es_dil_exact(e.m, xi, ui, xj[q], uj[q], vj[q],
-                                          c3, fs);
+            double xs[3], us[3], vs;
+            if constexpr (R32) {
+              // FP64 values of the neighbor from its halo entry
+              const uint32_t g = uint32_t(a.halo[hb + jc[q]]);
+              load3<DIM>(ld32(a.x, g), xs);
+              load3<DIM>(ld32(a.u, g), us);
+              vs = ld32(a.vol, g);
+            } else {
+              for (int d = 0; d < 3; d++) {
+                xs[d] = xj[q][d];
+                us[d] = uj[q][d];
+              }
+              vs = vj[q];
+            }
+            const double d = es_dil_exact(e.m, xi, ui, xs, us, vs, c3, fs);
             fsn = fs && !fs0;
             if (!fs) pf[q / 2] += Real(d);
           }
@@ -406,16 +515,26 @@ __global__ __launch_bounds__(256) void k_es_force(
   const int64_t lb = xcd_block(nblk);
   if (lb >= nblk) return;
   extern __shared__ EsRec es_smem[];
-  // fast staged passes: split arrays (FP64 x, u; volume and force
-  // coefficient)
-  constexpr bool SPLIT = TILED && MODE != kModeExact;
+  // fast staged passes: split arrays (FP64 mode: FP64 x, u; volume and
+  // force coefficient.  FP32-force mode: FP64 u; FP32 block-relative x and
+  // volume; FP32 force coefficient)
+  constexpr bool R32 = TILED && MODE == kModeFp32;
+  constexpr bool SPLIT = TILED && MODE == kModeFp64;
   typedef EsVPOf<MODE> VP;
   EsXU* const es_xu = reinterpret_cast<EsXU*>(es_smem);
   VP* const es_vp = reinterpret_cast<VP*>(es_xu + a.halo_cap);
-  if constexpr (SPLIT)
+  EsU64* const es_u64 = reinterpret_cast<EsU64*>(es_smem);
+  EsXV32* const es_xv = reinterpret_cast<EsXV32*>(es_u64 + a.halo_cap);
+  float* const es_p32 = reinterpret_cast<float*>(es_xv + a.halo_cap);
+  double4 org = make_double4(0., 0., 0., 0.);
+  if constexpr (R32) {
+    org = reinterpret_cast<const double4*>(a.blk_org)[lb];
+    es_stage_halo_r32<true>(a, e.P, lb, org, e.m, es_u64, es_xv, es_p32);
+  } else if constexpr (SPLIT) {
     es_stage_halo_split<true>(a, e.P, lb, es_xu, es_vp);
-  else if constexpr (TILED)
+  } else if constexpr (TILED) {
     es_stage_halo<true>(a, e.P, lb, es_smem);
+  }
   const int lane = threadIdx.x & 63;
   const int64_t s = lb * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (s >= a.nslices) return;
@@ -438,7 +557,18 @@ __global__ __launch_bounds__(256) void k_es_force(
     if (kExact) thi = e.theta[i];
     fl = a.flags[i];
   }
-  const float Pif = float(Pi), Qif = float(Qi);
+  // FP32-force staged pass: force coefficient times q (the bond arithmetic
+  // runs in units of q), fixed-point position, scaled displacement
+  const float Pif = float(R32 ? Pi * e.m.q : Pi), Qif = float(Qi);
+  int32_t xqi[3] = {0, 0, 0};
+  double uqi[3] = {0., 0., 0.};
+  if constexpr (R32) {
+    xqi[0] = es_fix(xi[0], org.x, e.m.inv_q);
+    xqi[1] = es_fix(xi[1], org.y, e.m.inv_q);
+    xqi[2] = es_fix(xi[2], org.z, e.m.inv_q);
+    for (int d = 0; d < 3; d++) uqi[d] = ui[d] * e.m.inv_q;
+  }
+  const int64_t hb = TILED ? a.halo_ptr[lb] : 0;
   const int32_t* __restrict__ colS = a.col + cb;
   const uint16_t* __restrict__ colL = a.colL + cb;
   const unsigned long long* __restrict__ bits_in = bits_ro + bb;
@@ -460,11 +590,23 @@ __global__ __launch_bounds__(256) void k_es_force(
       jn[q] = slot_index(k0 + U + q);
     }
     double xj[U][3], uj[U][3], vj[U], Pj[U], Qj[U], thj[U];
+    float dxf[U][3], duf[U][3];
     unsigned long long w[U];
 #pragma unroll
     for (int q = 0; q < U; ++q) {
       const uint32_t j = jc[q];
-      if constexpr (SPLIT) {
+      if constexpr (R32) {
+        const EsU64& ru = es_u64[j];
+        const EsXV32 rx = es_xv[j];
+        for (int d = 0; d < 3; d++) {
+          dxf[q][d] = float(rx.x[d] - xqi[d]);
+          duf[q][d] = float(ru.u[d] - uqi[d]);
+        }
+        vj[q] = double(rx.vol);
+        Pj[q] = double(es_p32[j]);
+        Qj[q] = 0.;
+        thj[q] = 0.;
+      } else if constexpr (SPLIT) {
         const EsXU& r = es_xu[j];
         for (int d = 0; d < 3; d++) {
           xj[q][d] = r.x[d];
@@ -555,8 +697,13 @@ __global__ __launch_bounds__(256) void k_es_force(
             __builtin_amdgcn_ballot_w64(k0 + q < len) & ~w[q];
         float dx[3], du[3], Y[3];
         for (int d = 0; d < 3; d++) {
-          dx[d] = d < DIM ? float(xj[q][d] - xi[d]) : 0.f;
-          du[d] = d < DIM ? float(uj[q][d] - ui[d]) : 0.f;
+          if constexpr (R32) {
+            dx[d] = dxf[q][d];
+            du[d] = duf[q][d];
+          } else {
+            dx[d] = d < DIM ? float(xj[q][d] - xi[d]) : 0.f;
+            du[d] = d < DIM ? float(uj[q][d] - ui[d]) : 0.f;
+          }
           Y[d] = dx[d] + du[d];
         }
         float yy = __builtin_fmaf(Y[1], Y[1], Y[0] * Y[0]);
@@ -568,15 +715,24 @@ __global__ __launch_bounds__(256) void k_es_force(
         const float ry = __builtin_amdgcn_rsqf(yy);
         const float rx = __builtin_amdgcn_rsqf(dd);
         const float lx = dd * rx;
+        const float lowb = R32 ? e.m.low_band_qf : e.m.low_band_f;
+        const float vtol = R32 ? e.m.vex_tol_qf : 3e-5f * e.m.low_band_f;
+        const float cup = R32 ? e.m.check_up_qf : e.m.check_up_f;
+        const float rch = R32 ? e.m.rcp_h_qf : e.m.rcp_h_f;
         const unsigned long long vexm =
-            __builtin_amdgcn_ballot_w64(fabsf(lx - e.m.low_band_f) <=
-                                        3e-5f * e.m.low_band_f) &
-            live;
-        float c = fminf((e.m.check_up_f - lx) * e.m.rcp_h_f, 1.0f);
+            __builtin_amdgcn_ballot_w64(fabsf(lx - lowb) <= vtol) & live;
+        float c = fminf((cup - lx) * rch, 1.0f);
         if (vexm != 0ull) {  // rare: lengths at the correction's edge
           if (__builtin_amdgcn_inverse_ballot_w64(vexm)) {
-            double X0 = xj[q][0] - xi[0], X1 = xj[q][1] - xi[1];
-            double X2 = DIM == 3 ? xj[q][2] - xi[2] : 0.;
+            double xs[3];
+            if constexpr (R32) {
+              // FP64 position of the neighbor from its halo entry
+              load3<DIM>(ld32(a.x, uint32_t(a.halo[hb + jc[q]])), xs);
+            } else {
+              for (int d = 0; d < 3; d++) xs[d] = xj[q][d];
+            }
+            double X0 = xs[0] - xi[0], X1 = xs[1] - xi[1];
+            double X2 = DIM == 3 ? xs[2] - xi[2] : 0.;
             c = float(es_corr_exact(e.m, sqrt(X0 * X0 + X1 * X1 + X2 * X2)));
           }
         }
@@ -631,7 +787,7 @@ EsArgs<DIM> make_es_args(nlm_ctx* c) {
 // the two split arrays; the exact pass 1 keeps one record).
 size_t es_lds_bytes(const FdState& s) {
   const size_t fast = s.precision == kModeFp32
-                          ? sizeof(EsXU) + sizeof(EsVPOf<kModeFp32>)
+                          ? sizeof(EsU64) + sizeof(EsXV32) + sizeof(float)
                           : sizeof(EsXU) + sizeof(EsVPOf<kModeFp64>);
   return size_t(s.halo_max + 1) *
          (s.precision == kModeExact ? sizeof(EsRec) : fast);

@@ -1262,21 +1262,35 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
     hv[a] = p->vol[i];
   }
   // Per 256-node block: the centre of its nodes' bounding box, the origin of
-  // the FP32 block-relative positions.
-  std::vector<double> borg(size_t(L.nblocks) * 2, 0.);
-  if (dim == 2) {
-    for (int64_t b = 0; b < L.nblocks; b++) {
-      const int64_t a0 = 256 * b, a1 = std::min<int64_t>(n, a0 + 256);
-      double lo[2] = {hx[a0 * 2], hx[a0 * 2 + 1]}, hi[2] = {lo[0], lo[1]};
-      for (int64_t a = a0; a < a1; a++)
-        for (int d = 0; d < 2; d++) {
-          lo[d] = std::min(lo[d], hx[a * 2 + d]);
-          hi[d] = std::max(hi[d], hx[a * 2 + d]);
-        }
-      const double o[2] = {0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1])};
-      borg[2 * b] = o[0];
-      borg[2 * b + 1] = o[1];
-    }
+  // the FP32 block-relative positions (double2 per block in 2D, double4 in
+  // 3D).
+  std::vector<double> borg(size_t(L.nblocks) * stride, 0.);
+  for (int64_t b = 0; b < L.nblocks; b++) {
+    const int64_t a0 = 256 * b, a1 = std::min<int64_t>(n, a0 + 256);
+    if (a0 >= n) break;
+    double lo[3], hi[3];
+    for (int d = 0; d < dim; d++) lo[d] = hi[d] = hx[a0 * stride + d];
+    for (int64_t a = a0; a < a1; a++)
+      for (int d = 0; d < dim; d++) {
+        lo[d] = std::min(lo[d], hx[a * stride + d]);
+        hi[d] = std::max(hi[d], hx[a * stride + d]);
+      }
+    for (int d = 0; d < dim; d++) borg[stride * b + d] = 0.5 * (lo[d] + hi[d]);
+  }
+  // largest offset of a block's node or halo node from the block's origin
+  // (per component): the range of the fixed-point positions of the
+  // state-based FP32-force passes
+  double org_range = 0.;
+  for (int64_t b = 0; b < L.nblocks; b++) {
+    auto span = [&](int64_t a) {
+      for (int d = 0; d < dim; d++)
+        org_range = std::max(
+            org_range, std::fabs(hx[a * stride + d] - borg[stride * b + d]));
+    };
+    for (int64_t a = 256 * b; a < std::min<int64_t>(n, 256 * b + 256); a++)
+      span(a);
+    for (int64_t t = L.halo_ptr[b]; t < L.halo_ptr[b + 1]; t++)
+      span(L.halo[size_t(t)]);
   }
   if ((rc = up(s.blk_org, borg.data(), sizeof(double) * borg.size())))
     return rc;
@@ -1459,6 +1473,19 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
     // exact-evaluation band of both fast modes covers that whole window.
     e.vex_tol_f = 3e-5f * e.low_band_f;
     e.vex_tol = 3e-5 * e.low_band;
+    // quantum of the fixed-point positions: a power of two with every
+    // offset from a block's origin below 2^29 q, so the difference of two
+    // (at most 2^30 q) is an exact int32 and the quantization (q/2) is ~1e-9
+    // of the range
+    const double rng = std::max(org_range, q.h);
+    e.q = std::ldexp(1.0, std::ilogb(rng) + 1 - 29);
+    e.inv_q = 1.0 / e.q;
+    e.q_f = float(e.q);
+    e.q2_f = float(e.q * e.q);
+    e.check_up_qf = float(e.check_up * e.inv_q);
+    e.low_band_qf = float(e.low_band * e.inv_q);
+    e.rcp_h_qf = float(e.q / q.h);
+    e.vex_tol_qf = 3e-5f * e.low_band_qf;
     e.brk_lo = e.breaking ? e.sc_break - kEsGuard64 * e.sc_break
                           : std::numeric_limits<double>::infinity();
     e.brk_hi = e.breaking ? e.sc_break + kEsGuard64 * e.sc_break

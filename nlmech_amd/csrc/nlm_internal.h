@@ -117,6 +117,13 @@ struct EsDev {
   float vex_tol_f, brk_lo_f, brk_hi_f;
   // the same bounds in FP64 for the FP64 mode
   double vex_tol, brk_lo, brk_hi;
+  // FP32-force staged passes (3D): positions are fixed-point offsets from
+  // the block's origin in units of the quantum q (a power of two), and the
+  // bond arithmetic runs in units of q: 1/q, q^2 (dilatation factor) and q
+  // (force coefficient) scale it back; lengths of the edge tests in q
+  double q, inv_q;
+  float q2_f, q_f;
+  float check_up_qf, low_band_qf, rcp_h_qf, vex_tol_qf;
 };
 
 // Slice layout of the neighbor lists: 64 consecutive nodes (one wavefront)
