@@ -1225,6 +1225,212 @@ struct ES {
     return corr[k] * r;
   }
 
+  // Strain energy of bond k of node i: the second value of
+  // ElasticState::getBondEF (ElasticState.cpp:250-262).
+  double bond_energy(int64_t i, int64_t k) {
+    int64_t j = col[k];
+    double w = 1;
+    double alpha_s = 0., alpha_d = 0., e_s = 0., e_d = 0., alpha = 0.;
+    P3 X = x[j] - x[i];
+    switch (dim) {
+      case 1:
+        alpha = m.E / mw[i];
+        break;
+      case 2:
+        if (!m.plane_strain)
+          alpha_s = (9. / mw[i]) *
+                    (m.K + std::pow((m.nu + 1.) / (2. * m.nu - 1.), 2) * m.mu /
+                               9.);
+        else
+          alpha_s = (9. / mw[i]) * (m.K + m.mu / 9.);
+        alpha_d = (8. / mw[i]) * m.mu;
+        e_s = theta[i] * length(X) / 3.;
+        e_d = ext[k] - e_s;
+        break;
+      default:
+        alpha_s = (9. / mw[i]) * m.K;
+        alpha_d = (15. / mw[i]) * m.mu;
+        e_s = theta[i] * length(X) / 3.;
+        break;
+    }
+    if (dim == 1)
+      return 0.5 * alpha * w * ext[k] * ext[k] * corr[k] * vol[j];
+    return 0.5 * w * (alpha_s * e_s * e_s + alpha_d * e_d * e_d) * corr[k] *
+           vol[j];
+  }
+
+  // 3x3 matrices of ElasticState's strain and stress (blaze::StaticMatrix
+  // in the reference)
+  struct M33 {
+    double a[3][3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};
+  };
+  static M33 outer(const P3& p, const P3& q) {  // Point3::toMatrix, point.h
+    const double pv[3] = {p.x, p.y, p.z}, qv[3] = {q.x, q.y, q.z};
+    M33 r;
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) r.a[a][b] = pv[a] * qv[b];
+    return r;
+  }
+
+  // ElasticState::getStrain (ElasticState.cpp:336-345) with
+  // deformation_gradient (:304-334): F = (sum_j Y X^T w c_ij V_j) (K^-1)^T
+  // with the shape tensor K = sum_j X X^T w c V_j of K_shape_tensor
+  // (:287-302), whose volume-correction index is never
+  // advanced (every neighbor takes the first one's correction: kept);
+  // strain = (F + F^T)/2 - I on the dim x dim block.
+  M33 strain(int64_t i) {
+    const double w = 1;
+    const int64_t k0 = row_ptr[i];
+    M33 K, T;
+    for (int64_t k = row_ptr[i]; k < row_ptr[i + 1]; k++) {
+      const int64_t j = col[k];
+      const P3 X = x[j] - x[i];
+      const P3 Y = (x[j] + u[j]) - (x[i] + u[i]);
+      const M33 xx = outer(X, X), yx = outer(Y, X);
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+          K.a[a][b] += xx.a[a][b] * w * corr[k0] * vol[j];
+          T.a[a][b] += yx.a[a][b] * w * corr[k] * vol[j];
+        }
+    }
+    // blaze::inv of the dim x dim block, transposed into a zero 3x3
+    M33 Si;
+    if (dim == 1) {
+      Si.a[0][0] = 1. / K.a[0][0];
+    } else if (dim == 2) {
+      const double det = K.a[0][0] * K.a[1][1] - K.a[0][1] * K.a[1][0];
+      const double inv[2][2] = {{K.a[1][1] / det, -K.a[0][1] / det},
+                                {-K.a[1][0] / det, K.a[0][0] / det}};
+      for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) Si.a[a][b] = inv[b][a];
+    } else {
+      const auto& A = K.a;
+      const double c00 = A[1][1] * A[2][2] - A[1][2] * A[2][1];
+      const double c01 = A[1][2] * A[2][0] - A[1][0] * A[2][2];
+      const double c02 = A[1][0] * A[2][1] - A[1][1] * A[2][0];
+      const double det = A[0][0] * c00 + A[0][1] * c01 + A[0][2] * c02;
+      double inv[3][3];
+      inv[0][0] = c00 / det;
+      inv[1][0] = c01 / det;
+      inv[2][0] = c02 / det;
+      inv[0][1] = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) / det;
+      inv[1][1] = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) / det;
+      inv[2][1] = (A[0][1] * A[2][0] - A[0][0] * A[2][1]) / det;
+      inv[0][2] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) / det;
+      inv[1][2] = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) / det;
+      inv[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) / det;
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) Si.a[a][b] = inv[b][a];
+    }
+    M33 F, E;
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++)
+        for (int c = 0; c < 3; c++) F.a[a][b] += T.a[a][c] * Si.a[c][b];
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++)
+        E.a[a][b] = (F.a[a][b] + F.a[b][a]) * 0.5 -
+                    ((a == b && a < dim) ? 1. : 0.);
+    return E;
+  }
+
+  // ElasticState::getStress (ElasticState.cpp:432-471) with
+  // K_modulus_tensor (:359-430) and dirac_delta (:347-357):
+  // stress = sum_j sum_k (K(j, k) (strain X_k)) X_j^T c_k V_k c_j V_j.
+  M33 stress(int64_t i, const M33& eps_i) {
+    const double w = 1;
+    M33 S;
+    for (int64_t kn = row_ptr[i]; kn < row_ptr[i + 1]; kn++) {
+      const int64_t j = col[kn];
+      const P3 Xj = x[j] - x[i];
+      const P3 M = Xj / length(Xj);
+      for (int64_t km = row_ptr[i]; km < row_ptr[i + 1]; km++) {
+        const int64_t kk = col[km];
+        const P3 Xk = x[kk] - x[i];
+        const double volume = corr[km] * vol[kk] * corr[kn] * vol[j];
+        // dirac_delta: essentiallyEqual(|Xk - Xj|, 0) holds only at 0
+        const double delta =
+            length(Xk - Xj) == 0. ? 1. / vol[kk] * corr[km] : 0.;
+        M33 Km;
+        if (dim == 1) {
+          const double alpha = m.E / mw[i];
+          const M33 mm = outer(M, M);
+          for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++)
+              Km.a[a][b] = alpha * w * mm.a[a][b] * delta;
+        } else {
+          double alpha_s, alpha_d, lead;
+          if (dim == 2) {
+            const double Nu = (3. * m.K - 2. * m.mu) / (2. * (3. * m.K + m.mu));
+            alpha_s = (9. / mw[i]) *
+                      (m.K + std::pow((Nu + 1.) / (2. * Nu - 1.), 2) * m.mu / 9.);
+            alpha_d = (8. / mw[i]) * m.mu;
+            const double alpha_sb = (2. * m.factor2D * alpha_s -
+                                     (3. - 2. * m.factor2D) * alpha_d) /
+                                    3.;
+            lead = (alpha_sb - alpha_d) / mw[i];
+          } else {
+            alpha_s = (9. / mw[i]) * m.K;
+            alpha_d = (15. / mw[i]) * m.mu;
+            lead = (alpha_s - alpha_d) / mw[i];
+          }
+          const M33 jk = outer(Xj, Xk), mm = outer(M, M);
+          for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++)
+              Km.a[a][b] = lead * w * w * jk.a[a][b] +
+                           alpha_d * w * mm.a[a][b] * delta;
+        }
+        // res = K (strain X_k); tmp(o, :) = res[o] X_j
+        const double xk[3] = {Xk.x, Xk.y, Xk.z};
+        double ex[3], res[3];
+        for (int a = 0; a < 3; a++) {
+          ex[a] = 0.;
+          for (int b = 0; b < 3; b++) ex[a] += eps_i.a[a][b] * xk[b];
+        }
+        for (int a = 0; a < 3; a++) {
+          res[a] = 0.;
+          for (int b = 0; b < 3; b++) res[a] += Km.a[a][b] * ex[b];
+        }
+        const double xj[3] = {Xj.x, Xj.y, Xj.z};
+        for (int a = 0; a < 3; a++)
+          for (int b = 0; b < 3; b++) S.a[a][b] += res[a] * xj[b] * volume;
+      }
+    }
+    return S;
+  }
+
+  // The output fields of QuasiStaticModel::computeForces(true)
+  // (QuasiStaticModel.cpp:206-229) at the current state: the strain energy
+  // of every bond is added in float to energy_acc[i] (the reference keeps a
+  // std::vector<float> and adds (float) per bond, never clearing it), the
+  // strain and stress tensors are written as (00, 11, 22, 12, 02, 01)
+  // (vtkWriter.cpp:213-233).
+  void fields(float* energy_acc, double* strain6, double* stress6) {
+    dilatation();
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+      if (energy_acc) {
+        float e = energy_acc[i];
+        for (int64_t k = row_ptr[i]; k < row_ptr[i + 1]; k++) {
+          if (broken[k]) continue;
+          e += float(bond_energy(i, k));
+        }
+        energy_acc[i] = e;
+      }
+      if (!strain6 && !stress6) continue;
+      const M33 E = strain(i);
+      auto put = [&](double* o, const M33& A) {
+        o[6 * i + 0] = A.a[0][0];
+        o[6 * i + 1] = A.a[1][1];
+        o[6 * i + 2] = A.a[2][2];
+        o[6 * i + 3] = A.a[1][2];
+        o[6 * i + 4] = A.a[0][2];
+        o[6 * i + 5] = A.a[0][1];
+      };
+      if (strain6) put(strain6, E);
+      if (stress6) put(stress6, stress(i, E));
+    }
+  }
+
   // QuasiStaticModel::computeForces (QuasiStaticModel.cpp:169-239), run
   // serially: f_i += T_ij V_j, f_j -= T_ij V_i.  Adds into ff.
   void forces(std::vector<P3>& ff) {
@@ -1466,6 +1672,10 @@ void ora_es_get_node_fields(void* h, double* theta, double* mvol) {
 }
 double ora_es_time(void* h) { return static_cast<ES*>(h)->time; }
 void ora_es_set_time(void* h, double t) { static_cast<ES*>(h)->time = t; }
+void ora_es_fields(void* h, float* energy_acc, double* strain6,
+                   double* stress6) {
+  static_cast<ES*>(h)->fields(energy_acc, strain6, stress6);
+}
 void ora_es_extension_rounding_bound(void* h, double* out) {
   static_cast<ES*>(h)->extension_rounding_bound(out);
 }

@@ -234,6 +234,10 @@ double ora_es_time(void* h);
 /* Reference-faithful timing variant of the FD force loop (same results). */
 void ora_fd_set_faithful(void* h, int on);
 void ora_es_set_time(void* h, double t);
+/* Output fields of QuasiStaticModel::computeForces(true): strain energy added
+ * in float to energy_acc, strain and stress tensors (6 per node). */
+void ora_es_fields(void* h, float* energy_acc, double* strain6,
+                   double* stress6);
 /* Test tolerance: per-node bound on the force difference caused by the
  * reference's rounding of |Y| - |X| (3D). */
 void ora_es_extension_rounding_bound(void* h, double* out);

@@ -149,6 +149,7 @@ SIGNATURES = {
     "ora_es_get_node_fields": (None, [_P, _D, _D]),
     "ora_es_time": (C.c_double, [_P]),
     "ora_es_set_time": (None, [_P, C.c_double]),
+    "ora_es_fields": (None, [_P, C.POINTER(C.c_float), _D, _D]),
     "ora_es_extension_rounding_bound": (None, [_P, _D]),
     "ora_fd_set_faithful": (None, [_P, C.c_int]),
     "ora_es_force_abs_sum": (None, [_P, _D]),
@@ -512,6 +513,19 @@ class OracleES:
 
     def set_time(self, t):
         self.lib.ora_es_set_time(self.h, float(t))
+
+    def fields(self, energy=None):
+        """(strain energy, strain[n, 6], stress[n, 6]) at the current state:
+        the output fields of QuasiStaticModel::computeForces(true); the
+        strain energy is added in float to `energy` (zeros if None)."""
+        n = self.p.n
+        e = (np.zeros(n, np.float32) if energy is None
+             else np.ascontiguousarray(energy, np.float32).copy())
+        st = np.zeros((n, 6))
+        ss = np.zeros((n, 6))
+        self.lib.ora_es_fields(self.h, _p(e, C.c_float), _p(st, C.c_double),
+                               _p(ss, C.c_double))
+        return e, st, ss
 
     def extension_rounding_bound(self):
         """Per-node bound (3D) on the force difference that the reference's

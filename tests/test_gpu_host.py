@@ -450,3 +450,132 @@ def test_postprocessing_tags_match_oracle(host_bin, oracle, tmp_path):
                                                                rel=1e-12)
     assert (fE > 0).sum() > 0
     o.close()
+
+
+def _copy_decks(tmp_path, pairs):
+    src = GOLDEN / "decks"
+    for a, b in pairs:
+        shutil.copy(src / a, tmp_path / b)
+
+
+def test_reference_restart_ctest(host_bin, oracle, tmp_path):
+    """explicit.restart.{mesh,run,restart,compare} (test/CMakeLists.txt:
+    103-133) with the reference's own decks (examples/fdModel/restart):
+    uniform triangle mesh, 5 velocity-Verlet steps under a linear force
+    ramp, restart from output_4.vtu at step 4, dc Displacement L2 < 1e-7
+    between output_5.vtu and restart_output_5.vtu.  The full run is also
+    checked against the oracle on the same mesh (linear influence function,
+    Check_Sc_Factor 10, force BC), and the restarted output equals the
+    uninterrupted one bit for bit (outputs carry the broken-bond bits)."""
+    _copy_decks(tmp_path, (("restart_input_mesh.yaml", "input_mesh.yaml"),
+                           ("restart_input.yaml", "input.yaml"),
+                           ("restart_restart.yaml", "restart.yaml"),
+                           ("restart_input_compare.yaml",
+                            "input_compare.yaml")))
+    progress("[gpu] reference restart decks")
+    run_tool(host_bin / "nlm_mesh", "-i", "input_mesh.yaml", "-d", "2",
+             cwd=tmp_path)
+    run_tool(host_bin / "nlm_fd", "-i", "input.yaml", cwd=tmp_path)
+    run_tool(host_bin / "nlm_fd", "-i", "restart.yaml", cwd=tmp_path)
+    rc, _ = run_tool(host_bin / "nlm_dc", "-i", "input_compare.yaml", "-k",
+                     "fd_simple", cwd=tmp_path, check=False)
+    lines = (tmp_path / "compare.txt").read_text().splitlines()
+    progress(f"[gpu]   dc: {lines[-1]}")
+    l2 = float(lines[-1].split(",")[0])
+    assert rc == 0 and l2 < 1e-7
+    full = read_vtu(str(tmp_path / "output_5.vtu"))
+    rest = read_vtu(str(tmp_path / "restart_output_5.vtu"))
+    for k in ("Displacement", "Velocity", "Force", "Bond_State"):
+        assert np.array_equal(full[k], rest[k]), k
+    # the full run against the oracle: the same deck with Node_Volume added
+    # to the output tags (the nodal volumes the GPU computed from the
+    # triangles), run beside it
+    vdir = tmp_path / "vol"
+    vdir.mkdir()
+    deck = (tmp_path / "input.yaml").read_text()
+    deck = deck.replace("File: mesh.vtu", f"File: {tmp_path / 'mesh.vtu'}")
+    deck = deck.replace("    - Damage_Z\n", "    - Damage_Z\n    - Node_Volume\n")
+    assert "Node_Volume" in deck
+    (vdir / "input.yaml").write_text(deck)
+    run_tool(host_bin / "nlm_fd", "-i", "input.yaml", cwd=vdir)
+    tagged = read_vtu(str(vdir / "output_5.vtu"))
+    assert np.array_equal(tagged["Displacement"], full["Displacement"])
+    m = read_vtu(str(tmp_path / "mesh.vtu"))
+    x = m["Points"].astype(np.float64)
+    vol = tagged["Node_Volume"]
+    fbc = [pr.BC("rectangle", [2], "linear", [-5.0e9], "constant", [1.0],
+                 x1=0.0, y1=0.0, x2=0.1, y2=0.002)]
+    mat = pr.rnp_material(dim=2, E=3.24e9, Gc=500.0, density=1200.0,
+                          check_sc=10.0, inf_type=1, inf_params=())
+    p = pr.Problem(name="restart", dim=2, x=x, vol=np.asarray(vol, float),
+                   horizon=0.002, h=0.002 / 4, dt=1e-7 / 5, nsteps=5,
+                   material=mat, fbc=fbc)
+    o = oracle.OracleFD(p)
+    o.init()
+    o.step(5, scheme="verlet")
+    u_ref, v_ref, f_ref = o.get_state()
+    bytes_ref = o.get_bond_bytes()
+    o.close()
+    u = full["Displacement"]
+    err = np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref)
+    progress(f"[gpu]   full run vs oracle: rel L2 {err:.2e}")
+    assert err <= 1e-9
+    assert np.array_equal(full["Bond_State"], bytes_ref)
+
+
+def test_reference_crack_propagation_deck(host_bin, oracle, tmp_path):
+    """examples/fdModel/crack_propagation run by nlm_fd, shortened to its
+    first 500 of 35,000 steps (same dt = 4e-9 s), outputs at steps 250 and
+    500 (Node_Volume added to the tags, which the output writes with post-
+    processing on, as model/util.cpp does), checked against the oracle at both
+    outputs: relative L2 displacement error <= 1e-9 and identical broken-
+    bond bytes.  The deck exercises the uniform triangle mesh, the linear
+    influence function without parameters, Check_Sc_Factor 10, a vertical
+    (Orientation -1) pre-crack, a fixed top layer and bottom layers pulled
+    apart at -/+1 m/s, and the 'constant' spatial function without
+    parameters (multiplier 1; the reference reads params[0] there,
+    uLoading.cpp:206-208)."""
+    _copy_decks(tmp_path, (("crack_propagation_mesh.yaml", "input_mesh.yaml"),
+                           ("crack_propagation.yaml", "input.yaml")))
+    deck = (tmp_path / "input.yaml").read_text()
+    deck = (deck.replace("Final_Time: 0.000140", "Final_Time: 0.000002")
+                .replace("Time_Steps: 35000", "Time_Steps: 500")
+                .replace("Output_Interval: 500", "Output_Interval: 250")
+                .replace("Enable_PostProcessing: false",
+                         "Enable_PostProcessing: true")
+                .replace("    - Damage_Z\n", "    - Damage_Z\n    - Node_Volume\n"))
+    assert "Time_Steps: 500 " in deck or "Time_Steps: 500\n" in deck
+    (tmp_path / "input.yaml").write_text(deck)
+    progress("[gpu] reference crack_propagation deck (500 steps)")
+    run_tool(host_bin / "nlm_mesh", "-i", "input_mesh.yaml", "-d", "2",
+             cwd=tmp_path)
+    run_tool(host_bin / "nlm_fd", "-i", "input.yaml", cwd=tmp_path)
+    outs = [read_vtu(str(tmp_path / f"output_{k}.vtu")) for k in (1, 2)]
+    m = read_vtu(str(tmp_path / "mesh.vtu"))
+    x = m["Points"].astype(np.float64)
+    vol = np.asarray(outs[0]["Node_Volume"], np.float64)
+    ubc = [pr.BC("rectangle", [1, 2], "constant", [0.0], "constant", [1.0],
+                 x1=0.0, y1=0.098, x2=0.1, y2=0.1),
+           pr.BC("rectangle", [1], "linear", [-1.0], "constant", [1.0],
+                 x1=0.0, y1=0.0, x2=0.05, y2=0.002),
+           pr.BC("rectangle", [1], "linear", [1.0], "constant", [1.0],
+                 x1=0.050125, y1=0.0, x2=0.1, y2=0.002)]
+    cracks = [pr.CrackSpec(-1, (0.050005, 0.0), (0.050005, 0.02))]
+    mat = pr.rnp_material(dim=2, E=3.75e10, Gc=500.0, density=1200.0,
+                          check_sc=10.0, inf_type=1, inf_params=())
+    p = pr.Problem(name="crack_propagation", dim=2, x=x, vol=vol,
+                   horizon=0.002, h=0.002 / 4, dt=0.000002 / 500,
+                   nsteps=500, material=mat, ubc=ubc, cracks=cracks)
+    o = oracle.OracleFD(p)
+    o.init()
+    assert o.get_bonds().sum() > 0          # the pre-crack
+    for k, out in enumerate(outs):
+        o.step(250)
+        u_ref, _, _ = o.get_state()
+        err = np.linalg.norm(out["Displacement"] - u_ref) / \
+            np.linalg.norm(u_ref)
+        progress(f"[gpu]   step {250 * (k + 1)}: rel L2 {err:.2e}, "
+                 f"{int(o.get_bonds().sum())} broken bonds")
+        assert err <= 1e-9
+        assert np.array_equal(out["Bond_State"], o.get_bond_bytes())
+    o.close()
