@@ -162,7 +162,8 @@ int nlm_neighbor_get(nlm_ctx* ctx, int64_t* row_ptr, int32_t* col);
  * otherwise interior[i] != 0 marks node i as breakable
  * (InteriorFlags::getInteriorFlag, interiorFlags.cpp:228-231). */
 typedef struct {
-  int dim; /* 2 or 3 */
+  int dim; /* 1, 2 or 3; a 1D model (nodes on a line, y = z = 0) runs on the
+              2D kernels with every y dof held at zero */
   int64_t n;
   const double* x;   /* [3n] reference coordinates, Mesh::getNodes */
   const double* vol; /* [n] nodal volumes, Mesh::getNodalVolumes */
@@ -324,6 +325,18 @@ int nlm_fd_reaction_force(nlm_ctx* ctx, const double* box, double horizon,
  * the weighted volume m_i (VolumeCorrection::weightedVolume,
  * volumeCorrection.cpp:51-67), reference node order; either may be NULL. */
 int nlm_fd_get_node_fields(nlm_ctx* ctx, double* theta, double* mw);
+/* State-based material: the output fields of QuasiStaticModel::
+ * computeForces(true) (QuasiStaticModel.cpp:206-229) at the current state,
+ * reference node order.  strain_energy (in/out, float like the reference's
+ * vector): the strain energy of every unbroken bond (ElasticState::getBondEF,
+ * ElasticState.cpp:250-262) is added to it, as the reference adds at every
+ * load step without clearing; strain, stress: 6 per node, (xx, yy, zz, yz,
+ * xz, xy) = entries (0,0), (1,1), (2,2), (1,2), (0,2), (0,1) of
+ * ElasticState::getStrain / getStress (:336-345, :432-471), the layout the
+ * reference's .vtu writer uses (vtkWriter.cpp:213-233).  Any pointer may be
+ * NULL. */
+int nlm_fd_es_fields(nlm_ctx* ctx, float* strain_energy, double* strain,
+                     double* stress);
 
 /* Measurement: runs nlm_fd_step(nsteps) bracketed by HIP events recorded on
  * the context's stream; returns the elapsed milliseconds of the whole call

@@ -170,6 +170,7 @@ SIGNATURES = {
     "nlm_fd_compute_forces": (C.c_int, [_P]),
     "nlm_fd_postproc": (C.c_int, [_P, _F, _F, _D]),
     "nlm_fd_postproc_noalloc": (C.c_int, [_P, _F, _F, _D]),
+    "nlm_fd_es_fields": (C.c_int, [_P, _F, _D, _D]),
     "nlm_fd_external_force": (C.c_int, [_P, C.c_double, _D]),
     "nlm_fd_reaction_force": (C.c_int, [_P, _D, C.c_double, _D, _D]),
     "nlm_fd_timed_steps": (C.c_int, [_P, C.c_int, _D, _D]),

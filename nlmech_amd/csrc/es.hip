@@ -488,6 +488,10 @@ __global__ __launch_bounds__(256) void k_es_dil(
     // EXACT keeps alpha_s = (9/m) K (ElasticState.cpp:230-233); the fast
     // modes fold t_ij = alpha_s theta_i |X| / 3 into P_i |X|.
     e.P[i] = MODE == kModeExact ? (9. / mwi) * e.m.K : 3. * e.m.K * th / mwi;
+  } else if (e.m.one_d) {
+    // 1D: t_ij = (E/m_i) e_ij (ElasticState.cpp:187-193)
+    e.P[i] = 0.;
+    e.Q[i] = e.m.E / mwi;
   } else {
     const double as = (9. / mwi) * e.m.Kc;
     const double ad = (8. / mwi) * e.m.mu;
@@ -554,7 +558,8 @@ __global__ __launch_bounds__(256) void k_es_force(
     load3<DIM>(a.u[i], ui);
     Pi = e.P[i];
     if (kNeedQ) Qi = e.Q[i];
-    if (kExact) thi = e.theta[i];
+    // (1D: no dilatation term, t = Q e)
+    if (kExact) thi = e.m.one_d ? 0. : e.theta[i];
     fl = a.flags[i];
   }
   // FP32-force staged pass: force coefficient times q (the bond arithmetic
@@ -635,7 +640,7 @@ __global__ __launch_bounds__(256) void k_es_force(
         vj[q] = ld32(a.vol, j);
         Pj[q] = ld32(e.P, j);
         Qj[q] = kNeedQ ? ld32(e.Q, j) : 0.;
-        thj[q] = kExact ? ld32(e.theta, j) : 0.;
+        thj[q] = kExact && !e.m.one_d ? ld32(e.theta, j) : 0.;
       }
       const int kq = k0 + q;
       w[q] = bits_in[kq < wlast ? kq : wlast];
@@ -992,7 +997,256 @@ int es_autotune(nlm_ctx* c, int* chosen, double* best_us) {
   return NLM_OK;
 }
 
+// ---- output fields of the quasi-static model ------------------------------
+
+namespace {
+
+// Output fields of QuasiStaticModel::computeForces(true)
+// (QuasiStaticModel.cpp:206-229) at the current state, one thread per node,
+// rows in list order: the strain energy of every unbroken bond
+// (ElasticState::getBondEF's second value, ElasticState.cpp:250-262) added
+// in FP32 to energy[i], the strain tensor (getStrain, :336-345, with
+// deformation_gradient :304-334 and K_shape_tensor :287-302, whose
+// volume-correction index stays at the row's first bond) and the stress
+// tensor (getStress :432-471 with K_modulus_tensor :359-430 and dirac_delta
+// :347-357), written as (00, 11, 22, 12, 02, 01).  The stress is summed in
+// the factorized form of the reference's double loop over (j, k):
+//   sum_jk K(j,k) (eps X_k) X_j^T c_k V_k c_j V_j
+//     = lead w^2 (sum_k (X_k^T eps X_k) c_k V_k) (sum_j X_j X_j^T c_j V_j)
+//     + sum_j a_d w (M_j^T eps X_j) M_j X_j^T c_j^3 V_j,
+// since dirac_delta(X_k - X_j) is nonzero (c_k / V_k) only for k = j.
+// edim is the model's dimension (1, 2 or 3).
+template <int DIM>
+__global__ __launch_bounds__(256) void k_es_fields(
+    FdArgs<DIM> a, const int64_t* __restrict__ rp,
+    const int32_t* __restrict__ col, EsDev m, const double* __restrict__ mw,
+    int edim, float* __restrict__ energy, double* __restrict__ strain6,
+    double* __restrict__ stress6) {
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= a.n) return;
+  const int64_t sl = i >> 6;
+  const int lane = int(i & 63);
+  const int64_t bb = a.sl_ptr[sl] >> 6;
+  auto broken = [&](int64_t k) {
+    return ((a.bits[bb + (k - rp[i])] >> lane) & 1ull) != 0ull;
+  };
+  double xi[3], ui[3];
+  load3<DIM>(a.x[i], xi);
+  load3<DIM>(a.u[i], ui);
+  const double mwi = mw[i];
+  const double w = 1.;
+  const double c3 = edim == 3 ? 3. / mwi
+                    : edim == 2 ? (2. / mwi) * m.f2d
+                                : 1. / mwi;
+  // dilatation of the current state (the exact pass-1 expression)
+  double th = 0.;
+  for (int64_t k = rp[i]; k < rp[i + 1]; k++) {
+    if (broken(k)) continue;
+    const int32_t j = col[k];
+    double xj[3], uj[3];
+    load3<DIM>(a.x[j], xj);
+    load3<DIM>(a.u[j], uj);
+    bool fs = true;  // no break test here
+    th += es_dil_exact(m, xi, ui, xj, uj, a.vol[j], c3, fs);
+  }
+  // material constants of getBondEF (energy) and K_modulus_tensor (stress)
+  double alpha = 0., as_e = 0., ad_e = 0., lead = 0., ad_s = 0.;
+  if (edim == 1) {
+    alpha = m.E / mwi;
+  } else if (edim == 2) {
+    as_e = (9. / mwi) * m.Kc;
+    ad_e = (8. / mwi) * m.mu;
+    // K_modulus_tensor takes plane stress and nu from K and mu
+    const double Nu = (3. * m.K - 2. * m.mu) / (2. * (3. * m.K + m.mu));
+    const double as_s =
+        (9. / mwi) * (m.K + pow((Nu + 1.) / (2. * Nu - 1.), 2) * m.mu / 9.);
+    ad_s = (8. / mwi) * m.mu;
+    const double asb = (2. * m.f2d * as_s - (3. - 2. * m.f2d) * ad_s) / 3.;
+    lead = (asb - ad_s) / mwi;
+  } else {
+    as_e = (9. / mwi) * m.K;
+    ad_e = (15. / mwi) * m.mu;
+    ad_s = ad_e;
+    lead = (as_e - ad_e) / mwi;
+  }
+  float en = energy ? energy[i] : 0.f;
+  double K[3][3] = {}, T[3][3] = {}, XX[3][3] = {};
+  const int64_t k0 = rp[i];
+  double c0 = 1.;
+  if (rp[i + 1] > k0) {
+    double x0[3];
+    load3<DIM>(a.x[col[k0]], x0);
+    const double X0[3] = {x0[0] - xi[0], x0[1] - xi[1], x0[2] - xi[2]};
+    c0 = es_corr_exact(m, sqrt(X0[0] * X0[0] + X0[1] * X0[1] + X0[2] * X0[2]));
+  }
+  for (int64_t k = rp[i]; k < rp[i + 1]; k++) {
+    const int32_t j = col[k];
+    double xj[3], uj[3];
+    load3<DIM>(a.x[j], xj);
+    load3<DIM>(a.u[j], uj);
+    double Y[3], lx, ly, ext;
+    es_geom_exact(xi, ui, xj, uj, Y, lx, ly, ext);
+    const double X[3] = {xj[0] - xi[0], xj[1] - xi[1], xj[2] - xi[2]};
+    const double c = es_corr_exact(m, lx);
+    const double vj = a.vol[j];
+    if (energy && !broken(k)) {
+      double eb;
+      if (edim == 1) {
+        eb = 0.5 * alpha * w * ext * ext * c * vj;
+      } else {
+        const double e_s = th * lx / 3.;
+        const double e_d = edim == 2 ? ext - e_s : 0.;
+        eb = 0.5 * w * (as_e * e_s * e_s + ad_e * e_d * e_d) * c * vj;
+      }
+      en += float(eb);
+    }
+    for (int p = 0; p < 3; p++)
+      for (int q = 0; q < 3; q++) {
+        K[p][q] += X[p] * X[q] * w * c0 * vj;
+        T[p][q] += Y[p] * X[q] * w * c * vj;
+        XX[p][q] += X[p] * X[q] * c * vj;
+      }
+  }
+  if (energy) energy[i] = en;
+  if (!strain6 && !stress6) return;
+  // (K^-1)^T on the edim x edim block
+  double Si[3][3] = {};
+  if (edim == 1) {
+    Si[0][0] = 1. / K[0][0];
+  } else if (edim == 2) {
+    const double det = K[0][0] * K[1][1] - K[0][1] * K[1][0];
+    const double inv[2][2] = {{K[1][1] / det, -K[0][1] / det},
+                              {-K[1][0] / det, K[0][0] / det}};
+    for (int p = 0; p < 2; p++)
+      for (int q = 0; q < 2; q++) Si[p][q] = inv[q][p];
+  } else {
+    const double c00 = K[1][1] * K[2][2] - K[1][2] * K[2][1];
+    const double c01 = K[1][2] * K[2][0] - K[1][0] * K[2][2];
+    const double c02 = K[1][0] * K[2][1] - K[1][1] * K[2][0];
+    const double det = K[0][0] * c00 + K[0][1] * c01 + K[0][2] * c02;
+    double inv[3][3];
+    inv[0][0] = c00 / det;
+    inv[1][0] = c01 / det;
+    inv[2][0] = c02 / det;
+    inv[0][1] = (K[0][2] * K[2][1] - K[0][1] * K[2][2]) / det;
+    inv[1][1] = (K[0][0] * K[2][2] - K[0][2] * K[2][0]) / det;
+    inv[2][1] = (K[0][1] * K[2][0] - K[0][0] * K[2][1]) / det;
+    inv[0][2] = (K[0][1] * K[1][2] - K[0][2] * K[1][1]) / det;
+    inv[1][2] = (K[0][2] * K[1][0] - K[0][0] * K[1][2]) / det;
+    inv[2][2] = (K[0][0] * K[1][1] - K[0][1] * K[1][0]) / det;
+    for (int p = 0; p < 3; p++)
+      for (int q = 0; q < 3; q++) Si[p][q] = inv[q][p];
+  }
+  double E[3][3];
+  {
+    double F[3][3] = {};
+    for (int p = 0; p < 3; p++)
+      for (int q = 0; q < 3; q++)
+        for (int r = 0; r < 3; r++) F[p][q] += T[p][r] * Si[r][q];
+    for (int p = 0; p < 3; p++)
+      for (int q = 0; q < 3; q++)
+        E[p][q] = (F[p][q] + F[q][p]) * 0.5 - ((p == q && p < edim) ? 1. : 0.);
+  }
+  auto put = [&](double* o, const double (&A)[3][3]) {
+    o[6 * i + 0] = A[0][0];
+    o[6 * i + 1] = A[1][1];
+    o[6 * i + 2] = A[2][2];
+    o[6 * i + 3] = A[1][2];
+    o[6 * i + 4] = A[0][2];
+    o[6 * i + 5] = A[0][1];
+  };
+  if (strain6) put(strain6, E);
+  if (!stress6) return;
+  // second sweep: sum_k (X_k^T eps X_k) c_k V_k and the k = j terms
+  double qsum = 0.;
+  double S[3][3] = {};
+  const double ad_w = (edim == 1 ? alpha : ad_s) * w;
+  for (int64_t k = rp[i]; k < rp[i + 1]; k++) {
+    const int32_t j = col[k];
+    double xj[3];
+    load3<DIM>(a.x[j], xj);
+    const double X[3] = {xj[0] - xi[0], xj[1] - xi[1], xj[2] - xi[2]};
+    const double lx = sqrt(X[0] * X[0] + X[1] * X[1] + X[2] * X[2]);
+    const double c = es_corr_exact(m, lx);
+    const double vj = a.vol[j];
+    double eX[3];
+    for (int p = 0; p < 3; p++)
+      eX[p] = E[p][0] * X[0] + E[p][1] * X[1] + E[p][2] * X[2];
+    qsum += (X[0] * eX[0] + X[1] * eX[1] + X[2] * eX[2]) * c * vj;
+    const double M[3] = {X[0] / lx, X[1] / lx, X[2] / lx};
+    const double me = M[0] * eX[0] + M[1] * eX[1] + M[2] * eX[2];
+    const double coef = ad_w * me * c * c * c * vj;
+    for (int p = 0; p < 3; p++)
+      for (int q = 0; q < 3; q++) S[p][q] += coef * M[p] * X[q];
+  }
+  if (edim > 1) {
+    const double t1 = lead * w * w * qsum;
+    for (int p = 0; p < 3; p++)
+      for (int q = 0; q < 3; q++) S[p][q] += t1 * XX[p][q];
+  }
+  put(stress6, S);
+}
+
+}  // namespace
+
 }  // namespace nlm
+
+extern "C" int nlm_fd_es_fields(nlm_ctx* c, float* strain_energy,
+                                double* strain, double* stress) {
+  using namespace nlm;
+  if (!c) return NLM_ERR_INVALID;
+  FdState& s = c->fd;
+  if (!s.ready) return fail(c, NLM_ERR_STATE, "nlm_fd_es_fields: no setup");
+  if (s.material != NLM_MATERIAL_ELASTIC_STATE)
+    return fail(c, NLM_ERR_INVALID,
+                "nlm_fd_es_fields: state-based material only");
+  int rc;
+  DevBuf de, dst, dss;
+  if ((rc = alloc(c, de, sizeof(float) * size_t(s.n)))) return rc;
+  if ((rc = alloc(c, dst, sizeof(double) * 6 * size_t(s.n)))) return rc;
+  if ((rc = alloc(c, dss, sizeof(double) * 6 * size_t(s.n)))) return rc;
+  std::vector<float> he(size_t(s.n), 0.f);
+  if (strain_energy) {
+    for (int64_t a = 0; a < s.n; a++) he[a] = strain_energy[s.perm_host[a]];
+    NLM_HIP(c, hipMemcpyAsync(de.p, he.data(), sizeof(float) * s.n,
+                              hipMemcpyHostToDevice, c->stream));
+  }
+  const unsigned g = unsigned((s.n + 255) / 256);
+  float* pe = strain_energy ? de.as<float>() : nullptr;
+  double* pst = strain ? dst.as<double>() : nullptr;
+  double* pss = stress ? dss.as<double>() : nullptr;
+  if (s.dim == 2)
+    hipLaunchKernelGGL(k_es_fields<2>, dim3(g), dim3(256), 0, c->stream,
+                       make_args<2>(c), s.row_ptr.as<int64_t>(),
+                       s.col.as<int32_t>(), s.es, s.es_mw.as<double>(),
+                       s.udim, pe, pst, pss);
+  else
+    hipLaunchKernelGGL(k_es_fields<3>, dim3(g), dim3(256), 0, c->stream,
+                       make_args<3>(c), s.row_ptr.as<int64_t>(),
+                       s.col.as<int32_t>(), s.es, s.es_mw.as<double>(),
+                       s.udim, pe, pst, pss);
+  NLM_HIP(c, hipGetLastError());
+  std::vector<double> hs(6 * size_t(s.n)), hq(6 * size_t(s.n));
+  if (strain_energy)
+    NLM_HIP(c, hipMemcpyAsync(he.data(), de.p, sizeof(float) * s.n,
+                              hipMemcpyDeviceToHost, c->stream));
+  if (strain)
+    NLM_HIP(c, hipMemcpyAsync(hs.data(), dst.p, sizeof(double) * 6 * s.n,
+                              hipMemcpyDeviceToHost, c->stream));
+  if (stress)
+    NLM_HIP(c, hipMemcpyAsync(hq.data(), dss.p, sizeof(double) * 6 * s.n,
+                              hipMemcpyDeviceToHost, c->stream));
+  NLM_HIP(c, hipStreamSynchronize(c->stream));
+  for (int64_t a = 0; a < s.n; a++) {
+    const int64_t i = s.perm_host[a];
+    if (strain_energy) strain_energy[i] = he[a];
+    for (int k = 0; k < 6; k++) {
+      if (strain) strain[6 * i + k] = hs[6 * a + k];
+      if (stress) stress[6 * i + k] = hq[6 * a + k];
+    }
+  }
+  return NLM_OK;
+}
 
 extern "C" int nlm_fd_get_node_fields(nlm_ctx* c, double* theta, double* mw) {
   using namespace nlm;

@@ -1047,6 +1047,16 @@ int do_steps(nlm_ctx* c, int nsteps, double* ms_fused) {
   return NLM_OK;
 }
 
+// 1D models: the y dof of every node is held at zero through the node-list
+// displacement BC (per-node rate 0), kept across BC updates; the fixity the
+// library reports (the reference's 1D fixity) does not include it.
+void pin_y_dim1(const FdState& s, std::vector<uint32_t>& fl) {
+  if (!s.dim1) return;
+  for (auto& f : fl)
+    f = (f & ~(0xfu << (kFlagUbcShift + 4))) |
+        (kNodeListBc << (kFlagUbcShift + 4));
+}
+
 int check_ready(nlm_ctx* c, const char* fn) {
   if (!c) return NLM_ERR_INVALID;
   if (!c->fd.ready)
@@ -1126,8 +1136,8 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
   if (!c) return NLM_ERR_INVALID;
   if (!p || p->n <= 0 || !p->x || !p->vol)
     return fail(c, NLM_ERR_INVALID, "nlm_fd_setup: missing mesh data");
-  if (p->dim != 2 && p->dim != 3)
-    return fail(c, NLM_ERR_INVALID, "nlm_fd_setup: dim must be 2 or 3");
+  if (p->dim < 1 || p->dim > 3)
+    return fail(c, NLM_ERR_INVALID, "nlm_fd_setup: dim must be 1, 2 or 3");
   if (p->precision < NLM_PREC_FP64_EXACT || p->precision > NLM_PREC_FP32_FORCE)
     return fail(c, NLM_ERR_INVALID, "nlm_fd_setup: unknown precision mode");
   if (p->time_scheme != NLM_TIME_CENTRAL_DIFFERENCE &&
@@ -1147,17 +1157,22 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
     return fail(c, NLM_ERR_INVALID,
                 "nlm_fd_setup: the state-based material uses central "
                 "difference");
-  if (es_mat && (p->es.dim != p->dim || !(p->es.K > 0.) ||
-                 !(p->es.mu > 0.) || !(p->es.factor2D > 0.)))
+  if (es_mat && (p->es.dim != p->dim ||
+                 (p->dim == 1 ? !(p->es.E > 0.)
+                              : (!(p->es.K > 0.) || !(p->es.mu > 0.) ||
+                                 !(p->es.factor2D > 0.)))))
     return fail(c, NLM_ERR_INVALID,
-                "nlm_fd_setup: state-based material needs dim, K, G "
-                "(nlm_es_params_from_deck)");
+                "nlm_fd_setup: state-based material needs dim and E (1D) or "
+                "K, G (2D, 3D) (nlm_es_params_from_deck)");
   if (p->n >= (int64_t(1) << 31))
     return fail(c, NLM_ERR_INVALID, "nlm_fd_setup: n >= 2^31");
   FdState& s = c->fd;
   s.ready = false;
   const int64_t n = p->n;
-  const int dim = p->dim;
+  // a 1D model runs on the 2D kernels (nodes on y = 0, y dofs held at 0)
+  const int dim = p->dim == 1 ? 2 : p->dim;
+  s.udim = p->dim;
+  s.dim1 = p->dim == 1;
   hipStream_t st = c->stream;
   int rc;
   // neighbor lists: caller's CSR or the context's last build
@@ -1318,6 +1333,7 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
         fl[a] |= kFlagNonInterior;
         s.has_nofail = true;
       }
+  pin_y_dim1(s, fl);
   NLM_HIP(c, hipMemcpyAsync(s.flags.p, fl.data(), sizeof(uint32_t) * n,
                             hipMemcpyHostToDevice, st));
   if ((rc = alloc(c, s.node_rate, sizeof(double) * 3 * size_t(n) + 256)))
@@ -1448,6 +1464,11 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
     e.K = q.K;
     e.mu = q.mu;
     e.f2d = q.factor2D;
+    e.E = q.E;
+    e.one_d = s.dim1 ? 1 : 0;
+    // 1D on the 2D kernels: (2/m_i) f2d = 1/m_i exactly (a power-of-two
+    // factor), the reference's 1D dilatation factor
+    if (s.dim1) e.f2d = 0.5;
     // alpha_s = (9/m) (K + ((nu+1)/(2 nu-1))^2 mu / 9) for plane stress,
     // (9/m) (K + mu/9) for plane strain (ElasticState.cpp:201-212)
     e.Kc = q.plane_strain
@@ -1540,7 +1561,7 @@ extern "C" int nlm_fd_set_ubc(nlm_ctx* c, int nsets, const nlm_bc* sets) {
     if (b.ndir < 1 || b.ndir > 3)
       return fail(c, NLM_ERR_INVALID, "nlm_fd_set_ubc: 1..3 directions");
     for (int d = 0; d < b.ndir; d++)
-      if (b.dir[d] < 1 || b.dir[d] > s.dim)
+      if (b.dir[d] < 1 || b.dir[d] > s.udim)
         return fail(c, NLM_ERR_INVALID, "nlm_fd_set_ubc: bad direction");
     s.bc_host.u[k] = to_dev(b);
     for (int64_t i = 0; i < s.n; i++) {
@@ -1578,6 +1599,7 @@ extern "C" int nlm_fd_set_ubc(nlm_ctx* c, int nsets, const nlm_bc* sets) {
     }
   }
   s.bc_host.nu = nsets;
+  pin_y_dim1(s, fl);
   NLM_HIP(c, hipMemcpy(s.flags.p, fl.data(), sizeof(uint32_t) * s.n,
                        hipMemcpyHostToDevice));
   NLM_HIP(c, hipMemcpy(s.bctab.p, &s.bc_host, sizeof(BcTable),
@@ -1632,7 +1654,7 @@ extern "C" int nlm_fd_set_fbc(nlm_ctx* c, int nsets, const nlm_bc* sets) {
     if (b.ndir < 1 || b.ndir > 3)
       return fail(c, NLM_ERR_INVALID, "nlm_fd_set_fbc: 1..3 directions");
     for (int d = 0; d < b.ndir; d++)
-      if (b.dir[d] < 1 || b.dir[d] > s.dim)
+      if (b.dir[d] < 1 || b.dir[d] > s.udim)
         return fail(c, NLM_ERR_INVALID, "nlm_fd_set_fbc: bad direction");
     s.bc_host.f[k] = to_dev(b);
     // mesh size of FLoading's line_load
@@ -1668,7 +1690,7 @@ extern "C" int nlm_fd_add_cracks(nlm_ctx* c, int ncracks,
   FdState& s = c->fd;
   if (ncracks < 0 || (ncracks > 0 && !cracks))
     return fail(c, NLM_ERR_INVALID, "nlm_fd_add_cracks: bad arguments");
-  if (ncracks > 0 && s.dim != 2)
+  if (ncracks > 0 && s.udim != 2)
     return fail(c, NLM_ERR_INVALID,
                 "nlm_fd_add_cracks: pre-cracks need dimension 2");
   for (int k = 0; k < ncracks; k++) {
@@ -2193,7 +2215,7 @@ extern "C" int nlm_fd_set_node_ubc(nlm_ctx* c, int64_t count,
     const int64_t i = nodes[k];
     if (i < 0 || i >= s.n)
       return fail(c, NLM_ERR_INVALID, "nlm_fd_set_node_ubc: node out of range");
-    if (dofs[k] >> s.dim)
+    if (dofs[k] >> s.udim)
       return fail(c, NLM_ERR_INVALID, "nlm_fd_set_node_ubc: bad dof mask");
     const int64_t a = s.inv_host[i];
     for (int d = 0; d < s.dim; d++) {
@@ -2204,6 +2226,7 @@ extern "C" int nlm_fd_set_node_ubc(nlm_ctx* c, int64_t count,
       s.fix_host[i] |= uint8_t(1u << d);
     }
   }
+  pin_y_dim1(s, fl);
   NLM_HIP(c, hipMemcpy(s.flags.p, fl.data(), sizeof(uint32_t) * s.n,
                        hipMemcpyHostToDevice));
   NLM_HIP(c, hipMemcpy(s.node_rate.p, rate.data(), sizeof(double) * rate.size(),

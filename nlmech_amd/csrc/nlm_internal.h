@@ -124,6 +124,12 @@ struct EsDev {
   double q, inv_q;
   float q2_f, q_f;
   float check_up_qf, low_band_qf, rcp_h_qf, vex_tol_qf;
+  // 1D model on the 2D kernels: dilatation (1/m_i) sum |X| e c V (f2d is
+  // then 1/2, so (2/m_i) f2d = 1/m_i exactly) and t = (E/m_i) e
+  // (ElasticState.cpp:187-193), i.e. no dilatation term in the force
+  int one_d;
+  int pad_es;
+  double E;
 };
 
 // Slice layout of the neighbor lists: 64 consecutive nodes (one wavefront)
@@ -142,7 +148,11 @@ struct SellDev {
 struct FdState {
   bool ready = false;
   int material = NLM_MATERIAL_RNP_BOND;
+  // internal dimension of the kernels (2 or 3); a 1D model (udim 1) runs as
+  // a 2D one with every node on y = 0 and every y dof held at zero
   int dim = 2;
+  int udim = 2;  // the model's own dimension (1, 2 or 3)
+  bool dim1 = false;
   int64_t n = 0;
   int64_t nnz = 0;
   int precision = NLM_PREC_FP64_EXACT;

@@ -433,6 +433,21 @@ class GpuFD:
                    ptr(mw, C.c_double))
         return th, mw
 
+    def es_fields(self, energy=None):
+        """State-based material: (strain energy, strain[n, 6], stress[n, 6])
+        of QuasiStaticModel::computeForces(true) at the current state; the
+        strain energy is added in float to `energy` (zeros if None)."""
+        n = self.p.n
+        e = (np.zeros(n, np.float32) if energy is None
+             else np.ascontiguousarray(energy, np.float32).copy())
+        if e.shape != (n,):
+            raise ValueError(f"es_fields: energy must have {n} entries")
+        st = np.zeros((n, 6))
+        ss = np.zeros((n, 6))
+        self._call("nlm_fd_es_fields", ptr(e, C.c_float), ptr(st, C.c_double),
+                   ptr(ss, C.c_double))
+        return e, st, ss
+
     def postproc(self):
         n = self.p.n
         Z = np.zeros(n, np.float32)

@@ -287,8 +287,8 @@ void FDModel<T>::initHObjects() {
   const size_t dim = d_mesh->getDimension();
   std::cout << "number of nodes = " << d_mesh->getNumNodes()
             << " number of elements = " << d_mesh->getNumElements() << "\n";
-  if (dim != 2 && dim != 3)
-    deck_error("Error: the GPU engine runs 2D and 3D models.");
+  if (dim < 1 || dim > 3)
+    deck_error("Error: the GPU engine runs 1D, 2D and 3D models.");
   const std::string& td = d_modelDeck_p->d_timeDiscretization;
   if (td != "central_difference" && td != "velocity_verlet" &&
       td != "quasi_static")

@@ -48,8 +48,13 @@ void QuasiStaticModel<T>::initHObjects() {
   if (d_input_p->getOutputDeck()->d_dtOut == 0)
     deck_error("Error: Output: Output_Interval must be given and positive.");
   d_fd.initHObjects();
-  if (d_fd.getMesh().getDimension() != 2)
-    deck_error("Error: the GPU quasi-static solver runs 2D models.");
+  const size_t dim = d_fd.getMesh().getDimension();
+  if (dim != 1 && dim != 2)
+    deck_error("Error: the GPU quasi-static solver runs 1D and 2D models.");
+  const size_t n = d_fd.getMesh().getNumNodes();
+  d_strainEnergy.assign(n, 0.f);
+  d_strain.assign(6 * n, 0.);
+  d_stress.assign(6 * n, 0.);
 }
 
 template <class T>
@@ -86,6 +91,19 @@ void QuasiStaticModel<T>::solver() {
                                                                d_stats.residual0)
       std::cout << "Warning: Newton stopped at Max_Iteration before the "
                    "tolerance.\n";
+    // computeForces(true) after the load step's Newton iterations
+    // (QuasiStaticModel.cpp:589): strain energy, strain and stress
+    if (T::kMaterial == NLM_MATERIAL_ELASTIC_STATE) {
+      const inp::OutputDeck* od = d_input_p->getOutputDeck();
+      const bool e = od->isTagInOutput("Strain_Energy");
+      const bool st = od->isTagInOutput("Strain_Tensor");
+      const bool ss = od->isTagInOutput("Stress_Tensor");
+      if ((e || st || ss) &&
+          nlm_fd_es_fields(c, e ? d_strainEnergy.data() : nullptr,
+                           st ? d_strain.data() : nullptr,
+                           ss ? d_stress.data() : nullptr) != NLM_OK)
+        throw Error(std::string("nlm_fd_es_fields: ") + nlm_last_error(c));
+    }
     output();
   }
 }
@@ -142,11 +160,24 @@ void QuasiStaticModel<T>::output() {
       for (size_t i = 0; i < n; i++) cnt[i] = double(rp[i + 1] - rp[i]);
       w.appendPointData("Neighbors", cnt.data(), 1);
     }
-    if (d_n == 0)
+    if (T::kMaterial == NLM_MATERIAL_ELASTIC_STATE) {
+      // model/util.cpp:176-186 (zeros before the first load step, as the
+      // reference's freshly allocated vectors)
+      if (od->isTagInOutput("Strain_Energy")) {
+        std::vector<double> e(d_strainEnergy.begin(), d_strainEnergy.end());
+        w.appendPointData("Strain_Energy", e.data(), 1);
+      }
+      if (od->isTagInOutput("Strain_Tensor"))
+        w.appendPointData("Strain_Tensor", d_strain.data(), 6);
+      if (od->isTagInOutput("Stress_Tensor"))
+        w.appendPointData("Stress_Tensor", d_stress.data(), 6);
+    } else if (d_n == 0) {
       for (const char* t : {"Strain_Energy", "Strain_Tensor", "Stress_Tensor"})
         if (od->isTagInOutput(t))
           std::cout << "Warning: output tag " << t
-                    << " is not computed by the GPU engine.\n";
+                    << " is not computed by the GPU engine for the "
+                       "bond-based material.\n";
+    }
   }
   w.close();
 }

@@ -40,6 +40,11 @@ class QuasiStaticModel : public Model {
   inp::Input* d_input_p;
   FDModel<T> d_fd;
   nlm_qs_stats d_stats{};
+  // state-based material: the strain energy (float, added to at every load
+  // step like the reference's vector) and the strain / stress tensors of
+  // the last load step (6 per node)
+  std::vector<float> d_strainEnergy;
+  std::vector<double> d_strain, d_stress;
 };
 
 extern template class QuasiStaticModel<material::pd::RNPBond>;

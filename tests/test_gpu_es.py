@@ -19,7 +19,7 @@ import numpy as np
 import pytest
 
 from conftest import progress
-from es_cases import qs2d, small_cube
+from es_cases import qs1d, qs2d, small_cube
 from nlmech_amd import problems as pr
 from nlmech_amd.capi import (NLM_PREC_FP32_FORCE, NLM_PREC_FP64,
                              NLM_PREC_FP64_EXACT)
@@ -317,3 +317,47 @@ def test_es_cube3d_4e6_stepwise(oracle, mode):
             progress(f"  max |df|/S {force_err(fg, fr, scale):.2e}")
     o.close()
     assert new_total > 0
+
+
+@pytest.mark.parametrize("mode", MODES, ids=lambda m: NAMES[m])
+def test_es_qs1d_golden_force_gpu(oracle, mode):
+    """1D model (33 nodes on a line, run on the 2D kernels with the y dofs
+    held at zero): the force at the reference's converged displacement of
+    examples/qsModel/1D reproduces its output_res.vtu Force (t = E e / m,
+    ElasticState.cpp:187-193)."""
+    progress(f"[gpu] es qs1d golden force {NAMES[mode]}")
+    p, g = qs1d()
+    csr = oracle.neighbors(p.x, p.horizon, "brute")
+    f, _, _, _ = gpu_forces(p, g["Displacement"], mode, csr)
+    F = f * p.vol[:, None]
+    err = np.abs(F - g["Force"]).max() / np.abs(g["Force"]).max()
+    assert err <= FORCE_TOL[mode] * 10, err
+    assert np.all(f[:, 1:] == 0.0)
+
+
+@pytest.mark.parametrize("case", ["qs1d", "qs2d"])
+def test_es_output_fields_gpu(oracle, case):
+    """nlm_fd_es_fields at the reference's converged displacement: strain
+    energy bit for bit against the reference-held output_res.vtu (float
+    accumulation per bond, as QuasiStaticModel::computeForces(true) does),
+    strain and stress tensors to rounding; and against the oracle; the
+    energy is added to what the array holds."""
+    progress(f"[gpu] es output fields {case}")
+    from nlmech_amd.engine import GpuFD
+    p, g = qs1d() if case == "qs1d" else qs2d()
+    csr = oracle.neighbors(p.x, p.horizon, "brute")
+    u = g["Displacement"]
+    z = np.zeros_like(u)
+    with GpuFD(p, precision=NLM_PREC_FP64_EXACT, csr=csr) as gg:
+        gg.set_state(u=u, v=z, f=z)
+        e, st, ss = gg.es_fields()
+        e2, _, _ = gg.es_fields(energy=e)
+    assert np.array_equal(e.astype(np.float64), g["Strain_Energy"])
+    smax = np.abs(g["Stress_Tensor"]).max()
+    assert np.abs(st - g["Strain_Tensor"]).max() <= 1e-14
+    assert np.abs(ss - g["Stress_Tensor"]).max() <= 1e-12 * smax
+    o = oracle.OracleES(p, csr)
+    o.set_state(u=u, v=z, f=z)
+    eo, sto, sso = o.fields(energy=e)
+    o.close()
+    assert np.array_equal(e2, eo)

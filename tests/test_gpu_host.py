@@ -579,3 +579,31 @@ def test_reference_crack_propagation_deck(host_bin, oracle, tmp_path):
         assert err <= 1e-9
         assert np.array_equal(out["Bond_State"], o.get_bond_bytes())
     o.close()
+
+
+@pytest.mark.parametrize("case", ["qs1d", "qs2d"])
+def test_reference_quasi_static_ctest(host_bin, tmp_path, case):
+    """quasistatic.{1D,2D}.elastic.{mesh,run,compare} (test/CMakeLists.txt:
+    11-54) with the reference's own decks, unchanged: nlm_mesh -d 1|2,
+    nlm_fd -i input.yaml, then nlm_dc -i input_compare.yaml -k fd_simple,
+    which compares Displacement, Force, Strain_Energy, Strain_Tensor and
+    Stress_Tensor against the reference-held output_res.vtu with L2 < 1e-7
+    (its exit status says whether every tag passed)."""
+    src = GOLDEN / case
+    for f in ("input.yaml", "input_mesh.yaml", "input_compare.yaml",
+              "output_res.vtu"):
+        shutil.copy(src / f, tmp_path / f)
+    progress(f"[gpu] reference quasi-static CTest {case}")
+    run_tool(host_bin / "nlm_mesh", "-i", "input_mesh.yaml", "-d",
+             "1" if case == "qs1d" else "2", cwd=tmp_path)
+    out = run_deck(host_bin, tmp_path)
+    progress("[gpu]   " + " | ".join(l for l in out.splitlines()
+                                     if l.startswith("Load step")))
+    rc, dc = run_tool(host_bin / "nlm_dc", "-i", "input_compare.yaml", "-k",
+                      "fd_simple", cwd=tmp_path, check=False)
+    lines = (tmp_path / "compare.txt").read_text().splitlines()
+    progress(f"[gpu]   dc: {lines[-1]}")
+    vals = [float(v) for v in lines[-1].split(",")]
+    l2 = vals[0::2]
+    assert len(l2) == 5 and max(l2) < 1e-7, l2
+    assert rc == 0
