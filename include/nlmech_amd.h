@@ -462,6 +462,15 @@ int nlm_qs_coarse_apply(nlm_ctx* ctx, const double* r, double* z);
  * environment variable NLM_QS_GRAPH=0 makes off the default of new
  * contexts. */
 int nlm_qs_set_graph(nlm_ctx* ctx, int on);
+/* BiCGSTAB configuration (no counterpart in the reference): graph as
+ * nlm_qs_set_graph; bond_scalar (default on): the products with the
+ * analytic bond tangent (NLM_QS_TANGENT_FULL / _COMPONENT) read one scalar
+ * per bond, g with -B_ij = g dx dx^T (dx the reference bond vector), and
+ * form the block from the positions, instead of reading the 2x2 block --
+ * 12 bytes per bond slot instead of 36.  The two forms agree to rounding
+ * (each block entry within a few units in the last place); graph replays
+ * and kernel-by-kernel launches of one form agree bit for bit. */
+int nlm_qs_set_launch(nlm_ctx* ctx, int graph, int bond_scalar);
 /* Solves A_ff x = b with the assembled tangent (b, x AoS double[3n]). */
 int nlm_qs_linear_solve(nlm_ctx* ctx, const double* b, double* x,
                         int precond, double tol, int max_iter, int* iters,

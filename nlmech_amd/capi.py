@@ -201,6 +201,7 @@ SIGNATURES = {
     "nlm_qs_linear_solve": (C.c_int, [_P, _D, _D, C.c_int, C.c_double,
                                       C.c_int, C.POINTER(C.c_int), _D]),
     "nlm_qs_set_graph": (C.c_int, [_P, C.c_int]),
+    "nlm_qs_set_launch": (C.c_int, [_P, C.c_int, C.c_int]),
 }
 
 _lib = None

@@ -71,6 +71,7 @@ extern "C" void nlm_ctx_destroy(nlm_ctx* c) {
   if (c->qs.bicg_graph) (void)hipGraphExecDestroy(c->qs.bicg_graph);
   if (c->qs.e0) (void)hipEventDestroy(c->qs.e0);
   if (c->qs.e1) (void)hipEventDestroy(c->qs.e1);
+  if (c->qs.hctl) (void)hipHostFree(c->qs.hctl);
   hipStream_t st = c->stream;
   delete c;  // frees every device buffer
   if (st) (void)hipStreamDestroy(st);

@@ -1168,6 +1168,13 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
     return fail(c, NLM_ERR_INVALID, "nlm_fd_setup: n >= 2^31");
   FdState& s = c->fd;
   s.ready = false;
+  // the quasi-static solver's captured batch refers to the buffers a new
+  // setup replaces
+  if (c->qs.bicg_graph) {
+    (void)hipGraphExecDestroy(c->qs.bicg_graph);
+    c->qs.bicg_graph = nullptr;
+    c->qs.bicg_key.clear();
+  }
   const int64_t n = p->n;
   // a 1D model runs on the 2D kernels (nodes on y = 0, y dofs held at 0)
   const int dim = p->dim == 1 ? 2 : p->dim;

@@ -215,6 +215,11 @@ struct QsState {
   bool has_m = false;
   int tangent = 0;
   DevBuf blk, diag, dinv;    // A = -J blocks, diagonal blocks, Jacobi 1/a_dd
+  // analytic bond tangent: one scalar per slot, -B_ij = g dx dx^T (bs_mode
+  // 1) or its diagonal (bs_mode 2, component tangent); 0 = blocks only
+  DevBuf bg;
+  int bs_mode = 0;
+  bool bond_scalar = true;   // products of the tangent from bg
   DevBuf mblk, mdiag;        // preconditioner blocks
   DevBuf r, rh, p, v, s, t, x, ph, sh, b, y, dup, u0;
   DevBuf part;               // 4 * nb partial sums
@@ -227,7 +232,8 @@ struct QsState {
   DevBuf adj_ptr, adj, cA, cB, rc, zcp, zcs;
   double* cinv = nullptr;
   int m = 0;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;  // BiCGSTAB batches in flight
+  int* hctl = nullptr;       // pinned: control words of two batches
   // finite-difference tangent: node colors (no two nodes of one color
   // within 4 horizons), perturbation factor (eps = factor * h)
   DevBuf color;

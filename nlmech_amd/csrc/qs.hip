@@ -82,10 +82,48 @@ __device__ void block_partial(double v, double* __restrict__ P, int64_t b) {
   __syncthreads();
 }
 
+// Workgroup sums of N values per thread, every thread receiving the N sums:
+// a butterfly within each wavefront, then the four wavefronts' sums in a
+// fixed order -- one barrier for all N, the same order in every workgroup.
+template <int N>
+__device__ __forceinline__ void block_reduce(double (&v)[N]) {
+  __shared__ double red[N][4];
+  for (int off = 32; off > 0; off >>= 1)
+    for (int q = 0; q < N; q++) v[q] += __shfl_xor(v[q], off);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int q = 0; q < N; q++) red[q][w] = v[q];
+  __syncthreads();
+  for (int q = 0; q < N; q++)
+    v[q] = (red[q][0] + red[q][1]) + (red[q][2] + red[q][3]);
+  __syncthreads();  // red is reused by the next call
+}
+
+// Sums of N partial arrays of nb values each (their loads in flight together).
+template <int N>
+__device__ __forceinline__ void block_sums(const double* const (&P)[N],
+                                           int64_t nb, double (&out)[N]) {
+  for (int q = 0; q < N; q++) out[q] = 0.;
+  for (int64_t k = threadIdx.x; k < nb; k += 256)
+    for (int q = 0; q < N; q++) out[q] += P[q][k];
+  block_reduce<N>(out);
+}
+
+// Workgroup sums of N values per thread, stored to P[q][b] by thread 0.
+template <int N>
+__device__ __forceinline__ void block_partials(double (&v)[N],
+                                               double* const (&P)[N],
+                                               int64_t b) {
+  block_reduce<N>(v);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < N; q++) P[q][b] = v[q];
+}
+
 // Tangent blocks of every slice entry and the diagonal blocks.
 __global__ __launch_bounds__(256) void k_qs_tangent(
     QsArgs a, int component, double4* __restrict__ blk,
-    double4* __restrict__ diag, double2* __restrict__ dinv) {
+    double4* __restrict__ diag, double2* __restrict__ dinv,
+    double* __restrict__ bg) {
   const int64_t nblk = (a.nslices * 64 + 255) / 256;
   const int64_t lb = xcd_block(nblk);
   if (lb >= nblk) return;
@@ -108,6 +146,7 @@ __global__ __launch_bounds__(256) void k_qs_tangent(
     const int64_t e = cb + int64_t(k) * 64 + lane;
     const unsigned long long w = a.bits[bb + k];
     double4 B = make_double4(0., 0., 0., 0.);
+    double gb = 0.;
     if (k < len) {
       const int32_t j = a.col[e];
       const double2 xj = a.x[j], uj = a.u[j];
@@ -130,6 +169,9 @@ __global__ __launch_bounds__(256) void k_qs_tangent(
         kk = c / r;
       }
       const double e0 = dx0 / r, e1 = dx1 / r;
+      // the block is rank one (or diagonal): -B = gb dx dx^T, dx the
+      // reference bond vector
+      gb = -(kk / dd);
       B.x = kk * e0 * e0;
       B.y = component ? 0. : kk * e0 * e1;
       B.z = component ? 0. : kk * e1 * e0;
@@ -140,6 +182,7 @@ __global__ __launch_bounds__(256) void k_qs_tangent(
       D.w += B.w;
     }
     blk[e] = make_double4(-B.x, -B.y, -B.z, -B.w);
+    bg[e] = gb;
   }
   if (!valid) return;
   diag[i] = D;
@@ -300,14 +343,23 @@ __device__ double2 coarse_rows(const double* __restrict__ Ainv, int m,
   return z;
 }
 
-template <int DOT, class BT>
+// y = A x over the slices (one workgroup per slice, wavefront q taking
+// slots q, q+4, ...), A from its blocks (BS 0), or from one scalar per bond
+// (the analytic bond tangent: off-diagonal block -B_ij = g_ij dx dx^T, BS 1;
+// the component tangent diag(g dx_0^2, g dx_1^2), BS 2) with dx = x_j - x_i
+// formed from the positions -- 12 bytes per slot instead of 36.  DOT 1:
+// partial (w, y) per slice; DOT 2: also (y, y).
+template <int DOT, class BT, int BS = 0>
 __global__ __launch_bounds__(256) void k_qs_spmv4(
     QsArgs a, const BT* __restrict__ blk, const BT* __restrict__ dg,
     const double2* __restrict__ xv, double2* __restrict__ y,
     const double2* __restrict__ wv, double* __restrict__ P0,
     double* __restrict__ P1, const int* __restrict__ ctl,
-    const double* __restrict__ Ainv, int m, const double* __restrict__ rc) {
-  if (ctl && ctl[0]) return;
+    const double* __restrict__ Ainv, int m, const double* __restrict__ rc,
+    const double* __restrict__ bg) {
+  // the control word is read with the first loads and tested after the
+  // products (a finished solve only skips the stores)
+  const int done = ctl ? ctl[0] : 0;
   const int64_t s = xcd_block(a.nslices);
   if (s >= a.nslices) return;
   __shared__ double2 red[3][64];
@@ -321,16 +373,47 @@ __global__ __launch_bounds__(256) void k_qs_spmv4(
   const int64_t cb = a.sl_ptr[s];
   const int width = int((a.sl_ptr[s + 1] - cb) >> 6);
   double2 acc = make_double2(0., 0.);
-  for (int k = wq; k < width; k += 4) {
-    const int64_t e = cb + int64_t(k) * 64 + lane;
-    const BT B = blk[e];
-    const double2 xj = xv[a.col[e]];
-    acc.x += double(B.x) * xj.x + double(B.y) * xj.y;
-    acc.y += double(B.z) * xj.x + double(B.w) * xj.y;
+  if (BS == 0) {
+    for (int k = wq; k < width; k += 4) {
+      const int64_t e = cb + int64_t(k) * 64 + lane;
+      const BT B = blk[e];
+      const double2 xj = xv[a.col[e]];
+      acc.x += double(B.x) * xj.x + double(B.y) * xj.y;
+      acc.y += double(B.z) * xj.x + double(B.w) * xj.y;
+    }
+  } else {
+    // padding slots hold g = 0 and a valid column
+    const double2 pi = valid ? a.x[i] : make_double2(0., 0.);
+    auto slot = [&](double gb, double2 pj, double2 xj) {
+      const double d0 = pj.x - pi.x, d1 = pj.y - pi.y;
+      if (BS == 1) {
+        const double w = gb * (d0 * xj.x + d1 * xj.y);
+        acc.x += w * d0;
+        acc.y += w * d1;
+      } else {
+        acc.x += gb * d0 * d0 * xj.x;
+        acc.y += gb * d1 * d1 * xj.y;
+      }
+    };
+    // two slots per trip, both slots' loads in flight together
+    int k = wq;
+    for (; k + 4 < width; k += 8) {
+      const int64_t e0 = cb + int64_t(k) * 64 + lane, e1 = e0 + 256;
+      const double g0 = bg[e0], g1 = bg[e1];
+      const int32_t j0 = a.col[e0], j1 = a.col[e1];
+      const double2 p0 = a.x[j0], x0 = xv[j0], p1 = a.x[j1], x1 = xv[j1];
+      slot(g0, p0, x0);
+      slot(g1, p1, x1);
+    }
+    if (k < width) {
+      const int64_t e = cb + int64_t(k) * 64 + lane;
+      const int32_t j = a.col[e];
+      slot(bg[e], a.x[j], xv[j]);
+    }
   }
   if (wq > 0) red[wq - 1][lane] = acc;
   __syncthreads();
-  if (wq != 0) return;
+  if (wq != 0 || done) return;
   acc.x += red[0][lane].x + red[1][lane].x + red[2][lane].x;
   acc.y += red[0][lane].y + red[1][lane].y + red[2][lane].y;
   double p0 = 0., p1 = 0.;
@@ -365,6 +448,9 @@ __global__ __launch_bounds__(256) void k_qs_spmv4(
 
 // Scalars of iteration it: rho[it] = (rh, r_it), alpha[it], omega[it],
 // rr[it] = (r_it, r_it), at offsets 8 + {0, 1, 2, 3} * L + it.
+// The latest rho, alpha and omega are also kept in two slots each, by the
+// parity of their iteration, so a step can load both slots together with
+// its control words and pick one once they arrive (no dependent load).
 struct Scal {
   double* base;
   int L;
@@ -373,6 +459,9 @@ struct Scal {
   __device__ double& alpha(int it) const { return base[8 + L + it]; }
   __device__ double& omega(int it) const { return base[8 + 2 * L + it]; }
   __device__ double& rr(int it) const { return base[8 + 3 * L + it]; }
+  __device__ double& rho_p(int par) const { return base[2 + par]; }
+  __device__ double& alpha_p(int par) const { return base[4 + par]; }
+  __device__ double& omega_p(int par) const { return base[6 + par]; }
 };
 
 struct VecArgs {
@@ -417,8 +506,11 @@ __global__ __launch_bounds__(256) void k_bicg_init(VecArgs a,
     a.sh[i] = z;
     p0 = bi.x * bi.x + bi.y * bi.y;
   }
-  block_partial(p0, a.P0, blockIdx.x);
-  block_partial(p0, a.P1, blockIdx.x);
+  {
+    double v[2] = {p0, p0};
+    double* const P[2] = {a.P0, a.P1};
+    block_partials<2>(v, P, blockIdx.x);
+  }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     a.ctl[0] = 0;
     a.ctl[1] = 0;
@@ -433,18 +525,34 @@ __global__ void k_iter_advance(int* __restrict__ ctl, int n) {
 
 // Iteration it, step 1: convergence test on r_{it-1}; p = r + beta (p -
 // omega v); ph = M p for the diagonal preconditioners (PRE 0 none, 1 Jacobi;
-// PRE 2 leaves ph to a block product).
+// PRE 2 leaves ph to a block product).  Every load of the step -- control
+// words, scalar slots, the node's vectors, the partials -- is issued before
+// the first wait.
 template <int PRE, bool RESTRICT = false>
 __global__ __launch_bounds__(256) void k_bicg_p(VecArgs a, int k) {
-  if (a.ctl[0]) return;
+  const int done = a.ctl[0];
   const int it = a.ctl[2] + k;
-  if (it > a.max_it) return;
-  const double rho_i = block_sum(a.P0, a.nb);
-  const double rr = block_sum(a.P1, a.nb);
+  const double rho2[2] = {a.sc.rho_p(0), a.sc.rho_p(1)};
+  const double al1[2] = {a.sc.alpha_p(0), a.sc.alpha_p(1)};
+  const double om1[2] = {a.sc.omega_p(0), a.sc.omega_p(1)};
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  const bool own = i < a.n;
+  const double2 z = make_double2(0., 0.);
+  const double2 r = own ? a.r[i] : z, p = own ? a.p[i] : z,
+                v = own ? a.v[i] : z;
+  const double2 di = (PRE == 1 && own) ? a.dinv[i] : z;
+  double red[2];
+  {
+    const double* const P[2] = {a.P0, a.P1};
+    block_sums<2>(P, a.nb, red);
+  }
+  if (done || it > a.max_it) return;
+  const double rho_i = red[0], rr = red[1];
   const bool conv = !(rr > a.sc.tol2());
   const bool brk = !conv && rho_i == 0.;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     a.sc.rho(it - 1) = rho_i;
+    a.sc.rho_p((it - 1) & 1) = rho_i;
     a.sc.rr(it - 1) = rr;
     if (conv || brk) {
       a.ctl[0] = conv ? 1 : 2;
@@ -454,21 +562,16 @@ __global__ __launch_bounds__(256) void k_bicg_p(VecArgs a, int k) {
   if (conv || brk) return;
   double beta = 0., om = 0.;
   if (it > 1) {
-    om = a.sc.omega(it - 1);
-    beta = (rho_i / a.sc.rho(it - 2)) * (a.sc.alpha(it - 1) / om);
+    om = om1[(it - 1) & 1];
+    beta = (rho_i / rho2[it & 1]) * (al1[(it - 1) & 1] / om);
   }
-  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
   double2 pn = make_double2(0., 0.);
-  if (i < a.n) {
-    const double2 r = a.r[i], p = a.p[i], v = a.v[i];
+  if (own) {
     pn.x = r.x + beta * (p.x - om * v.x);
     pn.y = r.y + beta * (p.y - om * v.y);
     a.p[i] = pn;
     if (PRE == 0) a.ph[i] = pn;
-    if (PRE == 1) {
-      const double2 di = a.dinv[i];
-      a.ph[i] = make_double2(di.x * pn.x, di.y * pn.y);
-    }
+    if (PRE == 1) a.ph[i] = make_double2(di.x * pn.x, di.y * pn.y);
   }
   if (RESTRICT) block_restrict(pn.x, pn.y, a.rc, blockIdx.x);
 }
@@ -476,32 +579,39 @@ __global__ __launch_bounds__(256) void k_bicg_p(VecArgs a, int k) {
 // Step 2 (after v = A ph with partial (rh, v) in P2): alpha, s = r - alpha v.
 template <int PRE, bool RESTRICT = false>
 __global__ __launch_bounds__(256) void k_bicg_s(VecArgs a, int k) {
-  if (a.ctl[0]) return;
+  const int done = a.ctl[0];
   const int it = a.ctl[2] + k;
-  if (it > a.max_it) return;
-  const double rv = block_sum(a.P2, a.nbs);
-  const double rho_i = a.sc.rho(it - 1);
+  const double rho1[2] = {a.sc.rho_p(0), a.sc.rho_p(1)};
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  const bool own = i < a.n;
+  const double2 z = make_double2(0., 0.);
+  const double2 r = own ? a.r[i] : z, v = own ? a.v[i] : z;
+  const double2 di = (PRE == 1 && own) ? a.dinv[i] : z;
+  double rv1[1];
+  {
+    const double* const P[1] = {a.P2};
+    block_sums<1>(P, a.nbs, rv1);
+  }
+  if (done || it > a.max_it) return;
+  const double rv = rv1[0];
+  const double rho_i = rho1[(it - 1) & 1];
   const bool brk = rv == 0.;
   const double alpha = brk ? 0. : rho_i / rv;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     a.sc.alpha(it) = alpha;
+    a.sc.alpha_p(it & 1) = alpha;
     if (brk) {
       a.ctl[0] = 2;
       a.ctl[1] = it - 1;
     }
   }
   if (brk) return;
-  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
   double2 sn = make_double2(0., 0.);
-  if (i < a.n) {
-    const double2 r = a.r[i], v = a.v[i];
+  if (own) {
     sn = make_double2(r.x - alpha * v.x, r.y - alpha * v.y);
     a.s[i] = sn;
     if (PRE == 0) a.sh[i] = sn;
-    if (PRE == 1) {
-      const double2 di = a.dinv[i];
-      a.sh[i] = make_double2(di.x * sn.x, di.y * sn.y);
-    }
+    if (PRE == 1) a.sh[i] = make_double2(di.x * sn.x, di.y * sn.y);
   }
   if (RESTRICT) block_restrict(sn.x, sn.y, a.rc, blockIdx.x);
 }
@@ -665,30 +775,42 @@ __global__ void k_gj_sweep(const double* __restrict__ A, double* __restrict__ B,
 // Step 3 (after t = A sh with partials (t, s) in P2 and (t, t) in P3):
 // omega, x += alpha ph + omega sh, r = s - omega t; partials (rh, r), (r, r).
 __global__ __launch_bounds__(256) void k_bicg_x(VecArgs a, int k) {
-  if (a.ctl[0]) return;
+  const int done = a.ctl[0];
   const int it = a.ctl[2] + k;
-  if (it > a.max_it) return;
-  const double ts = block_sum(a.P2, a.nbs);
-  const double tt = block_sum(a.P3, a.nbs);
-  const double omega = tt > 0. ? ts / tt : 0.;
-  const double alpha = a.sc.alpha(it);
-  if (blockIdx.x == 0 && threadIdx.x == 0) a.sc.omega(it) = omega;
+  const double al[2] = {a.sc.alpha_p(0), a.sc.alpha_p(1)};
   const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  const bool own = i < a.n;
+  const double2 z = make_double2(0., 0.);
+  const double2 ph = own ? a.ph[i] : z, sh = own ? a.sh[i] : z,
+                s = own ? a.s[i] : z, t = own ? a.t[i] : z,
+                x0 = own ? a.x[i] : z, rh = own ? a.rh[i] : z;
+  double red[2];
+  {
+    const double* const P[2] = {a.P2, a.P3};
+    block_sums<2>(P, a.nbs, red);
+  }
+  if (done || it > a.max_it) return;
+  const double ts = red[0], tt = red[1];
+  const double omega = tt > 0. ? ts / tt : 0.;
+  const double alpha = al[it & 1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.sc.omega(it) = omega;
+    a.sc.omega_p(it & 1) = omega;
+  }
   double p0 = 0., p1 = 0.;
-  if (i < a.n) {
-    const double2 ph = a.ph[i], sh = a.sh[i], s = a.s[i], t = a.t[i];
-    double2 x = a.x[i];
+  if (own) {
+    double2 x = x0;
     x.x += alpha * ph.x + omega * sh.x;
     x.y += alpha * ph.y + omega * sh.y;
     a.x[i] = x;
     const double2 r = make_double2(s.x - omega * t.x, s.y - omega * t.y);
     a.r[i] = r;
-    const double2 rh = a.rh[i];
     p0 = rh.x * r.x + rh.y * r.y;
     p1 = r.x * r.x + r.y * r.y;
   }
-  block_partial(p0, a.P0, blockIdx.x);
-  block_partial(p1, a.P1, blockIdx.x);
+  double v[2] = {p0, p1};
+  double* const P[2] = {a.P0, a.P1};
+  block_partials<2>(v, P, blockIdx.x);
 }
 
 // Newton helpers ------------------------------------------------------------
@@ -836,8 +958,18 @@ int qs_alloc(nlm_ctx* c, int max_iter) {
   const int64_t n = s.n;
   const size_t nent = size_t(s.sell.nentries);
   int rc;
-  if (!q.blk.p || q.blk.bytes < nent * sizeof(double4)) {
+  const bool grow_vec = !q.blk.p || q.blk.bytes < nent * sizeof(double4);
+  if ((grow_vec || max_iter + 2 > q.max_iter_alloc) && q.bicg_graph) {
+    // buffers the captured batch references are about to be replaced: the
+    // executable graph goes first, so no live graph ever refers to freed
+    // memory
+    (void)hipGraphExecDestroy(q.bicg_graph);
+    q.bicg_graph = nullptr;
+    q.bicg_key.clear();
+  }
+  if (grow_vec) {
     if ((rc = alloc(c, q.blk, nent * sizeof(double4) + 256))) return rc;
+    if ((rc = alloc(c, q.bg, nent * sizeof(double) + 256))) return rc;
     if ((rc = alloc(c, q.diag, n * sizeof(double4) + 256))) return rc;
     if ((rc = alloc(c, q.dinv, n * sizeof(double2) + 256))) return rc;
     DevBuf* vec[] = {&q.r, &q.rh, &q.p, &q.v, &q.s, &q.t, &q.x,
@@ -851,6 +983,9 @@ int qs_alloc(nlm_ctx* c, int max_iter) {
     if ((rc = alloc(c, q.ctl, 64))) return rc;
     if (!q.e0) NLM_HIP(c, hipEventCreate(&q.e0));
     if (!q.e1) NLM_HIP(c, hipEventCreate(&q.e1));
+    if (!q.hctl)
+      NLM_HIP(c, hipHostMalloc(reinterpret_cast<void**>(&q.hctl),
+                               4 * sizeof(int), hipHostMallocDefault));
   }
   if (max_iter + 2 > q.max_iter_alloc) {
     q.max_iter_alloc = max_iter + 2;
@@ -1044,6 +1179,7 @@ int assemble(nlm_ctx* c, int tangent) {
   QsState& q = c->qs;
   int rc;
   if ((rc = qs_alloc(c, 8))) return rc;
+  q.bs_mode = 0;
   if (tangent == NLM_QS_TANGENT_FD) {
     if ((rc = assemble_fd(c))) return rc;
     q.assembled = true;
@@ -1052,10 +1188,12 @@ int assemble(nlm_ctx* c, int tangent) {
   }
   hipLaunchKernelGGL(k_qs_tangent, dim3(slice_grid(c->fd)), dim3(256), 0,
                      c->stream, qs_args(c), tangent, q.blk.as<double4>(),
-                     q.diag.as<double4>(), q.dinv.as<double2>());
+                     q.diag.as<double4>(), q.dinv.as<double2>(),
+                     q.bg.as<double>());
   NLM_HIP(c, hipGetLastError());
   q.assembled = true;
   q.tangent = tangent;
+  q.bs_mode = tangent == NLM_QS_TANGENT_COMPONENT ? 2 : 1;
   return NLM_OK;
 }
 
@@ -1064,9 +1202,23 @@ void spmv(nlm_ctx* c, const DevBuf& blk, const DevBuf& dg, const double2* x,
           double2* y, const double2* w, double* P0, double* P1,
           const int* ctl, const double* rc = nullptr) {
   const unsigned g = unsigned((c->fd.sell.nslices + 7) / 8 * 8);
-  hipLaunchKernelGGL((k_qs_spmv4<DOT, BT>), dim3(g), dim3(256), 0, c->stream,
-                     qs_args(c), blk.as<BT>(), dg.as<BT>(), x, y, w, P0, P1,
-                     ctl, rc ? c->qs.cinv : nullptr, c->qs.m, rc);
+  const QsState& q = c->qs;
+  // the tangent's products from its bond scalars where it has them
+  const int bs = (&blk == &q.blk && q.bond_scalar) ? q.bs_mode : 0;
+  const double* Ai = rc ? q.cinv : nullptr;
+  const double* bg = q.bg.as<double>();
+  if (bs == 1)
+    hipLaunchKernelGGL((k_qs_spmv4<DOT, BT, 1>), dim3(g), dim3(256), 0,
+                       c->stream, qs_args(c), blk.as<BT>(), dg.as<BT>(), x, y,
+                       w, P0, P1, ctl, Ai, q.m, rc, bg);
+  else if (bs == 2)
+    hipLaunchKernelGGL((k_qs_spmv4<DOT, BT, 2>), dim3(g), dim3(256), 0,
+                       c->stream, qs_args(c), blk.as<BT>(), dg.as<BT>(), x, y,
+                       w, P0, P1, ctl, Ai, q.m, rc, bg);
+  else
+    hipLaunchKernelGGL((k_qs_spmv4<DOT, BT>), dim3(g), dim3(256), 0,
+                       c->stream, qs_args(c), blk.as<BT>(), dg.as<BT>(), x, y,
+                       w, P0, P1, ctl, Ai, q.m, rc, nullptr);
 }
 // The preconditioner's blocks are FP32 (their rounding only perturbs the
 // preconditioner, never the operator).
@@ -1279,17 +1431,32 @@ int bicgstab(nlm_ctx* c, int precond, double tol, int max_iter, int* iters,
     }
     exec = q.bicg_graph;
   }
-  for (int it = 1; it <= max_iter; it += batch) {
+  // One batch stays queued ahead of the host's check of the previous one,
+  // so the device does not idle through the round trip; a batch queued
+  // after the finish (convergence, breakdown or the iteration cap) runs as
+  // no-ops.  Each batch's control words land in their own pinned slot.
+  const int nbatch = (max_iter + batch - 1) / batch;
+  hipEvent_t ev[2] = {q.e0, q.e1};
+  auto queue = [&](int b) -> int {
     if (exec)
       NLM_HIP(c, hipGraphLaunch(exec, c->stream));
     else
       launch_batch();
     NLM_HIP(c, hipGetLastError());
-    NLM_HIP(c, hipMemcpyAsync(ctl, q.ctl.p, sizeof(ctl), hipMemcpyDeviceToHost,
-                              c->stream));
-    NLM_HIP(c, hipStreamSynchronize(c->stream));
+    NLM_HIP(c, hipMemcpyAsync(q.hctl + 2 * (b & 1), q.ctl.p, 2 * sizeof(int),
+                              hipMemcpyDeviceToHost, c->stream));
+    NLM_HIP(c, hipEventRecord(ev[b & 1], c->stream));
+    return NLM_OK;
+  };
+  if ((rc = queue(0))) return rc;
+  for (int b = 0; b < nbatch; b++) {
+    if (b + 1 < nbatch && (rc = queue(b + 1))) return rc;
+    NLM_HIP(c, hipEventSynchronize(ev[b & 1]));
+    ctl[0] = q.hctl[2 * (b & 1)];
+    ctl[1] = q.hctl[2 * (b & 1) + 1];
     if (ctl[0]) break;
   }
+  NLM_HIP(c, hipStreamSynchronize(c->stream));
   double rr = 0.;
   int done_it = max_iter;
   if (ctl[0]) {
@@ -1339,6 +1506,13 @@ using namespace nlm;
 extern "C" int nlm_qs_set_graph(nlm_ctx* c, int on) {
   if (!c) return NLM_ERR_INVALID;
   c->qs.use_graph = on != 0;
+  return NLM_OK;
+}
+
+extern "C" int nlm_qs_set_launch(nlm_ctx* c, int graph, int bond_scalar) {
+  if (!c) return NLM_ERR_INVALID;
+  c->qs.use_graph = graph != 0;
+  c->qs.bond_scalar = bond_scalar != 0;
   return NLM_OK;
 }
 

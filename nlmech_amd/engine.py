@@ -341,6 +341,13 @@ class GpuFD:
         kernel by kernel (off)."""
         self._call("nlm_qs_set_graph", int(bool(on)))
 
+    def qs_set_launch(self, graph: bool, bond_scalar: bool = True):
+        """BiCGSTAB: batches from graph replays or kernel by kernel; products
+        with the analytic bond tangent from one scalar per bond (True) or
+        from the stored 2x2 blocks (False)."""
+        self._call("nlm_qs_set_launch", int(bool(graph)),
+                   int(bool(bond_scalar)))
+
     def qs_assemble(self, tangent=0):
         self._call("nlm_qs_assemble", int(tangent))
 

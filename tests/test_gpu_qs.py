@@ -66,16 +66,19 @@ def test_qs_tangent_blocks(tangent):
     scale = np.abs(ref).max()
     assert np.abs(blk - ref).max() <= 1e-13 * scale
     assert np.abs(dg - refd).max() <= 1e-13 * np.abs(refd).max()
-    # y = A_ff x
+    # y = A_ff x, from the bond scalars (g dx dx^T, dx from the jittered
+    # positions) and from the stored blocks
     rng = np.random.default_rng(3)
     x = np.zeros((p.n, 3))
     x[:, :2] = rng.standard_normal((p.n, 2))
-    y = g.qs_apply(x)
     free = q.free
     xf = x[:, :2].reshape(-1) * free
     yref = (A @ xf) * free
-    assert np.abs(y[:, :2].reshape(-1) - yref).max() <= \
-        1e-13 * np.abs(yref).max()
+    for bond_scalar in (True, False):
+        g.qs_set_launch(graph=True, bond_scalar=bond_scalar)
+        y = g.qs_apply(x)
+        assert np.abs(y[:, :2].reshape(-1) - yref).max() <= \
+            1e-13 * np.abs(yref).max(), bond_scalar
     g.close()
     q.close()
 
@@ -196,7 +199,8 @@ def test_qs_plate300_jacobi_runs():
                                      NLM_QS_PRECOND_BLOCKS | 16])
 def test_qs_bicgstab_graph_equals_eager(precond):
     """Batches replayed from the captured HIP graph give the eager launches'
-    result bit for bit, on every replay, with eager work in between; an
+    result bit for bit, on every replay, with eager work in between, with
+    the tangent's products from its bond scalars and from its blocks; an
     iteration cap inside a batch stops both at the same iteration."""
     progress(f"[gpu] qs bicgstab graph vs eager, precond {precond}")
     p = pr.qs_plate(40)
@@ -219,17 +223,19 @@ def test_qs_bicgstab_graph_equals_eager(precond):
     rng = np.random.default_rng(5)
     b = np.zeros((p.n, 3))
     b[:, :2] = rng.standard_normal((p.n, 2))
-    for max_iter, tol in ((4000, 1e-8), (13, 1e-30)):
-        g.qs_set_graph(False)
-        x0, it0, rr0 = g.qs_linear_solve(b, precond=precond, tol=tol,
-                                         max_iter=max_iter)
-        g.qs_set_graph(True)
-        for rep in range(3):
-            x, it, rr = g.qs_linear_solve(b, precond=precond, tol=tol,
-                                          max_iter=max_iter)
-            assert it == it0 and np.array_equal(x, x0) and rr == rr0, rep
-            g.qs_apply(x)          # eager work between replays
-        if max_iter == 13:
-            assert it0 == 13
+    for bond_scalar in (True, False):
+        for max_iter, tol in ((4000, 1e-8), (13, 1e-30)):
+            g.qs_set_launch(graph=False, bond_scalar=bond_scalar)
+            x0, it0, rr0 = g.qs_linear_solve(b, precond=precond, tol=tol,
+                                             max_iter=max_iter)
+            g.qs_set_launch(graph=True, bond_scalar=bond_scalar)
+            for rep in range(3):
+                x, it, rr = g.qs_linear_solve(b, precond=precond, tol=tol,
+                                              max_iter=max_iter)
+                assert it == it0 and np.array_equal(x, x0) and rr == rr0, \
+                    (bond_scalar, rep)
+                g.qs_apply(x)          # eager work between replays
+            if max_iter == 13:
+                assert it0 == 13
     g.close()
     q.close()
