@@ -245,11 +245,13 @@ def quasi_static(a):
     out = {"workload": a.qs_workload, "nodes": p.n, "load_steps": a.qs_steps,
            "newton_tol_rel": 1e-8, "linear_tol_rel": 1e-8,
            "training": {"pairs": net["pairs"], "specimens": 100,
+                        "loss_initial": net["loss_initial"],
                         "loss_first_epoch": net["loss"][0],
                         "loss_last_epoch": net["loss"][-1],
                         "s_data": net["s_data"], "s_train": net["s_train"],
                         "s_total": t_train},
            "training_two_level": {"pairs": net2["pairs"], "specimens": 60,
+                                  "loss_initial": net2["loss_initial"],
                                   "loss_first_epoch": net2["loss"][0],
                                   "loss_last_epoch": net2["loss"][-1],
                                   "s_total": t_train2}}
@@ -278,6 +280,14 @@ def quasi_static(a):
         out["learned_two_level"]["ms_total"])
     out["time_ratio_network_gain_one_level"] = (
         out["jacobi"]["ms_total"] / out["learned"]["ms_total"])
+    out["iteration_ratio_network_gain_two_level"] = (
+        out["jacobi_two_level"]["linear_iters"] /
+        out["learned_two_level"]["linear_iters"])
+    out["iteration_ratio_network_gain_one_level"] = (
+        out["jacobi"]["linear_iters"] / out["learned"]["linear_iters"])
+    out["linear_time_ratio_network_gain_two_level"] = (
+        out["jacobi_two_level"]["ms_linear"] /
+        out["learned_two_level"]["ms_linear"])
     return out
 
 

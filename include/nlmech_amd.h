@@ -509,6 +509,9 @@ int nlm_gnn_loss_grad(nlm_ctx* ctx, int K, const double* r, const double* e,
                       double* loss, float* grad);
 /* Adam step with the accumulated gradient (then cleared). */
 int nlm_gnn_adam_step(nlm_ctx* ctx, double lr);
+/* Clears the gradient accumulator (after loss evaluations that are not to
+ * be stepped on). */
+int nlm_gnn_zero_grad(nlm_ctx* ctx);
 
 #ifdef __cplusplus
 }

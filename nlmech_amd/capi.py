@@ -198,6 +198,7 @@ SIGNATURES = {
     "nlm_gnn_predict": (C.c_int, [_P]),
     "nlm_gnn_loss_grad": (C.c_int, [_P, C.c_int, _D, _D, _D, _F]),
     "nlm_gnn_adam_step": (C.c_int, [_P, C.c_double]),
+    "nlm_gnn_zero_grad": (C.c_int, [_P]),
     "nlm_qs_linear_solve": (C.c_int, [_P, _D, _D, C.c_int, C.c_double,
                                       C.c_int, C.POINTER(C.c_int), _D]),
     "nlm_qs_set_graph": (C.c_int, [_P, C.c_int]),

@@ -4,14 +4,15 @@
 // Graph: the neighbor graph in the slice layout (an edge is a slice entry
 // (i, slot k), j = col; padding entries are masked).  Layer products are
 // dense [rows x K] x [K x 64] matrix products on the matrix cores
-// (v_mfma_f32_32x32x8bf16_1k: BF16 inputs, FP32 accumulation); one wavefront
-// computes a 32-row x 64-column tile.  Weight gradients X^T dY reduce over the
-// rows with the same instruction on LDS-staged tiles, per workgroup, and a
-// second kernel adds the workgroups' partials in a fixed order.
+// (gfx950's v_mfma_f32_32x32x16_bf16: BF16 inputs, FP32 accumulation, K = 16
+// per instruction); one wavefront computes a 32-row x 64-column tile.  Weight
+// gradients X^T dY reduce over the rows with the same instruction on
+// LDS-staged tiles, per workgroup, and a second kernel adds the workgroups'
+// partials in a fixed order.
 //
-// Lane layout of v_mfma_f32_32x32x8bf16_1k (64 lanes): A[i][k] from lane
-// i + 32 (k / 4), element k % 4; B[k][j] from lane j + 32 (k / 4), element
-// k % 4; D[i][j] in lane j + 32 ((i / 4) % 2), element (i % 4) + 4 (i / 8).
+// Lane layout of v_mfma_f32_32x32x16_bf16 (64 lanes): A[i][k] from lane
+// i + 32 (k / 8), element k % 8; B[k][j] from lane j + 32 (k / 8), element
+// k % 8; D[i][j] in lane j + 32 ((i / 4) % 2), element (i % 4) + 4 (i / 8).
 
 #include <hip/hip_runtime.h>
 
@@ -53,6 +54,7 @@ constexpr int64_t kBWc = 0, kBWd = 4096, kBWf = 8192, kBWab = 12288;
 constexpr int64_t kBwdSize = kL * kBwdLayerSize;
 
 typedef short v4s __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 
 __host__ __device__ inline uint16_t f2bf(float f) {
@@ -103,6 +105,14 @@ __device__ __forceinline__ v4s load4(const uint16_t* p) {
   return r;
 }
 
+// Eight consecutive BF16 values (16-byte aligned).
+__device__ __forceinline__ v8bf load8(const uint16_t* p) {
+  return __builtin_bit_cast(v8bf, *reinterpret_cast<const uint4*>(p));
+}
+__device__ __forceinline__ v8bf zero8() {
+  return __builtin_bit_cast(v8bf, make_uint4(0u, 0u, 0u, 0u));
+}
+
 // Y = act(X W + bias + P[prow] + Q[qrow]) masked; one wavefront per 32 rows.
 __global__ __launch_bounds__(256) void k_mm(MmArgs a) {
   const int lane = threadIdx.x & 63;
@@ -115,21 +125,25 @@ __global__ __launch_bounds__(256) void k_mm(MmArgs a) {
   }
   const int64_t arow = r0 + (lane & 31);
   const bool arow_ok = arow < a.M;
-  const int kq = 4 * (lane >> 5);
+  const int kq = 8 * (lane >> 5);
   const int ld1 = a.K < 64 ? a.K : 64;
-  for (int k0 = 0; k0 < a.K; k0 += 8) {
+  // K = 8: the upper half of the 16-deep step is zero
+  for (int k0 = 0; k0 < a.K; k0 += 16) {
     const int kk = k0 + kq;
-    v4s av = {0, 0, 0, 0};
-    if (arow_ok) {
+    const bool kin = kk < a.K;
+    v8bf av = zero8(), b0 = zero8(), b1 = zero8();
+    if (arow_ok && kin) {
       if (kk < 64)
-        av = load4(a.X1 + arow * ld1 + kk);
+        av = load8(a.X1 + arow * ld1 + kk);
       else
-        av = load4(a.X2 + arow * 64 + (kk - 64));
+        av = load8(a.X2 + arow * 64 + (kk - 64));
     }
-    const v4s b0 = load4(a.Wt + int64_t(lane & 31) * a.K + kk);
-    const v4s b1 = load4(a.Wt + int64_t(32 + (lane & 31)) * a.K + kk);
-    acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, b0, acc0, 0, 0, 0);
-    acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, b1, acc1, 0, 0, 0);
+    if (kin) {
+      b0 = load8(a.Wt + int64_t(lane & 31) * a.K + kk);
+      b1 = load8(a.Wt + int64_t(32 + (lane & 31)) * a.K + kk);
+    }
+    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, b0, acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, b1, acc1, 0, 0, 0);
   }
   for (int t = 0; t < 16; t++) {
     const int64_t row = r0 + 8 * (t >> 2) + 4 * (lane >> 5) + (t & 3);
@@ -197,15 +211,16 @@ __global__ __launch_bounds__(256) void k_mm_tn(int64_t M, int K,
       const int tile = wave + 4 * u;
       if (tile >= ntiles) continue;
       const int kt = tile >> 1, nt = tile & 1;
-      for (int kd = 0; kd < kTnRows; kd += 8) {
-        v4s av, bv;
-        for (int e = 0; e < 4; e++) {
-          const int r = kd + 4 * (lane >> 5) + e;
-          av[e] = short(sX[r * 128 + kt * 32 + (lane & 31)]);
-          bv[e] = short(sD[r * 64 + nt * 32 + (lane & 31)]);
+      for (int kd = 0; kd < kTnRows; kd += 16) {
+        uint16_t av[8], bv[8];
+        for (int e = 0; e < 8; e++) {
+          const int r = kd + 8 * (lane >> 5) + e;
+          av[e] = sX[r * 128 + kt * 32 + (lane & 31)];
+          bv[e] = sD[r * 64 + nt * 32 + (lane & 31)];
         }
-        acc[u] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, acc[u], 0, 0,
-                                                          0);
+        acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+            __builtin_bit_cast(v8bf, av), __builtin_bit_cast(v8bf, bv), acc[u],
+            0, 0, 0);
       }
     }
     __syncthreads();
@@ -1008,6 +1023,7 @@ int gnn_forward(nlm_ctx* c) {
   hipLaunchKernelGGL(k_gnn_decode, dim3(gs), dim3(256), 0, c->stream, d);
   NLM_HIP(c, hipGetLastError());
   q.has_m = true;
+  q.mhalf_ok = false;
   return NLM_OK;
 }
 
@@ -1253,6 +1269,16 @@ extern "C" int nlm_gnn_loss_grad(nlm_ctx* c, int K, const double* r,
   if (grad)
     NLM_HIP(c, hipMemcpy(grad, G.grad.p, sizeof(float) * kNumParams,
                          hipMemcpyDeviceToHost));
+  return NLM_OK;
+}
+
+extern "C" int nlm_gnn_zero_grad(nlm_ctx* c) {
+  int rc = gnn_check(c, "nlm_gnn_zero_grad");
+  if (rc) return rc;
+  GnnState& G = c->gnn;
+  NLM_HIP(c, hipMemsetAsync(G.grad.p, 0, sizeof(float) * kNumParams,
+                            c->stream));
+  NLM_HIP(c, hipStreamSynchronize(c->stream));
   return NLM_OK;
 }
 

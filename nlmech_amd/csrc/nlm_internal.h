@@ -220,7 +220,9 @@ struct QsState {
   DevBuf bg;
   int bs_mode = 0;
   bool bond_scalar = true;   // products of the tangent from bg
-  DevBuf mblk, mdiag;        // preconditioner blocks
+  DevBuf mblk, mdiag;        // preconditioner blocks (FP32)
+  DevBuf mhalf;              // BF16 copy of mblk read by the solver
+  bool mhalf_ok = false;     // mhalf matches mblk
   DevBuf r, rh, p, v, s, t, x, ph, sh, b, y, dup, u0;
   DevBuf part;               // 4 * nb partial sums
   DevBuf scal;               // per-iteration scalars (rho, alpha, omega, rr)

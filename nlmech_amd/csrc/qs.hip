@@ -343,15 +343,52 @@ __device__ double2 coarse_rows(const double* __restrict__ Ainv, int m,
   return z;
 }
 
+// BF16 copy of the preconditioner's bond blocks (8 bytes per slot): the
+// rounding (relative 2^-9) perturbs the preconditioner only, never the
+// operator.
+struct __align__(8) Bf4 {
+  uint16_t x, y, z, w;
+};
+__device__ __forceinline__ float bf16_to_f(uint16_t b) {
+  return __uint_as_float(uint32_t(b) << 16);
+}
+__device__ __forceinline__ uint16_t f_to_bf16(float f) {  // nearest even
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return uint16_t(u >> 16);
+}
+__device__ __forceinline__ double4 d4(const double4& b) { return b; }
+__device__ __forceinline__ double4 d4(const float4& b) {
+  return make_double4(b.x, b.y, b.z, b.w);
+}
+__device__ __forceinline__ double4 d4(const Bf4& b) {
+  return make_double4(bf16_to_f(b.x), bf16_to_f(b.y), bf16_to_f(b.z),
+                      bf16_to_f(b.w));
+}
+
+__global__ __launch_bounds__(256) void k_blocks_bf16(
+    int64_t nent, const float4* __restrict__ src, Bf4* __restrict__ dst) {
+  const int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (e >= nent) return;
+  const float4 b = src[e];
+  Bf4 o;
+  o.x = f_to_bf16(b.x);
+  o.y = f_to_bf16(b.y);
+  o.z = f_to_bf16(b.z);
+  o.w = f_to_bf16(b.w);
+  dst[e] = o;
+}
+
 // y = A x over the slices (one workgroup per slice, wavefront q taking
 // slots q, q+4, ...), A from its blocks (BS 0), or from one scalar per bond
 // (the analytic bond tangent: off-diagonal block -B_ij = g_ij dx dx^T, BS 1;
 // the component tangent diag(g dx_0^2, g dx_1^2), BS 2) with dx = x_j - x_i
 // formed from the positions -- 12 bytes per slot instead of 36.  DOT 1:
 // partial (w, y) per slice; DOT 2: also (y, y).
-template <int DOT, class BT, int BS = 0>
+template <int DOT, class BT, int BS = 0, class DT = BT>
 __global__ __launch_bounds__(256) void k_qs_spmv4(
-    QsArgs a, const BT* __restrict__ blk, const BT* __restrict__ dg,
+    QsArgs a, const BT* __restrict__ blk, const DT* __restrict__ dg,
     const double2* __restrict__ xv, double2* __restrict__ y,
     const double2* __restrict__ wv, double* __restrict__ P0,
     double* __restrict__ P1, const int* __restrict__ ctl,
@@ -376,10 +413,10 @@ __global__ __launch_bounds__(256) void k_qs_spmv4(
   if (BS == 0) {
     for (int k = wq; k < width; k += 4) {
       const int64_t e = cb + int64_t(k) * 64 + lane;
-      const BT B = blk[e];
+      const double4 B = d4(blk[e]);
       const double2 xj = xv[a.col[e]];
-      acc.x += double(B.x) * xj.x + double(B.y) * xj.y;
-      acc.y += double(B.z) * xj.x + double(B.w) * xj.y;
+      acc.x += B.x * xj.x + B.y * xj.y;
+      acc.y += B.z * xj.x + B.w * xj.y;
     }
   } else {
     // padding slots hold g = 0 and a valid column
@@ -418,10 +455,10 @@ __global__ __launch_bounds__(256) void k_qs_spmv4(
   acc.y += red[0][lane].y + red[1][lane].y + red[2][lane].y;
   double p0 = 0., p1 = 0.;
   if (valid) {
-    const BT D = dg[i];
+    const double4 D = d4(dg[i]);
     const double2 xi = xv[i];
-    acc.x += double(D.x) * xi.x + double(D.y) * xi.y;
-    acc.y += double(D.z) * xi.x + double(D.w) * xi.y;
+    acc.x += D.x * xi.x + D.y * xi.y;
+    acc.y += D.z * xi.x + D.w * xi.y;
     acc.x += zc.x;
     acc.y += zc.y;
     const uint32_t fl = a.flags[i];
@@ -1197,7 +1234,7 @@ int assemble(nlm_ctx* c, int tangent) {
   return NLM_OK;
 }
 
-template <int DOT, class BT = double4>
+template <int DOT, class BT = double4, class DT = BT>
 void spmv(nlm_ctx* c, const DevBuf& blk, const DevBuf& dg, const double2* x,
           double2* y, const double2* w, double* P0, double* P1,
           const int* ctl, const double* rc = nullptr) {
@@ -1208,25 +1245,49 @@ void spmv(nlm_ctx* c, const DevBuf& blk, const DevBuf& dg, const double2* x,
   const double* Ai = rc ? q.cinv : nullptr;
   const double* bg = q.bg.as<double>();
   if (bs == 1)
-    hipLaunchKernelGGL((k_qs_spmv4<DOT, BT, 1>), dim3(g), dim3(256), 0,
-                       c->stream, qs_args(c), blk.as<BT>(), dg.as<BT>(), x, y,
+    hipLaunchKernelGGL((k_qs_spmv4<DOT, BT, 1, DT>), dim3(g), dim3(256), 0,
+                       c->stream, qs_args(c), blk.as<BT>(), dg.as<DT>(), x, y,
                        w, P0, P1, ctl, Ai, q.m, rc, bg);
   else if (bs == 2)
-    hipLaunchKernelGGL((k_qs_spmv4<DOT, BT, 2>), dim3(g), dim3(256), 0,
-                       c->stream, qs_args(c), blk.as<BT>(), dg.as<BT>(), x, y,
+    hipLaunchKernelGGL((k_qs_spmv4<DOT, BT, 2, DT>), dim3(g), dim3(256), 0,
+                       c->stream, qs_args(c), blk.as<BT>(), dg.as<DT>(), x, y,
                        w, P0, P1, ctl, Ai, q.m, rc, bg);
   else
-    hipLaunchKernelGGL((k_qs_spmv4<DOT, BT>), dim3(g), dim3(256), 0,
-                       c->stream, qs_args(c), blk.as<BT>(), dg.as<BT>(), x, y,
+    hipLaunchKernelGGL((k_qs_spmv4<DOT, BT, 0, DT>), dim3(g), dim3(256), 0,
+                       c->stream, qs_args(c), blk.as<BT>(), dg.as<DT>(), x, y,
                        w, P0, P1, ctl, Ai, q.m, rc, nullptr);
 }
-// The preconditioner's blocks are FP32 (their rounding only perturbs the
-// preconditioner, never the operator).
-// rc non-null: add the coarse correction of the restricted vector rc.
+// The preconditioner's bond blocks are read from their BF16 copy, its node
+// blocks in FP32 (rounding perturbs the preconditioner only, never the
+// operator).  rc non-null: add the coarse correction of the restricted
+// vector rc.
 void mapply(nlm_ctx* c, const double2* x, double2* y, const int* ctl,
             const double* rc) {
-  spmv<0, float4>(c, c->qs.mblk, c->qs.mdiag, x, y, nullptr, nullptr, nullptr,
-                  ctl, rc);
+  spmv<0, Bf4, float4>(c, c->qs.mhalf, c->qs.mdiag, x, y, nullptr, nullptr,
+                       nullptr, ctl, rc);
+}
+
+// The BF16 copy of the preconditioner's bond blocks, made once after they
+// change.
+int precond_bf16(nlm_ctx* c) {
+  QsState& q = c->qs;
+  if (q.mhalf_ok) return NLM_OK;
+  const int64_t nent = c->fd.sell.nentries;
+  int rc;
+  if (!q.mhalf.p || q.mhalf.bytes < sizeof(Bf4) * nent) {
+    if (q.bicg_graph) {  // it may reference the buffer being replaced
+      (void)hipGraphExecDestroy(q.bicg_graph);
+      q.bicg_graph = nullptr;
+      q.bicg_key.clear();
+    }
+    if ((rc = alloc(c, q.mhalf, sizeof(Bf4) * nent + 256))) return rc;
+  }
+  hipLaunchKernelGGL(k_blocks_bf16, dim3(unsigned((nent + 255) / 256)),
+                     dim3(256), 0, c->stream, nent, q.mblk.as<float4>(),
+                     q.mhalf.as<Bf4>());
+  NLM_HIP(c, hipGetLastError());
+  q.mhalf_ok = true;
+  return NLM_OK;
 }
 
 // Coarse operator of the assembled tangent: adjacency of the aggregates
@@ -1311,6 +1372,7 @@ int bicgstab(nlm_ctx* c, int precond, double tol, int max_iter, int* iters,
   precond &= ~NLM_QS_PRECOND_COARSE;
   if (precond == NLM_QS_PRECOND_BLOCKS && !q.has_m)
     return fail(c, NLM_ERR_STATE, "BiCGSTAB: no preconditioner blocks set");
+  if (precond == NLM_QS_PRECOND_BLOCKS && (rc = precond_bf16(c))) return rc;
   if (coarse && (!q.cinv || precond == NLM_QS_PRECOND_NONE))
     return fail(c, NLM_ERR_STATE, "BiCGSTAB: no coarse operator set up");
   VecArgs a = vec_args(c);
@@ -1404,7 +1466,7 @@ int bicgstab(nlm_ctx* c, int precond, double tol, int max_iter, int* iters,
     key.cinv = q.cinv;
     key.blk = q.blk.p;
     key.diag = q.diag.p;
-    key.mblk = q.mblk.p;
+    key.mblk = q.mhalf.p;
     key.mdiag = q.mdiag.p;
     key.zcp = q.zcp.p;
     key.zcs = q.zcs.p;
@@ -1626,6 +1688,7 @@ extern "C" int nlm_qs_set_precond(nlm_ctx* c, const double* blocks,
   NLM_HIP(c, hipMemcpy(q.mdiag.p, hd.data(), sizeof(float4) * s.n,
                        hipMemcpyHostToDevice));
   q.has_m = true;
+  q.mhalf_ok = false;
   return NLM_OK;
 }
 

@@ -422,6 +422,9 @@ class GpuFD:
     def gnn_adam_step(self, lr):
         self._call("nlm_gnn_adam_step", float(lr))
 
+    def gnn_zero_grad(self):
+        self._call("nlm_gnn_zero_grad")
+
     def qs_linear_solve(self, b, precond=1, tol=1e-8, max_iter=5000):
         b = self._node_vec(b, "b")
         x = np.zeros((self.p.n, 3))
