@@ -653,6 +653,71 @@ __global__ __launch_bounds__(256) void k_bicg_s(VecArgs a, int k) {
   if (RESTRICT) block_restrict(sn.x, sn.y, a.rc, blockIdx.x);
 }
 
+// Step 2 with the preconditioner's bond blocks, one level: alpha, s = r -
+// alpha v and sh = M s in one launch.  Each row forms its neighbors' s_j =
+// r_j - alpha v_j itself (r and v are not written by this step, so every
+// row reads the same operands the owner does); the owner stores its s_i.
+// One workgroup per slice, wavefront q taking slots q, q+4, ...
+__global__ __launch_bounds__(256) void k_bicg_s_blocks(
+    VecArgs a, QsArgs q, const Bf4* __restrict__ mb,
+    const float4* __restrict__ md, int k) {
+  const int done = a.ctl[0];
+  const int it = a.ctl[2] + k;
+  const double rho1[2] = {a.sc.rho_p(0), a.sc.rho_p(1)};
+  const int64_t sl = xcd_block(q.nslices);
+  if (sl >= q.nslices) return;  // whole workgroups
+  double rv1[1];
+  {
+    const double* const P[1] = {a.P2};
+    block_sums<1>(P, a.nbs, rv1);
+  }
+  if (done || it > a.max_it) return;
+  const double rv = rv1[0];
+  const bool brk = rv == 0.;
+  const double alpha = brk ? 0. : rho1[(it - 1) & 1] / rv;
+  if (sl == 0 && threadIdx.x == 0) {
+    a.sc.alpha(it) = alpha;
+    a.sc.alpha_p(it & 1) = alpha;
+    if (brk) {
+      a.ctl[0] = 2;
+      a.ctl[1] = it - 1;
+    }
+  }
+  if (brk) return;
+  __shared__ double2 red[3][64];
+  const int lane = threadIdx.x & 63;
+  const int wq = threadIdx.x >> 6;
+  const int64_t i = sl * 64 + lane;
+  const bool valid = i < a.n;
+  const int64_t cb = q.sl_ptr[sl];
+  const int width = int((q.sl_ptr[sl + 1] - cb) >> 6);
+  double2 acc = make_double2(0., 0.);
+  for (int kk = wq; kk < width; kk += 4) {
+    const int64_t e = cb + int64_t(kk) * 64 + lane;
+    const double4 B = d4(mb[e]);
+    const int32_t j = q.col[e];
+    const double2 rj = a.r[j], vj = a.v[j];
+    const double2 sj = make_double2(rj.x - alpha * vj.x, rj.y - alpha * vj.y);
+    acc.x += B.x * sj.x + B.y * sj.y;
+    acc.y += B.z * sj.x + B.w * sj.y;
+  }
+  if (wq > 0) red[wq - 1][lane] = acc;
+  __syncthreads();
+  if (wq != 0 || !valid) return;
+  acc.x += red[0][lane].x + red[1][lane].x + red[2][lane].x;
+  acc.y += red[0][lane].y + red[1][lane].y + red[2][lane].y;
+  const double2 r = a.r[i], v = a.v[i];
+  const double2 si = make_double2(r.x - alpha * v.x, r.y - alpha * v.y);
+  a.s[i] = si;
+  const double4 D = d4(md[i]);
+  acc.x += D.x * si.x + D.y * si.y;
+  acc.y += D.z * si.x + D.w * si.y;
+  const uint32_t fl = a.flags[i];
+  if (dof_fixed(fl, 0)) acc.x = 0.;
+  if (dof_fixed(fl, 1)) acc.y = 0.;
+  a.sh[i] = acc;
+}
+
 // ---- coarse level ------------------------------------------------------------
 // Aggregates are the layout's 256-node blocks (compact Morton patches); the
 // coarse space is piecewise constant per aggregate and dof.  A_c = R^T A R
@@ -1379,6 +1444,7 @@ int bicgstab(nlm_ctx* c, int precond, double tol, int max_iter, int* iters,
   a.rc = coarse ? q.rc.as<double>() : nullptr;
   const unsigned gmv = unsigned((q.m + 3) / 4);
   const unsigned g = node_grid(s);
+  const unsigned gsl = unsigned((s.sell.nslices + 7) / 8 * 8);
   hipLaunchKernelGGL(k_bicg_init, dim3(g), dim3(256), 0, c->stream, a,
                      q.b.as<double2>());
   NLM_HIP(c, hipGetLastError());
@@ -1440,11 +1506,10 @@ int bicgstab(nlm_ctx* c, int precond, double tol, int max_iter, int* iters,
       else if (precond == NLM_QS_PRECOND_NONE)
         hipLaunchKernelGGL(k_bicg_s<0>, dim3(g), dim3(256), 0, c->stream, a,
                            k);
-      else {
-        hipLaunchKernelGGL(k_bicg_s<2>, dim3(g), dim3(256), 0, c->stream, a,
-                           k);
-        mapply(c, a.s, a.sh, a.ctl, nullptr);
-      }
+      else
+        hipLaunchKernelGGL(k_bicg_s_blocks, dim3(gsl), dim3(256), 0,
+                           c->stream, a, qs_args(c), q.mhalf.as<Bf4>(),
+                           q.mdiag.as<float4>(), k);
       spmv<2>(c, q.blk, q.diag, a.sh, a.t, a.s, a.P2, a.P3, a.ctl);
       hipLaunchKernelGGL(k_bicg_x, dim3(g), dim3(256), 0, c->stream, a, k);
     }

@@ -196,6 +196,7 @@ def test_qs_plate300_jacobi_runs():
 @pytest.mark.parametrize("precond", [NLM_QS_PRECOND_JACOBI,
                                      NLM_QS_PRECOND_JACOBI | 16,
                                      NLM_QS_PRECOND_NONE,
+                                     NLM_QS_PRECOND_BLOCKS,
                                      NLM_QS_PRECOND_BLOCKS | 16])
 def test_qs_bicgstab_graph_equals_eager(precond):
     """Batches replayed from the captured HIP graph give the eager launches'
