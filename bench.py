@@ -97,7 +97,7 @@ def run_mode(p, precision, steps, warmup):
     from nlmech_amd.engine import GpuFD
     g = GpuFD(p, precision=precision)
     g.init()
-    unroll, tune_us = g.autotune()
+    tile, unroll, table = g.autotune_tiles()
     if warmup > 0:
         g.step(warmup)
     t0 = time.perf_counter()
@@ -107,7 +107,8 @@ def run_mode(p, precision, steps, warmup):
     bonds = g.get_bonds()
     nnz = g.nnz
     return g, dict(ms_total=ms_total, ms_fused=ms_fused, wall=wall, u=u,
-                   bonds=bonds, nnz=nnz, unroll=unroll)
+                   bonds=bonds, nnz=nnz, unroll=unroll, tile=tile,
+                   tune_table_us=table.tolist())
 
 
 def state_based(a):
@@ -172,8 +173,9 @@ def ensemble(a):
     """256 uneven plates (sides U{50..400} nodes, random angled notches)
     packed into one specimen and advanced together (FP32-force mode): one
     fused launch per step for all plates, the steps replayed from a captured
-    HIP graph, the chunk width autotuned; against the same plates stepped one
-    at a time (one context each, direct launches)."""
+    HIP graph, the staged kernel's tile and chunk width autotuned jointly;
+    against the same plates stepped one at a time (one context each, direct
+    launches)."""
     from nlmech_amd import problems as pr
     from nlmech_amd.capi import NLM_PREC_FP32_FORCE
     from nlmech_amd.engine import GpuFD
@@ -182,7 +184,7 @@ def ensemble(a):
     g = GpuFD(p, precision=NLM_PREC_FP32_FORCE)
     t_setup = time.perf_counter() - t0
     g.init()
-    unroll, _ = g.autotune()
+    tile, unroll, table = g.autotune_tiles()
     g.step_graph(10, 10)
     steps = a.ens_steps
     ms_total, ms_step = g.timed_steps_graph(steps, 50)
@@ -210,7 +212,9 @@ def ensemble(a):
           file=sys.stderr, flush=True)
     return {"workload": "ensemble256", "plates": len(p.specimens),
             "nodes": p.n, "bonds": nnz, "steps": steps, "dt": p.dt,
-            "mode": "fp32", "autotuned_unroll": unroll,
+            "mode": "fp32", "autotuned_unroll": unroll, "autotuned_tile": tile,
+            "tune_table_us": {"tiles": [128, 256, 512], "chunks": [2, 4, 8],
+                              "us_per_step": table.tolist()},
             "graph_chunk": 50, "setup_s": t_setup,
             "packed": {"value": value, "unit": "bond updates/s",
                        "ms_per_step": ms_total / steps,
@@ -484,7 +488,9 @@ def main():
         out["modes"][name] = {"value": v, "ms_per_step": rr["ms_total"] / a.steps,
                               "fused_kernel_us": rr["ms_fused"] * 1e3,
                               "roofline_frac": ach / HBM_PEAK_GBS,
-                              "autotuned_unroll": rr["unroll"]}
+                              "autotuned_unroll": rr["unroll"],
+                              "autotuned_tile": rr["tile"],
+                              "tune_table_us": rr["tune_table_us"]}
     if "fp32" in res and "fp64" in res:
         out["fp32_over_fp64"] = (out["modes"]["fp32"]["value"] /
                                  out["modes"]["fp64"]["value"])

@@ -361,6 +361,21 @@ int nlm_fd_timed_steps_graph(nlm_ctx* ctx, int nsteps, int chunk,
 int nlm_fd_autotune(nlm_ctx* ctx, int* chosen, double* best_us);
 /* Sets the chunk width by hand (2, 4 or 8). */
 int nlm_fd_set_unroll(nlm_ctx* ctx, int unroll);
+/* Joint autotuner (no counterpart in the reference) of the 2D bond-based
+ * LDS-staged step kernel: for each tile (nodes per workgroup and halo block:
+ * 128, 256, 512) the halos are rebuilt and each chunk width timed as in
+ * nlm_fd_autotune; the fastest pair is kept.  us (optional, 9 entries,
+ * tile-major: us[3 t + k] for tile {128, 256, 512}[t] and chunk {2, 4, 8}[k])
+ * receives the times per step in microseconds, 0 where a pair was not run
+ * (a tile whose halo exceeds the 64 KiB LDS budget; chunks other than 4 in
+ * the FP64 exact mode).  Models without the staged kernel tune the chunk
+ * width alone.  The tile changes the data movement only: FP64 modes give
+ * the same bits at every tile; the FP32-force mode's block-relative FP32
+ * positions take their origin from the tile, so its rounding (within that
+ * mode's tolerance) depends on it. */
+int nlm_fd_autotune_tiles(nlm_ctx* ctx, int* tile, int* chunk, double* us);
+/* Sets the staged kernel's tile by hand (128, 256 or 512). */
+int nlm_fd_set_tile(nlm_ctx* ctx, int tile);
 /* Number of directed bonds of the configured model. */
 int nlm_fd_num_bonds(nlm_ctx* ctx, int64_t* nnz);
 /* Name of the fused step kernel launched for the configured model. */

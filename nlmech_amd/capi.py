@@ -177,6 +177,9 @@ SIGNATURES = {
     "nlm_fd_num_bonds": (C.c_int, [_P, _I64]),
     "nlm_fd_autotune": (C.c_int, [_P, C.POINTER(C.c_int), _D]),
     "nlm_fd_set_unroll": (C.c_int, [_P, C.c_int]),
+    "nlm_fd_autotune_tiles": (C.c_int, [_P, C.POINTER(C.c_int),
+                                        C.POINTER(C.c_int), _D]),
+    "nlm_fd_set_tile": (C.c_int, [_P, C.c_int]),
     "nlm_fd_kernel_name": (C.c_char_p, [_P]),
     "nlm_es_params_from_deck": (C.c_int, [C.POINTER(EsDeck),
                                           C.POINTER(EsParams)]),

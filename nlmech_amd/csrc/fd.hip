@@ -129,12 +129,14 @@ __device__ __forceinline__ void fd_step_body(
     t_force = a.tdev[0];
     t_next = a.tdev[1];
   }
-  int64_t nblk = (a.nslices * 64 + 255) / 256;
+  // TILED: the workgroup's nodes (its size: 128, 256 or 512, the tuned
+  // tile) are one block of the layout's halos; the halo (every node its rows
+  // reference) is staged in LDS before the bond loop and neighbors are read
+  // through halo-local entries (LDS byte offsets).
+  const int bn = TILED ? int(blockDim.x) : 256;
+  int64_t nblk = (a.nslices * 64 + bn - 1) / bn;
   int64_t lb = xcd_block(nblk);
   if (lb >= nblk) return;
-  // TILED: the workgroup's 256 nodes are one block of the layout; its halo
-  // (every node its rows reference) is staged in LDS before the bond loop
-  // and neighbors are read through halo-local entries (LDS byte offsets).
   static_assert(!TILED || DIM == 2, "the LDS-staged kernel is 2D");
   constexpr bool REL = TILED && MODE == kModeFp32;  // block-relative FP32
   typedef HaloRecOf<MODE> Rec;
@@ -146,7 +148,7 @@ __device__ __forceinline__ void fd_step_body(
     if constexpr (REL) org = a.blk_org[lb];
     hb = a.halo_ptr[lb];
     const int hn = int(a.halo_ptr[lb + 1] - hb);
-    for (int t = threadIdx.x; t < hn; t += 256) {
+    for (int t = threadIdx.x; t < hn; t += bn) {
       const uint32_t g = uint32_t(a.halo[hb + t]);
       Rec r;
       const double2 xg = ld32(a.x, g), ug = ld32(a.u, g);
@@ -169,7 +171,8 @@ __device__ __forceinline__ void fd_step_body(
   // from the first lane makes the slice bounds, the slot loop and the
   // bond-bit addresses scalar.
   const int lane = threadIdx.x & 63;
-  const int64_t s = lb * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t s =
+      lb * (bn >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (s >= a.nslices) return;  // whole wavefront
   const int64_t i = s * 64 + lane;
   bool valid = i < a.n;
@@ -461,7 +464,8 @@ __device__ __forceinline__ void fd_step_body(
   a.v_out[i] = make_v<DIM>(vn);
 }
 
-// The fused step kernel (one integrateCD iteration or a force evaluation).
+// The fused step kernel (one integrateCD iteration or a force evaluation),
+// 256-node workgroups.
 template <int DIM, int MODE, int KIND, bool BASE_F, int INF, int U,
           bool NOFAIL, bool TILED>
 __global__ __launch_bounds__(256) void k_fd(
@@ -472,17 +476,28 @@ __global__ __launch_bounds__(256) void k_fd(
       a, bits_ro, vex_ro, t_force, t_next);
 }
 
-// The same kernel held to at least 6 wavefronts per SIMD (at most 80
-// vector registers): the FP32-force staged kernel with two slots per chunk
-// fits without spilling, and the compiler's own register choice can miss
-// that occupancy step by a register.
+// The staged kernel with tiles of up to 512 nodes (the workgroup size).
 template <int DIM, int MODE, int KIND, bool BASE_F, int INF, int U,
-          bool NOFAIL, bool TILED>
-__global__ __launch_bounds__(256, 6) void k_fd_w6(
+          bool NOFAIL>
+__global__ __launch_bounds__(512) void k_fd_t(
     FdArgs<DIM> a, const unsigned long long* __restrict__ bits_ro,
     const unsigned long long* __restrict__ vex_ro, double t_force,
     double t_next) {
-  fd_step_body<DIM, MODE, KIND, BASE_F, INF, U, NOFAIL, TILED>(
+  fd_step_body<DIM, MODE, KIND, BASE_F, INF, U, NOFAIL, true>(
+      a, bits_ro, vex_ro, t_force, t_next);
+}
+
+// The staged kernel held to at least 6 wavefronts per SIMD (at most 80
+// vector registers; 3 workgroups of 512 or their equivalent): the FP32-force
+// staged kernel with two slots per chunk fits without spilling, and the
+// compiler's own register choice can miss that occupancy step by a register.
+template <int DIM, int MODE, int KIND, bool BASE_F, int INF, int U,
+          bool NOFAIL>
+__global__ __launch_bounds__(512, 3) void k_fd_w6(
+    FdArgs<DIM> a, const unsigned long long* __restrict__ bits_ro,
+    const unsigned long long* __restrict__ vex_ro, double t_force,
+    double t_next) {
+  fd_step_body<DIM, MODE, KIND, BASE_F, INF, U, NOFAIL, true>(
       a, bits_ro, vex_ro, t_force, t_next);
 }
 
@@ -875,15 +890,18 @@ void launch_t(nlm_ctx* c, const FdArgs<DIM>& a, double tf, double tn) {
   const unsigned long long* bits_ro = a.bits;
   if constexpr (DIM == 2) {
     if (c->fd.tiled && c->fd.use_tiled) {
+      // one workgroup per tile of the layout's halos
+      const int bn = c->fd.tile;
+      const unsigned gt = grid_xcd(c->fd.sell.nslices * 64, bn);
       size_t lds = size_t(a.halo_cap) * sizeof(HaloRecOf<MODE>);
       if constexpr (MODE == kModeFp32 && U == 2) {
-        hipLaunchKernelGGL(
-            (k_fd_w6<DIM, MODE, KIND, BASE_F, INF, U, NOFAIL, true>), dim3(g),
-            dim3(256), lds, c->stream, a, bits_ro, a.vexw, tf, tn);
+        hipLaunchKernelGGL((k_fd_w6<DIM, MODE, KIND, BASE_F, INF, U, NOFAIL>),
+                           dim3(gt), dim3(bn), lds, c->stream, a, bits_ro,
+                           a.vexw, tf, tn);
         return;
       }
-      hipLaunchKernelGGL((k_fd<DIM, MODE, KIND, BASE_F, INF, U, NOFAIL, true>),
-                         dim3(g), dim3(256), lds, c->stream, a, bits_ro,
+      hipLaunchKernelGGL((k_fd_t<DIM, MODE, KIND, BASE_F, INF, U, NOFAIL>),
+                         dim3(gt), dim3(bn), lds, c->stream, a, bits_ro,
                          a.vexw, tf, tn);
       return;
     }
@@ -1245,6 +1263,7 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
     return rc;
   if ((rc = up(s.halo, L.halo.data(), sizeof(int32_t) * L.halo.size()))) return rc;
   s.halo_max = L.halo_max;
+  s.tile = 256;
   // LDS budget of the staged kernel: one 48-byte record per halo node
   s.tiled = !es_mat && dim == 2 && L.halo_max > 0 &&
             (L.halo_max + 1) * int64_t(sizeof(HaloRec)) <= 64 * 1024;
@@ -1254,7 +1273,7 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
     hipFuncAttributes fa;
     if (hipFuncGetAttributes(
             &fa, reinterpret_cast<const void*>(
-                     &k_fd_w6<2, kModeFp32, KIND_CD, false, 0, 2, false, true>)) !=
+                     &k_fd_w6<2, kModeFp32, KIND_CD, false, 0, 2, false>)) !=
             hipSuccess ||
         fa.sharedSizeBytes != 0)
       s.tiled = false;
@@ -2115,8 +2134,10 @@ namespace {
 // current state.  The trials write the spare u/v buffers (overwritten by the
 // next step anyway) and a scratch copy of the bond bits, so the model state
 // is unchanged.
+// us[k]: microseconds per step with chunk width {2, 4, 8}[k] (0 = not
+// timed: the FP64-exact mode runs U = 4 only).
 template <int DIM>
-int autotune(nlm_ctx* c, int* chosen, double* best_us) {
+int time_chunks(nlm_ctx* c, double us[3]) {
   FdState& s = c->fd;
   DevBuf bits_copy;
   int rc;
@@ -2126,13 +2147,14 @@ int autotune(nlm_ctx* c, int* chosen, double* best_us) {
   FdArgs<DIM> a = make_args<DIM>(c);
   a.bits = bits_copy.as<unsigned long long>();
   const int cands[3] = {2, 4, 8};
-  double best = 1e300;
-  int pick = 4;
   hipEvent_t e0, e1;
   NLM_HIP(c, hipEventCreate(&e0));
   NLM_HIP(c, hipEventCreate(&e1));
   const int reps = 6;
-  for (int U : cands) {
+  for (int k = 0; k < 3; k++) {
+    const int U = cands[k];
+    us[k] = 0.;
+    if (s.precision == NLM_PREC_FP64_EXACT && U != 4) continue;
     for (int r = 0; r < 2; r++)
       launch_mode<DIM, KIND_CD, false>(c, a, s.time, s.time + s.dt, U);
     NLM_HIP(c, hipEventRecord(e0, c->stream));
@@ -2142,18 +2164,104 @@ int autotune(nlm_ctx* c, int* chosen, double* best_us) {
     NLM_HIP(c, hipEventSynchronize(e1));
     float ms = 0.f;
     NLM_HIP(c, hipEventElapsedTime(&ms, e0, e1));
-    double us = 1e3 * double(ms) / reps;
-    if (us < best) {
-      best = us;
-      pick = U;
-    }
+    us[k] = 1e3 * double(ms) / reps;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   NLM_HIP(c, hipGetLastError());
-  s.unroll = pick;
+  return NLM_OK;
+}
+
+int best_chunk(const double us[3], double* best) {
+  const int cands[3] = {2, 4, 8};
+  int pick = 4;
+  *best = 1e300;
+  for (int k = 0; k < 3; k++)
+    if (us[k] > 0. && us[k] < *best) {
+      *best = us[k];
+      pick = cands[k];
+    }
+  return pick;
+}
+
+template <int DIM>
+int autotune(nlm_ctx* c, int* chosen, double* best_us) {
+  double us[3];
+  int rc = time_chunks<DIM>(c, us);
+  if (rc) return rc;
+  double best;
+  const int pick = best_chunk(us, &best);
+  c->fd.unroll = pick;
   if (chosen) *chosen = pick;
   if (best_us) *best_us = best;
+  return NLM_OK;
+}
+
+// Rebuilds the staged kernel's halos for tiles of bn nodes (128, 256 or
+// 512): halo lists and halo-local entries (LDS byte offsets), the FP32
+// block-relative origins, the LDS budget.  NLM_ERR_INVALID (state unchanged)
+// when the largest halo does not fit the 64 KiB budget.
+int retile(nlm_ctx* c, int bn) {
+  FdState& s = c->fd;
+  if (bn == s.tile) return NLM_OK;
+  const int64_t n = s.n, ns = s.sell.nslices, nent = s.sell.nentries;
+  std::vector<int64_t> slp(ns + 1);
+  std::vector<int32_t> colS(nent);
+  NLM_HIP(c, hipStreamSynchronize(c->stream));
+  NLM_HIP(c, hipMemcpy(slp.data(), s.sell.sl_ptr.p, sizeof(int64_t) * (ns + 1),
+                       hipMemcpyDeviceToHost));
+  NLM_HIP(c, hipMemcpy(colS.data(), s.sell.col.p, sizeof(int32_t) * nent,
+                       hipMemcpyDeviceToHost));
+  std::vector<int64_t> hp;
+  std::vector<int32_t> halo;
+  std::vector<uint16_t> colL;
+  int64_t hmax = 0;
+  build_halos(n, ns, slp.data(), colS.data(), bn / 64, hp, halo, colL, hmax);
+  if ((hmax + 1) * int64_t(sizeof(HaloRec)) > 64 * 1024)
+    return fail(c, NLM_ERR_INVALID,
+                "tile of " + std::to_string(bn) +
+                    " nodes: its largest halo exceeds the LDS budget");
+  const uint32_t rec = s.precision == NLM_PREC_FP32_FORCE
+                           ? uint32_t(sizeof(HaloRec32))
+                           : uint32_t(sizeof(HaloRec));
+  for (auto& v : colL) v = uint16_t(uint32_t(v) * rec);
+  // origins: the centre of each block's bounding box (2D positions)
+  const int64_t nb = int64_t(hp.size()) - 1;
+  std::vector<double> hx(size_t(n) * 2);
+  NLM_HIP(c, hipMemcpy(hx.data(), s.x.p, sizeof(double) * 2 * size_t(n),
+                       hipMemcpyDeviceToHost));
+  std::vector<double> borg(size_t(nb) * 2, 0.);
+  for (int64_t b = 0; b < nb; b++) {
+    const int64_t a0 = int64_t(bn) * b, a1 = std::min<int64_t>(n, a0 + bn);
+    if (a0 >= n) break;
+    double lo[2] = {hx[2 * a0], hx[2 * a0 + 1]}, hi[2] = {lo[0], lo[1]};
+    for (int64_t a = a0; a < a1; a++)
+      for (int d = 0; d < 2; d++) {
+        lo[d] = std::min(lo[d], hx[2 * a + d]);
+        hi[d] = std::max(hi[d], hx[2 * a + d]);
+      }
+    for (int d = 0; d < 2; d++) borg[2 * b + d] = 0.5 * (lo[d] + hi[d]);
+  }
+  // a captured step graph references the buffers replaced below
+  if (s.graph) {
+    (void)hipGraphExecDestroy(s.graph);
+    s.graph = nullptr;
+  }
+  s.graph_chunk = 0;
+  int rc;
+  auto up = [&](DevBuf& d, const void* src, size_t bytes) -> int {
+    if (!d.p || d.bytes < bytes + 256)
+      if (int r = alloc(c, d, bytes + 256)) return r;
+    NLM_HIP(c, hipMemcpy(d.p, src, bytes, hipMemcpyHostToDevice));
+    return NLM_OK;
+  };
+  if ((rc = up(s.halo_ptr, hp.data(), sizeof(int64_t) * hp.size())) ||
+      (rc = up(s.halo, halo.data(), sizeof(int32_t) * halo.size())) ||
+      (rc = up(s.colL, colL.data(), sizeof(uint16_t) * nent)) ||
+      (rc = up(s.blk_org, borg.data(), sizeof(double) * borg.size())))
+    return rc;
+  s.halo_max = hmax;
+  s.tile = bn;
   return NLM_OK;
 }
 }  // namespace
@@ -2172,6 +2280,64 @@ extern "C" int nlm_fd_autotune(nlm_ctx* c, int* chosen, double* best_us) {
     return es_autotune(c, chosen, best_us);
   return c->fd.dim == 2 ? autotune<2>(c, chosen, best_us)
                         : autotune<3>(c, chosen, best_us);
+}
+
+extern "C" int nlm_fd_autotune_tiles(nlm_ctx* c, int* tile, int* chunk,
+                                     double* us) {
+  int rc = check_ready(c, "nlm_fd_autotune_tiles");
+  if (rc) return rc;
+  FdState& s = c->fd;
+  s.graph_chunk = 0;  // a captured step graph is stale
+  const int tiles[3] = {128, 256, 512};
+  double tab[9] = {0., 0., 0., 0., 0., 0., 0., 0., 0.};
+  if (!(s.tiled && s.use_tiled) || s.dim != 2 ||
+      s.material == NLM_MATERIAL_ELASTIC_STATE) {
+    // no staged kernel: the chunk width alone
+    int pick = 4;
+    double best = 0.;
+    if ((rc = nlm_fd_autotune(c, &pick, &best))) return rc;
+    tab[3 + (pick == 2 ? 0 : pick == 4 ? 1 : 2)] = best;
+    if (tile) *tile = s.tile;
+    if (chunk) *chunk = pick;
+    if (us) std::memcpy(us, tab, sizeof(tab));
+    return NLM_OK;
+  }
+  double best = 1e300;
+  int bt = s.tile, bu = s.unroll;
+  for (int k = 0; k < 3; k++) {
+    if ((rc = retile(c, tiles[k]))) {
+      if (rc != NLM_ERR_INVALID) return rc;
+      c->err.clear();  // over the LDS budget: not a candidate
+      continue;
+    }
+    if ((rc = time_chunks<2>(c, tab + 3 * k))) return rc;
+    double b;
+    const int u = best_chunk(tab + 3 * k, &b);
+    if (b < best) {
+      best = b;
+      bt = tiles[k];
+      bu = u;
+    }
+  }
+  if ((rc = retile(c, bt))) return rc;
+  s.unroll = bu;
+  if (tile) *tile = bt;
+  if (chunk) *chunk = bu;
+  if (us) std::memcpy(us, tab, sizeof(tab));
+  return NLM_OK;
+}
+
+extern "C" int nlm_fd_set_tile(nlm_ctx* c, int tile) {
+  int rc = check_ready(c, "nlm_fd_set_tile");
+  if (rc) return rc;
+  if (tile != 128 && tile != 256 && tile != 512)
+    return fail(c, NLM_ERR_INVALID, "nlm_fd_set_tile: 128, 256 or 512");
+  if (!c->fd.tiled || c->fd.dim != 2 ||
+      c->fd.material == NLM_MATERIAL_ELASTIC_STATE)
+    return fail(c, NLM_ERR_STATE,
+                "nlm_fd_set_tile: the model has no LDS-staged step kernel");
+  c->fd.graph_chunk = 0;
+  return retile(c, tile);
 }
 
 extern "C" int nlm_fd_set_unroll(nlm_ctx* c, int unroll) {

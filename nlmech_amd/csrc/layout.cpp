@@ -11,10 +11,11 @@
 // wavefront).  Slot k of lane l of slice s is entry sl_ptr[s] + 64 k + l;
 // slots past a row's length repeat the node itself.
 //
-// Halo.  For each block of 256 internal nodes the sorted set of internal ids
-// its rows reference (plus its own nodes) is the block's halo; the step kernel
-// stages the halo's positions, displacements and volumes in LDS and reads
-// neighbors through 16-bit halo-local indices (colL).
+// Halo.  For each block of 256 internal nodes (128 or 512 when the staged
+// kernel's tile is tuned) the sorted set of internal ids its rows reference
+// (plus its own nodes) is the block's halo; the step kernel stages the halo's
+// positions, displacements and volumes in LDS and reads neighbors through
+// 16-bit halo-local indices (colL).
 
 #include <algorithm>
 #include <cmath>
@@ -143,37 +144,46 @@ void build_layout(int dim, int64_t n, const double* x, double cell,
   });
 
   // --- halos of 256-node blocks ----------------------------------------------
-  const int64_t nblocks = (nslices + 3) / 4;
-  L.nblocks = nblocks;
-  L.halo_ptr.assign(nblocks + 1, 0);
-  L.colL.resize(nent);
+  L.nblocks = (nslices + 3) / 4;
+  build_halos(n, nslices, L.sl_ptr.data(), L.colS.data(), 4, L.halo_ptr,
+              L.halo, L.colL, L.halo_max);
+}
+
+void build_halos(int64_t n, int64_t nslices, const int64_t* sl_ptr,
+                 const int32_t* colS, int bs, std::vector<int64_t>& halo_ptr,
+                 std::vector<int32_t>& halo, std::vector<uint16_t>& colL,
+                 int64_t& halo_max) {
+  const int64_t nblocks = (nslices + bs - 1) / bs;
+  const int64_t nent = sl_ptr[nslices];
+  halo_ptr.assign(nblocks + 1, 0);
+  colL.resize(nent);
   std::vector<std::vector<int32_t>> hb(nblocks);
   parallel_chunks(nblocks, [&](int64_t b0, int64_t b1) {
     std::vector<int32_t> h;
     for (int64_t b = b0; b < b1; b++) {
-      int64_t s0 = 4 * b, s1 = std::min(nslices, s0 + 4);
-      int64_t e0 = L.sl_ptr[s0], e1 = L.sl_ptr[s1];
-      h.assign(L.colS.begin() + e0, L.colS.begin() + e1);
+      int64_t s0 = int64_t(bs) * b, s1 = std::min(nslices, s0 + bs);
+      int64_t e0 = sl_ptr[s0], e1 = sl_ptr[s1];
+      h.assign(colS + e0, colS + e1);
       for (int64_t a = s0 * 64; a < std::min(n, s1 * 64); a++)
         h.push_back(int32_t(a));
       std::sort(h.begin(), h.end());
       h.erase(std::unique(h.begin(), h.end()), h.end());
       for (int64_t e = e0; e < e1; e++) {
-        auto it = std::lower_bound(h.begin(), h.end(), L.colS[e]);
-        L.colL[e] = uint16_t(std::min<int64_t>(it - h.begin(), 65535));
+        auto it = std::lower_bound(h.begin(), h.end(), colS[e]);
+        colL[e] = uint16_t(std::min<int64_t>(it - h.begin(), 65535));
       }
       hb[b] = h;
     }
   });
-  L.halo_max = 0;
+  halo_max = 0;
   for (int64_t b = 0; b < nblocks; b++) {
-    L.halo_max = std::max<int64_t>(L.halo_max, int64_t(hb[b].size()));
-    L.halo_ptr[b + 1] = L.halo_ptr[b] + int64_t(hb[b].size());
+    halo_max = std::max<int64_t>(halo_max, int64_t(hb[b].size()));
+    halo_ptr[b + 1] = halo_ptr[b] + int64_t(hb[b].size());
   }
-  L.halo.resize(L.halo_ptr[nblocks]);
+  halo.resize(halo_ptr[nblocks]);
   parallel_chunks(nblocks, [&](int64_t b0, int64_t b1) {
     for (int64_t b = b0; b < b1; b++)
-      std::copy(hb[b].begin(), hb[b].end(), L.halo.begin() + L.halo_ptr[b]);
+      std::copy(hb[b].begin(), hb[b].end(), halo.begin() + halo_ptr[b]);
   });
 }
 

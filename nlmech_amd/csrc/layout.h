@@ -30,4 +30,13 @@ void build_layout(int dim, int64_t n, const double* x, double cell,
                   const int64_t* rp_ref, const int32_t* col_ref,
                   HostLayout& L);
 
+// Halos of blocks of bs consecutive slices (64 bs nodes, one workgroup of
+// the staged step kernel): each block's sorted internal ids referenced by its
+// rows plus its own nodes (halo_ptr / halo), the halo-local index of every
+// slice entry (colL, clamped to 65535) and the largest halo.
+void build_halos(int64_t n, int64_t nslices, const int64_t* sl_ptr,
+                 const int32_t* colS, int bs, std::vector<int64_t>& halo_ptr,
+                 std::vector<int32_t>& halo, std::vector<uint16_t>& colL,
+                 int64_t& halo_max);
+
 }  // namespace nlm

@@ -175,6 +175,8 @@ struct FdState {
   DevBuf u[2], v[2], f;
   int cur = 0;  // which of u[]/v[] holds the current state
   int unroll = 4;  // neighbor slots per chunk in the fused step kernel
+  int tile = 256;  // nodes per workgroup of the LDS-staged kernel (its halo
+                   // block): 128, 256 or 512
   DevBuf row_ptr, col;  // CSR in internal ids (int64, int32)
   SellDev sell;
   // internal node order (see layout.cpp): perm[a] = reference id of a

@@ -233,6 +233,19 @@ class GpuFD:
     def set_unroll(self, unroll: int):
         self._call("nlm_fd_set_unroll", int(unroll))
 
+    def autotune_tiles(self):
+        """(tile, chunk, table): the chosen staged-kernel tile and chunk
+        width and the 3 x 3 table of microseconds per step (rows: tiles 128,
+        256, 512; columns: chunks 2, 4, 8; 0 = not run)."""
+        t, u = C.c_int(), C.c_int()
+        us = np.zeros(9)
+        self._call("nlm_fd_autotune_tiles", C.byref(t), C.byref(u),
+                   ptr(us, C.c_double))
+        return t.value, u.value, us.reshape(3, 3)
+
+    def set_tile(self, tile: int):
+        self._call("nlm_fd_set_tile", int(tile))
+
     def timed_steps(self, nsteps: int):
         tot = C.c_double()
         per = C.c_double()

@@ -9,7 +9,10 @@ Bars:
   - a plate stepped inside the ensemble equals the plate stepped alone, bit
     for bit (rows keep the reference order, so per-node sums are the same);
   - graph-captured stepping equals direct stepping bit for bit, including a
-    remainder of steps outside whole graph chunks.
+    remainder of steps outside whole graph chunks, also on the full
+    256-plate workload with the jointly tuned tile and chunk width, where
+    the pulled rows follow their rates exactly, no bond re-forms and each
+    plate's internal forces sum to zero to rounding.
 """
 import numpy as np
 import pytest
@@ -85,3 +88,40 @@ def test_graph_steps_equal_direct_steps(mode):
         for x, y in zip(a[:4], b[:4]):
             assert np.array_equal(x, y)
         assert a[4] == b[4]
+
+
+def test_ensemble256_graph_equals_direct_tuned():
+    """ensemble256 (256 plates, 15.8M nodes, 376M bonds), FP32-force, tile
+    and chunk width from the joint autotuner: 9 steps replayed from graphs of
+    4 (plus a remainder of 1) equal 9 direct steps bit for bit."""
+    from nlmech_amd.engine import GpuFD
+    progress("[gpu] ensemble256 graph vs direct, tuned")
+    p = pr.workload("ensemble256")
+    steps = 9
+    with GpuFD(p, precision=NLM_PREC_FP32_FORCE) as g:
+        g.init()
+        b0 = g.get_bonds()
+        tile, unroll, tab = g.autotune_tiles()
+        progress(f"[gpu]   tuned: tile {tile} chunk {unroll}\n{tab}")
+        g.step(steps)
+        a = g.get_state() + (g.get_bonds(), g.time())
+    with GpuFD(p, precision=NLM_PREC_FP32_FORCE) as g:
+        g.set_tile(tile)
+        g.set_unroll(unroll)
+        g.init()
+        g.step_graph(steps, 4)
+        b = g.get_state() + (g.get_bonds(), g.time())
+    for x, y in zip(a[:4], b[:4]):
+        assert np.array_equal(x, y)
+    assert a[4] == b[4]
+    u, v, f, bits, (t, n) = a
+    assert n == steps and np.isfinite(u).all() and np.isfinite(v).all()
+    assert not np.any(bits & ~b0)  # no bond re-forms
+    # pulled rows: u_y = +/- pull t exactly (node-list BCs, linear in time)
+    ids, rates = p.node_bc[0], np.asarray(p.node_bc[2]).reshape(-1, 3)
+    assert np.allclose(u[ids, 1], rates[:, 1] * t, rtol=1e-12, atol=0.0)
+    # per plate: sum_i V_i f_i = 0 to the FP32-force rounding
+    for sp in p.specimens[::37]:
+        sl = slice(sp["offset"], sp["offset"] + sp["side"] ** 2)
+        vf = p.vol[sl, None] * f[sl]
+        assert np.all(np.abs(vf.sum(0)) <= 1e-6 * np.abs(vf).sum()), sp

@@ -369,6 +369,47 @@ def test_autotune_and_unroll_do_not_change_results(engine, mode):
             assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("mode", [NLM_PREC_FP64_EXACT, NLM_PREC_FP64,
+                                  NLM_PREC_FP32_FORCE])
+def test_tiles_and_joint_autotune(engine, mode):
+    """The LDS-staged kernel's tile (128, 256 or 512 nodes per workgroup and
+    halo block) changes the data movement only: in the FP64 modes every tile
+    gives the same bits; in the FP32-force mode the block-relative FP32
+    positions take their origin from the tile, so displacements agree to
+    1e-5 (relative L2) and the broken-bond sets to 1e-3 of the broken bonds.
+    The joint tile x chunk autotuner leaves the state untouched and times
+    every pair the mode runs."""
+    progress(f"[gpu] tiles mode={mode}")
+    p = pr.plate2d(160, 120, 1e-3, 4, 1e-7, 1, pull=30.0)
+    outs = {}
+    for tile in (256, 128, 512, "auto"):
+        with engine.GpuFD(p, precision=mode) as g:
+            g.init()
+            if tile == "auto":
+                t, u, tab = g.autotune_tiles()
+                progress(f"[gpu]   autotune: tile {t} chunk {u}\n{tab}")
+                assert t in (128, 256, 512) and u in (2, 4, 8)
+                cols = [1] if mode == NLM_PREC_FP64_EXACT else [0, 1, 2]
+                assert np.all(tab[:, cols] > 0)
+                tile = t
+            else:
+                g.set_tile(tile)
+            g.step(40)
+            u, v, f = g.get_state()
+            outs.setdefault(tile, []).append((u, v, f, g.get_bonds()))
+    ref = outs[256][0]
+    broken = int((~ref[3]).sum())
+    assert broken > 0
+    for tile, runs in outs.items():
+        for o in runs:
+            if mode != NLM_PREC_FP32_FORCE:
+                for a, b in zip(ref, o):
+                    assert np.array_equal(a, b), tile
+            else:
+                assert rel_l2(o[0], ref[0]) <= 1e-5, tile
+                assert int((o[3] != ref[3]).sum()) <= 1e-3 * broken + 2, tile
+
+
 def fp32_underflow_floor(o, p, csr):
     """Per-node force that the FP32-force mode may lose to underflow: its
     bond arithmetic (du, dx.du, S) runs in FP32, whose smallest normal
