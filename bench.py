@@ -73,6 +73,9 @@ def parse():
     ap.add_argument("--qs-workload", default="qs_plate300")
     ap.add_argument("--qs-steps", type=int, default=20)
     ap.add_argument("--no-qs", action="store_true")
+    ap.add_argument("--qs-no-network", action="store_true",
+                    help="quasi-static section without the network: point "
+                         "Jacobi one and two level only (for profiling)")
     ap.add_argument("--ens-steps", type=int, default=1000)
     ap.add_argument("--ens-alone-steps", type=int, default=200)
     ap.add_argument("--no-ens", action="store_true")
@@ -235,6 +238,17 @@ def quasi_static(a):
     from nlmech_amd.capi import (NLM_PREC_FP64_EXACT, NLM_QS_PRECOND_BLOCKS,
                                  NLM_QS_PRECOND_JACOBI)
     from nlmech_amd.engine import GpuFD
+    if a.qs_no_network:
+        p = pr.workload(a.qs_workload, nsteps=a.qs_steps)
+        out = {"workload": a.qs_workload, "nodes": p.n,
+               "load_steps": a.qs_steps}
+        for name, pre in (("jacobi", NLM_QS_PRECOND_JACOBI),
+                          ("jacobi_two_level", NLM_QS_PRECOND_JACOBI | 16)):
+            g = GpuFD(p, precision=NLM_PREC_FP64_EXACT)
+            out[name] = g.qs_solve(a.qs_steps, tol_rel=1e-8, precond=pre,
+                                   lin_tol=1e-8, lin_max_iter=50000)
+            g.close()
+        return out
     t0 = time.perf_counter()
     net = gnn_train.train(nx=48, specimens=100, load_steps=5, epochs=6,
                           seed=1, log=lambda m: print(m, file=sys.stderr,
