@@ -275,6 +275,10 @@ __global__ __launch_bounds__(256) void k_es_dil(
     else return uint32_t(ld32(colS, en));
   };
   double th = 0.;
+  // fast modes: the slot-pair sums accumulate without the node's factor c3
+  // (times q^2 in units of q), which multiplies their total once at the end
+  double thf = 0.;
+  const double sc3 = R32 ? c3 * e.m.q * e.m.q : c3;
   uint32_t jn[U];
 #pragma unroll
   for (int q = 0; q < U; ++q) jn[q] = slot_index(q);
@@ -355,9 +359,7 @@ __global__ __launch_bounds__(256) void k_es_dil(
       const Real rch = R32   ? Real(e.m.rcp_h_qf)
                        : F32 ? Real(e.m.rcp_h_f)
                              : Real(e.m.rcp_h);
-      const Real c3r = R32   ? Real(float(c3 * e.m.q * e.m.q))
-                       : F32 ? Real(c3f)
-                             : Real(c3);
+      const Real uprch = Real(e.m.up_rcp_h_qf);
 #pragma unroll
       for (int q = 0; q < U; ++q) {
         const unsigned long long act = __builtin_amdgcn_ballot_w64(k0 + q < len);
@@ -400,8 +402,15 @@ __global__ __launch_bounds__(256) void k_es_dil(
         band[q] = cand & v_lo & ~v_hi;
         ex[q] = (v_vex & act) | band[q];
         const unsigned long long keep = act & ~(w[q] | nb[q]) & ~ex[q];
-        const Real c = fmin((cup - lx) * rch, Real(1));
-        Real dterm = c3r * lx * ext * c * Real(vj[q]);
+        // volume-correction factor (check_up - |X|)/h clamped to [0, 1]
+        // (in FP32 one multiply-add with the clamp as its output modifier)
+        Real c;
+        if constexpr (F32)
+          c = __builtin_amdgcn_fmed3f(__builtin_fmaf(lx, -rch, uprch), 0.f,
+                                      1.f);
+        else
+          c = fmin((cup - lx) * rch, Real(1));
+        Real dterm = lx * ext * c * Real(vj[q]);
         dterm = __builtin_amdgcn_inverse_ballot_w64(keep) ? dterm : Real(0);
         if (q % 2 == 0) pf[q / 2] = dterm;
         else pf[q / 2] += dterm;
@@ -431,14 +440,14 @@ __global__ __launch_bounds__(256) void k_es_dil(
             }
             const double d = es_dil_exact(e.m, xi, ui, xs, us, vs, c3, fs);
             fsn = fs && !fs0;
-            if (!fs) pf[q / 2] += Real(d);
+            if (!fs) pf[q / 2] += Real(d / sc3);
           }
           // the guard band's bonds were decided exactly here
           nb[q] |= band[q] & __builtin_amdgcn_ballot_w64(fsn);
         }
       }
 #pragma unroll
-      for (int mm = 0; mm < (U + 1) / 2; ++mm) th += double(pf[mm]);
+      for (int mm = 0; mm < (U + 1) / 2; ++mm) thf += double(pf[mm]);
       unsigned long long nb_any = 0ull;
 #pragma unroll
       for (int q = 0; q < U; ++q) nb_any |= nb[q];
@@ -482,6 +491,7 @@ __global__ __launch_bounds__(256) void k_es_dil(
       }
     }
   }
+  th += sc3 * thf;
   if (!valid) return;
   e.theta[i] = th;
   if constexpr (DIM == 3) {
@@ -722,11 +732,13 @@ __global__ __launch_bounds__(256) void k_es_force(
         const float lx = dd * rx;
         const float lowb = R32 ? e.m.low_band_qf : e.m.low_band_f;
         const float vtol = R32 ? e.m.vex_tol_qf : 3e-5f * e.m.low_band_f;
-        const float cup = R32 ? e.m.check_up_qf : e.m.check_up_f;
         const float rch = R32 ? e.m.rcp_h_qf : e.m.rcp_h_f;
         const unsigned long long vexm =
             __builtin_amdgcn_ballot_w64(fabsf(lx - lowb) <= vtol) & live;
-        float c = fminf((cup - lx) * rch, 1.0f);
+        // (check_up - |X|)/h clamped to [0, 1]: one multiply-add with the
+        // clamp as its output modifier
+        float c = __builtin_amdgcn_fmed3f(
+            __builtin_fmaf(lx, -rch, e.m.up_rcp_h_qf), 0.f, 1.f);
         if (vexm != 0ull) {  // rare: lengths at the correction's edge
           if (__builtin_amdgcn_inverse_ballot_w64(vexm)) {
             double xs[3];

@@ -1532,6 +1532,7 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
     e.check_up_qf = float(e.check_up * e.inv_q);
     e.low_band_qf = float(e.low_band * e.inv_q);
     e.rcp_h_qf = float(e.q / q.h);
+    e.up_rcp_h_qf = float(e.check_up / q.h);
     e.vex_tol_qf = 3e-5f * e.low_band_qf;
     e.brk_lo = e.breaking ? e.sc_break - kEsGuard64 * e.sc_break
                           : std::numeric_limits<double>::infinity();

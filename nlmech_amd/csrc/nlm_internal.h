@@ -128,7 +128,9 @@ struct EsDev {
   // then 1/2, so (2/m_i) f2d = 1/m_i exactly) and t = (E/m_i) e
   // (ElasticState.cpp:187-193), i.e. no dilatation term in the force
   int one_d;
-  int pad_es;
+  // check_up / h (FP32): the clamped volume-correction factor as one
+  // multiply-add, fma(|X|, -1/h, check_up/h), in either unit
+  float up_rcp_h_qf;
   double E;
 };
 
