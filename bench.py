@@ -134,12 +134,13 @@ def state_based(a):
         if csr is None:
             csr = g.neighbors()
         g.init()
-        unroll, _ = g.autotune()
+        tile, unroll, table = g.autotune_tiles()
         g.step(5)
         ms_total, ms_pair = g.timed_steps(steps)
         u, _, _ = g.get_state()
         res[name] = dict(ms_total=ms_total, ms_pair=ms_pair, u=u,
-                         bonds=g.get_bonds(), nnz=g.nnz, unroll=unroll)
+                         bonds=g.get_bonds(), nnz=g.nnz, unroll=unroll,
+                         tile=tile, table=table.tolist())
         g.close()
         print(f"[bench] state-based {name}: {ms_total / steps * 1e3:.1f} us "
               f"per step", file=sys.stderr, flush=True)
@@ -155,7 +156,9 @@ def state_based(a):
             "ms_per_step": r["ms_total"] / steps,
             "pass_pair_us": r["ms_pair"] * 1e3,
             "roofline_frac": ach / HBM_PEAK_GBS,
-            "autotuned_unroll": r["unroll"]}
+            "autotuned_unroll": r["unroll"],
+            "autotuned_tile": r["tile"],
+            "tune_table_us": r["table"]}
     if "exact" in res:
         uref, bref = res["exact"]["u"], res["exact"]["bonds"]
         out["accuracy_vs_fp64_exact"] = {

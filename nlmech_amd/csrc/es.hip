@@ -62,7 +62,7 @@ __device__ __forceinline__ void es_stage_halo(const FdArgs<3>& a,
                                               int64_t lb, EsRec* smem) {
   const int64_t hb = a.halo_ptr[lb];
   const int hn = int(a.halo_ptr[lb + 1] - hb);
-  for (int t = threadIdx.x; t < hn; t += 256) {
+  for (int t = threadIdx.x; t < hn; t += int(blockDim.x)) {
     const uint32_t g = uint32_t(a.halo[hb + t]);
     const double4 xx = ld32(a.x, g);
     const double4 uu = ld32(a.u, g);
@@ -109,7 +109,7 @@ __device__ __forceinline__ void es_stage_halo_split(
     VP* vp) {
   const int64_t hb = a.halo_ptr[lb];
   const int hn = int(a.halo_ptr[lb + 1] - hb);
-  for (int t = threadIdx.x; t < hn; t += 256) {
+  for (int t = threadIdx.x; t < hn; t += int(blockDim.x)) {
     const uint32_t g = uint32_t(a.halo[hb + t]);
     const double4 xx = ld32(a.x, g);
     const double4 uu = ld32(a.u, g);
@@ -158,7 +158,7 @@ __device__ __forceinline__ void es_stage_halo_r32(
     const double4& org, const EsDev& m, EsU64* su, EsXV32* sx, float* sp) {
   const int64_t hb = a.halo_ptr[lb];
   const int hn = int(a.halo_ptr[lb + 1] - hb);
-  for (int t = threadIdx.x; t < hn; t += 256) {
+  for (int t = threadIdx.x; t < hn; t += int(blockDim.x)) {
     const uint32_t g = uint32_t(a.halo[hb + t]);
     const double4 xx = ld32(a.x, g);
     const double4 uu = ld32(a.u, g);
@@ -208,11 +208,14 @@ __global__ __launch_bounds__(256) void k_es_mw(FdArgs<DIM> a,
 // TILED (3D): the block's halo is staged in LDS and neighbors are read
 // through 16-bit halo-local indices.
 template <int DIM, int MODE, int U, bool TILED>
-__global__ __launch_bounds__(256) void k_es_dil(
+__global__ __launch_bounds__(TILED ? 512 : 256) void k_es_dil(
     EsArgs<DIM> e, const unsigned long long* __restrict__ bits_ro) {
   static_assert(!TILED || DIM == 3, "the LDS-staged passes are 3D");
   const FdArgs<DIM>& a = e.f;
-  const int64_t nblk = (a.nslices * 64 + 255) / 256;
+  // staged passes: the workgroup's nodes (its size: 128, 256 or 512, the
+  // tuned tile) are one block of the layout's halos
+  const int bn = TILED ? int(blockDim.x) : 256;
+  const int64_t nblk = (a.nslices * 64 + bn - 1) / bn;
   const int64_t lb = xcd_block(nblk);
   if (lb >= nblk) return;
   extern __shared__ EsRec es_smem[];
@@ -235,7 +238,8 @@ __global__ __launch_bounds__(256) void k_es_dil(
     es_stage_halo<false>(a, e.P, lb, es_smem);
   }
   const int lane = threadIdx.x & 63;
-  const int64_t s = lb * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t s =
+      lb * (bn >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (s >= a.nslices) return;  // whole wavefront
   const int64_t i = s * 64 + lane;
   const bool valid = i < a.n;
@@ -515,7 +519,7 @@ __global__ __launch_bounds__(256) void k_es_dil(
 // Pass 2: bond forces of both directions gathered at node i, force BCs,
 // central-difference update.
 template <int DIM, int MODE, int KIND, bool BASE_F, int U, bool TILED>
-__global__ __launch_bounds__(256) void k_es_force(
+__global__ __launch_bounds__(TILED ? 512 : 256) void k_es_force(
     EsArgs<DIM> e, const unsigned long long* __restrict__ bits_ro,
     double t_force, double t_next) {
   if (e.f.tdev) {
@@ -525,7 +529,8 @@ __global__ __launch_bounds__(256) void k_es_force(
   static_assert(!TILED || (DIM == 3 && MODE != kModeExact),
                 "the LDS-staged force pass is 3D, fast modes");
   const FdArgs<DIM>& a = e.f;
-  const int64_t nblk = (a.nslices * 64 + 255) / 256;
+  const int bn = TILED ? int(blockDim.x) : 256;
+  const int64_t nblk = (a.nslices * 64 + bn - 1) / bn;
   const int64_t lb = xcd_block(nblk);
   if (lb >= nblk) return;
   extern __shared__ EsRec es_smem[];
@@ -550,7 +555,8 @@ __global__ __launch_bounds__(256) void k_es_force(
     es_stage_halo<true>(a, e.P, lb, es_smem);
   }
   const int lane = threadIdx.x & 63;
-  const int64_t s = lb * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t s =
+      lb * (bn >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (s >= a.nslices) return;
   const int64_t i = s * 64 + lane;
   const bool valid = i < a.n;
@@ -802,12 +808,14 @@ EsArgs<DIM> make_es_args(nlm_ctx* c) {
 
 // LDS bytes of a tiled launch: one record per halo entry (the fast passes:
 // the two split arrays; the exact pass 1 keeps one record).
-size_t es_lds_bytes(const FdState& s) {
-  const size_t fast = s.precision == kModeFp32
+size_t es_record_bytes(int precision) {
+  const size_t fast = precision == kModeFp32
                           ? sizeof(EsU64) + sizeof(EsXV32) + sizeof(float)
                           : sizeof(EsXU) + sizeof(EsVPOf<kModeFp64>);
-  return size_t(s.halo_max + 1) *
-         (s.precision == kModeExact ? sizeof(EsRec) : fast);
+  return precision == kModeExact ? sizeof(EsRec) : fast;
+}
+size_t es_lds_bytes(const FdState& s) {
+  return size_t(s.halo_max + 1) * es_record_bytes(s.precision);
 }
 
 // Raises a kernel's dynamic-LDS limit to what the launch needs.  The grant
@@ -833,10 +841,13 @@ void launch_dil(nlm_ctx* c, const EsArgs<DIM>& e) {
   unsigned g = grid_xcd(force_blocks(c->fd) * 256, 256);
   if constexpr (DIM == 3) {
     if (c->fd.es_tiled && c->fd.use_tiled) {
+      // one workgroup per tile of the layout's halos
+      const int bn = c->fd.tile;
+      const unsigned gt = grid_xcd(c->fd.sell.nslices * 64, bn);
       const size_t lds = es_lds_bytes(c->fd);
       allow_lds(reinterpret_cast<const void*>(k_es_dil<DIM, MODE, 4, true>),
                 lds);
-      hipLaunchKernelGGL((k_es_dil<DIM, MODE, 4, true>), dim3(g), dim3(256),
+      hipLaunchKernelGGL((k_es_dil<DIM, MODE, 4, true>), dim3(gt), dim3(bn),
                          lds, c->stream, e, e.f.bits);
       return;
     }
@@ -850,12 +861,14 @@ void launch_force_k(nlm_ctx* c, const EsArgs<DIM>& e, double tf, double tn) {
   unsigned g = grid_xcd(force_blocks(c->fd) * 256, 256);
   if constexpr (DIM == 3 && MODE != kModeExact) {
     if (c->fd.es_tiled && c->fd.use_tiled) {
+      const int bn = c->fd.tile;
+      const unsigned gt = grid_xcd(c->fd.sell.nslices * 64, bn);
       const size_t lds = es_lds_bytes(c->fd);
       allow_lds(reinterpret_cast<const void*>(
                     k_es_force<DIM, MODE, KIND, BASE_F, U, true>),
                 lds);
       hipLaunchKernelGGL((k_es_force<DIM, MODE, KIND, BASE_F, U, true>),
-                         dim3(g), dim3(256), lds, c->stream, e, e.f.bits, tf,
+                         dim3(gt), dim3(bn), lds, c->stream, e, e.f.bits, tf,
                          tn);
       return;
     }
@@ -944,6 +957,10 @@ int es_prepare(nlm_ctx* c) {
   return NLM_OK;
 }
 
+size_t es_halo_lds_bytes(int precision, int64_t halo_max) {
+  return size_t(halo_max + 1) * es_record_bytes(precision);
+}
+
 void es_launch_force(nlm_ctx* c, bool base_f, double t) {
   launch_any(c, KIND_FORCE, base_f, t, t, 4);
 }
@@ -960,7 +977,7 @@ void es_launch_fused_dev(nlm_ctx* c, const double* tdev) {
 // current state; the trials write the spare u/v buffers, the node
 // coefficient arrays (rewritten by the next pass 1) and a scratch copy of the
 // bond bits, so the model state is unchanged.
-int es_autotune(nlm_ctx* c, int* chosen, double* best_us) {
+int es_autotune(nlm_ctx* c, int* chosen, double* best_us, double* us3) {
   FdState& s = c->fd;
   DevBuf bits_copy;
   int rc;
@@ -980,7 +997,8 @@ int es_autotune(nlm_ctx* c, int* chosen, double* best_us) {
   NLM_HIP(c, hipEventCreate(&e0));
   NLM_HIP(c, hipEventCreate(&e1));
   const int reps = 4;
-  for (int U : cands) {
+  for (int k = 0; k < 3; k++) {
+    const int U = cands[k];
     launch_any(c, KIND_CD, false, s.time, s.time + s.dt, U);
     NLM_HIP(c, hipEventRecord(e0, c->stream));
     for (int r = 0; r < reps; r++)
@@ -990,6 +1008,7 @@ int es_autotune(nlm_ctx* c, int* chosen, double* best_us) {
     float ms = 0.f;
     NLM_HIP(c, hipEventElapsedTime(&ms, e0, e1));
     const double us = 1e3 * double(ms) / reps;
+    if (us3) us3[k] = us;
     if (us < best) {
       best = us;
       pick = U;

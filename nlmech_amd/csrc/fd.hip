@@ -72,6 +72,24 @@ int scratch_setup(int device, void** out, std::string& err) {
   return NLM_OK;
 }
 
+// Quantum of the state-based FP32-force passes' fixed-point positions and
+// the lengths in its units: a power of two with every offset of a block's
+// node or halo node from the block's origin (at most org_range) below 2^29 q,
+// so the difference of two (at most 2^30 q) is an exact int32 and the
+// quantization (q/2) is ~1e-9 of the range.
+void es_set_quantum(EsDev& e, double org_range) {
+  const double rng = std::max(org_range, e.h);
+  e.q = std::ldexp(1.0, std::ilogb(rng) + 1 - 29);
+  e.inv_q = 1.0 / e.q;
+  e.q_f = float(e.q);
+  e.q2_f = float(e.q * e.q);
+  e.check_up_qf = float(e.check_up * e.inv_q);
+  e.low_band_qf = float(e.low_band * e.inv_q);
+  e.rcp_h_qf = float(e.q / e.h);
+  e.up_rcp_h_qf = float(e.check_up / e.h);
+  e.vex_tol_qf = 3e-5f * e.low_band_qf;
+}
+
 namespace {
 
 template <int DIM, int MODE, int INF>
@@ -1520,20 +1538,7 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
     // exact-evaluation band of both fast modes covers that whole window.
     e.vex_tol_f = 3e-5f * e.low_band_f;
     e.vex_tol = 3e-5 * e.low_band;
-    // quantum of the fixed-point positions: a power of two with every
-    // offset from a block's origin below 2^29 q, so the difference of two
-    // (at most 2^30 q) is an exact int32 and the quantization (q/2) is ~1e-9
-    // of the range
-    const double rng = std::max(org_range, q.h);
-    e.q = std::ldexp(1.0, std::ilogb(rng) + 1 - 29);
-    e.inv_q = 1.0 / e.q;
-    e.q_f = float(e.q);
-    e.q2_f = float(e.q * e.q);
-    e.check_up_qf = float(e.check_up * e.inv_q);
-    e.low_band_qf = float(e.low_band * e.inv_q);
-    e.rcp_h_qf = float(e.q / q.h);
-    e.up_rcp_h_qf = float(e.check_up / q.h);
-    e.vex_tol_qf = 3e-5f * e.low_band_qf;
+    es_set_quantum(e, org_range);
     e.brk_lo = e.breaking ? e.sc_break - kEsGuard64 * e.sc_break
                           : std::numeric_limits<double>::infinity();
     e.brk_hi = e.breaking ? e.sc_break + kEsGuard64 * e.sc_break
@@ -2198,13 +2203,16 @@ int autotune(nlm_ctx* c, int* chosen, double* best_us) {
   return NLM_OK;
 }
 
-// Rebuilds the staged kernel's halos for tiles of bn nodes (128, 256 or
-// 512): halo lists and halo-local entries (LDS byte offsets), the FP32
-// block-relative origins, the LDS budget.  NLM_ERR_INVALID (state unchanged)
-// when the largest halo does not fit the 64 KiB budget.
+// Rebuilds the staged kernels' halos for tiles of bn nodes (128, 256 or
+// 512): halo lists and halo-local entries (the 2D bond-based kernel's: LDS
+// byte offsets), the tile origins, the LDS budget and, for the state-based
+// FP32-force passes, the fixed-point quantum (the origins' range changes
+// with the tile).  NLM_ERR_INVALID (state unchanged) when the largest halo
+// does not fit the LDS budget.
 int retile(nlm_ctx* c, int bn) {
   FdState& s = c->fd;
   if (bn == s.tile) return NLM_OK;
+  const bool es = s.material == NLM_MATERIAL_ELASTIC_STATE;
   const int64_t n = s.n, ns = s.sell.nslices, nent = s.sell.nentries;
   std::vector<int64_t> slp(ns + 1);
   std::vector<int32_t> colS(nent);
@@ -2218,30 +2226,53 @@ int retile(nlm_ctx* c, int bn) {
   std::vector<uint16_t> colL;
   int64_t hmax = 0;
   build_halos(n, ns, slp.data(), colS.data(), bn / 64, hp, halo, colL, hmax);
-  if ((hmax + 1) * int64_t(sizeof(HaloRec)) > 64 * 1024)
-    return fail(c, NLM_ERR_INVALID,
-                "tile of " + std::to_string(bn) +
-                    " nodes: its largest halo exceeds the LDS budget");
-  const uint32_t rec = s.precision == NLM_PREC_FP32_FORCE
-                           ? uint32_t(sizeof(HaloRec32))
-                           : uint32_t(sizeof(HaloRec));
-  for (auto& v : colL) v = uint16_t(uint32_t(v) * rec);
-  // origins: the centre of each block's bounding box (2D positions)
+  if (!es) {
+    if ((hmax + 1) * int64_t(sizeof(HaloRec)) > 64 * 1024)
+      return fail(c, NLM_ERR_INVALID,
+                  "tile of " + std::to_string(bn) +
+                      " nodes: its largest halo exceeds the LDS budget");
+    const uint32_t rec = s.precision == NLM_PREC_FP32_FORCE
+                             ? uint32_t(sizeof(HaloRec32))
+                             : uint32_t(sizeof(HaloRec));
+    for (auto& v : colL) v = uint16_t(uint32_t(v) * rec);
+  } else {
+    int lds_max = 0;
+    NLM_HIP(c, hipDeviceGetAttribute(&lds_max,
+                                     hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                     c->device));
+    if (hmax > 65535 ||
+        es_halo_lds_bytes(s.precision, hmax) > size_t(lds_max))
+      return fail(c, NLM_ERR_INVALID,
+                  "tile of " + std::to_string(bn) +
+                      " nodes: its largest halo exceeds the LDS budget");
+  }
+  // origins: the centre of each block's bounding box
+  const int dim = s.dim == 3 ? 3 : 2;
+  const int stride = dim == 2 ? 2 : 4;
   const int64_t nb = int64_t(hp.size()) - 1;
-  std::vector<double> hx(size_t(n) * 2);
-  NLM_HIP(c, hipMemcpy(hx.data(), s.x.p, sizeof(double) * 2 * size_t(n),
+  std::vector<double> hx(size_t(n) * stride);
+  NLM_HIP(c, hipMemcpy(hx.data(), s.x.p, sizeof(double) * stride * size_t(n),
                        hipMemcpyDeviceToHost));
-  std::vector<double> borg(size_t(nb) * 2, 0.);
+  std::vector<double> borg(size_t(nb) * stride, 0.);
+  double org_range = 0.;
   for (int64_t b = 0; b < nb; b++) {
     const int64_t a0 = int64_t(bn) * b, a1 = std::min<int64_t>(n, a0 + bn);
     if (a0 >= n) break;
-    double lo[2] = {hx[2 * a0], hx[2 * a0 + 1]}, hi[2] = {lo[0], lo[1]};
+    double lo[3], hi[3];
+    for (int d = 0; d < dim; d++) lo[d] = hi[d] = hx[stride * a0 + d];
     for (int64_t a = a0; a < a1; a++)
-      for (int d = 0; d < 2; d++) {
-        lo[d] = std::min(lo[d], hx[2 * a + d]);
-        hi[d] = std::max(hi[d], hx[2 * a + d]);
+      for (int d = 0; d < dim; d++) {
+        lo[d] = std::min(lo[d], hx[stride * a + d]);
+        hi[d] = std::max(hi[d], hx[stride * a + d]);
       }
-    for (int d = 0; d < 2; d++) borg[2 * b + d] = 0.5 * (lo[d] + hi[d]);
+    for (int d = 0; d < dim; d++) borg[stride * b + d] = 0.5 * (lo[d] + hi[d]);
+    auto span = [&](int64_t a) {
+      for (int d = 0; d < dim; d++)
+        org_range = std::max(
+            org_range, std::fabs(hx[stride * a + d] - borg[stride * b + d]));
+    };
+    for (int64_t a = a0; a < a1; a++) span(a);
+    for (int64_t t = hp[b]; t < hp[b + 1]; t++) span(halo[size_t(t)]);
   }
   // a captured step graph references the buffers replaced below
   if (s.graph) {
@@ -2263,8 +2294,10 @@ int retile(nlm_ctx* c, int bn) {
     return rc;
   s.halo_max = hmax;
   s.tile = bn;
+  if (es) es_set_quantum(s.es, org_range);
   return NLM_OK;
 }
+
 }  // namespace
 
 extern "C" int nlm_fd_autotune(nlm_ctx* c, int* chosen, double* best_us) {
@@ -2291,8 +2324,10 @@ extern "C" int nlm_fd_autotune_tiles(nlm_ctx* c, int* tile, int* chunk,
   s.graph_chunk = 0;  // a captured step graph is stale
   const int tiles[3] = {128, 256, 512};
   double tab[9] = {0., 0., 0., 0., 0., 0., 0., 0., 0.};
-  if (!(s.tiled && s.use_tiled) || s.dim != 2 ||
-      s.material == NLM_MATERIAL_ELASTIC_STATE) {
+  const bool es = s.material == NLM_MATERIAL_ELASTIC_STATE;
+  const bool staged = es ? (s.es_tiled && s.use_tiled && s.dim == 3)
+                         : (s.tiled && s.use_tiled && s.dim == 2);
+  if (!staged) {
     // no staged kernel: the chunk width alone
     int pick = 4;
     double best = 0.;
@@ -2311,7 +2346,13 @@ extern "C" int nlm_fd_autotune_tiles(nlm_ctx* c, int* tile, int* chunk,
       c->err.clear();  // over the LDS budget: not a candidate
       continue;
     }
-    if ((rc = time_chunks<2>(c, tab + 3 * k))) return rc;
+    if (es) {
+      int pick = 4;
+      double bu = 0.;
+      if ((rc = es_autotune(c, &pick, &bu, tab + 3 * k))) return rc;
+    } else if ((rc = time_chunks<2>(c, tab + 3 * k))) {
+      return rc;
+    }
     double b;
     const int u = best_chunk(tab + 3 * k, &b);
     if (b < best) {
@@ -2333,8 +2374,9 @@ extern "C" int nlm_fd_set_tile(nlm_ctx* c, int tile) {
   if (rc) return rc;
   if (tile != 128 && tile != 256 && tile != 512)
     return fail(c, NLM_ERR_INVALID, "nlm_fd_set_tile: 128, 256 or 512");
-  if (!c->fd.tiled || c->fd.dim != 2 ||
-      c->fd.material == NLM_MATERIAL_ELASTIC_STATE)
+  const bool es = c->fd.material == NLM_MATERIAL_ELASTIC_STATE;
+  if (es ? !(c->fd.es_tiled && c->fd.dim == 3)
+         : !(c->fd.tiled && c->fd.dim == 2))
     return fail(c, NLM_ERR_STATE,
                 "nlm_fd_set_tile: the model has no LDS-staged step kernel");
   c->fd.graph_chunk = 0;

@@ -199,10 +199,15 @@ void fd_launch_force(nlm_ctx* c, bool base_f, double t);
 
 // State-based material (es.hip).
 int es_prepare(nlm_ctx* c);
+// LDS bytes of a staged state-based launch for a halo of halo_max nodes
+size_t es_halo_lds_bytes(int precision, int64_t halo_max);
 void es_launch_force(nlm_ctx* c, bool base_f, double t);
 void es_launch_fused(nlm_ctx* c, double tf, double tn);
 // fused pass pair reading the step times from tdev (graph capture)
 void es_launch_fused_dev(nlm_ctx* c, const double* tdev);
-int es_autotune(nlm_ctx* c, int* chosen, double* best_us);
+// us3 (optional): microseconds per step of the pass pair with force-pass
+// chunk width 2, 4, 8
+int es_autotune(nlm_ctx* c, int* chosen, double* best_us,
+                double* us3 = nullptr);
 
 }  // namespace nlm

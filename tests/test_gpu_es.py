@@ -240,6 +240,48 @@ def test_es_autotune_keeps_state(oracle):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("mode", [NLM_PREC_FP64_EXACT, NLM_PREC_FP64,
+                                  NLM_PREC_FP32_FORCE])
+def test_es_tiles_and_joint_autotune(mode):
+    """The staged passes' tile (128, 256 or 512 nodes per workgroup and halo
+    block): the FP64 modes give the same bits at every tile; the
+    FP32-force mode's fixed-point quantum follows the tiles' origin range,
+    so displacements agree to 1e-6 (relative L2) and the broken-bond sets to
+    1e-3 of the broken bonds.  The joint autotuner leaves the state
+    untouched and times every tile."""
+    progress(f"[gpu] es tiles mode={mode}")
+    from nlmech_amd.engine import GpuFD
+    p = small_cube(n=24, dt=5e-8, pull=20.0, jitter=0.05)
+    outs = {}
+    for tile in (256, 128, 512, "auto"):
+        with GpuFD(p, precision=mode) as g:
+            g.init()
+            g.step(3)
+            if tile == "auto":
+                t, u, tab = g.autotune_tiles()
+                progress(f"[gpu]   autotune: tile {t} chunk {u}\n{tab}")
+                assert t in (128, 256, 512) and u in (2, 4, 8)
+                assert np.all(tab > 0)
+                tile = t
+            else:
+                g.set_tile(tile)
+            g.step(25)
+            u, v, f = g.get_state()
+            outs.setdefault(tile, []).append((u, v, f, g.get_bonds()))
+    ref = outs[256][0]
+    broken = int((~ref[3]).sum())
+    assert broken > 0
+    for tile, runs in outs.items():
+        for o in runs:
+            if mode != NLM_PREC_FP32_FORCE:
+                for a, b in zip(ref, o):
+                    assert np.array_equal(a, b), tile
+            else:
+                err = np.linalg.norm(o[0] - ref[0]) / np.linalg.norm(ref[0])
+                assert err <= 1e-6, (tile, err)
+                assert int((o[3] != ref[3]).sum()) <= 1e-3 * broken + 2, tile
+
+
 def test_es_cube_full_size_properties():
     """160^3 block (4.1e6 nodes): Newton's third law of the internal force
     and the closed-form dilatation of a homogeneous stretch."""
