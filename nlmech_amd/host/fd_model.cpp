@@ -355,11 +355,16 @@ void FDModel<T>::initHObjects() {
   d_fix.assign(size_t(n), 0);
   check(nlm_fd_get_fixity(d_ctx, d_fix.data()), "nlm_fd_get_fixity");
   if (gpu->d_autotune) {
-    int chosen = 0;
-    double us = 0.;
-    check(nlm_fd_autotune(d_ctx, &chosen, &us), "nlm_fd_autotune");
-    std::cout << "FDModel: autotuner chose " << chosen << " slots per pass ("
-              << us << " us per step)\n";
+    // staged kernels: tile and chunk width together; others: the chunk
+    int tile = 0, chosen = 0;
+    double us[9] = {0., 0., 0., 0., 0., 0., 0., 0., 0.};
+    check(nlm_fd_autotune_tiles(d_ctx, &tile, &chosen, us),
+          "nlm_fd_autotune_tiles");
+    const int t = tile == 128 ? 0 : tile == 512 ? 2 : 1;
+    const int k = chosen == 2 ? 0 : chosen == 8 ? 2 : 1;
+    std::cout << "FDModel: autotuner chose tiles of " << tile << " nodes, "
+              << chosen << " slots per pass (" << us[3 * t + k]
+              << " us per step)\n";
   }
 }
 
