@@ -11,7 +11,8 @@ Bars:
   - restarting from an output file written with Bond_State continues
     exactly: the restarted run's final output equals the uninterrupted
     run's, bit for bit (displacement, velocity, force, bond bits);
-  - stepping through a captured HIP graph (GPU: Graph_Chunk) writes the same
+  - stepping through a captured HIP graph (GPU: Graph_Chunk) after the
+    joint autotuner (GPU: Autotune) writes the same
     output bit for bit;
   - a 3D state-based deck (ElasticState with Critical_Stretch, .csv mesh,
     cuboid displacement sets, explicit dynamics) matches the oracle: relative
@@ -222,8 +223,12 @@ def test_restart_with_bond_state_is_exact(host_bin, tmp_path):
 def test_graph_stepping_deck_is_bitwise_equal(host_bin, tmp_path):
     a = plate_dir(host_bin, tmp_path, "a")
     run_deck(host_bin, a)
-    g = plate_dir(host_bin, tmp_path, "g", extra="GPU:\n  Graph_Chunk: 10\n")
-    run_deck(host_bin, g)
+    # graph replays after the joint tile x chunk autotuner (the deck's
+    # precision, FP64 exact, gives the same bits at every tile)
+    g = plate_dir(host_bin, tmp_path, "g",
+                  extra="GPU:\n  Graph_Chunk: 10\n  Autotune: true\n")
+    out = run_deck(host_bin, g)
+    assert "autotuner chose tiles of" in out
     ra = read_vtu(str(a / "output_2.vtu"))
     rg = read_vtu(str(g / "output_2.vtu"))
     for k in ("Displacement", "Velocity", "Force", "Bond_State"):
