@@ -410,6 +410,30 @@ def test_tiles_and_joint_autotune(engine, mode):
                 assert int((o[3] != ref[3]).sum()) <= 1e-3 * broken + 2, tile
 
 
+def test_tiles_on_tiny_models(engine):
+    """Models smaller than one tile (a 6 x 6 plate: one slice of 36 nodes;
+    a 10 x 10 plate: two slices): every tile gives the FP64 result of the
+    default tile bit for bit, and the joint autotuner runs."""
+    progress("[gpu] tiles on tiny models")
+    for nx in (6, 10):
+        p = pr.plate2d(nx, nx, 1e-3, 3, 1e-7, 1, notch_len=0.0, pull=30.0,
+                       layers=1)
+        outs = []
+        for tile in (256, 128, 512, "auto"):
+            with engine.GpuFD(p, precision=NLM_PREC_FP64) as g:
+                g.init()
+                if tile == "auto":
+                    t, u, tab = g.autotune_tiles()
+                    assert t in (128, 256, 512) and u in (2, 4, 8)
+                else:
+                    g.set_tile(tile)
+                g.step(20)
+                outs.append(g.get_state() + (g.get_bonds(),))
+        for o in outs[1:]:
+            for a, b in zip(outs[0], o):
+                assert np.array_equal(a, b), nx
+
+
 def fp32_underflow_floor(o, p, csr):
     """Per-node force that the FP32-force mode may lose to underflow: its
     bond arithmetic (du, dx.du, S) runs in FP32, whose smallest normal
