@@ -117,8 +117,10 @@ struct alignas(16) HaloRec {
   float cv;  // FP32-force mode: vol * c4 (* J when constant), in FP32
 };
 
-// The FP32-force mode's record (32 bytes): the displacement in FP64 and the
-// position relative to the block's origin in FP32.  A bond's displacement
+// The FP32-force mode's record (40 bytes: a 10-dword stride puts record
+// starts on 32 of the 64 LDS banks where an 8-dword stride reaches 8): the
+// displacement in FP64 and the position relative to the block's origin in
+// FP32.  A bond's displacement
 // difference is formed in FP64 and rounded once to FP32, so its error is
 // relative to the difference itself however large the displacements are
 // (cancellation-free).  Positions within a block and its halo lie within
@@ -126,12 +128,23 @@ struct alignas(16) HaloRec {
 // carry FP32 rounding of that extent: a relative error of the bond's
 // position difference of order 1e-6, far inside the guard bands of the
 // break and volume-correction decisions (which the exact path re-takes).
-struct alignas(16) HaloRec32 {
-  double2 u;
+// 8-byte aligned pair of doubles (the record's FP64 displacement)
+struct alignas(8) Dbl2 {
+  double x, y;
+  __device__ Dbl2& operator=(const double2& v) {
+    x = v.x;
+    y = v.y;
+    return *this;
+  }
+};
+struct alignas(8) HaloRec32 {
+  Dbl2 u;
   float2 xr;
   float cv;  // vol c4 (J)
   uint32_t flags;
+  uint32_t pad[2];
 };
+static_assert(sizeof(HaloRec32) == 40, "FP32-force record");
 
 template <int MODE>
 using HaloRecOf =
