@@ -423,6 +423,60 @@ __device__ __forceinline__ void bond_fp32_core(
   F[2] = DIM == 3 ? sr * dx2 : 0.f;
 }
 
+// FP32-force common path of the LDS-staged kernel, on scaled displacement
+// differences: dv = tau du with tau^2 = beta log2(e) (the halo records and
+// the node hold tau u in FP64), so q = tau S r, Ss = tau S and
+// b = q Ss = beta log2(e) r S^2 is at once the exponent of the bond force,
+// exp(-beta r S^2) = exp2(-b), and the break test's quantity (its guard
+// band break_lo_s/break_hi_s scaled on the host).  cvs = vol_j c4 (J) / tau.
+// The volume-correction factor enters as fac / r =
+// clamp((check_up/h) / r - 1/h, 0, 1/r), whose upper bound holds exactly
+// when r <= check_low; the edge band is tested on r^2.  ACC: the force is
+// added to F with fused multiply-adds (F = sr dx + F), else written.
+template <int DIM, int INF, bool NOFAIL, bool ACC>
+__device__ __forceinline__ void bond_fp32_scaled(
+    const RnpDev& m, float dx0, float dx1, float dx2, float dv0, float dv1,
+    float dv2, float cvs, const SlotMasks& k, unsigned long long& nb,
+    unsigned long long& band, unsigned long long& ex, float* F) {
+  float dd = __builtin_fmaf(dx1, dx1, dx0 * dx0);
+  float num = __builtin_fmaf(dx1, dv1, dx0 * dv0);
+  if (DIM == 3) {
+    dd = __builtin_fmaf(dx2, dx2, dd);
+    num = __builtin_fmaf(dx2, dv2, num);
+  }
+  float rinv = __builtin_amdgcn_rsqf(dd);
+  float q = num * rinv;           // tau S r
+  float Ss = q * rinv;            // tau S
+  float b = q * Ss;               // beta log2(e) r S^2
+  const float facr = __builtin_amdgcn_fmed3f(
+      __builtin_fmaf(m.up_rcp_h_f, rinv, -m.rcp_h_f), 0.f, rinv);
+  const unsigned long long v_hi = __builtin_amdgcn_ballot_w64(b > m.break_hi_s);
+  const unsigned long long v_lo =
+      __builtin_amdgcn_ballot_w64(b >= m.break_lo_s);
+  const unsigned long long cand = k.act & k.brk & ~k.w;
+  nb = cand & v_hi;
+  band = cand & v_lo & ~v_hi;
+  const unsigned long long fs = k.w | nb;
+  const unsigned long long v_vex =
+      __builtin_amdgcn_ballot_w64(fabsf(dd - m.low_dd_f) <= m.vex_dd_tol_f);
+  ex = (v_vex | band | (fs & k.brk & m.contact_mask)) & k.act;
+  const unsigned long long keep = k.act & ~(fs & k.brk) & ~ex;
+  float ex_ = __builtin_amdgcn_exp2f(-b);
+  if (NOFAIL) ex_ = __builtin_amdgcn_inverse_ballot_w64(k.brk) ? ex_ : 1.0f;
+  float sr = (Ss * ex_) * (cvs * facr);
+  if constexpr (INF != 0) sr *= influence_f32<INF>(m, dd * rinv);
+  sr = __builtin_amdgcn_inverse_ballot_w64(keep) ? sr : 0.f;
+  if constexpr (ACC) {
+    F[0] = __builtin_fmaf(sr, dx0, F[0]);
+    F[1] = __builtin_fmaf(sr, dx1, F[1]);
+    if (DIM == 3) F[2] = __builtin_fmaf(sr, dx2, F[2]);
+  } else {
+    F[0] = sr * dx0;
+    F[1] = sr * dx1;
+    F[2] = DIM == 3 ? sr * dx2 : 0.f;
+  }
+}
+
 // The same from FP64 positions and displacements: the differences are
 // formed in FP64 (no cancellation) and rounded to FP32.
 template <int DIM, int INF, bool NOFAIL>

@@ -185,11 +185,13 @@ __device__ __forceinline__ void fd_step_body(
       const double2 xg = ld32(a.x, g), ug = ld32(a.u, g);
       const double vg = ld32(a.vol, g);
       r.flags = NOFAIL ? ld32(a.flags, g) : 0u;
-      r.cv = float(vg) * a.m.c4f_infl;
       if constexpr (REL) {
+        // tau u and vol c4 (J) / tau (bond_fp32_scaled)
+        r.cv = float(vg) * a.m.c4f_infl_tau;
         r.xr = make_float2(float(xg.x - org.x), float(xg.y - org.y));
-        r.u = ug;
+        r.u = make_double2(ug.x * a.m.tau, ug.y * a.m.tau);
       } else {
+        r.cv = float(vg) * a.m.c4f_infl;
         r.x = xg;
         r.u = ug;
         r.vol = vg;
@@ -221,11 +223,14 @@ __device__ __forceinline__ void fd_step_body(
     fl = a.flags[i];
   }
   bool int_i = !(fl & kFlagNonInterior);
-  // the node's block-relative FP32 position
+  // the node's block-relative FP32 position and its scaled displacement
   float xir[2] = {0.f, 0.f};
+  double uis[2] = {0., 0.};
   if constexpr (REL) {
     xir[0] = float(xi[0] - org.x);
     xir[1] = float(xi[1] - org.y);
+    uis[0] = ui[0] * a.m.tau;
+    uis[1] = ui[1] * a.m.tau;
   }
   // this slice's neighbor slots: slot k of this lane at entry 64 k + lane
   const int32_t* __restrict__ colS = a.col + cb;
@@ -292,9 +297,9 @@ __device__ __forceinline__ void fd_step_body(
         LdsRec* r = (LdsRec*)j;
         xrj[q][0] = r->xr.x;
         xrj[q][1] = r->xr.y;
-        // displacement difference in FP64, rounded once
-        duj[q][0] = float(r->u.x - ui[0]);
-        duj[q][1] = float(r->u.y - ui[1]);
+        // scaled displacement difference in FP64, rounded once
+        duj[q][0] = float(r->u.x - uis[0]);
+        duj[q][1] = float(r->u.y - uis[1]);
         cvj[q] = r->cv;
         if (NOFAIL) flj[q] = r->flags;
       } else if constexpr (TILED) {
@@ -343,9 +348,11 @@ __device__ __forceinline__ void fd_step_body(
       // decision inside a guard band, or a contact force, are left out of
       // the common path and evaluated after the chunk.  Forces are summed in
       // pairs of slots (2m, 2m+1) -- in FP32 in the FP32-force mode -- and
-      // the pairs are added to the FP64 accumulator in order: a two-term sum
-      // does not depend on the order of its terms, so neither the chunk
-      // width nor the deferred evaluation changes the result.
+      // the pairs are added to the FP64 accumulator in order (the staged
+      // FP32-force kernel fuses the odd slot's force into its pair,
+      // fma(s, dx, F_2m); a deferred evaluation is added to its pair after
+      // the chunk); a pair never spans two chunks, so the chunk width does
+      // not change the result.
       typedef typename std::conditional<MODE == kModeFp32, float, double>::type
           Real;
       constexpr int P = (U + 1) / 2;
@@ -362,17 +369,24 @@ __device__ __forceinline__ void fd_step_body(
         km.w = w[q];
         km.vex = vx[q];
         Real F[3];
-        if constexpr (MODE == kModeFp64)
+        if constexpr (REL) {
+          // written into (even slot) or fused into (odd slot) the pair sum
+          if (q % 2 == 1)
+            bond_fp32_scaled<DIM, INF, NOFAIL, true>(
+                a.m, xrj[q][0] - xir[0], xrj[q][1] - xir[1], 0.f, duj[q][0],
+                duj[q][1], 0.f, cvj[q], km, nb[q], band[q], ex[q], pf[q / 2]);
+          else
+            bond_fp32_scaled<DIM, INF, NOFAIL, false>(
+                a.m, xrj[q][0] - xir[0], xrj[q][1] - xir[1], 0.f, duj[q][0],
+                duj[q][1], 0.f, cvj[q], km, nb[q], band[q], ex[q], pf[q / 2]);
+        } else if constexpr (MODE == kModeFp64)
           bond_fp64_fast<DIM, INF, NOFAIL>(a.m, xi, ui, xj[q], uj[q], vj[q],
                                            km, nb[q], band[q], ex[q], F);
-        else if constexpr (REL)
-          bond_fp32_core<DIM, INF, NOFAIL>(
-              a.m, xrj[q][0] - xir[0], xrj[q][1] - xir[1], 0.f, duj[q][0],
-              duj[q][1], 0.f, cvj[q], km, nb[q], band[q], ex[q], F);
         else
           bond_fp32_fast<DIM, INF, NOFAIL>(a.m, xi, ui, xj[q], uj[q], cvj[q],
                                            km, nb[q], band[q], ex[q], F);
-        if (q % 2 == 0) {
+        if constexpr (REL) {
+        } else if (q % 2 == 0) {
           pf[q / 2][0] = F[0];
           pf[q / 2][1] = F[1];
           pf[q / 2][2] = F[2];
@@ -519,12 +533,14 @@ __global__ __launch_bounds__(512) void k_fd_t(
 }
 
 // The staged kernel held to at least 6 wavefronts per SIMD (at most 80
-// vector registers; 3 workgroups of 512 or their equivalent): the FP32-force
-// staged kernel with two slots per chunk fits without spilling, and the
-// compiler's own register choice can miss that occupancy step by a register.
+// vector registers): the FP32-force staged kernel with two slots per chunk
+// fits without spilling, and the compiler's own register choice can miss
+// that occupancy step by a register.  (HIP's second launch-bounds argument
+// is a minimum of wavefronts per SIMD, so the bound is stated directly.)
 template <int DIM, int MODE, int KIND, bool BASE_F, int INF, int U,
           bool NOFAIL>
-__global__ __launch_bounds__(512, 3) void k_fd_w6(
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6)))
+void k_fd_w6(
     FdArgs<DIM> a, const unsigned long long* __restrict__ bits_ro,
     const unsigned long long* __restrict__ vex_ro, double t_force,
     double t_next) {
@@ -1478,6 +1494,19 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
                                 : std::numeric_limits<float>::infinity();
   m.up_rcp_h_f = float(m.check_up * m.rcp_h);
   m.vex_tol_f = float(3e-5 * m.low_band);
+  {
+    const double b2 = m.beta * 1.4426950408889634;  // tau^2
+    m.tau = b2 > 0. ? std::sqrt(b2) : 1.0;  // (beta > 0 for RNP bonds)
+    m.c4f_infl_tau = float(double(m.c4f_infl) / m.tau);
+    m.break_lo_s = m.irreversible ? float(double(m.break_lo_f) * m.tau * m.tau)
+                                  : std::numeric_limits<float>::infinity();
+    m.break_hi_s = m.irreversible ? float(double(m.break_hi_f) * m.tau * m.tau)
+                                  : std::numeric_limits<float>::infinity();
+    // r within 3e-5 (relative) of low_band <=> r^2 within about 6e-5 of
+    // low_band^2; the band on r^2 is taken a little wider
+    m.low_dd_f = float(m.low_band * m.low_band);
+    m.vex_dd_tol_f = float(6.1e-5 * m.low_band * m.low_band);
+  }
   {
     // Bonds whose reference length lies within 3e-5 (relative) of the
     // volume-correction edge: a static mask per (slice, slot), padded like
