@@ -281,6 +281,9 @@ class OracleFD:
             _p(self.row_ptr, C.c_int64), _p(self.col, C.c_int32),
             C.byref(self.mat), p.dt, len(p.ubc), ub, len(p.fbc), fb,
             len(p.cracks), cr, _p(self._interior, C.c_uint8))
+        # initial velocity, as the GPU engine applies it at setup
+        if getattr(p, "v0", None) is not None:
+            self.set_state(v=p.v0)
 
     def close(self):
         if self.h:

@@ -434,6 +434,69 @@ def test_tiles_on_tiny_models(engine):
                 assert np.array_equal(a, b), nx
 
 
+def ragged_cloud(steps):
+    """A 40 x 40 plate (horizon 3h) whose nodes are jittered by up to 0.3 h
+    (rows of every length around the mean, bond lengths spread across the
+    volume-correction edge), with a cluster of 200 extra nodes in a 5h x 5h
+    square (rows up to several times the mean length) and 130 isolated
+    nodes far from the plate (empty rows; whole slices of them have no
+    slot) that fly at 1 m/s from the start."""
+    h = 1e-3
+    p = pr.plate2d(40, 40, h, 3, 1e-7, steps, pull=30.0, jitter=0.3, seed=3)
+    rng = np.random.default_rng(11)
+    xc = np.zeros((200, 3))
+    xc[:, :2] = 0.012 + 0.005 * rng.random((200, 2))
+    xi = np.zeros((130, 3))
+    xi[:, 0] = 1.0 + np.arange(130) * 5 * p.horizon
+    xi[:, 1] = 0.5
+    x = np.concatenate([p.x, xc, xi])
+    vol = np.concatenate([p.vol, np.full(200, 0.25 * h * h),
+                          np.full(130, h * h)])
+    v0 = np.zeros_like(x)
+    v0[-130:, 0] = 1.0
+    return pr.with_(p, x=x, vol=vol, v0=v0, name="ragged_cloud")
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_ragged_and_empty_rows(engine, oracle, mode):
+    """Irregular model (ragged_cloud) through the LDS-staged step kernel
+    against the oracle's trajectory: FP64 modes to 1e-9 with identical bond
+    sets, FP32-force to 1e-5; isolated nodes move freely (u = v0 t); in the
+    FP64 mode every tile gives the same bits."""
+    progress(f"[gpu] ragged and empty rows mode={mode}")
+    steps = 60
+    p = ragged_cloud(steps)
+    o = oracle.OracleFD(p)
+    o.init()
+    o.step(steps)
+    ur, vr, fr = o.get_state()
+    br = o.get_bonds()
+    rowlen = np.diff(o.row_ptr)
+    assert (rowlen[-130:] == 0).all() and rowlen.max() > 3 * rowlen[:1600].mean()
+    outs = []
+    for tile in ((256, 128, 512) if mode == NLM_PREC_FP64 else (256,)):
+        with engine.GpuFD(p, precision=mode) as g:
+            g.init()
+            g.set_tile(tile)  # raises unless the staged kernel is in use
+            g.step(steps)
+            outs.append(g.get_state() + (g.get_bonds(),))
+    ug, vg, fg, bg = outs[0]
+    t = steps * p.dt
+    assert np.allclose(ug[-130:, 0], t, rtol=1e-12, atol=0)
+    assert np.all(ug[-130:, 1:] == 0)
+    if mode == NLM_PREC_FP32_FORCE:
+        assert rel_l2(ug, ur) <= 1e-5
+        assert np.count_nonzero(bg != br) <= 1e-3 * max(br.sum(), 1)
+    else:
+        assert rel_l2(ug, ur) <= 1e-9
+        assert rel_l2(vg, vr) <= 1e-8
+        assert np.array_equal(bg, br)
+    assert br.sum() > 1000
+    for o2 in outs[1:]:
+        for a, b in zip(outs[0], o2):
+            assert np.array_equal(a, b)
+
+
 def fp32_underflow_floor(o, p, csr):
     """Per-node force that the FP32-force mode may lose to underflow: its
     bond arithmetic (du, dx.du, S) runs in FP32, whose smallest normal
