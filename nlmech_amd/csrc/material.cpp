@@ -142,6 +142,10 @@ extern "C" int nlm_rnp_params_from_deck(const nlm_material_deck* d,
     if (d->n_bond_params < 2) return NLM_ERR_INVALID;
     o->C = d->bond_params[0];
     o->beta = d->bond_params[1];
+    // rbar = sqrt(0.5 / beta) and the bond's exponent exp(-beta r S^2) need
+    // beta > 0 (the reference computes NaN / inf force laws otherwise; the
+    // FP32-force staged kernel scales displacements by sqrt(beta log2 e))
+    if (!(o->beta > 0.) || !std::isfinite(o->beta)) return NLM_ERR_INVALID;
     o->rbar = std::sqrt(0.5 / o->beta);
     nu = 0.33;  // computeMaterialProperties, rnpBond.cpp:191-212
     if (d->dim == 2) {

@@ -68,6 +68,18 @@ def test_rnp_params_rejects_bad_deck():
     bad = pr.with_(p, material=dict(pr.rnp_material(), inf_type=7))
     assert lib.nlm_rnp_params_from_deck(C.byref(material_deck(bad)),
                                         C.byref(out)) != 0
+    # explicit bond parameters [C, beta] need beta > 0
+    for beta in (0.0, -1.0):
+        bad = pr.with_(p, material=dict(pr.rnp_material(),
+                                        compute_from_classical=0,
+                                        bond_params=[100.0, beta]))
+        assert lib.nlm_rnp_params_from_deck(C.byref(material_deck(bad)),
+                                            C.byref(out)) != 0
+    good = pr.with_(p, material=dict(pr.rnp_material(),
+                                     compute_from_classical=0,
+                                     bond_params=[100.0, 2.0e7]))
+    assert lib.nlm_rnp_params_from_deck(C.byref(material_deck(good)),
+                                        C.byref(out)) == 0
 
 
 @pytest.mark.parametrize("case", ["qs2d", "qs1d", "cube", "plane_strain"])
