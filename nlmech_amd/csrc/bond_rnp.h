@@ -423,16 +423,19 @@ __device__ __forceinline__ void bond_fp32_core(
   F[2] = DIM == 3 ? sr * dx2 : 0.f;
 }
 
-// FP32-force common path of the LDS-staged kernel, on scaled displacement
-// differences: dv = tau du with tau^2 = beta log2(e) (the halo records and
-// the node hold tau u in FP64), so q = tau S r, Ss = tau S and
-// b = q Ss = beta log2(e) r S^2 is at once the exponent of the bond force,
-// exp(-beta r S^2) = exp2(-b), and the break test's quantity (its guard
-// band break_lo_s/break_hi_s scaled on the host).  cvs = vol_j c4 (J) / tau.
-// The volume-correction factor enters as fac / r =
-// clamp((check_up/h) / r - 1/h, 0, 1/r), whose upper bound holds exactly
-// when r <= check_low; the edge band is tested on r^2.  ACC: the force is
-// added to F with fused multiply-adds (F = sr dx + F), else written.
+// FP32-force common path of the LDS-staged kernel, on scaled differences:
+// dx = sigma (x_j - x_i) and dv = tau (u_j - u_i) with tau a power of two and
+// tau^2 / sigma = beta log2(e) (the halo records and the node hold sigma
+// (x - origin) in FP32 and tau u in FP64).  Then q = dx.dv / |dx| = tau S r,
+// Ss = q / |dx| = tau S / sigma and b = q Ss = beta log2(e) r S^2 is at once
+// the exponent of the bond force, exp(-beta r S^2) = exp2(-b), and the break
+// test's quantity (its guard band break_lo_s/break_hi_s scaled on the host).
+// The force S exp(-beta r S^2) V_j c4 J (fac / r) (x_j - x_i) equals
+// Ss exp2(-b) cvs (fac / |dx|) dx with cvs = V_j c4 J sigma / tau.  The
+// volume-correction factor enters as fac / |dx| =
+// clamp((check_up/h) / |dx| - 1/(sigma h), 0, 1/|dx|), whose upper bound holds
+// exactly when r <= check_low; the edge band is tested on |dx|^2.  ACC: the
+// force is added to F with fused multiply-adds (F = sr dx + F), else written.
 template <int DIM, int INF, bool NOFAIL, bool ACC>
 __device__ __forceinline__ void bond_fp32_scaled(
     const RnpDev& m, float dx0, float dx1, float dx2, float dv0, float dv1,
@@ -446,10 +449,10 @@ __device__ __forceinline__ void bond_fp32_scaled(
   }
   float rinv = __builtin_amdgcn_rsqf(dd);
   float q = num * rinv;           // tau S r
-  float Ss = q * rinv;            // tau S
+  float Ss = q * rinv;            // tau S / sigma
   float b = q * Ss;               // beta log2(e) r S^2
   const float facr = __builtin_amdgcn_fmed3f(
-      __builtin_fmaf(m.up_rcp_h_f, rinv, -m.rcp_h_f), 0.f, rinv);
+      __builtin_fmaf(m.up_rcp_h_f, rinv, -m.rcp_h_s), 0.f, rinv);
   const unsigned long long v_hi = __builtin_amdgcn_ballot_w64(b > m.break_hi_s);
   const unsigned long long v_lo =
       __builtin_amdgcn_ballot_w64(b >= m.break_lo_s);
@@ -464,7 +467,8 @@ __device__ __forceinline__ void bond_fp32_scaled(
   float ex_ = __builtin_amdgcn_exp2f(-b);
   if (NOFAIL) ex_ = __builtin_amdgcn_inverse_ballot_w64(k.brk) ? ex_ : 1.0f;
   float sr = (Ss * ex_) * (cvs * facr);
-  if constexpr (INF != 0) sr *= influence_f32<INF>(m, dd * rinv);
+  if constexpr (INF != 0)
+    sr *= influence_f32<INF>(m, (dd * rinv) * m.rcp_sigma_f);
   sr = __builtin_amdgcn_inverse_ballot_w64(keep) ? sr : 0.f;
   if constexpr (ACC) {
     F[0] = __builtin_fmaf(sr, dx0, F[0]);

@@ -186,9 +186,11 @@ __device__ __forceinline__ void fd_step_body(
       const double vg = ld32(a.vol, g);
       r.flags = NOFAIL ? ld32(a.flags, g) : 0u;
       if constexpr (REL) {
-        // tau u and vol c4 (J) / tau (bond_fp32_scaled)
-        r.cv = float(vg) * a.m.c4f_infl_tau;
-        r.xr = make_float2(float(xg.x - org.x), float(xg.y - org.y));
+        // sigma (x - origin), tau u, vol c4 (J) sigma / tau
+        // (bond_fp32_scaled)
+        r.cv = float(vg) * a.m.c4f_infl_s;
+        r.xr = make_float2(float((xg.x - org.x) * a.m.sigma),
+                           float((xg.y - org.y) * a.m.sigma));
         r.u = make_double2(ug.x * a.m.tau, ug.y * a.m.tau);
       } else {
         r.cv = float(vg) * a.m.c4f_infl;
@@ -223,12 +225,12 @@ __device__ __forceinline__ void fd_step_body(
     fl = a.flags[i];
   }
   bool int_i = !(fl & kFlagNonInterior);
-  // the node's block-relative FP32 position and its scaled displacement
+  // the node's scaled block-relative FP32 position and scaled displacement
   float xir[2] = {0.f, 0.f};
   double uis[2] = {0., 0.};
   if constexpr (REL) {
-    xir[0] = float(xi[0] - org.x);
-    xir[1] = float(xi[1] - org.y);
+    xir[0] = float((xi[0] - org.x) * a.m.sigma);
+    xir[1] = float((xi[1] - org.y) * a.m.sigma);
     uis[0] = ui[0] * a.m.tau;
     uis[1] = ui[1] * a.m.tau;
   }
@@ -297,7 +299,8 @@ __device__ __forceinline__ void fd_step_body(
         LdsRec* r = (LdsRec*)j;
         xrj[q][0] = r->xr.x;
         xrj[q][1] = r->xr.y;
-        // scaled displacement difference in FP64, rounded once
+        // scaled displacement difference in FP64 (tau a power of two: exact),
+        // rounded once
         duj[q][0] = float(r->u.x - uis[0]);
         duj[q][1] = float(r->u.y - uis[1]);
         cvj[q] = r->cv;
@@ -1495,17 +1498,26 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
   m.up_rcp_h_f = float(m.check_up * m.rcp_h);
   m.vex_tol_f = float(3e-5 * m.low_band);
   {
-    const double b2 = m.beta * 1.4426950408889634;  // tau^2
-    m.tau = b2 > 0. ? std::sqrt(b2) : 1.0;  // (beta > 0 for RNP bonds)
-    m.c4f_infl_tau = float(double(m.c4f_infl) / m.tau);
-    m.break_lo_s = m.irreversible ? float(double(m.break_lo_f) * m.tau * m.tau)
+    const double b2 = m.beta * 1.4426950408889634;  // beta log2 e
+    if (b2 > 0.) {  // (beta > 0 for RNP bonds)
+      m.tau = std::ldexp(1.0, int(std::lround(0.5 * std::log2(b2))));
+      m.sigma = m.tau * m.tau / b2;
+    } else {
+      m.tau = m.sigma = 1.0;
+    }
+    const double bl = m.tau * m.tau / m.sigma;  // beta log2 e
+    m.c4f_infl_s = float(double(m.c4f_infl) * m.sigma / m.tau);
+    m.rcp_h_s = float(1.0 / (m.sigma * m.h));
+    m.rcp_sigma_f = float(1.0 / m.sigma);
+    m.break_lo_s = m.irreversible ? float(double(m.break_lo_f) * bl)
                                   : std::numeric_limits<float>::infinity();
-    m.break_hi_s = m.irreversible ? float(double(m.break_hi_f) * m.tau * m.tau)
+    m.break_hi_s = m.irreversible ? float(double(m.break_hi_f) * bl)
                                   : std::numeric_limits<float>::infinity();
     // r within 3e-5 (relative) of low_band <=> r^2 within about 6e-5 of
     // low_band^2; the band on r^2 is taken a little wider
-    m.low_dd_f = float(m.low_band * m.low_band);
-    m.vex_dd_tol_f = float(6.1e-5 * m.low_band * m.low_band);
+    const double lows = m.sigma * m.low_band;
+    m.low_dd_f = float(lows * lows);
+    m.vex_dd_tol_f = float(6.1e-5 * lows * lows);
   }
   {
     // Bonds whose reference length lies within 3e-5 (relative) of the

@@ -96,14 +96,19 @@ struct RnpDev {
   // r S^2 (FP32 path); +inf when bonds never break
   double break_lo, break_hi;
   float break_lo_f, break_hi_f;
-  // FP32-force staged kernels: displacements are held as tau u with
-  // tau^2 = beta log2(e), so (tau S)(tau S) r = beta log2(e) r S^2 is the
-  // exponent of exp2 and the break ratio at once
-  double tau;              // sqrt(beta log2 e)
-  float c4f_infl_tau;      // c4f_infl / tau
-  float break_lo_s, break_hi_s;  // break_lo_f, break_hi_f times tau^2
-  float low_dd_f;          // low_band^2
-  float vex_dd_tol_f;      // half-width of the edge band on r^2
+  // FP32-force staged kernels: displacements are held as tau u (tau a power
+  // of two, so tau u is exact) and block-relative positions as sigma x with
+  // tau^2 / sigma = beta log2(e), so the product (tau S / sigma)(tau S r)
+  // they form is beta log2(e) r S^2: the exponent of exp2 and the break
+  // ratio at once
+  double tau;              // power of two nearest sqrt(beta log2 e)
+  double sigma;            // tau^2 / (beta log2 e), in [1/2, 2]
+  float c4f_infl_s;        // c4f_infl sigma / tau
+  float rcp_h_s;           // 1 / (sigma h)
+  float rcp_sigma_f;       // 1 / sigma
+  float break_lo_s, break_hi_s;  // break_lo_f, break_hi_f times beta log2 e
+  float low_dd_f;          // (sigma low_band)^2
+  float vex_dd_tol_f;      // half-width of the edge band on (sigma r)^2
 };
 
 // ElasticState constants as used by the kernels.

@@ -5,7 +5,7 @@ Bars (all through the C-ABI, libnlmech_amd.so):
   - bond bits (pre-crack and every break decision): bit-exact;
   - per-node force from an identical state, FP64 modes:
         |f_gpu - f_ref|_inf <= 1e-12 * sum_j |F_ij^ref|   (per node)
-    FP32-force mode: <= 2e-6 * sum_j |F_ij^ref|;
+    FP32-force mode: <= 1e-5 * sum_j |F_ij^ref| (FORCE_TOL);
   - trajectories, FP64 modes: relative L2 displacement error <= 1e-9 and
     identical broken-bond sets; FP32-force mode: <= 1e-5 and < 0.1 %
     broken-bond mismatch.
@@ -23,8 +23,15 @@ from nlmech_amd.capi import (NLM_PREC_FP32_FORCE, NLM_PREC_FP64,
 pytestmark = pytest.mark.gpu
 
 MODES = [NLM_PREC_FP64_EXACT, NLM_PREC_FP64, NLM_PREC_FP32_FORCE]
+# FP32-force mode: a bond's position difference comes from tile-relative
+# FP32 coordinates (within ~25 h of their origin), each rounded by at most
+# 2^-24 of its size: up to 2^-23 * 25 h / r relative, 3e-6 for the shortest
+# bonds (r >= h).  With ~15 FP32 roundings and the rsq / exp2 approximations
+# per bond force, the per-node bar is the mode's accuracy target, 1e-5 of
+# sum_j |F_ij| (five random states of the force-parity test read 0.5-4.6e-6
+# at their worst node).
 FORCE_TOL = {NLM_PREC_FP64_EXACT: 1e-12, NLM_PREC_FP64: 1e-12,
-             NLM_PREC_FP32_FORCE: 2e-6}
+             NLM_PREC_FP32_FORCE: 1e-5}
 
 
 @pytest.fixture(scope="module")
@@ -555,8 +562,8 @@ def test_plate2d_1e6_stepwise_breaking(engine, oracle, mode):
     in the first 40 steps): the GPU's neighbor lists equal the oracle's,
     and at three checkpoints the GPU state is loaded into the oracle and one
     step is taken on both -- bond bits identical (including the bonds that
-    break in that step) and per-node force within 1e-12 (FP64) / 2e-6
-    (FP32-force) of sum_j |F_ij|, plus in FP32 the underflow floor of nodes
+    break in that step) and per-node force within 1e-12 (FP64) / 1e-5
+    (FP32-force, FORCE_TOL) of sum_j |F_ij|, plus in FP32 the underflow floor of nodes
     the waves have not reached (fp32_underflow_floor)."""
     progress(f"[gpu] plate2d_1e6 stepwise breaking mode {mode}")
     p = pr.plate2d(1000, 1000, 1e-4, 4, 1e-8, 100, pull=30.0)
