@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <fstream>
 #include <sstream>
 #include <unordered_map>
@@ -71,6 +72,152 @@ void read_csv(const std::string& file, size_t dim, std::vector<double>* x,
   }
 }
 
+// rw::reader::MshReader::readMesh (src/rw/mshReader.cpp:20-114) reads a gmsh
+// mesh through the gmsh library: every node at index tag - 1, and of the
+// elements the 3-node triangles if the mesh has any, else the 4-node
+// quadrangles (a mesh with both keeps the triangles after an error message;
+// one with neither is an error), connectivity in the order the file lists
+// them.  This reads the same from the ASCII formats 2.2 and 4.1 directly
+// (binary files are refused).
+void read_msh(const std::string& file, std::vector<double>* x,
+              std::vector<int64_t>* enc, int* etype, size_t* nelems) {
+  std::ifstream f(file);
+  if (!f) error("Error: cannot open mesh file " + file);
+  auto bad = [&](const std::string& what) {
+    error("Error: mesh file " + file + ": " + what);
+  };
+  std::string line;
+  auto next = [&]() -> std::string {
+    if (!std::getline(f, line)) bad("unexpected end of file");
+    if (!line.empty() && line.back() == '\r') line.pop_back();
+    return line;
+  };
+  auto skip_to = [&](const std::string& end) {
+    while (next() != end) {
+    }
+  };
+  double version = 0.;
+  bool have_nodes = false, have_elems = false;
+  std::vector<int64_t> tri, quad;
+  x->clear();
+  auto put_node = [&](int64_t tag, double px, double py, double pz) {
+    if (tag < 1) bad("node tag < 1");
+    if (size_t(tag) * 3 > x->size()) bad("node tag beyond the node count");
+    (*x)[3 * (tag - 1)] = px;
+    (*x)[3 * (tag - 1) + 1] = py;
+    (*x)[3 * (tag - 1) + 2] = pz;
+  };
+  // one element: gmsh type, node tags (1-based) from the stream
+  auto put_elem = [&](int type, std::istringstream& ss) {
+    std::vector<int64_t>* dst = type == 2 ? &tri : type == 3 ? &quad : nullptr;
+    if (!dst) return;  // points, lines, other types: not used
+    const int nv = type == 2 ? 3 : 4;
+    for (int k = 0; k < nv; k++) {
+      int64_t t = 0;
+      if (!(ss >> t) || t < 1) bad("bad element connectivity");
+      dst->push_back(t - 1);
+    }
+  };
+  while (std::getline(f, line)) {
+    if (!line.empty() && line.back() == '\r') line.pop_back();
+    if (line == "$MeshFormat") {
+      std::istringstream ss(next());
+      int ftype = -1, dsize = 0;
+      ss >> version >> ftype >> dsize;
+      if (ftype != 0) bad("binary gmsh files are not supported");
+      if (!(version >= 2. && version < 3.) && !(version >= 4. && version < 5.))
+        bad("gmsh format version " + std::to_string(version) +
+            " is not supported (2.2 or 4.1)");
+      skip_to("$EndMeshFormat");
+    } else if (line == "$Nodes") {
+      if (version == 0.) bad("$Nodes before $MeshFormat");
+      if (version < 3.) {
+        int64_t n = 0;
+        std::istringstream(next()) >> n;
+        if (n <= 0) bad("no nodes");
+        x->assign(size_t(3 * n), 0.);
+        for (int64_t i = 0; i < n; i++) {
+          std::istringstream ss(next());
+          int64_t tag;
+          double px = 0, py = 0, pz = 0;
+          if (!(ss >> tag >> px >> py >> pz)) bad("bad node line");
+          put_node(tag, px, py, pz);
+        }
+      } else {
+        int64_t nblk = 0, n = 0, tmin = 0, tmax = 0;
+        std::istringstream(next()) >> nblk >> n >> tmin >> tmax;
+        if (n <= 0) bad("no nodes");
+        if (tmax != n) bad("node tags are not 1..N");
+        x->assign(size_t(3 * n), 0.);
+        for (int64_t b = 0; b < nblk; b++) {
+          int edim = 0, etag = 0, param = 0;
+          int64_t nb = 0;
+          std::istringstream(next()) >> edim >> etag >> param >> nb;
+          std::vector<int64_t> tags(static_cast<size_t>(nb));
+          for (int64_t k = 0; k < nb; k++)
+            std::istringstream(next()) >> tags[size_t(k)];
+          for (int64_t k = 0; k < nb; k++) {
+            std::istringstream ss(next());
+            double px = 0, py = 0, pz = 0;
+            if (!(ss >> px >> py >> pz)) bad("bad node coordinates");
+            put_node(tags[size_t(k)], px, py, pz);
+          }
+        }
+      }
+      skip_to("$EndNodes");
+      have_nodes = true;
+    } else if (line == "$Elements") {
+      if (version == 0.) bad("$Elements before $MeshFormat");
+      if (version < 3.) {
+        int64_t n = 0;
+        std::istringstream(next()) >> n;
+        for (int64_t i = 0; i < n; i++) {
+          std::istringstream ss(next());
+          int64_t tag = 0;
+          int type = 0, ntags = 0;
+          if (!(ss >> tag >> type >> ntags)) bad("bad element line");
+          for (int k = 0; k < ntags; k++) {
+            int64_t t;
+            ss >> t;
+          }
+          put_elem(type, ss);
+        }
+      } else {
+        int64_t nblk = 0, n = 0, tmin = 0, tmax = 0;
+        std::istringstream(next()) >> nblk >> n >> tmin >> tmax;
+        for (int64_t b = 0; b < nblk; b++) {
+          int edim = 0, etag = 0, type = 0;
+          int64_t nb = 0;
+          std::istringstream(next()) >> edim >> etag >> type >> nb;
+          for (int64_t k = 0; k < nb; k++) {
+            std::istringstream ss(next());
+            int64_t tag = 0;
+            ss >> tag;
+            put_elem(type, ss);
+          }
+        }
+      }
+      skip_to("$EndElements");
+      have_elems = true;
+    }
+  }
+  if (!have_nodes) bad("no $Nodes section");
+  if (!have_elems) bad("no $Elements section");
+  if (!tri.empty() && !quad.empty())
+    std::fprintf(stderr,
+                 "Error: Only mesh with one type of elements is supported!\n");
+  if (tri.empty() && quad.empty())
+    error("Error: Only 3-node triangle or 4-node square elements are "
+          "supported!");
+  const size_t nn = x->size() / 3;
+  const bool use_tri = !tri.empty();
+  *enc = use_tri ? std::move(tri) : std::move(quad);
+  for (int64_t c : *enc)
+    if (size_t(c) >= nn) bad("element refers to a missing node");
+  *etype = use_tri ? rw::kVtkTriangle : rw::kVtkQuad;
+  *nelems = enc->size() / (use_tri ? 3 : 4);
+}
+
 }  // namespace
 
 Mesh::Mesh(const inp::MeshDeck& deck) : d_dim(deck.d_dim), d_h(deck.d_h) {
@@ -108,8 +255,13 @@ Mesh::Mesh(const inp::MeshDeck& deck) : d_dim(deck.d_dim), d_h(deck.d_h) {
       error(std::string("Error: ") + ex.what());
     }
   } else if (e == "msh") {
-    error("Error: gmsh .msh meshes are not supported; convert the mesh to "
-          ".vtu or .csv.");
+    // the reference reads 2D triangle / quadrangle meshes; a 3D mesh would
+    // yield its surface triangles
+    if (d_dim != 2)
+      error("Error: MshReader currently only supports reading of "
+            "triangle/quadrangle elements in dimension 2.");
+    read_msh(deck.d_filename, &d_nodes, &d_enc, &d_eType, &d_numElems);
+    d_vol.clear();
   } else {
     error("Error: unknown mesh file type " + deck.d_filename);
   }
@@ -122,7 +274,9 @@ Mesh::Mesh(const inp::MeshDeck& deck) : d_dim(deck.d_dim), d_h(deck.d_h) {
         << 0.01 * std::pow(d_h, double(d_dim)) << ", Node = " << i;
       error(m.str());
     }
-  if (!deck.d_keepElementConn) {
+  // (the reference keeps a gmsh mesh's elements: fe::Mesh::createData,
+  // mesh.cpp:127-129)
+  if (!deck.d_keepElementConn && e != "msh") {
     d_enc.clear();
     d_numElems = 0;
   }

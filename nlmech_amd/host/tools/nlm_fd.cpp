@@ -134,13 +134,32 @@ void dump(nlm::inp::Input& d) {
   std::cout << s << std::endl;
 }
 
+// The mesh the deck names, as the model reads it (nlm_fd --dump-mesh): node
+// coordinates, nodal volumes and element connectivity as JSON.
+void dump_mesh(nlm::inp::Input& d) {
+  nlm::fe::Mesh mesh(*d.getMeshDeck());
+  std::string s = "{\"dim\": " + num(double(mesh.getDimension())) +
+                  ", \"h\": " + num(mesh.getMeshSize()) + ", \"nodes\": [";
+  const auto& x = mesh.getNodes();
+  for (size_t k = 0; k < x.size(); k++) s += (k ? ", " : "") + num(x[k]);
+  s += "], \"volumes\": " + list(mesh.getNodalVolumes()) +
+       ", \"element_type\": " + num(double(mesh.getElementType())) +
+       ", \"elements\": [";
+  const auto& e = mesh.getElementConnectivities();
+  for (size_t k = 0; k < e.size(); k++)
+    s += (k ? ", " : "") + std::to_string(e[k]);
+  s += "]}";
+  std::cout << s << std::endl;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
   std::string file;
-  bool dump_only = false;
+  bool dump_only = false, dump_mesh_only = false;
   for (int k = 1; k < argc; k++) {
     if (!std::strcmp(argv[k], "--dump-deck")) dump_only = true;
+    if (!std::strcmp(argv[k], "--dump-mesh")) dump_mesh_only = true;
     if ((!std::strcmp(argv[k], "-i") || !std::strcmp(argv[k], "--input-file")) &&
         k + 1 < argc)
       file = argv[++k];
@@ -152,10 +171,11 @@ int main(int argc, char** argv) {
               << std::endl;
     return 1;
   }
-  if (dump_only) {
+  if (dump_only || dump_mesh_only) {
     try {
       nlm::inp::Input deck(file);
-      dump(deck);
+      if (dump_only) dump(deck);
+      else dump_mesh(deck);
     } catch (const nlm::inp::DeckError& e) {
       std::cerr << e.what() << std::endl;
       return 1;

@@ -16,6 +16,11 @@ container, where the reference tree is mounted):
        compare.txt (the dc tool's known answer, test/CMakeLists.txt:61-64)
    - testFracture_nodes.csv <- data/test/geometry/testFracture_nodes.csv
        (the node set of TestGeom, src/test/geometry/testGeomLib.cpp:42-86)
+   - gmsh/ <- examples/io/{input_2d_gmsh_quad,input_2d_gmsh_triangle}.yaml
+       and their meshes mesh_2d_{quad,triangle}.msh (gmsh format 4.1; the
+       inputs of mesh.2d.read.{quad,triangle}.gmsh, test/CMakeLists.txt:
+       72-82), examples/fdModel/logo/input.yaml and mesh.msh (format 2.2)
+       as logo_input.yaml / logo_mesh.msh, copied unchanged
 2. Oracle-generated trajectories (oracle_*.npz), pinning the CPU restatement
    against later edits; written by make_oracle_fixtures().
 """
@@ -55,6 +60,14 @@ def reference_fixtures(ref: Path) -> None:
                         expected_l2_sup=np.array(vals))
     shutil.copyfile(ref / "data/test/geometry/testFracture_nodes.csv",
                     HERE / "testFracture_nodes.csv")
+    g = HERE / "gmsh"
+    g.mkdir(exist_ok=True)
+    for f in ("input_2d_gmsh_quad.yaml", "input_2d_gmsh_triangle.yaml",
+              "mesh_2d_quad.msh", "mesh_2d_triangle.msh"):
+        shutil.copyfile(ref / "examples/io" / f, g / f)
+    shutil.copyfile(ref / "examples/fdModel/logo/input.yaml",
+                    g / "logo_input.yaml")
+    shutil.copyfile(ref / "examples/fdModel/logo/mesh.msh", g / "logo_mesh.msh")
 
 
 def make_oracle_fixtures() -> None:

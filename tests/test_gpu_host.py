@@ -24,7 +24,11 @@ Bars:
     based material, its mesh made by nlm_mesh) reproduces the reference-held
     result output_res.vtu within the reference test's own dc tolerance
     (displacement L2 < 1e-7, test/CMakeLists.txt:39-52); the internal force
-    too.
+    too;
+  - gmsh meshes: the reference's mesh.2d.read.{quad,triangle}.gmsh CTests
+    (examples/io decks, format 4.1, unchanged) run and write the nodal
+    volumes of their meshes; the logo deck (examples/fdModel/logo, format
+    2.2, shortened) matches the oracle on the same mesh.
 """
 import shutil
 from pathlib import Path
@@ -612,3 +616,99 @@ def test_reference_quasi_static_ctest(host_bin, tmp_path, case):
     l2 = vals[0::2]
     assert len(l2) == 5 and max(l2) < 1e-7, l2
     assert rc == 0
+
+
+# ------------------------------------------------------------ gmsh meshes
+GMSH = GOLDEN / "gmsh"
+
+
+@pytest.mark.parametrize("case", ["quad", "triangle"])
+def test_reference_gmsh_read_ctest(host_bin, tmp_path, case):
+    """mesh.2d.read.{quad,triangle}.gmsh (test/CMakeLists.txt:72-82): nlm_fd
+    -i input_2d_gmsh_<case>.yaml on the reference's deck and gmsh 4.1 mesh,
+    unchanged (a state-based quasi-static model without loads, one load
+    step, Node_Volume output): the run succeeds and the written volumes sum
+    to the 5 x 5 domain's area."""
+    deck = f"input_2d_gmsh_{case}.yaml"
+    shutil.copy(GMSH / deck, tmp_path / deck)
+    shutil.copy(GMSH / f"mesh_2d_{case}.msh", tmp_path / f"mesh_2d_{case}.msh")
+    progress(f"[gpu] reference gmsh {case} deck")
+    run_tool(host_bin / "nlm_fd", "-i", deck, cwd=tmp_path)
+    outs = sorted(tmp_path.glob("output_*.vtu"))
+    assert outs
+    v = np.asarray(read_vtu(str(outs[-1]))["Node_Volume"], np.float64)
+    assert v.size == (36 if case == "quad" else 45)
+    assert abs(v.sum() - 25.0) <= 1e-12 * 25.0
+
+
+def read_msh22(path):
+    """Nodes and 3-node triangles of a gmsh 2.2 ASCII file (node tags 1..N,
+    the triangles in file order)."""
+    lines = Path(path).read_text().splitlines()
+    i = lines.index("$Nodes")
+    n = int(lines[i + 1])
+    x = np.zeros((n, 3))
+    for ln in lines[i + 2:i + 2 + n]:
+        t = ln.split()
+        x[int(t[0]) - 1] = [float(t[1]), float(t[2]), float(t[3])]
+    j = lines.index("$Elements")
+    tri = []
+    for ln in lines[j + 2:j + 2 + int(lines[j + 1])]:
+        t = [int(a) for a in ln.split()]
+        if t[1] == 2:
+            tri.append([a - 1 for a in t[3 + t[2]:]])
+    return x, np.asarray(tri, np.int64)
+
+
+def test_reference_logo_deck(host_bin, oracle, tmp_path):
+    """examples/fdModel/logo (its gmsh 2.2 mesh of 5107 nodes and 9486
+    triangles, PDBond = RNPBond, linear influence function, Check_Sc_Factor
+    10, a body force 1.5e9 t sin(1.090909 x) in x over the whole logo)
+    run by nlm_fd, shortened to its first 300 of 50,000 steps (same dt =
+    2e-7 s), outputs at steps 150 and 300 written to ./ with Node_Volume
+    added: nodal volumes equal the triangles' areas split in thirds, and the
+    displacements match the oracle's on the same mesh (relative L2 <= 1e-9)
+    with identical broken-bond bytes."""
+    shutil.copy(GMSH / "logo_input.yaml", tmp_path / "input.yaml")
+    shutil.copy(GMSH / "logo_mesh.msh", tmp_path / "mesh.msh")
+    deck = (tmp_path / "input.yaml").read_text()
+    deck = (deck.replace("Final_Time: 0.010000", "Final_Time: 0.000060")
+                .replace("Time_Steps: 50000", "Time_Steps: 300")
+                .replace("Output_Interval: 500", "Output_Interval: 150")
+                .replace("Path: ../out/", "Path: ./")
+                .replace("    - Force\n", "    - Force\n    - Node_Volume\n"))
+    assert "Time_Steps: 300\n" in deck and "Node_Volume" in deck
+    (tmp_path / "input.yaml").write_text(deck)
+    progress("[gpu] reference logo deck (300 steps)")
+    run_tool(host_bin / "nlm_fd", "-i", "input.yaml", cwd=tmp_path)
+    outs = [read_vtu(str(tmp_path / f"output_{k}.vtu")) for k in (1, 2)]
+    x, tri = read_msh22(tmp_path / "mesh.msh")
+    a, b, c = x[tri[:, 0], :2], x[tri[:, 1], :2], x[tri[:, 2], :2]
+    area = 0.5 * np.abs((b[:, 0] - a[:, 0]) * (c[:, 1] - a[:, 1]) -
+                        (c[:, 0] - a[:, 0]) * (b[:, 1] - a[:, 1]))
+    vol = np.zeros(x.shape[0])
+    for k in range(3):
+        np.add.at(vol, tri[:, k], area / 3.0)
+    assert np.allclose(np.asarray(outs[0]["Node_Volume"], np.float64), vol,
+                       rtol=1e-12, atol=0)
+    fbc = [pr.BC("rectangle", [1], "linear", [1.5e9], "sin_x", [1.090909],
+                 x1=0.0, y1=0.0, x2=2.75, y2=1.5)]
+    mat = pr.rnp_material(dim=2, E=88000.0, Gc=999.802380,
+                          density=1011.203677, check_sc=10.0, inf_type=1,
+                          inf_params=())
+    p = pr.Problem(name="logo", dim=2, x=x, vol=vol, horizon=0.1,
+                   h=0.1 / 4, dt=0.000060 / 300, nsteps=300, material=mat,
+                   fbc=fbc)
+    o = oracle.OracleFD(p)
+    o.init()
+    for k, out in enumerate(outs):
+        o.step(150)
+        u_ref, _, _ = o.get_state()
+        err = np.linalg.norm(out["Displacement"] - u_ref) / \
+            np.linalg.norm(u_ref)
+        progress(f"[gpu]   step {150 * (k + 1)}: rel L2 {err:.2e}, "
+                 f"{int(o.get_bonds().sum())} broken bonds")
+        assert err <= 1e-9
+        assert np.array_equal(out["Bond_State"], o.get_bond_bytes())
+    o.close()
+

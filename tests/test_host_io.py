@@ -13,6 +13,13 @@ the .vtu writer / reader, the mesh tool and the dc tool.
   size from Horizon / Horizon_h_Ratio, Rectangle [x1, y1, x2, y2], PDBond
   renamed RNPBond, a repeated key keeping its first value; error messages
   and exit status 1 where the reference exits.
+- gmsh meshes (formats 4.1 and 2.2, the reference's examples/io and
+  examples/fdModel/logo meshes): node counts, element counts, and nodal
+  volumes pinned by closed forms -- the unit quadrangles' corner / edge /
+  interior volumes 1/4, 1/2, 1, every mesh's volumes summing to its domain's
+  area (5 x 5, and 2.25 for the logo polygon of logo.geo) -- and by the
+  element areas split in thirds (triangles); binary files and 3D decks are
+  refused.
 No GPU work is issued here.
 """
 import json
@@ -214,3 +221,78 @@ def test_every_reference_example_deck_parses(host_bin):
     for p in decks:
         d = dump(host_bin, p)
         assert d["model"]["dim"] in (1, 2, 3), p
+
+
+# ------------------------------------------------------------ gmsh meshes
+GMSH = GOLDEN / "gmsh"
+
+
+def dump_mesh(host_bin, deck: Path):
+    rc, out = run_tool(host_bin / "nlm_fd", "--dump-mesh", "-i", deck.name,
+                       cwd=deck.parent)
+    line = [l for l in out.splitlines() if l.startswith("{")][-1]
+    d = json.loads(line)
+    d["nodes"] = np.asarray(d["nodes"]).reshape(-1, 3)
+    d["volumes"] = np.asarray(d["volumes"])
+    d["elements"] = np.asarray(d["elements"], np.int64)
+    return d
+
+
+def gmsh_case(tmp_path, case):
+    """The deck of case (its mesh copied beside it) and the mesh's domain
+    area: examples/io covers [0, 5]^2; the logo is the polygon of logo.geo,
+    whose shoelace area is 2.25."""
+    if case == "logo":
+        shutil.copy(GMSH / "logo_input.yaml", tmp_path / "input.yaml")
+        shutil.copy(GMSH / "logo_mesh.msh", tmp_path / "mesh.msh")
+        return tmp_path / "input.yaml", 2.25
+    deck = f"input_2d_gmsh_{case}.yaml"
+    shutil.copy(GMSH / deck, tmp_path / deck)
+    shutil.copy(GMSH / f"mesh_2d_{case}.msh", tmp_path / f"mesh_2d_{case}.msh")
+    return tmp_path / deck, 25.0
+
+
+@pytest.mark.parametrize("case, nodes, elems, etype",
+                         [("quad", 36, 25, 9), ("triangle", 45, 68, 5),
+                          ("logo", 5107, 9486, 5)])
+def test_gmsh_mesh_volumes(host_bin, tmp_path, case, nodes, elems, etype):
+    deck, area = gmsh_case(tmp_path, case)
+    d = dump_mesh(host_bin, deck)
+    x, v, e = d["nodes"], d["volumes"], d["elements"]
+    assert x.shape == (nodes, 3) and v.shape == (nodes,)
+    assert d["element_type"] == etype
+    nv = 3 if etype == 5 else 4
+    assert e.size == nv * elems and e.min() >= 0 and e.max() < nodes
+    assert abs(v.sum() - area) <= 1e-12 * area
+    assert (v > 0).all()
+    c = e.reshape(-1, nv)
+    if etype == 5:
+        a, b, q = x[c[:, 0], :2], x[c[:, 1], :2], x[c[:, 2], :2]
+        area_e = 0.5 * np.abs((b[:, 0] - a[:, 0]) * (q[:, 1] - a[:, 1]) -
+                              (q[:, 0] - a[:, 0]) * (b[:, 1] - a[:, 1]))
+        ref = np.zeros(nodes)
+        for k in range(3):
+            np.add.at(ref, c[:, k], area_e / 3.0)
+        assert np.allclose(v, ref, rtol=1e-12, atol=0)
+    else:
+        # unit squares: corner nodes 1/4, edge nodes 1/2, interior nodes 1
+        on_x = np.isclose(x[:, 0], 0) | np.isclose(x[:, 0], 5)
+        on_y = np.isclose(x[:, 1], 0) | np.isclose(x[:, 1], 5)
+        ref = np.where(on_x & on_y, 0.25, np.where(on_x | on_y, 0.5, 1.0))
+        assert np.allclose(v, ref, rtol=1e-10, atol=0)
+
+
+def test_gmsh_refusals(host_bin, tmp_path):
+    deck, _ = gmsh_case(tmp_path, "triangle")
+    msh = tmp_path / "mesh_2d_triangle.msh"
+    text = msh.read_text()
+    msh.write_text(text.replace("4.1 0 8", "4.1 1 8", 1))
+    rc, out = run_tool(host_bin / "nlm_fd", "--dump-mesh", "-i", deck.name,
+                       cwd=tmp_path, check=False)
+    assert rc != 0 and "binary" in out
+    msh.write_text(text)
+    deck.write_text(deck.read_text().replace("Dimension: 2", "Dimension: 3"))
+    rc, out = run_tool(host_bin / "nlm_fd", "--dump-mesh", "-i", deck.name,
+                       cwd=tmp_path, check=False)
+    assert rc != 0 and "dimension 2" in out
+
