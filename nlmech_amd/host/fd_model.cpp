@@ -581,7 +581,7 @@ void FDModel<T>::output() {
   getState(&u, &v, &f);
   const std::string file =
       od->d_path + "output_" + std::to_string(d_n / od->d_dtOutCriteria);
-  rw::VtuWriter w(file, od->d_compressType);
+  rw::VtuWriter w(file, od->d_compressType, od->d_outFormat == "legacy_vtk");
   w.appendNodes(d_mesh->getNodes(), u.data());
   if (d_mesh->getNumElements() != 0 && od->d_performFEOut)
     w.appendCells(d_mesh->getElementType(),

@@ -370,6 +370,11 @@ void Input::setOutputDeck(const yaml::Node& config) {
   const auto& e = config["Output"];
   if (!e) return;
   if (e["File_Format"]) d_output.d_outFormat = e["File_Format"].str();
+  // rw::writer::Writer::open (src/rw/writer.cpp:30-40) also knows "msh" (the
+  // gmsh writer) and writes nothing for other names
+  if (d_output.d_outFormat != "vtu" && d_output.d_outFormat != "legacy_vtk")
+    throw DeckError("Error: Output File_Format " + d_output.d_outFormat +
+                    " is not supported (vtu or legacy_vtk).");
   if (e["Path"]) d_output.d_path = e["Path"].str();
   if (e["Tags"]) d_output.d_outTags = e["Tags"].as_strings();
   if (e["Output_Interval"]) d_output.d_dtOut = e["Output_Interval"].as_size();

@@ -126,7 +126,7 @@ void QuasiStaticModel<T>::output() {
   std::cout << "Output: time step = " << d_n << "\n";
   const std::string file =
       od->d_path + "output_" + std::to_string(d_n / od->d_dtOutCriteria);
-  rw::VtuWriter w(file, od->d_compressType);
+  rw::VtuWriter w(file, od->d_compressType, od->d_outFormat == "legacy_vtk");
   w.appendNodes(mesh.getNodes(), u.data());
   if (mesh.getNumElements() != 0 && od->d_performFEOut)
     w.appendCells(mesh.getElementType(), mesh.getElementConnectivities());

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstring>
 #include <fstream>
+#include <iostream>
 #include <sstream>
 #include <stdexcept>
 
@@ -135,11 +136,52 @@ int vtk_cell_vertices(int type) {
 // Writer
 // ---------------------------------------------------------------------------
 
-VtuWriter::VtuWriter(const std::string& filename, const std::string& compress)
-    : file_(filename + ".vtu"), zlib_(compress == "zlib") {}
+namespace {
+// stream formatting of the legacy writer (std::ofstream defaults)
+template <class T>
+std::string legacy_rows(const T* data, int64_t n, int ncomp) {
+  std::ostringstream o;
+  for (int64_t i = 0; i < n; i++)
+    for (int c = 0; c < ncomp; c++)
+      o << data[i * ncomp + c] << (c + 1 < ncomp ? " " : "\n");
+  return o.str();
+}
+void legacy_warning() {
+  std::cerr << "Warning: Field data is not implementend in the legacy vtk "
+               "writer"
+            << std::endl;
+}
+}  // namespace
+
+VtuWriter::VtuWriter(const std::string& filename, const std::string& compress,
+                     bool legacy)
+    : file_(filename + (legacy ? ".vtk" : ".vtu")),
+      zlib_(compress == "zlib"),
+      legacy_(legacy) {
+  if (legacy_)
+    legacy_text_ =
+        "# vtk DataFile Version 3.0\nNLMech legacy vtk writer\nASCII\n";
+}
 
 void VtuWriter::appendNodes(const std::vector<double>& nodes, const double* u) {
   npoints_ = int64_t(nodes.size() / 3);
+  if (legacy_) {
+    std::ostringstream o;
+    o << "DATASET UNSTRUCTURED_GRID\nPOINTS " << npoints_ << " double\n";
+    for (int64_t i = 0; i < npoints_; i++) {
+      double p[3];
+      for (int d = 0; d < 3; d++)
+        p[d] = u ? nodes[3 * i + d] + u[3 * i + d] : nodes[3 * i + d];
+      o << p[0] << " " << p[1] << " " << p[2] << "\n";
+    }
+    o << "CELLS " << npoints_ << " " << 2 * npoints_ << "\n";
+    for (int64_t i = 0; i < npoints_; i++) o << "1 " << i << "\n";
+    o << "CELL_TYPES " << npoints_ << "\n";
+    for (int64_t i = 0; i < npoints_; i++) o << "1\n";
+    o << "POINT_DATA " << npoints_ << "\n";
+    legacy_text_ += o.str();
+    return;
+  }
   std::vector<float> p(nodes.size());
   for (size_t k = 0; k < nodes.size(); k++) {
     double v = nodes[k];
@@ -150,6 +192,10 @@ void VtuWriter::appendNodes(const std::vector<double>& nodes, const double* u) {
 }
 
 void VtuWriter::appendCells(int type, const std::vector<int64_t>& conn) {
+  if (legacy_) {
+    legacy_warning();
+    return;
+  }
   const int nv = vtk_cell_vertices(type);
   if (nv == 0) throw std::runtime_error("vtu: unsupported cell type");
   ncells_ = int64_t(conn.size()) / nv;
@@ -164,35 +210,79 @@ void VtuWriter::appendCells(int type, const std::vector<int64_t>& conn) {
 
 void VtuWriter::appendPointData(const std::string& name, const double* data,
                                 int ncomp) {
+  if (legacy_) {
+    if (ncomp == 6) {
+      // (xx, yy, zz, yz, xz, xy) as the full 3 x 3 matrix, row by row
+      std::vector<double> m(size_t(npoints_) * 9);
+      const int at[9] = {0, 5, 4, 5, 1, 3, 4, 3, 2};
+      for (int64_t i = 0; i < npoints_; i++)
+        for (int k = 0; k < 9; k++) m[size_t(9 * i + k)] = data[6 * i + at[k]];
+      legacy_text_ += "TENSORS " + name + " double \n";
+      legacy_text_ += legacy_rows(m.data(), 3 * npoints_, 3);
+    } else {
+      legacy_text_ += "SCALARS " + name + " double " + std::to_string(ncomp) +
+                      "\nLOOKUP_TABLE default\n";
+      legacy_text_ += legacy_rows(data, npoints_, ncomp);
+    }
+    return;
+  }
   point_.push_back({name, "Float64", ncomp, npoints_,
                     bytes_of(data, size_t(npoints_) * ncomp)});
 }
 
 void VtuWriter::appendPointData(const std::string& name, const float* data,
                                 int ncomp) {
+  if (legacy_) {
+    legacy_text_ += "SCALARS " + name + " float " + std::to_string(ncomp) +
+                    "\nLOOKUP_TABLE default\n";
+    legacy_text_ += legacy_rows(data, npoints_, ncomp);
+    return;
+  }
   std::vector<double> d(size_t(npoints_) * ncomp);
   for (size_t k = 0; k < d.size(); k++) d[k] = data[k];
   appendPointData(name, d.data(), ncomp);
 }
 
 void VtuWriter::appendPointData(const std::string& name, const uint8_t* data) {
+  if (legacy_) {
+    // (as integers; the reference streams the bytes as characters)
+    std::vector<int> d(data, data + npoints_);
+    legacy_text_ += "SCALARS " + name + " int 1\nLOOKUP_TABLE default\n";
+    legacy_text_ += legacy_rows(d.data(), npoints_, 1);
+    return;
+  }
   std::vector<double> d(static_cast<size_t>(npoints_));
   for (size_t k = 0; k < d.size(); k++) d[k] = data[k];
   appendPointData(name, d.data(), 1);
 }
 
 void VtuWriter::appendFieldData(const std::string& name, double value) {
+  if (legacy_) {
+    legacy_warning();
+    return;
+  }
   field_.push_back({name, "Float64", 1, 1, bytes_of(&value, 1)});
 }
 
 void VtuWriter::appendFieldData(const std::string& name,
                                 const std::vector<uint8_t>& bytes) {
+  if (legacy_) {
+    legacy_warning();
+    return;
+  }
   field_.push_back({name, "UInt8", 1, int64_t(bytes.size()), bytes});
 }
 
 void VtuWriter::close() {
   if (closed_) return;
   closed_ = true;
+  if (legacy_) {
+    std::ofstream f(file_);
+    if (!f) throw std::runtime_error("cannot write " + file_);
+    f << legacy_text_;
+    if (!f) throw std::runtime_error("write error on " + file_);
+    return;
+  }
   std::vector<const Array*> order;
   for (const auto& a : field_) order.push_back(&a);
   for (const auto& a : point_) order.push_back(&a);

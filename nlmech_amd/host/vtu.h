@@ -31,8 +31,14 @@ int vtk_cell_vertices(int type);
 class VtuWriter {
  public:
   // filename without the ".vtu" extension (VtkWriter::VtkWriter appends it);
-  // compress: "zlib" or anything else for no compression.
-  VtuWriter(const std::string& filename, const std::string& compress);
+  // compress: "zlib" or anything else for no compression.  legacy: the ASCII
+  // legacy format instead (File_Format: legacy_vtk, LegacyVtkWriter,
+  // src/rw/legacyVtkWriter.cpp): "<filename>.vtk", points x + u, one vertex
+  // cell per node, point data as SCALARS (TENSORS for the 6-component
+  // symmetric tensors), numbers in the stream's default format; cells and
+  // field data are left out with the reference's warning.
+  VtuWriter(const std::string& filename, const std::string& compress,
+            bool legacy = false);
 
   // Nodes (AoS [3n]) plus optional displacement (AoS [3n] or null):
   // VtkWriter::appendNodes.
@@ -49,8 +55,11 @@ class VtuWriter {
   void appendFieldData(const std::string& name, double value);
   void appendFieldData(const std::string& name,
                        const std::vector<uint8_t>& bytes);
-  // VtkWriter::addTimeStep: field array "TIME".
-  void addTimeStep(double t) { appendFieldData("TIME", t); }
+  // VtkWriter::addTimeStep: field array "TIME" (nothing in the legacy
+  // format).
+  void addTimeStep(double t) {
+    if (!legacy_) appendFieldData("TIME", t);
+  }
   // Writes the file; throws std::runtime_error on an I/O error.
   void close();
 
@@ -64,6 +73,8 @@ class VtuWriter {
   };
   std::string file_;
   bool zlib_;
+  bool legacy_ = false;
+  std::string legacy_text_;  // the legacy file's text so far
   int64_t npoints_ = 0;
   std::vector<Array> field_, point_;
   Array points_;

@@ -16,6 +16,9 @@ container, where the reference tree is mounted):
        compare.txt (the dc tool's known answer, test/CMakeLists.txt:61-64)
    - testFracture_nodes.csv <- data/test/geometry/testFracture_nodes.csv
        (the node set of TestGeom, src/test/geometry/testGeomLib.cpp:42-86)
+   - qs1d/input_legacy_vtk.yaml <- examples/qsModel/1D/input_legacy_vtk.yaml
+       (the deck of quasistatic.1D.elastic_legacy_vtk, test/CMakeLists.txt:
+       88-91)
    - gmsh/ <- examples/io/{input_2d_gmsh_quad,input_2d_gmsh_triangle}.yaml
        and their meshes mesh_2d_{quad,triangle}.msh (gmsh format 4.1; the
        inputs of mesh.2d.read.{quad,triangle}.gmsh, test/CMakeLists.txt:
@@ -60,6 +63,8 @@ def reference_fixtures(ref: Path) -> None:
                         expected_l2_sup=np.array(vals))
     shutil.copyfile(ref / "data/test/geometry/testFracture_nodes.csv",
                     HERE / "testFracture_nodes.csv")
+    shutil.copyfile(ref / "examples/qsModel/1D/input_legacy_vtk.yaml",
+                    HERE / "qs1d/input_legacy_vtk.yaml")
     g = HERE / "gmsh"
     g.mkdir(exist_ok=True)
     for f in ("input_2d_gmsh_quad.yaml", "input_2d_gmsh_triangle.yaml",

@@ -28,7 +28,10 @@ Bars:
   - gmsh meshes: the reference's mesh.2d.read.{quad,triangle}.gmsh CTests
     (examples/io decks, format 4.1, unchanged) run and write the nodal
     volumes of their meshes; the logo deck (examples/fdModel/logo, format
-    2.2, shortened) matches the oracle on the same mesh.
+    2.2, shortened) matches the oracle on the same mesh;
+  - the legacy VTK output (File_Format: legacy_vtk) of the reference's 1D
+    quasi-static CTest equals the reference-held result to its printed
+    precision.
 """
 import shutil
 from pathlib import Path
@@ -616,6 +619,68 @@ def test_reference_quasi_static_ctest(host_bin, tmp_path, case):
     l2 = vals[0::2]
     assert len(l2) == 5 and max(l2) < 1e-7, l2
     assert rc == 0
+
+
+def read_legacy_vtk(path):
+    """Points and point data of an ASCII legacy .vtk unstructured grid:
+    {"Points": (n, 3), name: (n,) / (n, k) for SCALARS, (n, 3, 3) for
+    TENSORS}."""
+    lines = Path(path).read_text().splitlines()
+    out, n, i = {}, 0, 0
+    while i < len(lines):
+        t = lines[i].split()
+        if t and t[0] == "POINTS":
+            n = int(t[1])
+            out["Points"] = np.loadtxt(lines[i + 1:i + 1 + n], ndmin=2)
+            i += 1 + n
+        elif t and t[0] == "SCALARS":
+            a = np.loadtxt(lines[i + 2:i + 2 + n], ndmin=2)
+            out[t[1]] = a[:, 0] if int(t[3]) == 1 else a
+            i += 2 + n
+        elif t and t[0] == "TENSORS":
+            out[t[1]] = np.loadtxt(lines[i + 1:i + 1 + 3 * n],
+                                   ndmin=2).reshape(n, 3, 3)
+            i += 1 + 3 * n
+        else:
+            i += 1
+    return out
+
+
+def test_reference_legacy_vtk_ctest(host_bin, tmp_path):
+    """quasistatic.1D.elastic_legacy_vtk (test/CMakeLists.txt:88-91): the
+    1D quasi-static deck with File_Format: legacy_vtk, unchanged, writes
+    output_1.vtk in the ASCII legacy format (points x + u, SCALARS, TENSORS
+    for the stress and strain tensors, numbers at the stream's default 6
+    significant digits), whose fields equal the reference-held
+    output_res.vtu to that precision: Displacement, Force, Strain_Energy,
+    the tensors (the .vtu's (xx, yy, zz, yz, xz, xy) as full matrices),
+    Fixity and Neighbors."""
+    src = GOLDEN / "qs1d"
+    for f in ("input_mesh.yaml", "input_legacy_vtk.yaml", "output_res.vtu"):
+        shutil.copy(src / f, tmp_path / f)
+    progress("[gpu] reference legacy vtk CTest (qs1d)")
+    run_tool(host_bin / "nlm_mesh", "-i", "input_mesh.yaml", "-d", "1",
+             cwd=tmp_path)
+    run_tool(host_bin / "nlm_fd", "-i", "input_legacy_vtk.yaml", cwd=tmp_path)
+    assert not list(tmp_path.glob("output_[0-9]*.vtu"))
+    got = read_legacy_vtk(tmp_path / "output_1.vtk")
+    ref = read_vtu(str(tmp_path / "output_res.vtu"))
+
+    def close(a, b):
+        a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+        return np.allclose(a, b, rtol=1e-5, atol=1e-5 * np.abs(b).max())
+
+    # (both files hold the deformed positions x + u)
+    assert close(got["Points"], ref["Points"])
+    for tag in ("Displacement", "Force"):
+        assert close(got[tag], ref[tag]), tag
+    assert close(got["Strain_Energy"], ref["Strain_Energy"])
+    for tag in ("Strain_Tensor", "Stress_Tensor"):
+        v = np.asarray(ref[tag], np.float64).reshape(-1, 6)
+        m = v[:, [0, 5, 4, 5, 1, 3, 4, 3, 2]].reshape(-1, 3, 3)
+        assert close(got[tag], m), tag
+    assert np.array_equal(got["Fixity"], np.asarray(ref["Fixity"]))
+    assert np.array_equal(got["Neighbors"], np.asarray(ref["Neighbors"]))
 
 
 # ------------------------------------------------------------ gmsh meshes
