@@ -154,6 +154,32 @@ def test_force_parity_identical_state(engine, oracle, mode):
         assert force_err(fg, fr, absF) <= FORCE_TOL[mode]
 
 
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("inf", [1, 2], ids=["linear", "gaussian"])
+def test_force_parity_influence_functions(engine, oracle, mode, inf):
+    """The same from an identical deformed state with the linear and the
+    Gaussian influence functions (default parameters, influenceFn.h), whose
+    value the fast paths evaluate from the bond length (the staged
+    FP32-force kernel from its scaled length)."""
+    progress(f"[gpu] force parity influence={inf} mode={mode}")
+    p = pr.plate2d(40, 40, 1e-3, 3, 1e-7, 1, inf_type=inf, inf_params=())
+    o = oracle.OracleFD(p)
+    u, v = _deformed_state(p, 3, sigma=4e-6)
+    with engine.GpuFD(p, precision=mode) as g:
+        f0 = np.zeros((p.n, 3))
+        g.set_state(u=u, v=v, f=f0)
+        o.set_state(u=u, v=v, f=f0)
+        absF = o.force_abs_sum()
+        o.compute_forces()
+        g.compute_forces()
+        fr = o.get_state()[2]
+        fg = g.get_state()[2]
+        br, bg = o.get_bonds(), g.get_bonds()
+        assert br.sum() > 1000
+        assert np.array_equal(bg, br)
+        assert force_err(fg, fr, absF) <= FORCE_TOL[mode]
+
+
 # ------------------------------------------------------------ trajectories
 @pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("pull", [0.5, 30.0])
