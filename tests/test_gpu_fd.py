@@ -323,19 +323,30 @@ def test_reaction_and_external_force_parity(engine, oracle, scale):
     assert np.array_equal(fg, fr) and np.abs(fr).max() > 0
 
 
-def test_verlet_parity(engine, oracle):
-    progress("[gpu] velocity verlet")
+@pytest.mark.parametrize("mode", MODES)
+def test_verlet_parity(engine, oracle, mode):
+    """velocity_verlet (fDModel.cpp:446-525: half-step velocity, position,
+    force-only kernel, second half-step) against the oracle's Verlet run;
+    FP64 modes to 1e-9 with identical bonds, FP32-force to 1e-5."""
+    progress(f"[gpu] velocity verlet mode={mode}")
     p = pr.plate2d(30, 30, 1e-3, 3, 1e-7, 40, pull=10.0)
     o = oracle.OracleFD(p)
     o.init()
     o.step(40, scheme="verlet")
     ur, vr, _ = o.get_state()
-    with engine.GpuFD(p, scheme=NLM_TIME_VELOCITY_VERLET) as g:
+    br = o.get_bonds()
+    with engine.GpuFD(p, precision=mode,
+                      scheme=NLM_TIME_VELOCITY_VERLET) as g:
         g.init()
         g.step(40)
         ug, vg, _ = g.get_state()
-        assert np.array_equal(g.get_bonds(), o.get_bonds())
-    assert rel_l2(ug, ur) <= 1e-9 and rel_l2(vg, vr) <= 1e-8
+        bg = g.get_bonds()
+    if mode == NLM_PREC_FP32_FORCE:
+        assert rel_l2(ug, ur) <= 1e-5
+        assert np.count_nonzero(bg != br) <= 1e-3 * max(br.sum(), 1)
+    else:
+        assert np.array_equal(bg, br)
+        assert rel_l2(ug, ur) <= 1e-9 and rel_l2(vg, vr) <= 1e-8
 
 
 @pytest.mark.parametrize("mode", [NLM_PREC_FP64_EXACT, NLM_PREC_FP64,
