@@ -179,6 +179,19 @@ __device__ __forceinline__ void fd_step_body(
     if constexpr (REL) org = a.blk_org[lb];
     hb = a.halo_ptr[lb];
     const int hn = int(a.halo_ptr[lb + 1] - hb);
+    if constexpr (REL) {
+      // record 0: zero force constant, far from every node (its bonds come
+      // out as exactly zero forces, never near a decision)
+      if (threadIdx.x == 0) {
+        HaloRec32 r0;
+        r0.u = make_double2(0., 0.);
+        r0.xr = make_float2(1e15f, 1e15f);
+        r0.cv = 0.f;
+        r0.flags = 0u;
+        r0.pad[0] = r0.pad[1] = 0u;
+        halo_lds[0] = r0;
+      }
+    }
     for (int t = threadIdx.x; t < hn; t += bn) {
       const uint32_t g = uint32_t(a.halo[hb + t]);
       Rec r;
@@ -198,7 +211,7 @@ __device__ __forceinline__ void fd_step_body(
         r.u = ug;
         r.vol = vg;
       }
-      halo_lds[t] = r;
+      halo_lds[REL ? t + 1 : t] = r;
     }
     __syncthreads();
   }
@@ -250,7 +263,9 @@ __device__ __forceinline__ void fd_step_body(
   // whose positions and displacements are gathered, so the index latency
   // overlaps the previous chunk's arithmetic.  Every load is unconditional:
   // index fetches past the slice return 0 (a valid halo entry or node), and
-  // slots at or past the row length are masked out of the arithmetic.  A
+  // slots at or past the row length are masked out of the arithmetic (in the
+  // staged FP32-force kernel entry 0 and the padded entries address the
+  // zero-weight record 0 instead, which needs no mask).  A
   // slot's 64 entries start at a wavefront-uniform address; the lane offset
   // is the only per-lane part.
   // The indices are fetched through a buffer resource over the slice: the
@@ -365,7 +380,9 @@ __device__ __forceinline__ void fd_step_body(
 #pragma unroll
       for (int q = 0; q < U; ++q) {
         SlotMasks km;
-        km.act = __builtin_amdgcn_ballot_w64(k0 + q < len);
+        // (staged FP32-force: slots past a row's length address the zero-
+        // weight record 0, whose force is exactly zero)
+        km.act = REL ? ~0ull : __builtin_amdgcn_ballot_w64(k0 + q < len);
         km.brk = NOFAIL ? (int_m & __builtin_amdgcn_ballot_w64(
                                        (flj[q] & kFlagNonInterior) == 0u))
                         : ~0ull;
@@ -405,7 +422,10 @@ __device__ __forceinline__ void fd_step_body(
         for (int q = 0; q < U; ++q) {
           if (ex[q] == 0ull) continue;
           bool fsn = false;
-          if (__builtin_amdgcn_inverse_ballot_w64(ex[q])) {
+          // (lanes whose slot is past their row, in the staged FP32-force
+          // kernel, evaluate nothing)
+          if (__builtin_amdgcn_inverse_ballot_w64(ex[q]) &&
+              (!REL || k0 + q < len)) {
             bool brk = NOFAIL ? (int_i && !(flj[q] & kFlagNonInterior)) : true;
             bool fs0 = __builtin_amdgcn_inverse_ballot_w64(w[q]);
             bool fs = fs0;
@@ -414,7 +434,7 @@ __device__ __forceinline__ void fd_step_body(
             if constexpr (REL) {
               // FP64 values of the neighbor from its halo entry
               const uint32_t g = uint32_t(
-                  a.halo[hb + jc[q] / uint32_t(sizeof(HaloRec32))]);
+                  a.halo[hb + jc[q] / uint32_t(sizeof(HaloRec32)) - 1]);
               load3<DIM>(ld32(a.x, g), xs);
               load3<DIM>(ld32(a.u, g), us);
               vs = ld32(a.vol, g);
@@ -943,7 +963,9 @@ void launch_t(nlm_ctx* c, const FdArgs<DIM>& a, double tf, double tn) {
       // one workgroup per tile of the layout's halos
       const int bn = c->fd.tile;
       const unsigned gt = grid_xcd(c->fd.sell.nslices * 64, bn);
-      size_t lds = size_t(a.halo_cap) * sizeof(HaloRecOf<MODE>);
+      // (the FP32-force records follow a zero-weight record 0)
+      size_t lds = size_t(a.halo_cap + (MODE == kModeFp32 ? 1 : 0)) *
+                   sizeof(HaloRecOf<MODE>);
       if constexpr (MODE == kModeFp32 && U == 2) {
         hipLaunchKernelGGL((k_fd_w6<DIM, MODE, KIND, BASE_F, INF, U, NOFAIL>),
                            dim3(gt), dim3(bn), lds, c->stream, a, bits_ro,
@@ -1200,6 +1222,39 @@ int n_time_params_u(int fn) {
 
 using namespace nlm;
 
+namespace {
+
+// LDS byte offsets of the staged step kernel's slot entries (halo-local
+// record index times the record size).  In the FP32-force mode the records
+// start at index 1: record 0 is a zero-weight record far from every node,
+// addressed by the padded slots (past a row's length) and by the fetches
+// past a slice, so the common path needs no per-slot row-length test.
+std::vector<uint16_t> staged_entries(const std::vector<uint16_t>& colL,
+                                     int precision, const int64_t* sl_ptr,
+                                     int64_t nslices, const uint16_t* rowlen,
+                                     int64_t n) {
+  std::vector<uint16_t> out(colL.size(), 0);
+  if (precision != NLM_PREC_FP32_FORCE) {
+    for (size_t e = 0; e < colL.size(); e++)
+      out[e] = uint16_t(uint32_t(colL[e]) * uint32_t(sizeof(HaloRec)));
+    return out;
+  }
+  for (int64_t sl = 0; sl < nslices; sl++) {
+    const int64_t e0 = sl_ptr[sl];
+    const int64_t width = (sl_ptr[sl + 1] - e0) / 64;
+    for (int64_t k = 0; k < width; k++)
+      for (int lane = 0; lane < 64; lane++) {
+        const int64_t e = e0 + 64 * k + lane, a = sl * 64 + lane;
+        if (a < n && k < int64_t(rowlen[a]))
+          out[size_t(e)] = uint16_t((uint32_t(colL[size_t(e)]) + 1u) *
+                                    uint32_t(sizeof(HaloRec32)));
+      }
+  }
+  return out;
+}
+
+}  // namespace
+
 extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
   if (!c) return NLM_ERR_INVALID;
   if (!p || p->n <= 0 || !p->x || !p->vol)
@@ -1330,15 +1385,12 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
   }
   // the bond-based staged kernel reads halo-local entries as LDS byte
   // offsets of its records (below 64 KiB by the budget above; the FP32-force
-  // mode's records are the 32-byte block-relative ones)
-  std::vector<uint16_t> colL_off;
+  // mode's records are the 40-byte block-relative ones after a zero-weight
+  // record 0)
   if (s.tiled) {
-    const uint32_t rec = p->precision == NLM_PREC_FP32_FORCE
-                             ? uint32_t(sizeof(HaloRec32))
-                             : uint32_t(sizeof(HaloRec));
-    colL_off.resize(L.colL.size());
-    for (size_t e = 0; e < colL_off.size(); e++)
-      colL_off[e] = uint16_t(uint32_t(L.colL[e]) * rec);
+    const std::vector<uint16_t> colL_off =
+        staged_entries(L.colL, p->precision, L.sl_ptr.data(), L.nslices,
+                       L.rowlen.data(), n);
     if ((rc = up(s.colL, colL_off.data(), sizeof(uint16_t) * nent))) return rc;
   } else if ((rc = up(s.colL, L.colL.data(), sizeof(uint16_t) * nent))) {
     return rc;
@@ -2285,10 +2337,10 @@ int retile(nlm_ctx* c, int bn) {
       return fail(c, NLM_ERR_INVALID,
                   "tile of " + std::to_string(bn) +
                       " nodes: its largest halo exceeds the LDS budget");
-    const uint32_t rec = s.precision == NLM_PREC_FP32_FORCE
-                             ? uint32_t(sizeof(HaloRec32))
-                             : uint32_t(sizeof(HaloRec));
-    for (auto& v : colL) v = uint16_t(uint32_t(v) * rec);
+    std::vector<uint16_t> rl(static_cast<size_t>(n));
+    NLM_HIP(c, hipMemcpy(rl.data(), s.sell.rowlen.p, sizeof(uint16_t) * n,
+                         hipMemcpyDeviceToHost));
+    colL = staged_entries(colL, s.precision, slp.data(), ns, rl.data(), n);
   } else {
     int lds_max = 0;
     NLM_HIP(c, hipDeviceGetAttribute(&lds_max,
