@@ -429,7 +429,8 @@ __device__ __forceinline__ void bond_fp32_core(
 // (x - origin) in FP32 and tau u in FP64).  Then q = dx.dv / |dx| = tau S r,
 // Ss = q / |dx| = tau S / sigma and b = q Ss = beta log2(e) r S^2 is at once
 // the exponent of the bond force, exp(-beta r S^2) = exp2(-b), and the break
-// test's quantity (its guard band break_lo_s/break_hi_s scaled on the host).
+// test's quantity (its guard band's lower bound break_lo_s scaled on the
+// host).
 // The force S exp(-beta r S^2) V_j c4 J (fac / r) (x_j - x_i) equals
 // Ss exp2(-b) cvs (fac / |dx|) dx with cvs = V_j c4 J sigma / tau.  The
 // volume-correction factor enters as fac / |dx| =
@@ -453,17 +454,19 @@ __device__ __forceinline__ void bond_fp32_scaled(
   float b = q * Ss;               // beta log2(e) r S^2
   const float facr = __builtin_amdgcn_fmed3f(
       __builtin_fmaf(m.up_rcp_h_f, rinv, -m.rcp_h_s), 0.f, rinv);
-  const unsigned long long v_hi = __builtin_amdgcn_ballot_w64(b > m.break_hi_s);
+  // One vote on the guard band's lower bound: a bond that may break (inside
+  // the band, or beyond it: breaking in this step) is decided, and its force
+  // taken, by the deferred evaluation, which also reports the new broken
+  // bits; band carries those bonds, nb stays empty here.
   const unsigned long long v_lo =
       __builtin_amdgcn_ballot_w64(b >= m.break_lo_s);
-  const unsigned long long cand = k.act & k.brk & ~k.w;
-  nb = cand & v_hi;
-  band = cand & v_lo & ~v_hi;
-  const unsigned long long fs = k.w | nb;
+  band = k.act & k.brk & ~k.w & v_lo;
+  nb = 0ull;
+  const unsigned long long fsb = k.w & k.brk;
   const unsigned long long v_vex =
       __builtin_amdgcn_ballot_w64(fabsf(dd - m.low_dd_f) <= m.vex_dd_tol_f);
-  ex = (v_vex | band | (fs & k.brk & m.contact_mask)) & k.act;
-  const unsigned long long keep = k.act & ~(fs & k.brk) & ~ex;
+  ex = (v_vex | band | (fsb & m.contact_mask)) & k.act;
+  const unsigned long long keep = k.act & ~fsb & ~ex;
   float ex_ = __builtin_amdgcn_exp2f(-b);
   if (NOFAIL) ex_ = __builtin_amdgcn_inverse_ballot_w64(k.brk) ? ex_ : 1.0f;
   float sr = (Ss * ex_) * (cvs * facr);
