@@ -1563,8 +1563,6 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
     m.rcp_sigma_f = float(1.0 / m.sigma);
     m.break_lo_s = m.irreversible ? float(double(m.break_lo_f) * bl)
                                   : std::numeric_limits<float>::infinity();
-    m.break_hi_s = m.irreversible ? float(double(m.break_hi_f) * bl)
-                                  : std::numeric_limits<float>::infinity();
     // r within 3e-5 (relative) of low_band <=> r^2 within about 6e-5 of
     // low_band^2; the band on r^2 is taken a little wider
     const double lows = m.sigma * m.low_band;

@@ -106,7 +106,7 @@ struct RnpDev {
   float c4f_infl_s;        // c4f_infl sigma / tau
   float rcp_h_s;           // 1 / (sigma h)
   float rcp_sigma_f;       // 1 / sigma
-  float break_lo_s, break_hi_s;  // break_lo_f, break_hi_f times beta log2 e
+  float break_lo_s;        // break_lo_f times beta log2 e
   float low_dd_f;          // (sigma low_band)^2
   float vex_dd_tol_f;      // half-width of the edge band on (sigma r)^2
 };
