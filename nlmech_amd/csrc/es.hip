@@ -414,10 +414,11 @@ __global__ __launch_bounds__(TILED ? 512 : 256) void k_es_dil(
                                       1.f);
         else
           c = fmin((cup - lx) * rch, Real(1));
-        Real dterm = lx * ext * c * Real(vj[q]);
-        dterm = __builtin_amdgcn_inverse_ballot_w64(keep) ? dterm : Real(0);
-        if (q % 2 == 0) pf[q / 2] = dterm;
-        else pf[q / 2] += dterm;
+        // (the odd slot's term fused into its pair: fma(l e c, V_j, T_2m))
+        Real lec = lx * ext * c;
+        lec = __builtin_amdgcn_inverse_ballot_w64(keep) ? lec : Real(0);
+        if (q % 2 == 0) pf[q / 2] = lec * Real(vj[q]);
+        else pf[q / 2] = fmaR(lec, Real(vj[q]), pf[q / 2]);
         ex_any |= ex[q];
       }
       if (ex_any != 0ull) {  // rare: whole wavefronts skip it
@@ -768,9 +769,10 @@ __global__ __launch_bounds__(TILED ? 512 : 256) void k_es_force(
         }
         float sr = t * c * float(vj[q]) * ry;
         sr = __builtin_amdgcn_inverse_ballot_w64(live) ? sr : 0.f;
+        // the odd slot's force fused into its pair: fma(s, Y, F_2m)
         for (int d = 0; d < DIM; d++) {
           if (q % 2 == 0) pf[q / 2][d] = sr * Y[d];
-          else pf[q / 2][d] += sr * Y[d];
+          else pf[q / 2][d] = __builtin_fmaf(sr, Y[d], pf[q / 2][d]);
         }
       }
     }
