@@ -98,6 +98,8 @@ void read_msh(const std::string& file, std::vector<double>* x,
   };
   double version = 0.;
   bool have_nodes = false, have_elems = false;
+  // counts read from the file are checked before they size anything
+  const int64_t kMaxCount = int64_t(1) << 36;
   std::vector<int64_t> tri, quad;
   x->clear();
   auto put_node = [&](int64_t tag, double px, double py, double pz) {
@@ -135,6 +137,7 @@ void read_msh(const std::string& file, std::vector<double>* x,
         int64_t n = 0;
         std::istringstream(next()) >> n;
         if (n <= 0) bad("no nodes");
+        if (n > kMaxCount) bad("node count out of range");
         x->assign(size_t(3 * n), 0.);
         for (int64_t i = 0; i < n; i++) {
           std::istringstream ss(next());
@@ -147,12 +150,14 @@ void read_msh(const std::string& file, std::vector<double>* x,
         int64_t nblk = 0, n = 0, tmin = 0, tmax = 0;
         std::istringstream(next()) >> nblk >> n >> tmin >> tmax;
         if (n <= 0) bad("no nodes");
+        if (n > kMaxCount) bad("node count out of range");
         if (tmax != n) bad("node tags are not 1..N");
         x->assign(size_t(3 * n), 0.);
         for (int64_t b = 0; b < nblk; b++) {
           int edim = 0, etag = 0, param = 0;
           int64_t nb = 0;
           std::istringstream(next()) >> edim >> etag >> param >> nb;
+          if (nb < 0 || nb > n) bad("bad node block size");
           std::vector<int64_t> tags(static_cast<size_t>(nb));
           for (int64_t k = 0; k < nb; k++)
             std::istringstream(next()) >> tags[size_t(k)];
@@ -176,6 +181,7 @@ void read_msh(const std::string& file, std::vector<double>* x,
           int64_t tag = 0;
           int type = 0, ntags = 0;
           if (!(ss >> tag >> type >> ntags)) bad("bad element line");
+          if (ntags < 0 || ntags > 64) bad("bad element tag count");
           for (int k = 0; k < ntags; k++) {
             int64_t t;
             ss >> t;

@@ -296,3 +296,30 @@ def test_gmsh_refusals(host_bin, tmp_path):
                        cwd=tmp_path, check=False)
     assert rc != 0 and "dimension 2" in out
 
+
+def test_gmsh_malformed_counts(host_bin, tmp_path):
+    """Counts read from a damaged gmsh file are checked before they size
+    anything: a negative node-block size (format 4.1) and an absurd element
+    tag count (format 2.2) end in a deck error, not an allocation failure or
+    a run-away read loop."""
+    deck, _ = gmsh_case(tmp_path, "triangle")
+    msh = tmp_path / "mesh_2d_triangle.msh"
+    lines = msh.read_text().split("\n")
+    k = lines.index("$Nodes") + 2          # first node block's header
+    head = lines[k].split()
+    lines[k] = " ".join(head[:3] + ["-5"])
+    msh.write_text("\n".join(lines))
+    rc, out = run_tool(host_bin / "nlm_fd", "--dump-mesh", "-i", deck.name,
+                       cwd=tmp_path, check=False)
+    assert rc != 0 and "bad node block size" in out
+    deck, _ = gmsh_case(tmp_path, "logo")
+    msh = tmp_path / "mesh.msh"
+    lines = msh.read_text().split("\n")
+    k = lines.index("$Elements") + 2       # first element line
+    e = lines[k].split()
+    lines[k] = " ".join(e[:2] + ["2000000000"] + e[3:])
+    msh.write_text("\n".join(lines))
+    rc, out = run_tool(host_bin / "nlm_fd", "--dump-mesh", "-i", deck.name,
+                       cwd=tmp_path, check=False)
+    assert rc != 0 and "bad element tag count" in out
+
