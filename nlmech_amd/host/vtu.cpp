@@ -55,7 +55,7 @@ int b64_value(char c) {
 // removed).
 std::vector<uint8_t> b64_decode(const std::string& s, size_t pos,
                                 size_t nchars) {
-  if (pos + nchars > s.size())
+  if (pos > s.size() || nchars > s.size() - pos)
     throw std::runtime_error("vtu: base64 data truncated");
   std::vector<uint8_t> out;
   out.reserve(nchars / 4 * 3);
@@ -495,7 +495,10 @@ VtuReader::VtuReader(const std::string& filename) : file_(filename) {
 
 std::vector<uint8_t> VtuReader::decode(const ArrayRef& a) const {
   const size_t hs = header64_ ? 8 : 4;
+  // header words and sizes come from the file: every read of one is bounds
+  // checked and every size compared without overflow before it is used
   auto hval = [&](const std::vector<uint8_t>& b, size_t k) -> uint64_t {
+    if (k >= b.size() / hs) throw std::runtime_error("vtu: header truncated");
     if (header64_) {
       uint64_t v;
       std::memcpy(&v, b.data() + 8 * k, 8);
@@ -521,16 +524,18 @@ std::vector<uint8_t> VtuReader::decode(const ArrayRef& a) const {
     if (b64) {
       h = b64_decode(src, start, b64_len(hs));
       nbytes = size_t(hval(h, 0));
+      if (nbytes > src.size())
+        throw std::runtime_error("vtu: array data truncated");
       std::vector<uint8_t> all = b64_decode(src, start, b64_len(hs + nbytes));
       if (all.size() < hs + nbytes)
         throw std::runtime_error("vtu: array data truncated");
       out.assign(all.begin() + hs, all.begin() + hs + nbytes);
     } else {
-      if (start + hs > src.size())
+      if (start > src.size() || hs > src.size() - start)
         throw std::runtime_error("vtu: raw data truncated");
       h.assign(src.begin() + start, src.begin() + start + hs);
       nbytes = size_t(hval(h, 0));
-      if (start + hs + nbytes > src.size())
+      if (nbytes > src.size() - start - hs)
         throw std::runtime_error("vtu: raw data truncated");
       out.assign(src.begin() + start + hs, src.begin() + start + hs + nbytes);
     }
@@ -540,30 +545,41 @@ std::vector<uint8_t> VtuReader::decode(const ArrayRef& a) const {
   std::vector<uint8_t> first;
   size_t hdr_chars = 0;
   std::vector<uint8_t> hdr;
+  if (start > src.size()) throw std::runtime_error("vtu: truncated");
   if (b64) {
     first = b64_decode(src, start, b64_len(hs));
     const size_t nb = size_t(hval(first, 0));
+    if (nb > src.size() / hs) throw std::runtime_error("vtu: truncated");
     hdr_chars = b64_len(hs * (3 + nb));
     hdr = b64_decode(src, start, hdr_chars);
   } else {
-    if (start + hs > src.size()) throw std::runtime_error("vtu: truncated");
+    if (hs > src.size() - start) throw std::runtime_error("vtu: truncated");
     first.assign(src.begin() + start, src.begin() + start + hs);
     const size_t nb = size_t(hval(first, 0));
+    if (nb > src.size() / hs) throw std::runtime_error("vtu: truncated");
     hdr_chars = hs * (3 + nb);
-    if (start + hdr_chars > src.size())
+    if (hdr_chars > src.size() - start)
       throw std::runtime_error("vtu: truncated");
     hdr.assign(src.begin() + start, src.begin() + start + hdr_chars);
   }
   const size_t nb = size_t(hval(hdr, 0));
   const size_t bs = size_t(hval(hdr, 1));
   const size_t last = size_t(hval(hdr, 2));
+  // (zlib expands a block at most ~1032 times; the writer's blocks are
+  // 32 KiB)
+  if (bs > (size_t(1) << 31) || last > bs)
+    throw std::runtime_error("vtu: bad compressed block size");
   size_t total = 0;
-  for (size_t b = 0; b < nb; b++) total += size_t(hval(hdr, 3 + b));
+  for (size_t b = 0; b < nb; b++) {
+    const size_t cs = size_t(hval(hdr, 3 + b));
+    if (cs > src.size() - total) throw std::runtime_error("vtu: truncated");
+    total += cs;
+  }
   std::vector<uint8_t> comp;
   if (b64) {
     comp = b64_decode(src, start + hdr_chars, b64_len(total));
   } else {
-    if (start + hdr_chars + total > src.size())
+    if (total > src.size() - start - hdr_chars)
       throw std::runtime_error("vtu: truncated");
     comp.assign(src.begin() + start + hdr_chars,
                 src.begin() + start + hdr_chars + total);

@@ -23,6 +23,7 @@ the .vtu writer / reader, the mesh tool and the dc tool.
 No GPU work is issued here.
 """
 import json
+import re
 import shutil
 from pathlib import Path
 
@@ -68,6 +69,35 @@ def test_dc_tolerance_exit_status(host_bin, tmp_path):
         rc, _ = run_tool(host_bin / "nlm_dc", "-i", "input.yaml", "-k",
                          "fd_simple", cwd=tmp_path, check=False)
         assert rc == rc_expect
+
+
+def test_dc_corrupt_vtu_headers(host_bin, tmp_path):
+    """Sizes read from a .vtu file's compressed-block header are checked
+    before they are used: a block count or a block size far beyond the file
+    ends in a reader error (non-zero exit, "vtu:" message), not a crash."""
+    import base64
+    import struct
+    src = GOLDEN / "dc_fd_simple"
+    data = (src / "output_4.vtu").read_bytes()
+    us = data.index(b"_", data.index(b"<AppendedData"))
+    m = re.search(rb'Name="Velocity"[^>]*offset="(\d+)"', data)
+    at = us + 1 + int(m.group(1))
+    hdr = base64.b64decode(data[at:at + 24])     # UInt32 [nb, bs, last, cs]
+    assert struct.unpack("<I", hdr[:4])[0] == 1
+    for word, value in ((0, 0xFFFFFFF0), (1, 0x90000000)):
+        h = bytearray(hdr)
+        h[4 * word:4 * word + 4] = struct.pack("<I", value)
+        bad = data[:at] + base64.b64encode(bytes(h)) + data[at + 24:]
+        (tmp_path / "bad.vtu").write_bytes(bad)
+        (tmp_path / "input.yaml").write_text(
+            "Dimension: 2\n"
+            f"Filename_1: {src / 'output_2.vtu'}\n"
+            f"Filename_2: {tmp_path / 'bad.vtu'}\n"
+            "Out_Filename: compare.txt\nPrint_Screen: false\n"
+            "Compare_Tags:\n    - Velocity\n")
+        rc, out = run_tool(host_bin / "nlm_dc", "-i", "input.yaml", "-k",
+                           "fd_simple", cwd=tmp_path, check=False)
+        assert rc > 0 and "vtu:" in out, (word, rc, out[-300:])
 
 
 def test_mesh_tool_1d_equals_reference_mesh(host_bin, tmp_path):
