@@ -355,7 +355,6 @@ __global__ __launch_bounds__(TILED ? 512 : 256) void k_es_dil(
       const Real lowb = R32   ? Real(e.m.low_band_qf)
                         : F32 ? Real(e.m.low_band_f)
                               : Real(e.m.low_band);
-      const Real bhi = F32 ? Real(e.m.brk_hi_f) : Real(e.m.brk_hi);
       const Real blo = F32 ? Real(e.m.brk_lo_f) : Real(e.m.brk_lo);
       const Real cup = R32   ? Real(e.m.check_up_qf)
                        : F32 ? Real(e.m.check_up_f)
@@ -399,13 +398,15 @@ __global__ __launch_bounds__(TILED ? 512 : 256) void k_es_dil(
         const Real st = ext * rx;
         const unsigned long long v_vex =
             __builtin_amdgcn_ballot_w64(fabs(lx - lowb) <= vtol);
-        const unsigned long long v_hi = __builtin_amdgcn_ballot_w64(st > bhi);
+        // one vote on the guard band's lower bound: a bond that may break
+        // (inside the band, or beyond it) is decided, and its term taken, by
+        // the exact evaluation after the chunk, which also reports the new
+        // broken bits
         const unsigned long long v_lo = __builtin_amdgcn_ballot_w64(st >= blo);
-        const unsigned long long cand = act & ~w[q];
-        nb[q] = cand & v_hi;
-        band[q] = cand & v_lo & ~v_hi;
+        nb[q] = 0ull;
+        band[q] = act & ~w[q] & v_lo;
         ex[q] = (v_vex & act) | band[q];
-        const unsigned long long keep = act & ~(w[q] | nb[q]) & ~ex[q];
+        const unsigned long long keep = act & ~w[q] & ~ex[q];
         // volume-correction factor (check_up - |X|)/h clamped to [0, 1]
         // (in FP32 one multiply-add with the clamp as its output modifier)
         Real c;

@@ -1645,11 +1645,7 @@ extern "C" int nlm_fd_setup(nlm_ctx* c, const nlm_fd_problem* p) {
     es_set_quantum(e, org_range);
     e.brk_lo = e.breaking ? e.sc_break - kEsGuard64 * e.sc_break
                           : std::numeric_limits<double>::infinity();
-    e.brk_hi = e.breaking ? e.sc_break + kEsGuard64 * e.sc_break
-                          : std::numeric_limits<double>::infinity();
     e.brk_lo_f = e.breaking ? e.sc_break_f - kEsGuard32 * e.sc_break_f
-                            : std::numeric_limits<float>::infinity();
-    e.brk_hi_f = e.breaking ? e.sc_break_f + kEsGuard32 * e.sc_break_f
                             : std::numeric_limits<float>::infinity();
     if ((rc = es_prepare(c))) return rc;
   }

@@ -125,11 +125,12 @@ struct EsDev {
   int breaking;           // sc > 0
   int pad;
   float check_up_f, low_band_f, rcp_h_f, sc_break_f;
-  // FP32 pass 1: half-width of the volume-correction edge band; bounds of
-  // the break test's guard band on e/|X| (+inf when bonds never break)
-  float vex_tol_f, brk_lo_f, brk_hi_f;
-  // the same bounds in FP64 for the FP64 mode
-  double vex_tol, brk_lo, brk_hi;
+  // FP32 pass 1: half-width of the volume-correction edge band; lower bound
+  // of the break test's guard band on e/|X| (+inf when bonds never break;
+  // stretches at or above it are decided by the exact evaluation)
+  float vex_tol_f, brk_lo_f;
+  // the same in FP64 for the FP64 mode
+  double vex_tol, brk_lo;
   // FP32-force staged passes (3D): positions are fixed-point offsets from
   // the block's origin in units of the quantum q (a power of two), and the
   // bond arithmetic runs in units of q: 1/q, q^2 (dilatation factor) and q
